@@ -1,0 +1,240 @@
+"""ctypes binding of liberp.so (include/erp.h).
+
+The library is loaded from this package directory.  There is no fallback:
+if the HIP library is missing or fails to load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+from ._abi import (ERP_E_NOTREADY, ERP_OK, PRISM_DTYPE, SHOT_DTYPE, ErpParams)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liberp.so")
+
+_lib = None
+_lock = threading.Lock()
+
+EXPORTS = ("erp_ctx_create", "erp_ctx_destroy", "erp_last_error", "erp_mesh_upload",
+           "erp_mesh_release", "erp_targets_upload", "erp_targets_release", "erp_run",
+           "erp_run_dev", "erp_submit", "erp_poll", "erp_wait", "erp_run_footprint")
+
+
+class ErpError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"erp error {code}: {msg}")
+        self.code = code
+
+
+def lib() -> ctypes.CDLL:
+    """Load liberp.so once (raises if it is not built)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`"
+                " or `make -C europaraypaths_amd/csrc`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64, i32, u32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_uint32
+        P = ctypes.POINTER
+        L.erp_ctx_create.argtypes = [i32, P(vp)]
+        L.erp_ctx_create.restype = i32
+        L.erp_ctx_destroy.argtypes = [vp]
+        L.erp_ctx_destroy.restype = None
+        L.erp_last_error.argtypes = [vp]
+        L.erp_last_error.restype = ctypes.c_char_p
+        L.erp_mesh_upload.argtypes = [vp, vp, i64, P(vp)]
+        L.erp_mesh_upload.restype = i32
+        L.erp_mesh_release.argtypes = [vp]
+        L.erp_mesh_release.restype = None
+        L.erp_targets_upload.argtypes = [vp, vp, vp, i64, P(vp)]
+        L.erp_targets_upload.restype = i32
+        L.erp_targets_release.argtypes = [vp]
+        L.erp_targets_release.restype = None
+        run_args = [vp, P(ErpParams), vp, vp, vp, i64, vp, i64, vp, vp, u32]
+        L.erp_run.argtypes = run_args
+        L.erp_run.restype = i32
+        L.erp_run_dev.argtypes = run_args
+        L.erp_run_dev.restype = i32
+        L.erp_submit.argtypes = run_args + [P(vp)]
+        L.erp_submit.restype = i32
+        L.erp_poll.argtypes = [vp]
+        L.erp_poll.restype = i32
+        L.erp_wait.argtypes = [vp]
+        L.erp_wait.restype = i32
+        L.erp_run_footprint.argtypes = [P(ErpParams), i64, i64, i64]
+        L.erp_run_footprint.restype = i64
+        _lib = L
+        return L
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Mesh:
+    """A facet mesh resident in HBM (erp_mesh_upload)."""
+
+    def __init__(self, ctx: "Context", soa: np.ndarray):
+        soa = np.ascontiguousarray(soa, dtype=np.float32)
+        if soa.ndim != 2 or soa.shape[0] != 12:
+            raise ValueError("mesh must be a (12, F) float32 SoA array")
+        self.ctx = ctx
+        self.F = soa.shape[1]
+        h = ctypes.c_void_p()
+        ctx._check(lib().erp_mesh_upload(ctx.handle, _ptr(soa), self.F, ctypes.byref(h)))
+        self.handle = h
+
+    def release(self):
+        if getattr(self, "handle", None):
+            lib().erp_mesh_release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class Targets:
+    """Subsurface targets resident in HBM (erp_targets_upload)."""
+
+    def __init__(self, ctx: "Context", xyzn: np.ndarray, ttype: np.ndarray):
+        xyzn = np.ascontiguousarray(xyzn, dtype=np.float32)
+        ttype = np.ascontiguousarray(ttype, dtype=np.uint8)
+        if xyzn.ndim != 2 or xyzn.shape[0] != 6 or ttype.shape[0] != xyzn.shape[1]:
+            raise ValueError("targets must be (6, T) float32 + (T,) uint8")
+        self.ctx = ctx
+        self.T = xyzn.shape[1]
+        h = ctypes.c_void_p()
+        ctx._check(lib().erp_targets_upload(ctx.handle, _ptr(xyzn), _ptr(ttype), self.T,
+                                            ctypes.byref(h)))
+        self.handle = h
+
+    def release(self):
+        if getattr(self, "handle", None):
+            lib().erp_targets_release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class Job:
+    """An asynchronous run (erp_submit); keeps its host buffers alive."""
+
+    def __init__(self, ctx, handle, out, nvalid, keep):
+        self.ctx, self.handle, self.out, self.nvalid, self._keep = ctx, handle, out, nvalid, keep
+        self.done = False
+
+    def poll(self) -> bool:
+        if self.done:
+            return True
+        rc = lib().erp_poll(self.handle)
+        if rc == ERP_E_NOTREADY:
+            return False
+        self.done = True
+        self.ctx._check(rc)
+        return True
+
+    def wait(self):
+        if not self.done:
+            rc = lib().erp_wait(self.handle)
+            self.done = True
+            self.ctx._check(rc)
+        return self.out, self.nvalid
+
+
+class Context:
+    """One HIP device context (erp_ctx_create)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        rc = lib().erp_ctx_create(device, ctypes.byref(h))
+        if rc != ERP_OK:
+            raise ErpError(rc, lib().erp_last_error(None).decode())
+        self.handle = h
+        self.device = device
+
+    def _check(self, rc: int):
+        if rc != ERP_OK:
+            raise ErpError(rc, lib().erp_last_error(self.handle).decode())
+
+    def upload_mesh(self, soa) -> Mesh:
+        return Mesh(self, soa)
+
+    def upload_targets(self, xyzn, ttype) -> Targets:
+        return Targets(self, xyzn, ttype)
+
+    @staticmethod
+    def _args(params: ErpParams, shots, prisms):
+        shots = np.ascontiguousarray(shots, dtype=SHOT_DTYPE)
+        if prisms is None or len(prisms) == 0:
+            prisms = None
+            npr = 0
+        else:
+            prisms = np.ascontiguousarray(prisms, dtype=PRISM_DTYPE)
+            npr = prisms.shape[0]
+        return shots, prisms, npr
+
+    def run(self, params: ErpParams, mesh: Mesh, targets: Targets | None, shots,
+            prisms=None, flags: int = 0):
+        """Synchronous erp_run -> (traces (ns, nr) complex64, nvalid (ns,) int32)."""
+        shots, prisms, npr = self._args(params, shots, prisms)
+        ns, nr = shots.shape[0], params.nr
+        out = np.zeros((ns, nr), dtype=np.complex64)
+        nvalid = np.zeros(ns, dtype=np.int32)
+        self._check(lib().erp_run(self.handle, ctypes.byref(params), mesh.handle,
+                                  targets.handle if targets is not None else None,
+                                  _ptr(shots), ns, _ptr(prisms), npr, _ptr(out), _ptr(nvalid),
+                                  flags))
+        return out, nvalid
+
+    def run_dev(self, params: ErpParams, mesh: Mesh, targets: Targets | None, shots,
+                out_ptr: int, nvalid_ptr: int | None = None, prisms=None, flags: int = 0):
+        """erp_run_dev with a caller-owned device output pointer."""
+        shots, prisms, npr = self._args(params, shots, prisms)
+        self._check(lib().erp_run_dev(self.handle, ctypes.byref(params), mesh.handle,
+                                      targets.handle if targets is not None else None,
+                                      _ptr(shots), shots.shape[0], _ptr(prisms), npr,
+                                      ctypes.c_void_p(out_ptr),
+                                      ctypes.c_void_p(nvalid_ptr) if nvalid_ptr else None,
+                                      flags))
+
+    def submit(self, params: ErpParams, mesh: Mesh, targets: Targets | None, shots,
+               prisms=None, flags: int = 0) -> Job:
+        shots, prisms, npr = self._args(params, shots, prisms)
+        ns, nr = shots.shape[0], params.nr
+        out = np.zeros((ns, nr), dtype=np.complex64)
+        nvalid = np.zeros(ns, dtype=np.int32)
+        h = ctypes.c_void_p()
+        self._check(lib().erp_submit(self.handle, ctypes.byref(params), mesh.handle,
+                                     targets.handle if targets is not None else None,
+                                     _ptr(shots), ns, _ptr(prisms), npr, _ptr(out),
+                                     _ptr(nvalid), flags, ctypes.byref(h)))
+        return Job(self, h, out, nvalid, (shots, prisms, params))
+
+    @staticmethod
+    def footprint(params: ErpParams, F: int, T: int, ns: int) -> int:
+        return int(lib().erp_run_footprint(ctypes.byref(params), F, T, ns))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().erp_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,183 @@
+// erp_internal.h — shared declarations of the gfx950 HIP implementation of
+// the per-shot raypath / phasor / range-compression path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include <mutex>
+
+#include "../../include/erp.h"
+
+// ---------------------------------------------------------------------------
+// error plumbing: every failure is returned as a status, never exit()
+// ---------------------------------------------------------------------------
+struct ErpError {
+    int code;
+    std::string msg;
+};
+
+#define ERP_HIP_TRY(expr)                                                     \
+    do {                                                                      \
+        hipError_t _e = (expr);                                               \
+        if (_e != hipSuccess)                                                 \
+            throw ErpError{ERP_E_HIP, std::string(#expr) + ": " +             \
+                                          hipGetErrorString(_e)};             \
+    } while (0)
+
+#define ERP_CHECK_LAUNCH(name)                                                \
+    do {                                                                      \
+        hipError_t _e = hipGetLastError();                                    \
+        if (_e != hipSuccess)                                                 \
+            throw ErpError{ERP_E_HIP, std::string("launch ") + (name) +       \
+                                          ": " + hipGetErrorString(_e)};      \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// constants shared by every kernel (the reference's literals)
+// ---------------------------------------------------------------------------
+// geometry_kernels.cu:45 `const float pi = 3.14159265f`
+#define ERP_PI_F 3.14159265f
+// geometry_kernels.cu:388,411 `pow(4 * 3.14159, 2)` evaluated in double
+#define ERP_FOURPI5_SQ (4.0 * 3.14159 * 4.0 * 3.14159)
+
+// FFT twiddle table size (largest supported transform)
+#define ERP_TWN (1 << 18)
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t n) {
+        if (n <= bytes) return;
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (n == 0) return;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw ErpError{ERP_E_NOMEM, "hipMalloc of " + std::to_string(n) +
+                                            " bytes failed: " + hipGetErrorString(e)};
+        }
+        bytes = n;
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct erp_mesh {
+    erp_ctx* ctx;
+    int device;
+    int64_t F;
+    DevBuf soa;  // 12 x F float32 (stride Fpad)
+    int64_t Fpad;
+};
+
+struct erp_targets {
+    erp_ctx* ctx;
+    int device;
+    int64_t T;
+    DevBuf xyzn;  // 6 x Tpad float32
+    DevBuf type;  // T uint8
+    int64_t Tpad;
+};
+
+// FFT plan for one power-of-two size N (see erp_fft.hip)
+struct FftPlan {
+    int N = 0, log2N = 0;
+    int N1 = 0, N2 = 0;  // two-pass split: N = N1 (row, contiguous) * N2 (rows)
+    bool two_pass = false;
+};
+
+// Per-(nr, dr, res) chirp spectra and per-L Bluestein tables
+struct ChirpKey {
+    int nr;
+    float dr, res;
+    int kind;  // 0 = padded (linear), 1 = circular (genChirp), rst in key2
+    float rst;
+    bool operator==(const ChirpKey& o) const {
+        return nr == o.nr && dr == o.dr && res == o.res && kind == o.kind && rst == o.rst;
+    }
+};
+
+struct ChirpSpec {
+    ChirpKey key;
+    DevBuf spec;  // P complex64, permuted forward order, scaled by 1/P
+};
+
+struct BlueTables {
+    int L = 0, M = 0;
+    float smpl = 0, B = 0;
+    DevBuf w;     // L complex64: exp(-i pi n^2 / L)
+    DevBuf Bw;    // M complex64: FFT_M of the conjugate chirp, scaled 1/M
+    DevBuf mask;  // L uint8: crop mask (1 = keep)
+};
+
+struct WorkBufs {
+    DevBuf cnt, off, V, Vd, tcnt, tlist, tcount;  // counts / offsets
+    DevBuf rec;        // compacted facet records
+    DevBuf bounds;     // per-shot bounds of illuminated facets
+    DevBuf psurf;      // S x P complex: surface phasor trace (padded)
+    DevBuf refl;       // S x nr complex
+    DevBuf shotS;      // S x P complex: per-shot spectral accumulator
+    DevBuf blu;        // S x M complex: Bluestein workspace
+    DevBuf tA, tB, tC; // per-target chunk buffers, C x P complex
+    DevBuf pairs;      // chunk pair list
+    DevBuf t0trace;    // S x P complex: summed type-0 target histograms
+    DevBuf shots;      // S erp_shot
+    DevBuf prisms;
+    DevBuf out;        // S x nr complex (device output staging)
+    DevBuf nvalid;     // S int32
+};
+
+struct erp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    DevBuf twiddle;  // ERP_TWN complex64 exp(-2 pi i k / TWN)
+    std::vector<ChirpSpec*> chirps;
+    std::vector<BlueTables*> blues;
+    WorkBufs work;
+    std::mutex mu;
+    int num_cu = 0;
+};
+
+// ---------------------------------------------------------------------------
+// FFT interface (erp_fft.hip)
+// ---------------------------------------------------------------------------
+FftPlan fft_plan(int N);
+// Forward transform of `batch` rows of length N (row stride N), in place.
+// Output in the plan's permuted order (natural order for single-pass sizes).
+void fft_forward(erp_ctx* ctx, const FftPlan& p, float2* data, int batch, hipStream_t s);
+// Inverse (unnormalised) of a permuted-order spectrum, in place, natural output.
+void fft_inverse(erp_ctx* ctx, const FftPlan& p, float2* data, int batch, hipStream_t s);
+void fft_init_twiddles(erp_ctx* ctx);
+
+// ---------------------------------------------------------------------------
+// pipeline (erp_pipeline.hip)
+// ---------------------------------------------------------------------------
+void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
+                  const erp_targets* targets, const erp_shot* shots, int64_t ns,
+                  const erp_prism* prisms, int64_t nprisms, float2* out_dev,
+                  int32_t* nvalid_dev, uint32_t flags, hipStream_t stream,
+                  WorkBufs& work);
+int64_t erp_footprint_impl(const erp_params* par, int64_t F, int64_t T, int64_t ns);
+
+inline int next_pow2(int64_t n) {
+    int p = 1;
+    while (p < n) p <<= 1;
+    return p;
+}
+inline int ilog2(int n) {
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return l;
+}

@@ -1,0 +1,1664 @@
+// erp_pipeline.hip — the per-shot path of sim.cu:615-1008 as gfx950 kernels.
+//
+// Per batch of S shots (all shots of a batch run together):
+//   k_illum_mask   aperture predicate of every (shot, facet): FP32 screen,
+//                  exact FP64 decision near the threshold; one 64-bit ballot
+//                  word per wave -> mask, per-block counts.
+//                  (compSourceInclination + cropByAperture,
+//                  geometry_kernels.cu:75-100, 143-204)
+//   k_scan_blocks  per-shot exclusive scan of the block counts -> stable
+//                  record offsets, V (illuminated facets) per shot.
+//   k_surface      fused compIncidentRays + compRefractedRays +
+//                  compReflectedEnergy + surfacePT / genReflPhasor
+//                  (geometry_kernels.cu:103-140, 426-517; radar_kernels.cu:
+//                  314-348, 714-764): one thread per illuminated facet, FP64
+//                  range and phase, two-bin split accumulated in an LDS window,
+//                  compacted facet records for the target stage.
+//   surface range compression: FFT -> x chirp spectrum -> IFFT, batched over
+//                  shots (convolvePhasorChirpLinear, radar_kernels.cu:554-649).
+//   k_target_select per-shot in-aperture target lists (sim.cu:808-826).
+//   k_target_accum one workgroup per (shot, target): compTargetRays +
+//                  compRefrEnergyIn/Out + accumulateTarget/radiateTarget
+//                  (geometry_kernels.cu:272-350, 520-584; radar_kernels.cu:
+//                  767-892) over all illuminated facets, LDS window histogram.
+//   target spectra: FFT, x chirp(2 res), IFFT, window, FFT, and the product
+//                  with the outward trace's spectrum summed over the shot's
+//                  targets in the frequency domain; one inverse + Bluestein
+//                  DFT_L / crop / IDFT_L per shot replaces the reference's
+//                  per-target convolveComplex (elementary_kernels.cu:258-353).
+//   k_finalize     refl + refr -> output (combineRadarSignals, radar_kernels.cu:
+//                  302-311).
+//
+// Every convolution of the reference (2nr circular + window, nr circular) is
+// evaluated as an exact linear convolution with a rearranged kernel g of
+// length 2nr-1 through a power-of-two FFT of size P = nextpow2(2nr-1):
+//   out[j] = sum_m x[m] g[j + nr - 1 - m],  j < nr.
+#include "erp_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace {
+
+constexpr int NT = 256;        // threads per workgroup
+constexpr int FB = 2048;       // facets per block in the surface kernels
+constexpr int FPT = FB / NT;   // facets per thread (8)
+constexpr int WPB = NT / 64;   // waves per block
+constexpr int MASKW = FB / 64; // mask words per block (32)
+constexpr int SWIN = 2048;     // surface LDS window (bins)
+constexpr int TWIN = 4096;     // target LDS window (bins)
+
+// Kernel-side parameter block (derived once per run on the host).
+struct KPar {
+    float lam, fs, P, ks, eps1, eps2, nu1, nu2, alpha1, alpha2, rng_res, dr;
+    float c, c1, c2;
+    float piLlam;      // PI_F*fs/lam (facet_G's sinc argument scale)
+    float Dmax;        // 4*PI_F*fs^2/lam^2
+    float fs_lam;      // fs/lam (facetReradiation)
+    float fs2_lam;     // fs*fs/lam
+    float nf;          // sqrt(eps_2) in float (accumulateTarget's n)
+    float sqrtP;
+    float invK;        // 1/(4*3.14159)^2
+    float sqrt_e1e2;   // sqrt(eps_1/eps_2)
+    float sqrt_e2e1;   // sqrt(eps_2/eps_1)
+    double drd, cos_ap, n_d, inv_cyc, lam_d;
+    double B, smpl;
+    int nr, P_fft, rerad;
+    uint32_t mode;
+    uint64_t seed;
+};
+
+struct ShotK {
+    double sx, sy, sz;
+    float snx, sny, snz, rst, grefl, grefr;
+};
+
+__device__ __forceinline__ ShotK load_shot(const erp_shot* __restrict__ shots, int s) {
+    const erp_shot& h = shots[s];
+    ShotK k;
+    k.sx = h.sx; k.sy = h.sy; k.sz = h.sz;
+    k.snx = h.snx; k.sny = h.sny; k.snz = h.snz;
+    k.rst = h.rst; k.grefl = h.grefl_lin; k.grefr = h.grefr_lin;
+    return k;
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// ---------------------------------------------------------------------------
+// exact FP64 pieces (no contraction: identical operation sequence to the
+// oracle's -ffp-contract=off build, so bins compare bit for bit)
+// ---------------------------------------------------------------------------
+__device__ __noinline__ bool illum_exact(float fx, float fy, float fz, const ShotK& s,
+                                         double cos_ap) {
+#pragma clang fp contract(off)
+    double dx = (double)fx - s.sx, dy = (double)fy - s.sy, dz = (double)fz - s.sz;
+    double Itd = sqrt(dx * dx + dy * dy + dz * dz);
+    double Ix = dx / Itd, Iy = dy / Itd, Iz = dz / Itd;
+    double c = (-Ix) * (double)s.snx + (-Iy) * (double)s.sny + (-Iz) * (double)s.snz;
+    return c > cos_ap && c <= 1.0;
+}
+
+__device__ __forceinline__ double dist3(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+    return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+__device__ __forceinline__ double madd_nc(double a, double b, double n) {
+#pragma clang fp contract(off)
+    return a + b * n;  // a + b*n, two roundings
+}
+
+// Range -> (bin, weight) of radar_kernels.cu:736-737 in FP64; valid iff
+// -1 < x < nr (the CUDA (short) conversion saturates).
+__device__ __forceinline__ bool bin_of(double rng, double rst, double dr, int nr, int* b,
+                                       float* w) {
+#pragma clang fp contract(off)
+    double x = (rng - rst) / dr;
+    if (!(x > -1.0) || !(x < (double)nr)) return false;
+    int bi = (int)x;
+    *b = bi;
+    *w = (float)(x - (double)bi);
+    return true;
+}
+
+// exp(-i 2 pi cyc): phase from FP64 cycles, reduced before FP32 sincospi.
+__device__ __forceinline__ float2 phasor_cycles(double cyc) {
+    double f = cyc - floor(cyc);
+    float s, c;
+    sincospif(-2.0f * (float)f, &s, &c);
+    return make_float2(c, s);
+}
+
+// |a x b| (sine of the angle between unit vectors)
+__device__ __forceinline__ float cross_norm(float ax, float ay, float az, float bx, float by,
+                                            float bz) {
+    const float cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+    return sqrtf(cx * cx + cy * cy + cz * cz);
+}
+
+// elementary_kernels.cu:50-53
+__device__ __forceinline__ float sinc_f(float x) {
+    if (fabsf(x) < 1e-6f) return 1.0f;
+    return sinpif(x) / (3.14159265358979f * x);
+}
+
+// splitmix64 hash of (seed, shot, facet): the incoherent phase u in [0,1)
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ float incoherent_u(uint64_t seed, int64_t shot, int64_t facet) {
+    uint64_t h = mix64(seed ^ mix64((uint64_t)shot * 0x100000001B3ull ^ mix64((uint64_t)facet)));
+    return (float)((double)(h >> 40) * (1.0 / 16777216.0));
+}
+
+// order-preserving float <-> uint for atomic min/max
+__device__ __forceinline__ unsigned f2o(float f) {
+    unsigned u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float o2f(unsigned u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+// per-shot bounds of the illuminated facets (for target windows)
+struct Bounds {
+    unsigned long long itd_min, itd_max;  // double bits (positive)
+    unsigned xmin, ymin, zmin, xmax, ymax, zmax;  // f2o-encoded
+};
+
+// ---------------------------------------------------------------------------
+// generic small kernels
+// ---------------------------------------------------------------------------
+__global__ void k_zero(float2* __restrict__ p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t st = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) p[i] = make_float2(0.f, 0.f);
+}
+
+__global__ void k_init_bounds(Bounds* b, int S) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < S) {
+        b[s].itd_min = ~0ull; b[s].itd_max = 0ull;
+        b[s].xmin = b[s].ymin = b[s].zmin = 0xFFFFFFFFu;
+        b[s].xmax = b[s].ymax = b[s].zmax = 0u;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 1. aperture mask + block counts
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(NT) k_illum_mask(
+    const float* __restrict__ mesh, int64_t Fpad, int64_t F,
+    const erp_shot* __restrict__ shots, double cos_ap,
+    unsigned long long* __restrict__ mask, int* __restrict__ cnt, int nblk) {
+    const int blk = blockIdx.x, s = blockIdx.y;
+    const ShotK sh = load_shot(shots, s);
+    const float sxf = (float)sh.sx, syf = (float)sh.sy, szf = (float)sh.sz;
+    const float cosf_ap = (float)cos_ap;
+    const float* X = mesh;
+    const float* Y = mesh + Fpad;
+    const float* Z = mesh + 2 * Fpad;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __shared__ int wc[WPB];
+    int total = 0;
+    const float srcmag = fabsf(sxf) + fabsf(syf) + fabsf(szf);
+#pragma unroll
+    for (int q = 0; q < FPT; ++q) {
+        const int64_t i = (int64_t)blk * FB + q * NT + threadIdx.x;
+        bool p = false;
+        if (i < F) {
+            const float fx = X[i], fy = Y[i], fz = Z[i];
+            const float dx = fx - sxf, dy = fy - syf, dz = fz - szf;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            const float rinv = rsqrtf(d2);
+            const float c = -(dx * sh.snx + dy * sh.sny + dz * sh.snz) * rinv;
+            const float margin =
+                1e-5f + 4e-7f * (srcmag + fabsf(fx) + fabsf(fy) + fabsf(fz)) * rinv;
+            if (c < cosf_ap - margin) {
+                p = false;
+            } else if (c > cosf_ap + margin && c < 1.0f - margin) {
+                p = true;
+            } else {
+                p = illum_exact(fx, fy, fz, sh, cos_ap);
+            }
+        }
+        const unsigned long long b = __ballot(p);
+        if (lane == 0) mask[((size_t)s * nblk + blk) * MASKW + q * WPB + wv] = b;
+        total += __popcll(b);
+    }
+    if (lane == 0) wc[wv] = total;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < WPB; ++w) t += wc[w];
+        cnt[(size_t)s * nblk + blk] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 2. per-shot exclusive scan of block counts
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_scan_blocks(const int* __restrict__ cnt,
+                                                      int* __restrict__ off,
+                                                      int* __restrict__ V, int nblk) {
+    const int s = blockIdx.x;
+    __shared__ int buf[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nblk; base += 1024) {
+        const int i = base + threadIdx.x;
+        const int v = (i < nblk) ? cnt[(size_t)s * nblk + i] : 0;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            int t = (threadIdx.x >= o) ? buf[threadIdx.x - o] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nblk) off[(size_t)s * nblk + i] = carry + buf[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += buf[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) V[s] = carry;
+}
+
+// ---------------------------------------------------------------------------
+// 3. fused surface kernel
+// ---------------------------------------------------------------------------
+// Compacted facet record (SoA over Vtot records of the batch).
+struct Rec {
+    float* geo;     // 12 x cap: x,y,z,nx,ny,nz,ux,uy,uz,vx,vy,vz
+    double* itd;    // cap
+    float* F;       // cap: sqrt(P n Grefr G1 [fRfrC]) lam^2 / (K Itd)
+    float* fRfrC;   // cap
+    float* ang;     // 3 x cap: Ith, Iph, Rth
+    int64_t cap;
+};
+
+__device__ __forceinline__ void lds_add2(float2* win, int idx, float2 v) {
+    atomicAdd(&win[idx].x, v.x);
+    atomicAdd(&win[idx].y, v.y);
+}
+__device__ __forceinline__ void glb_add2(float2* row, int idx, float2 v) {
+    atomicAdd(&row[idx].x, v.x);
+    atomicAdd(&row[idx].y, v.y);
+}
+
+__global__ void __launch_bounds__(NT) k_surface(
+    const float* __restrict__ mesh, int64_t Fpad, int64_t F, const erp_shot* __restrict__ shots,
+    const unsigned long long* __restrict__ mask, const int* __restrict__ off,
+    const int* __restrict__ Vs, const int64_t* __restrict__ rec_base, int nblk, KPar kp,
+    Rec rec, int write_rec, int do_trace, float2* __restrict__ psurf, int64_t shot0,
+    Bounds* __restrict__ bounds) {
+    const int blk = blockIdx.x, s = blockIdx.y;
+    const unsigned long long* mw = mask + ((size_t)s * nblk + blk) * MASKW;
+    __shared__ unsigned long long m_s[MASKW];
+    __shared__ int pre_s[MASKW + 1];
+    __shared__ float2 win[SWIN];
+    __shared__ int wmin_s;
+    __shared__ float red_f[6][WPB];
+    __shared__ double red_d[2][WPB];
+    if (threadIdx.x < MASKW) m_s[threadIdx.x] = mw[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < MASKW; ++w) {
+            pre_s[w] = t;
+            t += __popcll(m_s[w]);
+        }
+        pre_s[MASKW] = t;
+        wmin_s = 0x7FFFFFFF;
+    }
+    __syncthreads();
+    if (pre_s[MASKW] == 0) return;  // dark block (uniform exit)
+
+    const ShotK sh = load_shot(shots, s);
+    const int V = Vs[s];
+    const int64_t rbase = rec_base[s] + off[(size_t)s * nblk + blk];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const bool spec = kp.mode & ERP_MODE_SPECULAR;
+    const bool lossless = kp.mode & ERP_MODE_LOSSLESS;
+    const bool incoh = kp.mode & ERP_MODE_INCOHERENT;
+    const double rst = (double)sh.rst;
+    const float Grefr = sh.grefr;
+
+    // per-thread contributions kept for the windowed scatter
+    int bq[FPT];
+    float wq[FPT];
+    float2 cq[FPT];
+    float lxmin = INFINITY, lymin = INFINITY, lzmin = INFINITY;
+    float lxmax = -INFINITY, lymax = -INFINITY, lzmax = -INFINITY;
+    double litmin = INFINITY, litmax = 0.0;
+    int bmin = 0x7FFFFFFF;
+
+#pragma unroll
+    for (int q = 0; q < FPT; ++q) {
+        bq[q] = -1;
+        const int w = q * WPB + wv;
+        const unsigned long long m = m_s[w];
+        if (!((m >> lane) & 1ull)) continue;
+        const int64_t i = (int64_t)blk * FB + q * NT + threadIdx.x;
+        const int rank = pre_s[w] + __popcll(m & ((1ull << lane) - 1ull));
+        float g[12];
+#pragma unroll
+        for (int a = 0; a < 12; ++a) g[a] = mesh[(size_t)a * Fpad + i];
+        // compIncidentRays (FP64 range, FP32 directions)
+        const double dx = (double)g[0] - sh.sx, dy = (double)g[1] - sh.sy,
+                     dz = (double)g[2] - sh.sz;
+        const double Itd = dist3(dx, dy, dz);
+        const float inv = (float)(1.0 / Itd);
+        const float Ix = (float)dx * inv, Iy = (float)dy * inv, Iz = (float)dz * inv;
+        float cI = -(Ix * g[3] + Iy * g[4] + Iz * g[5]);
+        cI = fmaxf(-1.f, fminf(1.f, cI));
+        const float Iu = Ix * g[6] + Iy * g[7] + Iz * g[8];
+        const float Iv = Ix * g[9] + Iy * g[10] + Iz * g[11];
+        // sin(Ith) = |I x n|: well conditioned at every angle (acos of the FP32
+        // cosine loses ~1e-7/sin(Ith) relative precision near nadir)
+        const float sI = cross_norm(Ix, Iy, Iz, g[3], g[4], g[5]);
+        const float Ith = atan2f(sI, cI);
+        const float Iph = atan2f(Iu, Iv);
+        // compRefractedRays (snellsLaw)
+        const float sR = kp.sqrt_e1e2 * sI;
+        float Rth, cR;
+        if (sR > 1.0f) {
+            Rth = 1e4f;
+            cR = cosf(1e4f);
+        } else {
+            Rth = asinf(sR);
+            cR = sqrtf(fmaxf(0.f, 1.f - sR * sR));
+        }
+        // compReflectedEnergy (pol 0, H)
+        const float rho = (kp.nu2 * cI - kp.nu1 * cR) / (kp.nu2 * cI + kp.nu1 * cR);
+        const float Itdf = (float)Itd;
+        const float atm = expf(-4.0f * kp.alpha1 * Itdf);
+        float fRfrC = (Rth > 1e3f) ? 0.f : (1.f - rho * rho);
+        fRfrC *= atm;
+        const float kc = kp.ks * cI;
+        const float rough = expf(-4.f * (kc * kc));
+        float fRe = rho * rho * atm * rough;
+        // facet_G(Ith, Iph) (radar_kernels.cu:658-677)
+        // facet_G(Ith, Iph): sin(Ith)cos(Iph) = I.v and sin(Ith)sin(Iph) = I.u for the
+        // orthonormal facet basis (u, v, n)
+        const float s1 = sinc_f(kp.piLlam * Iv), s2 = sinc_f(kp.piLlam * Iu);
+        const float G1 = (s1 * s2) * (s1 * s2) * kp.Dmax;
+        if (spec) {
+            // facetReradiation(Itd, 2 Ith, -Iph) * radarEq(P, Grefl, fs, lam, Itd, V);
+            // sin(2 Ith) cos(-Iph) = 2 cos(Ith) I.v, sin(2 Ith) sin(-Iph) = -2 cos(Ith) I.u
+            const float r1 = sinc_f(kp.fs_lam * 2.f * cI * Iv),
+                        r2 = sinc_f(kp.fs_lam * -2.f * cI * Iu);
+            const float rr = kp.fs2_lam * r1 * r2;
+            const float Gl = sh.grefl;
+            const double Itd2 = Itd * Itd;
+            const float req = (float)((double)V *
+                                      ((double)(kp.P * Gl * Gl * kp.fs * kp.fs * kp.fs * kp.fs) /
+                                       (ERP_FOURPI5_SQ * Itd2 * Itd2)));
+            fRe *= rr * rr * req;
+        }
+        // bounds for the target windows
+        lxmin = fminf(lxmin, g[0]); lxmax = fmaxf(lxmax, g[0]);
+        lymin = fminf(lymin, g[1]); lymax = fmaxf(lymax, g[1]);
+        lzmin = fminf(lzmin, g[2]); lzmax = fmaxf(lzmax, g[2]);
+        litmin = fmin(litmin, Itd); litmax = fmax(litmax, Itd);
+
+        if (write_rec) {
+            const int64_t r = rbase + rank;
+#pragma unroll
+            for (int a = 0; a < 12; ++a) rec.geo[(size_t)a * rec.cap + r] = g[a];
+            rec.itd[r] = Itd;
+            // target amplitude factor: sqrt(P n Grefr G1 fRfrC) lam^2/(K Itd)
+            const float fac = lossless ? 1.f : fRfrC;
+            // NoConv specular keeps fRe here for reflRadarSignal's direct sum
+            rec.F[r] = (spec && !(kp.mode & ERP_MODE_CONVOLUTION))
+                           ? fRe
+                           : sqrtf(kp.P * kp.nf * Grefr * G1 * fac) * kp.lam * kp.lam * kp.invK / Itdf;
+            rec.fRfrC[r] = fRfrC;
+            rec.ang[r] = Ith;
+            rec.ang[rec.cap + r] = Iph;
+            rec.ang[2 * rec.cap + r] = Rth;
+        }
+        if (!do_trace) continue;
+        int b;
+        float wgt;
+        if (!spec) {
+            // surfacePT: sqrt(friis(friis(P,Gr,G,lam,R),G,Gr,lam,R) [* fRe])
+            if (!bin_of(Itd, rst, kp.drd, kp.nr, &b, &wgt)) continue;
+            float amp = kp.sqrtP * kp.lam * kp.lam * Grefr * G1 * kp.invK / (Itdf * Itdf);
+            if (!lossless) amp *= sqrtf(fRe);
+            float2 ph;
+            if (incoh) {
+                float sn, cs;
+                sincospif(2.0f * incoherent_u(kp.seed, shot0 + s, i), &sn, &cs);
+                ph = make_float2(cs, sn);
+            } else {
+                ph = phasor_cycles(2.0 * Itd * kp.inv_cyc);
+            }
+            bq[q] = b;
+            wq[q] = wgt;
+            cq[q] = make_float2(ph.x * amp, ph.y * amp);
+        } else {
+            // genReflPhasor: fRe * exp(+i 4 pi/lam Itd), one bin, no split
+            if (!bin_of(Itd, rst, kp.drd, kp.nr, &b, &wgt)) continue;
+            const float2 ph = phasor_cycles(-2.0 * Itd * kp.inv_cyc);
+            bq[q] = b;
+            wq[q] = 0.f;
+            cq[q] = make_float2(ph.x * fRe, ph.y * fRe);
+        }
+        bmin = min(bmin, b);
+    }
+
+    // block reductions: window base and bounds
+    for (int o = 32; o > 0; o >>= 1) {
+        bmin = min(bmin, __shfl_xor(bmin, o));
+        lxmin = fminf(lxmin, __shfl_xor(lxmin, o)); lxmax = fmaxf(lxmax, __shfl_xor(lxmax, o));
+        lymin = fminf(lymin, __shfl_xor(lymin, o)); lymax = fmaxf(lymax, __shfl_xor(lymax, o));
+        lzmin = fminf(lzmin, __shfl_xor(lzmin, o)); lzmax = fmaxf(lzmax, __shfl_xor(lzmax, o));
+        litmin = fmin(litmin, __shfl_xor(litmin, o)); litmax = fmax(litmax, __shfl_xor(litmax, o));
+    }
+    if (lane == 0) {
+        atomicMin(&wmin_s, bmin);
+        red_f[0][wv] = lxmin; red_f[1][wv] = lymin; red_f[2][wv] = lzmin;
+        red_f[3][wv] = lxmax; red_f[4][wv] = lymax; red_f[5][wv] = lzmax;
+        red_d[0][wv] = litmin; red_d[1][wv] = litmax;
+    }
+    for (int k = threadIdx.x; k < SWIN; k += NT) win[k] = make_float2(0.f, 0.f);
+    __syncthreads();
+    if (threadIdx.x == 0 && bounds) {
+        float a[6];
+        double d0 = red_d[0][0], d1 = red_d[1][0];
+        for (int k = 0; k < 6; ++k) a[k] = red_f[k][0];
+        for (int w = 1; w < WPB; ++w) {
+            a[0] = fminf(a[0], red_f[0][w]); a[1] = fminf(a[1], red_f[1][w]);
+            a[2] = fminf(a[2], red_f[2][w]); a[3] = fmaxf(a[3], red_f[3][w]);
+            a[4] = fmaxf(a[4], red_f[4][w]); a[5] = fmaxf(a[5], red_f[5][w]);
+            d0 = fmin(d0, red_d[0][w]); d1 = fmax(d1, red_d[1][w]);
+        }
+        Bounds* bd = bounds + s;
+        atomicMin(&bd->xmin, f2o(a[0])); atomicMin(&bd->ymin, f2o(a[1]));
+        atomicMin(&bd->zmin, f2o(a[2])); atomicMax(&bd->xmax, f2o(a[3]));
+        atomicMax(&bd->ymax, f2o(a[4])); atomicMax(&bd->zmax, f2o(a[5]));
+        atomicMin(&bd->itd_min, (unsigned long long)__double_as_longlong(d0));
+        atomicMax(&bd->itd_max, (unsigned long long)__double_as_longlong(d1));
+    }
+    if (!do_trace) return;
+    const int base = wmin_s;
+    if (base == 0x7FFFFFFF) return;
+    float2* row = psurf + (size_t)s * kp.P_fft;
+    const int nr = kp.nr;
+#pragma unroll
+    for (int q = 0; q < FPT; ++q) {
+        const int b = bq[q];
+        if (b < 0) continue;
+        const float w = wq[q];
+        const float2 c = cq[q];
+        const float2 c0 = make_float2(c.x * (1.f - w), c.y * (1.f - w));
+        const int k0 = b - base;
+        if (k0 < SWIN) lds_add2(win, k0, c0); else glb_add2(row, b, c0);
+        if (!spec && b + 1 < nr) {
+            const float2 c1 = make_float2(c.x * w, c.y * w);
+            if (k0 + 1 < SWIN) lds_add2(win, k0 + 1, c1); else glb_add2(row, b + 1, c1);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < SWIN; k += NT) {
+        const float2 v = win[k];
+        const int b = base + k;
+        if (b < nr && (v.x != 0.f || v.y != 0.f)) glb_add2(row, b, v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 4. pointwise spectral kernels
+// ---------------------------------------------------------------------------
+// dst[r][k] = src[r][k] * spec[k]
+__global__ void k_mul_spec(float2* __restrict__ dst, const float2* __restrict__ src,
+                           const float2* __restrict__ spec, int P, int rows) {
+    const size_t n = (size_t)P * rows;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        dst[i] = cmul(src[i], spec[i & (size_t)(P - 1)]);
+    }
+}
+
+// out[r][j] = scale * y[r][j + nr - 1] for j < nr (window of the exact linear
+// convolution); if pad, also zero out[r][nr .. P).
+__global__ void k_window(float2* __restrict__ out, int ostride, const float2* __restrict__ y,
+                         int P, int nr, int rows, float scale, int pad) {
+    const int lim = pad ? P : nr;
+    const size_t n = (size_t)lim * rows;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / lim), j = (int)(i - (size_t)r * lim);
+        float2 v = make_float2(0.f, 0.f);
+        if (j < nr) {
+            v = y[(size_t)r * P + j + nr - 1];
+            v.x *= scale;
+            v.y *= scale;
+        }
+        out[(size_t)r * ostride + j] = v;
+    }
+}
+
+// out[r][j] = in[r][j], j < ncols
+__global__ void k_copy_rows(float2* __restrict__ out, int ostride, const float2* __restrict__ in,
+                            int istride, int ncols, int rows) {
+    const size_t n = (size_t)ncols * rows;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / ncols), j = (int)(i - (size_t)r * ncols);
+        out[(size_t)r * ostride + j] = in[(size_t)r * istride + j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 5. chirp tables
+// ---------------------------------------------------------------------------
+// Rearranged kernel g[e], e < 2nr-1, of the reference convolutions:
+//  kind 0 (convolvePhasorChirpLinear with genCenteredChirpPadded):
+//     g[e] = h[(e + nr/2 - nr + 1) mod 2nr],  h[i] = sinc((i - nr/2) dr / res)
+//  kind 1 (convolvePhasorChirp with genChirp):
+//     g[e] = h[(e + 1 - nr) mod nr],  h[i] = sinc((r-rst)/res) + sinc((r-rst-nr dr)/res)
+__global__ void k_gen_kernel(float2* __restrict__ g, int P, int nr, double dr, double res,
+                             double rst, int kind) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= P) return;
+    float v = 0.f;
+    if (e < 2 * nr - 1) {
+        if (kind == 0) {
+            const int kc = nr / 2;
+            const int m = (int)(((long long)e + kc - nr + 1 + 2LL * nr) % (2LL * nr));
+            const double x = ((double)(m - kc) * dr) / res;
+            v = (fabs(x) < 1e-6) ? 1.f : (float)(sin(M_PI * x) / (M_PI * x));
+        } else {
+            const int m = (int)(((long long)e + 1 - nr + 2LL * nr) % (long long)nr);
+            const double r = rst + (double)m * dr;
+            const double x1 = (r - rst) / res;
+            const double x2 = (r - (rst + (double)nr * dr)) / res;
+            const double a = (fabs(x1) < 1e-6) ? 1.0 : sin(M_PI * x1) / (M_PI * x1);
+            const double b = (fabs(x2) < 1e-6) ? 1.0 : sin(M_PI * x2) / (M_PI * x2);
+            v = (float)(a + b);
+        }
+    }
+    g[e] = make_float2(v, 0.f);
+}
+
+__global__ void k_scale(float2* __restrict__ p, int n, float s) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        p[i].x *= s;
+        p[i].y *= s;
+    }
+}
+
+// Bluestein tables for DFT_L: w[n] = exp(-i pi n^2 / L); b[m] = conj(w[|m|])
+// circularly in M; crop mask of cropSpectrum (elementary_kernels.cu:258-278).
+__global__ void k_blue_tables(float2* __restrict__ w, float2* __restrict__ b,
+                              unsigned char* __restrict__ mask, int L, int M, double smpl,
+                              double Bw) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < L) {
+        const long long q = ((long long)i * i) % (2LL * L);
+        double sn, cs;
+        sincospi(-(double)q / (double)L, &sn, &cs);
+        w[i] = make_float2((float)cs, (float)sn);
+        const double freq = (i <= L / 2) ? ((double)i * smpl) / (double)L
+                                         : ((double)(i - L) * smpl) / (double)L;
+        mask[i] = (fabs(freq) > Bw) ? 0 : 1;
+    }
+    if (i < M) {
+        int m = -1;
+        if (i < L) m = i;
+        else if (i > M - L) m = M - i;
+        float2 v = make_float2(0.f, 0.f);
+        if (m >= 0) {
+            const long long q = ((long long)m * m) % (2LL * L);
+            double sn, cs;
+            sincospi((double)q / (double)L, &sn, &cs);
+            v = make_float2((float)cs, (float)sn);
+        }
+        b[i] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 6. target selection (host aperture test of sim.cu:808-826, in FP64)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_target_select(
+    const float* __restrict__ tg, int64_t Tpad, int T, const unsigned char* __restrict__ ttype,
+    const erp_shot* __restrict__ shots, const int* __restrict__ Vs, double cos_ap,
+    int* __restrict__ list1, int* __restrict__ list0, int* __restrict__ tcnt, int Tcap) {
+#pragma clang fp contract(off)
+    const int s = blockIdx.x;
+    const ShotK sh = load_shot(shots, s);
+    __shared__ int buf1[1024], buf0[1024];
+    __shared__ int c1, c0;
+    if (threadIdx.x == 0) { c1 = 0; c0 = 0; }
+    __syncthreads();
+    const bool live = Vs[s] > 0;
+    for (int base = 0; base < T; base += 1024) {
+        const int t = base + threadIdx.x;
+        int in1 = 0, in0 = 0;
+        if (live && t < T) {
+            const double vx = (double)tg[t] - sh.sx, vy = (double)tg[Tpad + t] - sh.sy,
+                         vz = (double)tg[2 * Tpad + t] - sh.sz;
+            const double mag = sqrt(vx * vx + vy * vy + vz * vz);
+            const double ux = vx / mag, uy = vy / mag, uz = vz / mag;
+            const double dp = (-(double)sh.snx) * ux + (-(double)sh.sny) * uy +
+                              (-(double)sh.snz) * uz;
+            if (!(dp < cos_ap)) {
+                if (ttype[t] != 0) in1 = 1; else in0 = 1;
+            }
+        }
+        buf1[threadIdx.x] = in1;
+        buf0[threadIdx.x] = in0;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            const int a = (threadIdx.x >= o) ? buf1[threadIdx.x - o] : 0;
+            const int b = (threadIdx.x >= o) ? buf0[threadIdx.x - o] : 0;
+            __syncthreads();
+            buf1[threadIdx.x] += a;
+            buf0[threadIdx.x] += b;
+            __syncthreads();
+        }
+        if (in1) list1[(size_t)s * Tcap + c1 + buf1[threadIdx.x] - 1] = t;
+        if (in0) list0[(size_t)s * Tcap + c0 + buf0[threadIdx.x] - 1] = t;
+        __syncthreads();
+        if (threadIdx.x == 1023) { c1 += buf1[1023]; c0 += buf0[1023]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        tcnt[2 * s] = c1;
+        tcnt[2 * s + 1] = c0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 7. facet x target accumulation
+// ---------------------------------------------------------------------------
+struct PairCtx {
+    const int* pair_shot;   // per pair in chunk: local shot index
+    const int* pair_tgt;    // per pair: target index
+};
+
+// geometry_kernels.cu:207-269
+__device__ __forceinline__ float prism_len(float ox, float oy, float oz, float dx, float dy,
+                                           float dz, const erp_prism& p, float t) {
+    const float INF = 1e30f;
+    float tx1, tx2, ty1, ty2, tz1, tz2;
+    if (fabsf(dx) < 1e-12f) {
+        if (ox < p.xmin || ox > p.xmax) return 0.f;
+        tx1 = -INF; tx2 = INF;
+    } else {
+        tx1 = (p.xmin - ox) / dx; tx2 = (p.xmax - ox) / dx;
+        if (tx1 > tx2) { float tt = tx1; tx1 = tx2; tx2 = tt; }
+    }
+    if (fabsf(dy) < 1e-12f) {
+        if (oy < p.ymin || oy > p.ymax) return 0.f;
+        ty1 = -INF; ty2 = INF;
+    } else {
+        ty1 = (p.ymin - oy) / dy; ty2 = (p.ymax - oy) / dy;
+        if (ty1 > ty2) { float tt = ty1; ty1 = ty2; ty2 = tt; }
+    }
+    if (fabsf(dz) < 1e-12f) {
+        if (oz < p.zmin || oz > p.zmax) return 0.f;
+        tz1 = -INF; tz2 = INF;
+    } else {
+        tz1 = (p.zmin - oz) / dz; tz2 = (p.zmax - oz) / dz;
+        if (tz1 > tz2) { float tt = tz1; tz1 = tz2; tz2 = tt; }
+    }
+    float te = fmaxf(fmaxf(tx1, ty1), tz1), tx = fminf(fminf(tx2, ty2), tz2);
+    if (te < 0.f) te = 0.f;
+    if (tx > t) tx = t;
+    if (te >= tx) return 0.f;
+    return tx - te;
+}
+
+__device__ __forceinline__ float rerad_f(int fn, float th) {
+    const float pi = ERP_PI_F;
+    switch (fn) {
+    case 0: return 1.f;
+    case 1: return (th * (180.f / pi) < 1.f) ? 1.f : 0.f;
+    case 2: return (th * (180.f / pi) < 0.5f) ? 1.f : 0.f;
+    case 3: return (th * (pi / 180.f) < 3.f) ? 1.f : 0.f;
+    case 4: {
+        const float sg = pi / 180.f;
+        return expf(-0.5f * (th * th) / (sg * sg));
+    }
+    default: return 1.f;
+    }
+}
+
+// MODE 0: type != 0, non-specular: one trace A (the inward trace PTtarg);
+//         the outward trace is -A/sqrt(P) exactly.
+// MODE 1: type != 0, specular: inward (A) and outward (B) traces separately.
+// MODE 2: type == 0 (genRefrPhasor): single bin, summed into one per-shot
+//         trace (out row = per shot, global atomics after the LDS window).
+template <int MODE>
+__global__ void __launch_bounds__(NT) k_target_accum(
+    const float* __restrict__ tg, int64_t Tpad, const int* __restrict__ pshot,
+    const int* __restrict__ ptgt, const erp_shot* __restrict__ shots,
+    const int* __restrict__ Vs, const int64_t* __restrict__ rec_base, Rec rec,
+    const Bounds* __restrict__ bounds, const erp_prism* __restrict__ prisms, int np, KPar kp,
+    float2* __restrict__ outA, float2* __restrict__ outB) {
+    extern __shared__ float2 lds[];
+    float2* winA = lds;
+    float2* winB = lds + TWIN;  // MODE 1 only
+    const int pi = blockIdx.x;
+    const int s = pshot[pi], t = ptgt[pi];
+    const ShotK sh = load_shot(shots, s);
+    const int V = Vs[s];
+    const int64_t r0 = rec_base[s];
+    const float txf = tg[t], tyf = tg[Tpad + t], tzf = tg[2 * Tpad + t];
+    const float tnx = tg[3 * Tpad + t], tny = tg[4 * Tpad + t], tnz = tg[5 * Tpad + t];
+    const double tx = txf, ty = tyf, tz = tzf;
+    const double rst = (double)sh.rst;
+    const int nr = kp.nr, P = kp.P_fft;
+    const bool lossless = kp.mode & ERP_MODE_LOSSLESS;
+
+    // window base from the shot bounds: rngt >= Itd_min + n * dist(target, AABB)
+    const Bounds bd = bounds[s];
+    const float bx = fmaxf(fmaxf(o2f(bd.xmin) - txf, txf - o2f(bd.xmax)), 0.f);
+    const float by = fmaxf(fmaxf(o2f(bd.ymin) - tyf, tyf - o2f(bd.ymax)), 0.f);
+    const float bz = fmaxf(fmaxf(o2f(bd.zmin) - tzf, tzf - o2f(bd.zmax)), 0.f);
+    const double tmin = sqrt((double)bx * bx + (double)by * by + (double)bz * bz);
+    const double itmin = __longlong_as_double((long long)bd.itd_min);
+    double rlo;
+    if (MODE == 2) rlo = (itmin + tmin) - tmin + tmin * (double)(kp.c1 / kp.c2);
+    else rlo = itmin + kp.n_d * tmin;
+    int base = (int)floor((rlo - rst) / kp.drd) - 2;
+    base = max(base, 0);
+
+    float2* rowA = (MODE == 2) ? outA + (size_t)s * P : outA + (size_t)pi * P;
+    float2* rowB = (MODE == 1) ? outB + (size_t)pi * P : nullptr;
+    // zero the output rows outside the window (MODE 2 rows are pre-zeroed)
+    if (MODE != 2) {
+        for (int k = threadIdx.x; k < P; k += NT) {
+            if (k < base || k >= base + TWIN || k >= nr) {
+                rowA[k] = make_float2(0.f, 0.f);
+                if (MODE == 1) rowB[k] = make_float2(0.f, 0.f);
+            }
+        }
+    }
+    for (int k = threadIdx.x; k < TWIN; k += NT) {
+        winA[k] = make_float2(0.f, 0.f);
+        if (MODE == 1) winB[k] = make_float2(0.f, 0.f);
+    }
+    __threadfence();
+    __syncthreads();
+
+    for (int k = threadIdx.x; k < V; k += NT) {
+        const int64_t r = r0 + k;
+        const float fx = rec.geo[r], fy = rec.geo[rec.cap + r], fz = rec.geo[2 * rec.cap + r];
+        const float nx = rec.geo[3 * rec.cap + r], ny = rec.geo[4 * rec.cap + r],
+                    nz = rec.geo[5 * rec.cap + r];
+        const float ux = rec.geo[6 * rec.cap + r], uy = rec.geo[7 * rec.cap + r],
+                    uz = rec.geo[8 * rec.cap + r];
+        const float vx = rec.geo[9 * rec.cap + r], vy = rec.geo[10 * rec.cap + r],
+                    vz = rec.geo[11 * rec.cap + r];
+        const double Itd = rec.itd[r];
+        // compTargetRays: forced ray facet -> target
+        const double dx = tx - (double)fx, dy = ty - (double)fy, dz = tz - (double)fz;
+        const double Ttd = dist3(dx, dy, dz);
+        double rngt;
+        if (MODE == 2) {
+            // genRefrPhasor: sltrng = Itd + Ttd; rngt = (sltrng - Ttd) + Ttd*(c1/c2)
+            const double sl = Itd + Ttd;
+            rngt = madd_nc(sl - Ttd, Ttd, (double)(kp.c1 / kp.c2));
+        } else {
+            rngt = madd_nc(Itd, Ttd, kp.n_d);
+        }
+        int b;
+        float w;
+        if (!bin_of(rngt, rst, kp.drd, nr, &b, &w)) continue;
+        const float Ttdf = (float)Ttd;
+        const float inv = (float)(1.0 / Ttd);
+        const float Ix = (float)dx * inv, Iy = (float)dy * inv, Iz = (float)dz * inv;
+        const float cT = fmaxf(-1.f, fminf(1.f, -(Ix * nx + Iy * ny + Iz * nz)));
+        const float cTT = fmaxf(-1.f, fminf(1.f, -(Ix * tnx + Iy * tny + Iz * tnz)));
+        const float Iu = Ix * ux + Iy * uy + Iz * uz;
+        const float Iv = Ix * vx + Iy * vy + Iz * vz;
+        // attenuation along the forced ray (prisms + background alpha2)
+        float expo = 0.f;
+        {
+            float tot = 0.f;
+            for (int p = 0; p < np; ++p) {
+                const float d = prism_len(fx, fy, fz, Ix, Iy, Iz, prisms[p], Ttdf);
+                tot += d;
+                expo += prisms[p].alpha * d;
+            }
+            expo += kp.alpha2 * (Ttdf - tot);
+        }
+        const float kc = kp.ks * cT;
+        if (MODE == 2) {
+            // EI = atten*fRfrC*rough (*rerad_spec), EO likewise; rerad(TargetTh)
+            const float fRfrC = rec.fRfrC[r];
+            float EI = expf(-4.f * expo) * fRfrC * expf(-4.f * kc * kc);
+            float EO = EI;
+            if (kp.mode & ERP_MODE_SPECULAR) {
+                const float Ith = rec.ang[r], Iph = rec.ang[rec.cap + r],
+                            Rth = rec.ang[2 * rec.cap + r];
+                const float Tth = atan2f(cross_norm(Ix, Iy, Iz, nx, ny, nz), cT);
+                const float Tph = atan2f(Iu, Iv);
+                const float dth = Rth - Tth, dph = Iph - Tph;
+                const float si = sinf(dth);
+                const float a1 = sinc_f(kp.fs_lam * si * cosf(dph)),
+                            a2 = sinc_f(kp.fs_lam * si * sinf(dph));
+                const float vi = kp.fs2_lam * a1 * a2;
+                EI *= vi * vi;
+                float sR = kp.sqrt_e2e1 * sinf(ERP_PI_F - Tth);
+                const float RrTh = (sR > 1.f) ? 1e4f : asinf(sR);
+                const float so = sinf(RrTh), dpo = Tph - Iph;
+                const float b1 = sinc_f(kp.fs_lam * so * cosf(dpo)),
+                            b2 = sinc_f(kp.fs_lam * so * sinf(dpo));
+                const float vo = kp.fs2_lam * b1 * b2;
+                EO *= vo * vo;
+                (void)Ith;
+            }
+            const float TTh = atan2f(cross_norm(Ix, Iy, Iz, tnx, tny, tnz), cTT);
+            const double sl = Itd + Ttd;
+            const double sl2 = sl * sl;
+            const float req = (float)((double)V * ((double)(kp.P * sh.grefr * sh.grefr * kp.fs *
+                                                           kp.fs * kp.fs * kp.fs) /
+                                                   (ERP_FOURPI5_SQ * sl2 * sl2)));
+            const float rc = rerad_f(kp.rerad, TTh) * EI * EO * req;
+            const float2 ph = phasor_cycles(-2.0 * rngt * kp.inv_cyc);
+            const float2 c = make_float2(ph.x * rc, ph.y * rc);
+            const int k0 = b - base;
+            if (k0 >= 0 && k0 < TWIN) lds_add2(winA, k0, c); else glb_add2(rowA, b, c);
+            continue;
+        }
+        // accumulateTarget / radiateTarget amplitude:
+        // sqrt(Pray n) = F_f * sqrt(G(Tth,Tph) G(TargetTh,Tph)) / Ttd * sqrt(EI-part)
+        // sin(Tth)cos(Tph) = I.v, sin(Tth)sin(Tph) = I.u; sin(TargetTh) = |I x tn|
+        const float rp = sqrtf(Iu * Iu + Iv * Iv);
+        const float cph = (rp > 0.f) ? Iv / rp : 1.f;
+        const float sph = (rp > 0.f) ? Iu / rp : 0.f;
+        const float sTT = cross_norm(Ix, Iy, Iz, tnx, tny, tnz);
+        const float s1 = sinc_f(kp.piLlam * Iv), s2 = sinc_f(kp.piLlam * Iu);
+        const float s3 = sinc_f(kp.piLlam * sTT * cph), s4 = sinc_f(kp.piLlam * sTT * sph);
+        const float gg = fabsf(s1 * s2) * fabsf(s3 * s4) * kp.Dmax;
+        float amp = rec.F[r] * gg / Ttdf;
+        float ampB = 0.f;
+        if (!lossless) {
+            const float e = expf(-2.f * (expo + kc * kc));
+            amp *= e;
+            if (MODE == 1) {
+                // specular EI/EO facet reradiation factors (geometry_kernels.cu:530-539,565-572)
+                const float Iph = rec.ang[rec.cap + r], Rth = rec.ang[2 * rec.cap + r];
+                const float Tth = atan2f(cross_norm(Ix, Iy, Iz, nx, ny, nz), cT);
+                const float Tph = atan2f(Iu, Iv);
+                const float si = sinf(Rth - Tth), dph = Iph - Tph;
+                const float a1 = sinc_f(kp.fs_lam * si * cosf(dph)),
+                            a2 = sinc_f(kp.fs_lam * si * sinf(dph));
+                const float vi = kp.fs2_lam * a1 * a2;
+                const float sR = kp.sqrt_e2e1 * sinf(ERP_PI_F - Tth);
+                const float RrTh = (sR > 1.f) ? 1e4f : asinf(sR);
+                const float so = sinf(RrTh), dpo = Tph - Iph;
+                const float b1 = sinc_f(kp.fs_lam * so * cosf(dpo)),
+                            b2 = sinc_f(kp.fs_lam * so * sinf(dpo));
+                const float vo = kp.fs2_lam * b1 * b2;
+                ampB = amp * fabsf(vo) / kp.sqrtP;
+                amp *= fabsf(vi);
+            }
+        } else if (MODE == 1) {
+            ampB = amp / kp.sqrtP;
+        }
+        const float2 ph = phasor_cycles(rngt * kp.inv_cyc);
+        const float2 c = make_float2(ph.x * amp, ph.y * amp);
+        const int k0 = b - base;
+        const float2 c0 = make_float2(c.x * (1.f - w), c.y * (1.f - w));
+        const float2 c1 = make_float2(c.x * w, c.y * w);
+        if (k0 >= 0 && k0 < TWIN) lds_add2(winA, k0, c0); else glb_add2(rowA, b, c0);
+        if (b + 1 < nr) {
+            if (k0 + 1 >= 0 && k0 + 1 < TWIN) lds_add2(winA, k0 + 1, c1);
+            else glb_add2(rowA, b + 1, c1);
+        }
+        if (MODE == 1) {
+            // radiateTarget: phasor(rngt - lam/2), P = 1 W
+            const float2 pb = phasor_cycles(rngt * kp.inv_cyc - 0.5 * (double)kp.lam * kp.inv_cyc);
+            const float2 d = make_float2(pb.x * ampB, pb.y * ampB);
+            const float2 d0 = make_float2(d.x * (1.f - w), d.y * (1.f - w));
+            const float2 d1 = make_float2(d.x * w, d.y * w);
+            if (k0 >= 0 && k0 < TWIN) lds_add2(winB, k0, d0); else glb_add2(rowB, b, d0);
+            if (b + 1 < nr) {
+                if (k0 + 1 >= 0 && k0 + 1 < TWIN) lds_add2(winB, k0 + 1, d1);
+                else glb_add2(rowB, b + 1, d1);
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < TWIN; k += NT) {
+        const int b = base + k;
+        if (b >= nr) break;
+        if (MODE == 2) {
+            const float2 v = winA[k];
+            if (v.x != 0.f || v.y != 0.f) glb_add2(rowA, b, v);
+        } else {
+            rowA[b] = winA[k];
+            if (MODE == 1) rowB[b] = winB[k];
+        }
+    }
+}
+
+// S[s][k] += scale * sum over the shot's pairs of X[p][k] * Z[p][k]
+__global__ void k_accum_spectra(float2* __restrict__ S, const float2* __restrict__ X,
+                                const float2* __restrict__ Z, const int* __restrict__ seg_shot,
+                                const int* __restrict__ seg_beg, const int* __restrict__ seg_end,
+                                int P, float scale) {
+    const int sg = blockIdx.y;
+    const int s = seg_shot[sg], p0 = seg_beg[sg], p1 = seg_end[sg];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < P; k += gridDim.x * blockDim.x) {
+        float2 acc = make_float2(0.f, 0.f);
+        for (int p = p0; p < p1; ++p) {
+            const float2 x = X[(size_t)p * P + k], z = Z[(size_t)p * P + k];
+            acc.x += x.x * z.x - x.y * z.y;
+            acc.y += x.x * z.y + x.y * z.x;
+        }
+        float2* d = S + (size_t)s * P + k;
+        d->x += scale * acc.x;
+        d->y += scale * acc.y;
+    }
+}
+
+// Bluestein step 1: a[n] = c[n] * cscale * w[n] (n < L), 0 for L <= n < M
+__global__ void k_blue_in(float2* __restrict__ a, int M, const float2* __restrict__ c, int P,
+                          const float2* __restrict__ w, int L, float cscale, int rows) {
+    const size_t n = (size_t)M * rows;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / M), k = (int)(i - (size_t)r * M);
+        float2 v = make_float2(0.f, 0.f);
+        if (k < L) {
+            float2 x = c[(size_t)r * P + k];
+            x.x *= cscale;
+            x.y *= cscale;
+            v = cmul(x, w[k]);
+        }
+        a[i] = v;
+    }
+}
+
+// Bluestein step 2 (between DFT_L and IDFT_L): a[n] = mask[n] conj(conv[n])
+__global__ void k_blue_mid(float2* __restrict__ a, int M, const unsigned char* __restrict__ mask,
+                           int L, int rows) {
+    const size_t n = (size_t)M * rows;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(i % M);
+        float2 v = make_float2(0.f, 0.f);
+        if (k < L && mask[k]) {
+            v = a[i];
+            v.y = -v.y;
+        }
+        a[i] = v;
+    }
+}
+
+// Bluestein output + decimation: refr[i] = conj(w[2i] conv2[2i]) / L
+__global__ void k_blue_out(float2* __restrict__ refr, int nr, const float2* __restrict__ a,
+                           int M, const float2* __restrict__ w, float invL, int rows,
+                           const int* __restrict__ live) {
+    const size_t n = (size_t)nr * rows;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / nr), j = (int)(i - (size_t)r * nr);
+        float2 v = make_float2(0.f, 0.f);
+        if (live[r]) {
+            v = cmul(w[2 * j], a[(size_t)r * M + 2 * j]);
+            v = make_float2(v.x * invL, -v.y * invL);
+        }
+        refr[i] = v;
+    }
+}
+
+// out = refl + refr (+ type-0 conv), zeros for shots with V == 0
+__global__ void k_finalize(float2* __restrict__ out, const float2* __restrict__ refl,
+                           const float2* __restrict__ refr, const float2* __restrict__ refr0,
+                           const int* __restrict__ Vs, int nr, int rows,
+                           int32_t* __restrict__ nvalid) {
+    const size_t n = (size_t)nr * rows;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / nr);
+        float2 v = make_float2(0.f, 0.f);
+        if (Vs[r] > 0) {
+            const float2 a = refl ? refl[i] : make_float2(0.f, 0.f);
+            const float2 b = refr ? refr[i] : make_float2(0.f, 0.f);
+            const float2 c = refr0 ? refr0[i] : make_float2(0.f, 0.f);
+            v = make_float2(a.x + b.x + c.x, a.y + b.y + c.y);
+        }
+        out[i] = v;
+        if (nvalid && (i % nr) == 0) nvalid[r] = Vs[r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// NoConv direct sums (reflRadarSignal / refrRadarSignal, radar_kernels.cu:
+// 58-159, 201-299): one thread per (shot, bin), facets streamed through LDS.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(NT) k_refl_direct(
+    float2* __restrict__ refl, const erp_shot* __restrict__ shots, const int* __restrict__ Vs,
+    const int64_t* __restrict__ rec_base, Rec rec, KPar kp) {
+    const int s = blockIdx.y;
+    const int ir = blockIdx.x * NT + threadIdx.x;
+    const ShotK sh = load_shot(shots, s);
+    const int V = Vs[s];
+    const int64_t r0 = rec_base[s];
+    __shared__ double it_s[NT];
+    __shared__ float fre_s[NT];
+    const double r = (double)sh.rst + (double)ir * kp.drd;
+    const bool lossless = kp.mode & ERP_MODE_LOSSLESS;
+    (void)lossless;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int k0 = 0; k0 < V; k0 += NT) {
+        const int k = k0 + threadIdx.x;
+        if (k < V) {
+            it_s[threadIdx.x] = rec.itd[r0 + k];
+            fre_s[threadIdx.x] = rec.F[r0 + k];  // fRe stored in F for this mode
+        }
+        __syncthreads();
+        const int m = min(NT, V - k0);
+        for (int j = 0; j < m; ++j) {
+            const double itd = it_s[j];
+            const float dl = (float)((r - itd) / (double)kp.rng_res);
+            const float2 ph = phasor_cycles(-2.0 * itd * kp.inv_cyc);
+            const float sc = sinc_f(dl) * fre_s[j];
+            acc.x += ph.x * sc;
+            acc.y += ph.y * sc;
+        }
+        __syncthreads();
+    }
+    if (ir < kp.nr) refl[(size_t)s * kp.nr + ir] = acc;
+}
+
+__global__ void __launch_bounds__(NT) k_refr0_direct(
+    float2* __restrict__ refr, const float* __restrict__ tg, int64_t Tpad,
+    const int* __restrict__ list0, const int* __restrict__ tcnt, int Tcap,
+    const erp_shot* __restrict__ shots, const int* __restrict__ Vs,
+    const int64_t* __restrict__ rec_base, Rec rec, const erp_prism* __restrict__ prisms, int np,
+    KPar kp) {
+    const int s = blockIdx.y;
+    const int ir = blockIdx.x * NT + threadIdx.x;
+    const ShotK sh = load_shot(shots, s);
+    const int V = Vs[s];
+    const int n0 = tcnt[2 * s + 1];
+    const int64_t r0 = rec_base[s];
+    __shared__ double rng_s[NT];
+    __shared__ float rc_s[NT];
+    const double r = (double)sh.rst + (double)ir * kp.drd;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int ti = 0; ti < n0; ++ti) {
+        const int t = list0[(size_t)s * Tcap + ti];
+        const double tx = tg[t], ty = tg[Tpad + t], tz = tg[2 * Tpad + t];
+        for (int k0 = 0; k0 < V; k0 += NT) {
+            const int k = k0 + threadIdx.x;
+            if (k < V) {
+                const int64_t rr = r0 + k;
+                const float fx = rec.geo[rr], fy = rec.geo[rec.cap + rr], fz = rec.geo[2 * rec.cap + rr];
+                const float nx = rec.geo[3 * rec.cap + rr], ny = rec.geo[4 * rec.cap + rr],
+                            nz = rec.geo[5 * rec.cap + rr];
+                const double Itd = rec.itd[rr];
+                const double dx = tx - fx, dy = ty - fy, dz = tz - fz;
+                const double Ttd = dist3(dx, dy, dz);
+                const float inv = (float)(1.0 / Ttd);
+                const float Ix = (float)dx * inv, Iy = (float)dy * inv, Iz = (float)dz * inv;
+                const float cT = fmaxf(-1.f, fminf(1.f, -(Ix * nx + Iy * ny + Iz * nz)));
+                float expo = 0.f, tot = 0.f;
+                for (int p = 0; p < np; ++p) {
+                    const float d = prism_len(fx, fy, fz, Ix, Iy, Iz, prisms[p], (float)Ttd);
+                    tot += d;
+                    expo += prisms[p].alpha * d;
+                }
+                expo += kp.alpha2 * ((float)Ttd - tot);
+                const float kc = kp.ks * cT;
+                const float fRfrC = rec.fRfrC[rr];
+                float EI = expf(-4.f * expo) * fRfrC * expf(-4.f * kc * kc);
+                float EO = EI;
+                {
+                    const float Iu = Ix * rec.geo[6 * rec.cap + rr] + Iy * rec.geo[7 * rec.cap + rr] +
+                                     Iz * rec.geo[8 * rec.cap + rr];
+                    const float Iv = Ix * rec.geo[9 * rec.cap + rr] + Iy * rec.geo[10 * rec.cap + rr] +
+                                     Iz * rec.geo[11 * rec.cap + rr];
+                    const float Iph = rec.ang[rec.cap + rr], Rth = rec.ang[2 * rec.cap + rr];
+                    const float Tth = atan2f(cross_norm(Ix, Iy, Iz, nx, ny, nz), cT);
+                    const float Tph = atan2f(Iu, Iv);
+                    const float si = sinf(Rth - Tth), dph = Iph - Tph;
+                    const float a1 = sinc_f(kp.fs_lam * si * cosf(dph)),
+                                a2 = sinc_f(kp.fs_lam * si * sinf(dph));
+                    const float vi = kp.fs2_lam * a1 * a2;
+                    EI *= vi * vi;
+                    const float sR = kp.sqrt_e2e1 * sinf(ERP_PI_F - Tth);
+                    const float RrTh = (sR > 1.f) ? 1e4f : asinf(sR);
+                    const float so = sinf(RrTh), dpo = Tph - Iph;
+                    const float b1 = sinc_f(kp.fs_lam * so * cosf(dpo)),
+                                b2 = sinc_f(kp.fs_lam * so * sinf(dpo));
+                    const float vo = kp.fs2_lam * b1 * b2;
+                    EO *= vo * vo;
+                }
+                const double sl = Itd + Ttd;
+                const double sl2 = sl * sl;
+                const float req = (float)((double)V * ((double)(kp.P * sh.grefr * sh.grefr * kp.fs *
+                                                               kp.fs * kp.fs * kp.fs) /
+                                                       (ERP_FOURPI5_SQ * sl2 * sl2)));
+                // refrRadarSignal passes d_Tth to rerad_funct (sim.cu:930-934)
+                rc_s[threadIdx.x] =
+                    rerad_f(kp.rerad, atan2f(cross_norm(Ix, Iy, Iz, nx, ny, nz), cT)) * EI * EO * req;
+                rng_s[threadIdx.x] = madd_nc(sl - Ttd, Ttd, (double)(kp.c / kp.c2));
+            }
+            __syncthreads();
+            const int m = min(NT, V - k0);
+            for (int j = 0; j < m; ++j) {
+                const double rngt = rng_s[j];
+                const float dl = (float)((r - rngt) / (double)kp.rng_res);
+                const float2 ph = phasor_cycles(-2.0 * rngt * kp.inv_cyc);
+                const float sc = sinc_f(dl) * rc_s[j];
+                acc.x += ph.x * sc;
+                acc.y += ph.y * sc;
+            }
+            __syncthreads();
+        }
+    }
+    if (ir < kp.nr) refr[(size_t)s * kp.nr + ir] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+inline int grid_for(size_t n, int per = 256, int cap = 1 << 16) {
+    size_t g = (n + per - 1) / per;
+    if (g < 1) g = 1;
+    if (g > (size_t)cap) g = cap;
+    return (int)g;
+}
+
+KPar make_kpar(const erp_params& p, int P_fft) {
+    KPar k;
+    k.lam = p.lam; k.fs = p.fs; k.P = p.P; k.ks = p.ks; k.eps1 = p.eps_1; k.eps2 = p.eps_2;
+    k.nu1 = p.nu_1; k.nu2 = p.nu_2; k.alpha1 = p.alpha1; k.alpha2 = p.alpha2;
+    k.rng_res = p.rng_res; k.dr = p.dr; k.c = p.c; k.c1 = p.c_1; k.c2 = p.c_2;
+    k.piLlam = (ERP_PI_F * p.fs) / p.lam;
+    k.Dmax = (4.f * ERP_PI_F * p.fs * p.fs) / (p.lam * p.lam);
+    k.fs_lam = p.fs / p.lam;
+    k.fs2_lam = p.fs * p.fs / p.lam;
+    k.nf = sqrtf(p.eps_2);
+    k.sqrtP = sqrtf(p.P);
+    k.invK = (float)(1.0 / ERP_FOURPI5_SQ);
+    k.sqrt_e1e2 = sqrtf(p.eps_1 / p.eps_2);
+    k.sqrt_e2e1 = sqrtf(p.eps_2 / p.eps_1);
+    k.drd = (double)p.dr;
+    k.cos_ap = p.cos_ap;
+    k.n_d = std::sqrt((double)p.eps_2);
+    // phasor(r) = exp(-i (2 PI_F / lam) r) = exp(-i 2 pi r * inv_cyc)
+    k.inv_cyc = (double)ERP_PI_F / (M_PI * (double)p.lam);
+    k.lam_d = p.lam;
+    k.B = p.B;
+    k.smpl = p.smpl;
+    k.nr = p.nr;
+    k.P_fft = P_fft;
+    k.rerad = p.rerad_funct;
+    k.mode = p.mode;
+    k.seed = p.seed;
+    return k;
+}
+
+const float2* get_chirp(erp_ctx* ctx, int nr, float dr, float res, int kind, float rst, int P,
+                        const FftPlan& plan, hipStream_t st) {
+    ChirpKey key{nr, dr, res, kind, kind == 1 ? rst : 0.f};
+    for (ChirpSpec* c : ctx->chirps)
+        if (c->key == key) return c->spec.as<float2>();
+    ChirpSpec* c = new ChirpSpec();
+    c->key = key;
+    c->spec.ensure(sizeof(float2) * (size_t)P);
+    hipLaunchKernelGGL(k_gen_kernel, dim3((P + 255) / 256), dim3(256), 0, st,
+                       c->spec.as<float2>(), P, nr, (double)dr, (double)res, (double)rst, kind);
+    ERP_CHECK_LAUNCH("k_gen_kernel");
+    fft_forward(ctx, plan, c->spec.as<float2>(), 1, st);
+    hipLaunchKernelGGL(k_scale, dim3((P + 255) / 256), dim3(256), 0, st, c->spec.as<float2>(), P,
+                       1.0f / (float)P);
+    ERP_CHECK_LAUNCH("k_scale");
+    if (ctx->chirps.size() > 16) {
+        delete ctx->chirps.front();
+        ctx->chirps.erase(ctx->chirps.begin());
+    }
+    ctx->chirps.push_back(c);
+    return c->spec.as<float2>();
+}
+
+BlueTables* get_blue(erp_ctx* ctx, int L, float smpl, float B, hipStream_t st) {
+    for (BlueTables* b : ctx->blues)
+        if (b->L == L && b->smpl == smpl && b->B == B) return b;
+    BlueTables* b = new BlueTables();
+    b->L = L;
+    b->M = next_pow2(2 * (int64_t)L - 1);
+    b->smpl = smpl;
+    b->B = B;
+    b->w.ensure(sizeof(float2) * (size_t)L);
+    b->Bw.ensure(sizeof(float2) * (size_t)b->M);
+    b->mask.ensure((size_t)L);
+    hipLaunchKernelGGL(k_blue_tables, dim3((b->M + 255) / 256), dim3(256), 0, st,
+                       b->w.as<float2>(), b->Bw.as<float2>(), b->mask.as<unsigned char>(), L,
+                       b->M, (double)smpl, (double)B);
+    ERP_CHECK_LAUNCH("k_blue_tables");
+    FftPlan pm = fft_plan(b->M);
+    fft_forward(ctx, pm, b->Bw.as<float2>(), 1, st);
+    hipLaunchKernelGGL(k_scale, dim3((b->M + 255) / 256), dim3(256), 0, st, b->Bw.as<float2>(),
+                       b->M, 1.0f / (float)b->M);
+    ERP_CHECK_LAUNCH("k_scale");
+    if (ctx->blues.size() > 8) {
+        delete ctx->blues.front();
+        ctx->blues.erase(ctx->blues.begin());
+    }
+    ctx->blues.push_back(b);
+    return b;
+}
+
+void zero(float2* p, size_t n, hipStream_t st) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_zero, dim3(grid_for(n)), dim3(256), 0, st, p, n);
+    ERP_CHECK_LAUNCH("k_zero");
+}
+
+// windowed convolution of `rows` padded rows (P each) with spectrum g:
+// data := FFT, *g, IFFT in place; then out[r][j] = data[r][j + nr - 1]
+void window_conv(erp_ctx* ctx, const FftPlan& plan, float2* data, int rows, const float2* g,
+                 float2* out, int nr, hipStream_t st) {
+    if (rows <= 0) return;
+    const int P = plan.N;
+    fft_forward(ctx, plan, data, rows, st);
+    hipLaunchKernelGGL(k_mul_spec, dim3(grid_for((size_t)P * rows)), dim3(256), 0, st, data, data,
+                       g, P, rows);
+    ERP_CHECK_LAUNCH("k_mul_spec");
+    fft_inverse(ctx, plan, data, rows, st);
+    hipLaunchKernelGGL(k_window, dim3(grid_for((size_t)nr * rows)), dim3(256), 0, st, out, nr,
+                       data, P, nr, rows, 1.0f, 0);
+    ERP_CHECK_LAUNCH("k_window");
+}
+
+struct BatchPlan {
+    int S;
+    int64_t shot0;
+};
+
+}  // namespace
+
+int64_t erp_footprint_impl(const erp_params* par, int64_t F, int64_t T, int64_t ns) {
+    const int64_t nr = par->nr;
+    const int64_t P = next_pow2(2 * nr - 1);
+    const int64_t M = next_pow2(2 * (2 * nr - 1) - 1);
+    const int64_t S = std::min<int64_t>(ns, 32);
+    const int64_t nblk = (F + FB - 1) / FB;
+    int64_t b = 0;
+    b += S * nblk * (MASKW * 8 + 8);          // mask + cnt + off
+    b += S * std::min<int64_t>(F, 1 << 20) * 76;  // facet records (typical cap)
+    b += S * P * 8 * 3 + S * M * 8 + S * nr * 8 * 3;
+    if (T > 0) {
+        b += S * T * 8;                       // target lists
+        b += 3 * 256 * P * 8;                 // target chunk buffers
+    }
+    return b;
+}
+
+void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
+                  const erp_targets* targets, const erp_shot* shots, int64_t ns,
+                  const erp_prism* prisms, int64_t nprisms, float2* out_dev,
+                  int32_t* nvalid_dev, uint32_t flags, hipStream_t st, WorkBufs& W) {
+    const erp_params& p = *par;
+    const uint32_t mode = p.mode;
+    const bool conv = mode & ERP_MODE_CONVOLUTION;
+    const bool lin = mode & ERP_MODE_CONV_LINEAR;
+    const bool spec = mode & ERP_MODE_SPECULAR;
+    const int nr = p.nr;
+    if (nr <= 0 || nr > 32767) throw ErpError{ERP_E_INVALID, "nr must be in [1, 32767]"};
+    if (!spec && !(conv && lin))
+        throw ErpError{ERP_E_UNSUPPORTED,
+                       "non-specular runs need convolution and convolution_linear "
+                       "(sim.cu:431-439,778 otherwise write past the chirp buffer)"};
+    const int64_t F = mesh->F;
+    const int64_t T = targets ? targets->T : 0;
+    bool any_typed = false, any_t0 = false;
+    std::vector<unsigned char> htype;
+    if (T > 0) {
+        htype.resize(T);
+        ERP_HIP_TRY(hipMemcpyAsync(htype.data(), targets->type.p, T, hipMemcpyDeviceToHost, st));
+        ERP_HIP_TRY(hipStreamSynchronize(st));
+        for (auto t : htype) (t ? any_typed : any_t0) = true;
+    }
+    if (any_typed && !(conv && lin))
+        throw ErpError{ERP_E_UNSUPPORTED,
+                       "targets of type != 0 need convolution and convolution_linear "
+                       "(sim.cu:410,872,912 otherwise use unallocated buffers)"};
+
+    const int P = next_pow2(2 * (int64_t)nr - 1);
+    const FftPlan planP = fft_plan(P);
+    const KPar kp = make_kpar(p, P);
+    const int L = 2 * nr - 1;
+    const int nblk = (int)((F + FB - 1) / FB);
+    const bool need_rec = (T > 0) || (spec && !conv);
+    const bool do_trace = !spec || (conv && lin);
+    const float res_t = spec ? p.rng_res : 2.f * p.rng_res;  // sim.cu:853-858
+
+    // chirp spectra (cached per context)
+    const float2* g_surf = nullptr;
+    if (!spec || (conv && lin)) g_surf = get_chirp(ctx, nr, p.dr, p.rng_res, 0, 0.f, P, planP, st);
+    const float2* g_targ = nullptr;
+    if (T > 0 && conv && lin) g_targ = get_chirp(ctx, nr, p.dr, res_t, 0, 0.f, P, planP, st);
+    BlueTables* blue = nullptr;
+    FftPlan planM;
+    if (any_typed) {
+        blue = get_blue(ctx, L, p.smpl, p.B, st);
+        planM = fft_plan(blue->M);
+    }
+
+    // batch size: bounded by memory of per-shot buffers
+    int Smax = 32;
+    if ((int64_t)P * 8 * 4 * Smax > (int64_t)1 << 30) Smax = std::max<int>(1, (int)(((int64_t)1 << 30) / ((int64_t)P * 32)));
+    const int64_t Tpad = targets ? targets->Tpad : 0;
+
+    W.shots.ensure(sizeof(erp_shot) * (size_t)Smax);
+    if (nprisms > 0) {
+        W.prisms.ensure(sizeof(erp_prism) * (size_t)nprisms);
+        ERP_HIP_TRY(hipMemcpyAsync(W.prisms.p, prisms, sizeof(erp_prism) * nprisms,
+                                   hipMemcpyHostToDevice, st));
+    }
+    W.cnt.ensure(sizeof(int) * (size_t)Smax * nblk);
+    W.off.ensure(sizeof(int) * (size_t)Smax * nblk);
+    W.V.ensure(sizeof(int) * (size_t)Smax);
+    W.Vd.ensure(sizeof(unsigned long long) * (size_t)Smax * nblk * MASKW);  // mask
+    W.bounds.ensure(sizeof(Bounds) * (size_t)Smax);
+    W.tcount.ensure(sizeof(int64_t) * (size_t)Smax);  // rec_base
+    W.psurf.ensure(sizeof(float2) * (size_t)Smax * P);
+    W.refl.ensure(sizeof(float2) * (size_t)Smax * nr);
+    W.out.ensure(sizeof(float2) * (size_t)Smax * nr * 2);  // refr + refr0
+    if (any_typed) {
+        W.shotS.ensure(sizeof(float2) * (size_t)Smax * P);
+        W.blu.ensure(sizeof(float2) * (size_t)Smax * planM.N);
+    }
+    if (any_t0) W.t0trace.ensure(sizeof(float2) * (size_t)Smax * P);
+    if (T > 0) {
+        W.tlist.ensure(sizeof(int) * (size_t)Smax * T * 2);
+        W.tcnt.ensure(sizeof(int) * (size_t)Smax * 2);
+    }
+
+    std::vector<int> hV(Smax), htc(2 * Smax);
+    std::vector<int64_t> hbase(Smax + 1);
+    int* dV = W.V.as<int>();
+    int64_t* dbase = W.tcount.as<int64_t>();
+    unsigned long long* dmask = W.Vd.as<unsigned long long>();
+    Bounds* dbounds = W.bounds.as<Bounds>();
+    float2* refr = W.out.as<float2>();
+    float2* refr0 = refr + (size_t)Smax * nr;
+
+    for (int64_t s0 = 0; s0 < ns; s0 += Smax) {
+        const int S = (int)std::min<int64_t>(Smax, ns - s0);
+        ERP_HIP_TRY(hipMemcpyAsync(W.shots.p, shots + s0, sizeof(erp_shot) * S,
+                                   hipMemcpyHostToDevice, st));
+        const erp_shot* dshots = W.shots.as<erp_shot>();
+
+        // 1-2: aperture mask, counts, offsets
+        hipLaunchKernelGGL(k_illum_mask, dim3(nblk, S), dim3(NT), 0, st, mesh->soa.as<float>(),
+                           mesh->Fpad, F, dshots, p.cos_ap, dmask, W.cnt.as<int>(), nblk);
+        ERP_CHECK_LAUNCH("k_illum_mask");
+        hipLaunchKernelGGL(k_scan_blocks, dim3(S), dim3(1024), 0, st, W.cnt.as<int>(),
+                           W.off.as<int>(), dV, nblk);
+        ERP_CHECK_LAUNCH("k_scan_blocks");
+        ERP_HIP_TRY(hipMemcpyAsync(hV.data(), dV, sizeof(int) * S, hipMemcpyDeviceToHost, st));
+        ERP_HIP_TRY(hipStreamSynchronize(st));
+        hbase[0] = 0;
+        for (int s = 0; s < S; ++s) hbase[s + 1] = hbase[s] + (need_rec ? hV[s] : 0);
+        const int64_t Vtot = hbase[S];
+        ERP_HIP_TRY(hipMemcpyAsync(dbase, hbase.data(), sizeof(int64_t) * S,
+                                   hipMemcpyHostToDevice, st));
+        Rec rec{};
+        if (need_rec && Vtot > 0) {
+            const size_t cap = (size_t)Vtot;
+            const size_t bytes = cap * (12 * 4 + 8 + 4 + 4 + 3 * 4) + 256 * 6;
+            W.rec.ensure(bytes);
+            char* b = W.rec.as<char>();
+            auto take = [&](size_t n) { char* r = b; b += (n + 255) & ~(size_t)255; return r; };
+            rec.itd = (double*)take(cap * 8);
+            rec.geo = (float*)take(cap * 12 * 4);
+            rec.F = (float*)take(cap * 4);
+            rec.fRfrC = (float*)take(cap * 4);
+            rec.ang = (float*)take(cap * 3 * 4);
+            rec.cap = (int64_t)cap;
+        }
+
+        // 3: fused surface kernel
+        zero(W.psurf.as<float2>(), (size_t)S * P, st);
+        hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, st, dbounds, S);
+        ERP_CHECK_LAUNCH("k_init_bounds");
+        hipLaunchKernelGGL(k_surface, dim3(nblk, S), dim3(NT), 0, st, mesh->soa.as<float>(),
+                           mesh->Fpad, F, dshots, dmask, W.off.as<int>(), dV, dbase, nblk, kp, rec,
+                           (need_rec && Vtot > 0) ? 1 : 0, do_trace ? 1 : 0, W.psurf.as<float2>(),
+                           s0, dbounds);
+        ERP_CHECK_LAUNCH("k_surface");
+
+        if (flags & ERP_RUN_RAW_SURFACE) {
+            // debug_surface: the raw surface phasor trace, no range compression
+            hipLaunchKernelGGL(k_copy_rows, dim3(grid_for((size_t)nr * S)), dim3(256), 0, st,
+                               out_dev + (size_t)s0 * nr, nr, W.psurf.as<float2>(), P, nr, S);
+            ERP_CHECK_LAUNCH("k_copy_rows");
+            if (nvalid_dev)
+                ERP_HIP_TRY(hipMemcpyAsync(nvalid_dev + s0, dV, sizeof(int) * S,
+                                           hipMemcpyDeviceToDevice, st));
+            continue;
+        }
+
+        // 4: reflected signal
+        float2* refl = W.refl.as<float2>();
+        bool have_refl = false;
+        if (!spec || (conv && lin)) {
+            if (mode & ERP_MODE_DISABLE_SURFACE && !spec) {
+                zero(refl, (size_t)S * nr, st);
+            } else {
+                window_conv(ctx, planP, W.psurf.as<float2>(), S, g_surf, refl, nr, st);
+            }
+            have_refl = true;
+        } else if (spec && !conv) {
+            hipLaunchKernelGGL(k_refl_direct, dim3((nr + NT - 1) / NT, S), dim3(NT), 0, st, refl,
+                               dshots, dV, dbase, rec, kp);
+            ERP_CHECK_LAUNCH("k_refl_direct");
+            have_refl = true;
+        }
+
+        // 5-7: targets
+        bool have_refr = false, have_refr0 = false;
+        if (T > 0) {
+            const int Tcap = (int)T;
+            int* list1 = W.tlist.as<int>();
+            int* list0 = list1 + (size_t)S * Tcap;
+            hipLaunchKernelGGL(k_target_select, dim3(S), dim3(1024), 0, st,
+                               targets->xyzn.as<float>(), Tpad, (int)T,
+                               targets->type.as<unsigned char>(), dshots, dV, p.cos_ap, list1,
+                               list0, W.tcnt.as<int>(), Tcap);
+            ERP_CHECK_LAUNCH("k_target_select");
+            ERP_HIP_TRY(hipMemcpyAsync(htc.data(), W.tcnt.p, sizeof(int) * 2 * S,
+                                       hipMemcpyDeviceToHost, st));
+            ERP_HIP_TRY(hipStreamSynchronize(st));
+            const erp_prism* dpr = nprisms > 0 ? W.prisms.as<erp_prism>() : nullptr;
+
+            // type != 0 targets: per-pair traces + spectral sum
+            int64_t npairs = 0;
+            for (int s = 0; s < S; ++s) npairs += htc[2 * s];
+            if (npairs > 0) {
+                float2* Sacc = W.shotS.as<float2>();
+                zero(Sacc, (size_t)S * P, st);
+                // chunk size: bounded by 3 (or 4) P-rows per pair
+                const int nbuf = spec ? 4 : 3;
+                int C = (int)std::max<int64_t>(1, ((int64_t)768 << 20) / ((int64_t)P * 8 * nbuf));
+                C = std::min<int>(C, 4096);
+                W.tA.ensure(sizeof(float2) * (size_t)C * P * (nbuf - 1));
+                W.tC.ensure(sizeof(float2) * (size_t)C * P);
+                float2* A = W.tA.as<float2>();
+                float2* Bq = A + (size_t)C * P;                 // y (and MODE 1: B)
+                float2* Bs = spec ? Bq + (size_t)C * P : nullptr;
+                float2* Z = W.tC.as<float2>();
+                std::vector<int> pshot, ptgt;
+                std::vector<int> hl1((size_t)S * Tcap);
+                ERP_HIP_TRY(hipMemcpyAsync(hl1.data(), list1, sizeof(int) * (size_t)S * Tcap,
+                                           hipMemcpyDeviceToHost, st));
+                ERP_HIP_TRY(hipStreamSynchronize(st));
+                W.pairs.ensure(sizeof(int) * (size_t)C * 2 + sizeof(int) * 3 * (size_t)S);
+                int* dps = W.pairs.as<int>();
+                int* dpt = dps + C;
+                int* dseg = dpt + C;
+                // iterate pairs in (shot, target) order
+                int cs = 0, cj = 0;
+                while (cs < S) {
+                    pshot.clear();
+                    ptgt.clear();
+                    while (cs < S && (int)pshot.size() < C) {
+                        if (cj >= htc[2 * cs]) { ++cs; cj = 0; continue; }
+                        pshot.push_back(cs);
+                        ptgt.push_back(hl1[(size_t)cs * Tcap + cj]);
+                        ++cj;
+                    }
+                    const int n = (int)pshot.size();
+                    if (n == 0) break;
+                    // segments (contiguous pairs of one shot)
+                    std::vector<int> sg_s, sg_b, sg_e;
+                    for (int i = 0; i < n; ++i) {
+                        if (i == 0 || pshot[i] != pshot[i - 1]) {
+                            sg_s.push_back(pshot[i]);
+                            sg_b.push_back(i);
+                            sg_e.push_back(i + 1);
+                        } else {
+                            sg_e.back() = i + 1;
+                        }
+                    }
+                    const int nseg = (int)sg_s.size();
+                    std::vector<int> up(2 * (size_t)C + 3 * (size_t)S, 0);
+                    std::copy(pshot.begin(), pshot.end(), up.begin());
+                    std::copy(ptgt.begin(), ptgt.end(), up.begin() + C);
+                    std::copy(sg_s.begin(), sg_s.end(), up.begin() + 2 * C);
+                    std::copy(sg_b.begin(), sg_b.end(), up.begin() + 2 * C + S);
+                    std::copy(sg_e.begin(), sg_e.end(), up.begin() + 2 * C + 2 * S);
+                    ERP_HIP_TRY(hipMemcpyAsync(dps, up.data(), sizeof(int) * up.size(),
+                                               hipMemcpyHostToDevice, st));
+                    if (!spec) {
+                        hipLaunchKernelGGL(k_target_accum<0>, dim3(n), dim3(NT),
+                                           sizeof(float2) * TWIN, st, targets->xyzn.as<float>(),
+                                           Tpad, dps, dpt, dshots, dV, dbase, rec, dbounds, dpr,
+                                           (int)nprisms, kp, A, nullptr);
+                        ERP_CHECK_LAUNCH("k_target_accum<0>");
+                    } else {
+                        hipLaunchKernelGGL(k_target_accum<1>, dim3(n), dim3(NT),
+                                           sizeof(float2) * TWIN * 2, st,
+                                           targets->xyzn.as<float>(), Tpad, dps, dpt, dshots, dV,
+                                           dbase, rec, dbounds, dpr, (int)nprisms, kp, A, Bs);
+                        ERP_CHECK_LAUNCH("k_target_accum<1>");
+                    }
+                    // X = FFT(A); y = IFFT(X * g_targ); Ptarg = window(y); Z = FFT(Ptarg)
+                    fft_forward(ctx, planP, A, n, st);
+                    hipLaunchKernelGGL(k_mul_spec, dim3(grid_for((size_t)P * n)), dim3(256), 0,
+                                       st, Bq, A, g_targ, P, n);
+                    ERP_CHECK_LAUNCH("k_mul_spec");
+                    fft_inverse(ctx, planP, Bq, n, st);
+                    hipLaunchKernelGGL(k_window, dim3(grid_for((size_t)P * n)), dim3(256), 0, st,
+                                       Z, P, Bq, P, nr, n, 1.0f, 1);
+                    ERP_CHECK_LAUNCH("k_window");
+                    fft_forward(ctx, planP, Z, n, st);
+                    const float2* Xs = A;
+                    float sc = -1.0f / kp.sqrtP;  // outward trace = -A/sqrt(P)
+                    if (spec) {
+                        fft_forward(ctx, planP, Bs, n, st);
+                        Xs = Bs;
+                        sc = 1.0f;
+                    }
+                    hipLaunchKernelGGL(k_accum_spectra, dim3(std::max(1, P / 256), nseg), dim3(256),
+                                       0, st, Sacc, Xs, Z, dseg, dseg + S, dseg + 2 * S, P, sc);
+                    ERP_CHECK_LAUNCH("k_accum_spectra");
+                }
+                // per shot: c = IFFT(S)/P; Bluestein DFT_L, crop, IDFT_L, decimate
+                fft_inverse(ctx, planP, Sacc, S, st);
+                float2* bu = W.blu.as<float2>();
+                const int M = planM.N;
+                hipLaunchKernelGGL(k_blue_in, dim3(grid_for((size_t)M * S)), dim3(256), 0, st, bu,
+                                   M, Sacc, P, blue->w.as<float2>(), L, 1.0f / (float)P, S);
+                ERP_CHECK_LAUNCH("k_blue_in");
+                fft_forward(ctx, planM, bu, S, st);
+                hipLaunchKernelGGL(k_mul_spec, dim3(grid_for((size_t)M * S)), dim3(256), 0, st, bu,
+                                   bu, blue->Bw.as<float2>(), M, S);
+                ERP_CHECK_LAUNCH("k_mul_spec");
+                fft_inverse(ctx, planM, bu, S, st);
+                hipLaunchKernelGGL(k_blue_mid, dim3(grid_for((size_t)M * S)), dim3(256), 0, st, bu,
+                                   M, blue->mask.as<unsigned char>(), L, S);
+                ERP_CHECK_LAUNCH("k_blue_mid");
+                fft_forward(ctx, planM, bu, S, st);
+                hipLaunchKernelGGL(k_mul_spec, dim3(grid_for((size_t)M * S)), dim3(256), 0, st, bu,
+                                   bu, blue->Bw.as<float2>(), M, S);
+                ERP_CHECK_LAUNCH("k_mul_spec");
+                fft_inverse(ctx, planM, bu, S, st);
+                hipLaunchKernelGGL(k_blue_out, dim3(grid_for((size_t)nr * S)), dim3(256), 0, st,
+                                   refr, nr, bu, M, blue->w.as<float2>(), 1.0f / (float)L, S, dV);
+                ERP_CHECK_LAUNCH("k_blue_out");
+                have_refr = true;
+            }
+
+            // type 0 targets
+            int64_t n0 = 0;
+            for (int s = 0; s < S; ++s) n0 += htc[2 * s + 1];
+            if (n0 > 0) {
+                if (!conv) {
+                    hipLaunchKernelGGL(k_refr0_direct, dim3((nr + NT - 1) / NT, S), dim3(NT), 0, st,
+                                       refr0, targets->xyzn.as<float>(), Tpad, list0,
+                                       W.tcnt.as<int>(), Tcap, dshots, dV, dbase, rec, dpr,
+                                       (int)nprisms, kp);
+                    ERP_CHECK_LAUNCH("k_refr0_direct");
+                } else {
+                    float2* t0 = W.t0trace.as<float2>();
+                    zero(t0, (size_t)S * P, st);
+                    std::vector<int> hl0((size_t)S * Tcap);
+                    ERP_HIP_TRY(hipMemcpyAsync(hl0.data(), list0, sizeof(int) * (size_t)S * Tcap,
+                                               hipMemcpyDeviceToHost, st));
+                    ERP_HIP_TRY(hipStreamSynchronize(st));
+                    std::vector<int> ps, pt;
+                    for (int s = 0; s < S; ++s)
+                        for (int j = 0; j < htc[2 * s + 1]; ++j) {
+                            ps.push_back(s);
+                            pt.push_back(hl0[(size_t)s * Tcap + j]);
+                        }
+                    const int n = (int)ps.size();
+                    W.pairs.ensure(sizeof(int) * 2 * (size_t)n + 64);
+                    int* dps = W.pairs.as<int>();
+                    std::vector<int> up(ps);
+                    up.insert(up.end(), pt.begin(), pt.end());
+                    ERP_HIP_TRY(hipMemcpyAsync(dps, up.data(), sizeof(int) * up.size(),
+                                               hipMemcpyHostToDevice, st));
+                    hipLaunchKernelGGL(k_target_accum<2>, dim3(n), dim3(NT), sizeof(float2) * TWIN,
+                                       st, targets->xyzn.as<float>(), Tpad, dps, dps + n, dshots,
+                                       dV, dbase, rec, dbounds, dpr, (int)nprisms, kp, t0, nullptr);
+                    ERP_CHECK_LAUNCH("k_target_accum<2>");
+                    if (lin) {
+                        window_conv(ctx, planP, t0, S, g_targ, refr0, nr, st);
+                    } else {
+                        // circular: genChirp table depends on rst (per shot)
+                        std::vector<erp_shot> hs(shots + s0, shots + s0 + S);
+                        for (int s = 0; s < S; ++s) {
+                            const float2* gc = get_chirp(ctx, nr, p.dr, p.rng_res, 1, hs[s].rst, P,
+                                                         planP, st);
+                            window_conv(ctx, planP, t0 + (size_t)s * P, 1, gc,
+                                        refr0 + (size_t)s * nr, nr, st);
+                        }
+                    }
+                }
+                have_refr0 = true;
+            }
+        }
+
+        // 8: combine
+        hipLaunchKernelGGL(k_finalize, dim3(grid_for((size_t)nr * S)), dim3(256), 0, st,
+                           out_dev + (size_t)s0 * nr, have_refl ? W.refl.as<float2>() : nullptr,
+                           have_refr ? refr : nullptr, have_refr0 ? refr0 : nullptr, dV, nr, S,
+                           nvalid_dev ? nvalid_dev + s0 : nullptr);
+        ERP_CHECK_LAUNCH("k_finalize");
+    }
+}

@@ -1,0 +1,181 @@
+/*
+ * erp.h — C ABI of the MI355X (gfx950) per-shot raypath / phasor / range-
+ * compression path of the EuropaRaypaths radar-sounder simulator.
+ *
+ * The reference has no library boundary: its path is reached through the
+ * process boundary `sim <params.json> <facets.fct> <targets.txt> <outdir>`
+ * (src/CUDA/sim.cu:16, 92-106, 187-224) and writes one text trace per shot
+ * (sim.cu:995-998, file_io.cu:284-296).  Each entry point below replaces one
+ * part of that process so that a ctypes caller can drive the path in-process:
+ *
+ *   erp_ctx_create / erp_ctx_destroy  - device set-up and tear-down that
+ *                                        sim.cu:272-607 performs inline.
+ *   erp_mesh_upload                    - loadFacetFile + the 12 SoA cudaMalloc/
+ *                                        cudaMemcpy calls (file_io.cu:337-362,
+ *                                        sim.cu:228-311).
+ *   erp_targets_upload                 - loadTargetFile + host target arrays
+ *                                        (file_io.cu:365-426, sim.cu:185-210).
+ *   erp_run / erp_run_dev              - the shot loop sim.cu:615-1008 and every
+ *                                        kernel it launches, for ns shots.
+ *   erp_submit / erp_poll / erp_wait   - asynchronous form of erp_run used by
+ *                                        the serving loop (no reference
+ *                                        counterpart: the reference is
+ *                                        synchronous, sim.cu:992-998).
+ *   erp_last_error                     - replaces checkCUDAError's exit()
+ *                                        (sim.cu:60-70): every call returns a
+ *                                        status code and never exits.
+ *
+ * Conventions: 0 = success, negative = error (see ERP_E_*).  The caller owns
+ * every host buffer; the library owns every device handle.  One context is
+ * driven by one submitting thread.  Output traces are shot-major: shot s
+ * occupies out[s*nr*2 .. (s+1)*nr*2) as interleaved (re, im) float32, the same
+ * order as the reference's s%06d.txt rows (file_io.cu:291-293).
+ */
+#ifndef ERP_H
+#define ERP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define ERP_OK             0
+#define ERP_E_INVALID     -1  /* bad argument or unsupported mode combination */
+#define ERP_E_HIP         -2  /* HIP runtime error (message in erp_last_error) */
+#define ERP_E_NOMEM       -3  /* device allocation failed */
+#define ERP_E_UNSUPPORTED -4  /* mode the reference itself cannot run */
+#define ERP_E_NOTREADY    -5  /* erp_poll: job still running */
+
+/* ---- processing mode bits (erp_params.mode), file_io.cu:238-246 --------- */
+#define ERP_MODE_CONVOLUTION     0x01u /* "convolution"        */
+#define ERP_MODE_CONV_LINEAR     0x02u /* "convolution_linear" */
+#define ERP_MODE_SPECULAR        0x04u /* "specular"           */
+#define ERP_MODE_LOSSLESS        0x08u /* "lossless"           */
+#define ERP_MODE_INCOHERENT      0x10u /* "incoherent"         */
+#define ERP_MODE_DISABLE_SURFACE 0x20u /* "disable_surface"    */
+
+/* ---- run flags (erp_run flags argument) -------------------------------- */
+#define ERP_RUN_DETERMINISTIC 0x01u /* fixed-order reductions: bitwise repeatable */
+#define ERP_RUN_PHASE_F32     0x02u /* precision study: ranges and phase in FP32 */
+#define ERP_RUN_PHASE_DF      0x04u /* precision study: phase in double-float    */
+#define ERP_RUN_RAW_SURFACE   0x08u /* output the raw surface phasor trace before
+                                       range compression (the Psurf dump of
+                                       debug_surface, sim.cu:782-788)           */
+
+/*
+ * Simulation parameters after the derivations of parseSimulationParameters
+ * (file_io.cu:122-249).  Every float field holds exactly the float value the
+ * reference stores in SimulationParameters (file_io.cu:40-120); the host side
+ * computes them with the reference's FP32 operation order.  Per-shot values
+ * (source position, rx window start, gains) live in erp_shot.
+ */
+typedef struct erp_params {
+    float lam;          /* c / f0                          file_io.cu:170 */
+    float k;            /* 2 pi / lam                      file_io.cu:171 */
+    float rng_res;      /* c / B                           file_io.cu:168 */
+    float P;            /* transmit power                  file_io.cu:154 */
+    float fs;           /* facet size                      file_io.cu:228 */
+    float ks;           /* k * rms_height                  file_io.cu:179 */
+    float eps_1, eps_2; /*                                 file_io.cu:182-183 */
+    float c, c_1, c_2;  /* c, c/sqrt(eps)                  file_io.cu:42,187-188 */
+    float nu_1, nu_2;   /* 376.7/sqrt(eps)                 file_io.cu:189-190 */
+    float alpha1, alpha2;/* attenuation                    file_io.cu:196-206 */
+    float aperture;     /* degrees                         file_io.cu:155 */
+    float smpl;         /* rx sample rate                  file_io.cu:208 */
+    float B;            /* bandwidth                       file_io.cu:153 */
+    float f0;           /* centre frequency                file_io.cu:152 */
+    float dr;           /* rx_window_m / nr                file_io.cu:213 */
+    int32_t nr;         /* range bins                      file_io.cu:212 */
+    int32_t rerad_funct;/* target reradiation function     file_io.cu:232 */
+    int32_t pol;        /* always 0 (never parsed)         file_io.cu:70  */
+    uint32_t mode;      /* ERP_MODE_* bits */
+    double cos_ap;      /* cos of the float aperture threshold (pi/180)*aperture,
+                           in double; the FP64 aperture predicate compares
+                           against it (geometry_kernels.cu:95-96,183) */
+    uint64_t seed;      /* incoherent-mode RNG seed (replaces clock64(),
+                           radar_kernels.cu:689-711) */
+} erp_params;
+
+/* One shot (source position).  sx, sy, sz carry the reference's float source
+ * coordinates promoted to double (sim.cu:642-651); snx..snz already include
+ * source_normal_multiplier; rst and the gains are the per-shot values of
+ * sim.cu:624-633. */
+typedef struct erp_shot {
+    double sx, sy, sz;
+    float snx, sny, snz;
+    float rst;
+    float grefl_lin, grefr_lin;
+} erp_shot;
+
+/* Attenuation prism: alpha already converted from conductivity
+ * (file_io.cu:477-517). */
+typedef struct erp_prism {
+    float alpha, xmin, ymin, zmin, xmax, ymax, zmax;
+} erp_prism;
+
+typedef struct erp_ctx erp_ctx;
+typedef struct erp_mesh erp_mesh;
+typedef struct erp_targets erp_targets;
+typedef struct erp_job erp_job;
+
+/* Create a context on HIP device `device`.  Owns its stream, FFT twiddle
+ * tables and the chirp-spectrum cache. */
+int erp_ctx_create(int device, erp_ctx** out);
+void erp_ctx_destroy(erp_ctx* ctx);
+/* Last error message of this context (never NULL). */
+const char* erp_last_error(const erp_ctx* ctx);
+
+/* Upload a facet mesh: soa holds 12 consecutive float arrays of length F in
+ * the order x, y, z, nx, ny, nz, ux, uy, uz, vx, vy, vz (file_io.cu:349,
+ * sim.cu:228-239).  The mesh stays resident in HBM until released. */
+int erp_mesh_upload(erp_ctx* ctx, const float* soa, int64_t F, erp_mesh** out);
+void erp_mesh_release(erp_mesh* mesh);
+
+/* Upload T targets: xyzn holds 6 consecutive float arrays x, y, z, nx, ny, nz
+ * of length T; type holds T target types (file_io.cu:365-426). */
+int erp_targets_upload(erp_ctx* ctx, const float* xyzn, const uint8_t* type,
+                       int64_t T, erp_targets** out);
+void erp_targets_release(erp_targets* targets);
+
+/* Run ns shots (sim.cu:615-1008).  targets may be NULL (no targets).
+ * out: host buffer of ns*nr*2 floats (shot-major, interleaved re/im).
+ * nvalid (optional, host, ns entries): illuminated facet count per shot; a
+ * shot with 0 illuminated facets has an all-zero trace (the reference writes
+ * no file for it, sim.cu:693-697).  Synchronous. */
+int erp_run(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
+            const erp_targets* targets, const erp_shot* shots, int64_t ns,
+            const erp_prism* prisms, int64_t nprisms,
+            float* out, int32_t* nvalid, uint32_t flags);
+
+/* Same as erp_run with the output left in device memory: out_dev is a device
+ * pointer to ns*nr*2 floats, nvalid_dev an optional device pointer to ns
+ * int32.  Returns after the work is queued and complete on the context's
+ * stream (synchronous). */
+int erp_run_dev(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
+                const erp_targets* targets, const erp_shot* shots, int64_t ns,
+                const erp_prism* prisms, int64_t nprisms,
+                void* out_dev, void* nvalid_dev, uint32_t flags);
+
+/* Asynchronous form: queue a job on its own stream and return at once.  The
+ * host buffers `shots`, `prisms` are copied before return; `out`/`nvalid`
+ * must stay valid until erp_wait/erp_poll reports completion. */
+int erp_submit(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
+               const erp_targets* targets, const erp_shot* shots, int64_t ns,
+               const erp_prism* prisms, int64_t nprisms,
+               float* out, int32_t* nvalid, uint32_t flags, erp_job** job);
+/* ERP_OK when done (job resources released), ERP_E_NOTREADY while running. */
+int erp_poll(erp_job* job);
+/* Block until the job is done; releases it. */
+int erp_wait(erp_job* job);
+
+/* Device bytes a run of (F facets, T targets, ns shots, nr bins) needs beyond
+ * the resident mesh and targets; used for admission by footprint. */
+int64_t erp_run_footprint(const erp_params* par, int64_t F, int64_t T, int64_t ns);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ERP_H */
