@@ -21,7 +21,11 @@ _lock = threading.Lock()
 
 EXPORTS = ("erp_ctx_create", "erp_ctx_destroy", "erp_last_error", "erp_mesh_upload",
            "erp_mesh_release", "erp_targets_upload", "erp_targets_release", "erp_run",
-           "erp_run_dev", "erp_submit", "erp_poll", "erp_wait", "erp_run_footprint")
+           "erp_run_dev", "erp_submit", "erp_poll", "erp_wait", "erp_run_footprint",
+           "erp_profile_enable", "erp_profile_read")
+
+STAGES = ("mask", "surface", "surface_conv", "target_select", "target_accum", "target_fft",
+          "shot_final", "other")
 
 
 class ErpError(RuntimeError):
@@ -70,6 +74,10 @@ def lib() -> ctypes.CDLL:
         L.erp_wait.restype = i32
         L.erp_run_footprint.argtypes = [P(ErpParams), i64, i64, i64]
         L.erp_run_footprint.restype = i64
+        L.erp_profile_enable.argtypes = [vp, i32]
+        L.erp_profile_enable.restype = i32
+        L.erp_profile_read.argtypes = [vp, i32, P(ctypes.c_double), P(i64), P(ctypes.c_double)]
+        L.erp_profile_read.restype = i32
         _lib = L
         return L
 
@@ -223,6 +231,20 @@ class Context:
                                      _ptr(shots), ns, _ptr(prisms), npr, _ptr(out),
                                      _ptr(nvalid), flags, ctypes.byref(h)))
         return Job(self, h, out, nvalid, (shots, prisms, params))
+
+    def profile(self, on: bool = True):
+        """Enable (and clear) or disable the per-stage hipEvent timers."""
+        self._check(lib().erp_profile_enable(self.handle, 1 if on else 0))
+
+    def profile_read(self) -> dict:
+        """{stage: (ms, launches, units)} accumulated since profile(True)."""
+        out = {}
+        for k, name in enumerate(STAGES):
+            ms, n, u = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
+            self._check(lib().erp_profile_read(self.handle, k, ctypes.byref(ms),
+                                               ctypes.byref(n), ctypes.byref(u)))
+            out[name] = (ms.value, n.value, u.value)
+        return out
 
     @staticmethod
     def footprint(params: ErpParams, F: int, T: int, ns: int) -> int:

@@ -42,6 +42,54 @@ void validate_params(const erp_params* p) {
 
 }  // namespace
 
+hipEvent_t StageTimer::get() {
+    if (!pool.empty()) {
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    ERP_HIP_TRY(hipEventCreate(&e));
+    return e;
+}
+
+void StageTimer::begin(int stage, hipStream_t s, double u) {
+    if (!on) return;
+    Pending p{stage, get(), get(), u};
+    ERP_HIP_TRY(hipEventRecord(p.a, s));
+    pending.push_back(p);
+    cur = (int)pending.size() - 1;
+}
+
+void StageTimer::end(hipStream_t s) {
+    if (!on || cur < 0) return;
+    ERP_HIP_TRY(hipEventRecord(pending[cur].b, s));
+    cur = -1;
+}
+
+void StageTimer::resolve() {
+    for (Pending& p : pending) {
+        float ms_ = 0.f;
+        if (hipEventElapsedTime(&ms_, p.a, p.b) == hipSuccess) {
+            ms[p.stage] += ms_;
+            launches[p.stage] += 1;
+            units[p.stage] += p.units;
+        }
+        pool.push_back(p.a);
+        pool.push_back(p.b);
+    }
+    pending.clear();
+    cur = -1;
+}
+
+void StageTimer::reset() {
+    for (int k = 0; k < ERP_NSTAGES; ++k) {
+        ms[k] = 0;
+        launches[k] = 0;
+        units[k] = 0;
+    }
+}
+
 struct erp_job {
     erp_ctx* ctx;
     hipStream_t stream;
@@ -101,6 +149,13 @@ void erp_ctx_destroy(erp_ctx* ctx) {
                       &w.pairs, &w.t0trace, &w.shots, &w.prisms, &w.out, &w.nvalid};
     for (DevBuf* b : bufs) b->release();
     ctx->twiddle.release();
+    for (auto& t : ctx->inter) {
+        t.second->release();
+        delete t.second;
+    }
+    ctx->inter.clear();
+    ctx->prof.resolve();
+    for (hipEvent_t e : ctx->prof.pool) hipEventDestroy(e);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -207,6 +262,7 @@ int erp_run_dev(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         erp_run_impl(ctx, par, mesh, targets, shots, ns, prisms, nprisms, (float2*)out_dev,
                      (int32_t*)nvalid_dev, flags, ctx->stream, ctx->work);
         ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        ctx->prof.resolve();
     });
 }
 
@@ -240,6 +296,7 @@ int erp_run(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
             ERP_HIP_TRY(hipMemcpyAsync(nvalid, ctx->work.nvalid.p, sizeof(int32_t) * ns,
                                        hipMemcpyDeviceToHost, ctx->stream));
         ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        ctx->prof.resolve();
     });
 }
 
@@ -269,8 +326,17 @@ int erp_submit(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
             j->nvalid.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(ns, 1));
             // make sure context-level tables exist before the job's stream uses them
             ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
-            erp_run_impl(ctx, par, mesh, targets, shots, ns, prisms, nprisms,
-                         j->out.as<float2>(), j->nvalid.as<int32_t>(), flags, j->stream, j->work);
+            const bool prof_on = ctx->prof.on;
+            ctx->prof.on = false;  // stage timers cover synchronous runs only
+            try {
+                erp_run_impl(ctx, par, mesh, targets, shots, ns, prisms, nprisms,
+                             j->out.as<float2>(), j->nvalid.as<int32_t>(), flags, j->stream,
+                             j->work);
+            } catch (...) {
+                ctx->prof.on = prof_on;
+                throw;
+            }
+            ctx->prof.on = prof_on;
             j->host_out = out;
             j->host_nvalid = nvalid;
             j->n_out = (int64_t)nout;
@@ -325,6 +391,23 @@ int erp_wait(erp_job* j) {
     if (rc != ERP_OK) j->ctx->last_error = hipGetErrorString(e);
     job_free(j);
     return rc;
+}
+
+int erp_profile_enable(erp_ctx* ctx, int on) {
+    if (!ctx) return ERP_E_INVALID;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->prof.on = on != 0;
+    ctx->prof.reset();
+    return ERP_OK;
+}
+
+int erp_profile_read(erp_ctx* ctx, int stage, double* ms, int64_t* launches, double* units) {
+    if (!ctx || stage < 0 || stage >= ERP_NSTAGES) return ERP_E_INVALID;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ms) *ms = ctx->prof.ms[stage];
+    if (launches) *launches = ctx->prof.launches[stage];
+    if (units) *units = ctx->prof.units[stage];
+    return ERP_OK;
 }
 
 int64_t erp_run_footprint(const erp_params* par, int64_t F, int64_t T, int64_t ns) {

@@ -138,11 +138,34 @@ struct WorkBufs {
     DevBuf nvalid;     // S int32
 };
 
+// stage timers: events recorded on the run stream, resolved after it syncs
+struct StageTimer {
+    bool on = false;
+    double ms[ERP_NSTAGES] = {};
+    int64_t launches[ERP_NSTAGES] = {};
+    double units[ERP_NSTAGES] = {};
+    struct Pending {
+        int stage;
+        hipEvent_t a, b;
+        double units;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t get();
+    void begin(int stage, hipStream_t s, double units);
+    void end(hipStream_t s);
+    void resolve();  // after a stream sync
+    void reset();
+    int cur = -1;
+};
+
 struct erp_ctx {
     int device = 0;
+    StageTimer prof;
     hipStream_t stream = nullptr;
     std::string last_error;
-    DevBuf twiddle;  // ERP_TWN complex64 exp(-2 pi i k / TWN)
+    DevBuf twiddle;  // ladder table of the LDS FFT stages (8192 complex64)
+    std::vector<std::pair<int, DevBuf*>> inter;  // inter-pass twiddles per size
     std::vector<ChirpSpec*> chirps;
     std::vector<BlueTables*> blues;
     WorkBufs work;
@@ -160,6 +183,16 @@ void fft_forward(erp_ctx* ctx, const FftPlan& p, float2* data, int batch, hipStr
 // Inverse (unnormalised) of a permuted-order spectrum, in place, natural output.
 void fft_inverse(erp_ctx* ctx, const FftPlan& p, float2* data, int batch, hipStream_t s);
 void fft_init_twiddles(erp_ctx* ctx);
+// Fused per-target spectral chain (erp_fft.hip): for n pairs with windowed
+// inward traces in A (rows of P), accumulates scale * sum_p FFT(A_p) *
+// FFT(window(IFFT(FFT(A_p) G))) into S[shot] for each shot segment.  Groups
+// [gbeg, gend) partition the pairs without crossing shots; segment k sums
+// groups (two-pass sizes) or pairs (single-pass sizes) [gseg_beg, gseg_end).
+void target_spectra_fused(erp_ctx* ctx, const FftPlan& p, float2* A, float2* Y,
+                          const int* pbase, int win, int nr, int n, const float2* G,
+                          float2* work, int ngroups, const int* gbeg, const int* gend,
+                          const int* gseg_shot, const int* gseg_beg, const int* gseg_end,
+                          int nseg, float2* S, float scale, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // pipeline (erp_pipeline.hip)

@@ -48,6 +48,10 @@ constexpr int WPB = NT / 64;   // waves per block
 constexpr int MASKW = FB / 64; // mask words per block (32)
 constexpr int SWIN = 2048;     // surface LDS window (bins)
 constexpr int TWIN = 4096;     // target LDS window (bins)
+constexpr int NTT = 512;       // threads per workgroup of the target kernels
+// Bin windows accumulate in FP64: LDS atomic adds of doubles (ds_add_f64) run
+// at more than twice the rate of FP32 ones (ds_add_f32) on gfx950 and remove
+// FP32 accumulation error from the bins.
 
 // Kernel-side parameter block (derived once per run on the host).
 struct KPar {
@@ -62,7 +66,7 @@ struct KPar {
     float invK;        // 1/(4*3.14159)^2
     float sqrt_e1e2;   // sqrt(eps_1/eps_2)
     float sqrt_e2e1;   // sqrt(eps_2/eps_1)
-    double drd, cos_ap, n_d, inv_cyc, lam_d;
+    double drd, inv_dr, cos_ap, n_d, inv_cyc, lam_d;
     double B, smpl;
     int nr, P_fft, rerad;
     uint32_t mode;
@@ -143,6 +147,65 @@ __device__ __forceinline__ float cross_norm(float ax, float ay, float az, float 
 __device__ __forceinline__ float sinc_f(float x) {
     if (fabsf(x) < 1e-6f) return 1.0f;
     return sinpif(x) / (3.14159265358979f * x);
+}
+
+// sin(pi r) and cos(pi r) for |r| <= 0.5 from their Taylor series (degree 13
+// and 14); truncation error < 7e-10, FP32 evaluation error < 2e-7.
+__device__ __forceinline__ void sincospi_red(float r, float* s, float* c) {
+    const float r2 = r * r;
+    float ps = 0.00046630280576761234f;
+    ps = fmaf(ps, r2, -0.0073704309457143478f);
+    ps = fmaf(ps, r2, 0.082145886611128191f);
+    ps = fmaf(ps, r2, -0.59926452932079188f);
+    ps = fmaf(ps, r2, 2.5501640398773451f);
+    ps = fmaf(ps, r2, -5.1677127800499694f);
+    ps = fmaf(ps, r2, 3.1415926535897931f);
+    *s = r * ps;
+    float pc = -0.00010463810492484565f;
+    pc = fmaf(pc, r2, 0.0019295743094039221f);
+    pc = fmaf(pc, r2, -0.025806891390014051f);
+    pc = fmaf(pc, r2, 0.23533063035889312f);
+    pc = fmaf(pc, r2, -1.3352627688545893f);
+    pc = fmaf(pc, r2, 4.0587121264167676f);
+    pc = fmaf(pc, r2, -4.934802200544679f);
+    *c = fmaf(pc, r2, 1.0f);
+}
+
+// sin(pi r) only, |r| <= 0.5
+__device__ __forceinline__ float sinpi_red(float r) {
+    const float r2 = r * r;
+    float ps = 0.00046630280576761234f;
+    ps = fmaf(ps, r2, -0.0073704309457143478f);
+    ps = fmaf(ps, r2, 0.082145886611128191f);
+    ps = fmaf(ps, r2, -0.59926452932079188f);
+    ps = fmaf(ps, r2, 2.5501640398773451f);
+    ps = fmaf(ps, r2, -5.1677127800499694f);
+    ps = fmaf(ps, r2, 3.1415926535897931f);
+    return r * ps;
+}
+
+// sinc(x) of elementary_kernels.cu:50-53 as numerator / denominator so that a
+// product of sincs needs one reciprocal: sinc = num / den.
+__device__ __forceinline__ void sinc_nd(float x, float* num, float* den) {
+    const bool tiny = fabsf(x) < 1e-6f;
+    const float n = rintf(x);
+    float s = sinpi_red(x - n);
+    s = ((int)n & 1) ? -s : s;
+    *num = tiny ? 1.0f : s;
+    *den = tiny ? 1.0f : 3.14159265358979f * x;
+}
+
+// exp(-i pi t) for FP64 t: exact reduction in double, FP32 polynomial
+__device__ __forceinline__ float2 phasor_half_cycles(double t) {
+    const double n = rint(t);
+    const float r = (float)(t - n);
+    float s, c;
+    sincospi_red(r, &s, &c);
+    if (((long long)n) & 1) {
+        s = -s;
+        c = -c;
+    }
+    return make_float2(c, -s);
 }
 
 // splitmix64 hash of (seed, shot, facet): the incoherent phase u in [0,1)
@@ -284,9 +347,9 @@ struct Rec {
     int64_t cap;
 };
 
-__device__ __forceinline__ void lds_add2(float2* win, int idx, float2 v) {
-    atomicAdd(&win[idx].x, v.x);
-    atomicAdd(&win[idx].y, v.y);
+__device__ __forceinline__ void lds_add2(double2* win, int idx, float2 v) {
+    atomicAdd(&win[idx].x, (double)v.x);
+    atomicAdd(&win[idx].y, (double)v.y);
 }
 __device__ __forceinline__ void glb_add2(float2* row, int idx, float2 v) {
     atomicAdd(&row[idx].x, v.x);
@@ -303,7 +366,7 @@ __global__ void __launch_bounds__(NT) k_surface(
     const unsigned long long* mw = mask + ((size_t)s * nblk + blk) * MASKW;
     __shared__ unsigned long long m_s[MASKW];
     __shared__ int pre_s[MASKW + 1];
-    __shared__ float2 win[SWIN];
+    __shared__ double2 win[SWIN];
     __shared__ int wmin_s;
     __shared__ float red_f[6][WPB];
     __shared__ double red_d[2][WPB];
@@ -469,7 +532,7 @@ __global__ void __launch_bounds__(NT) k_surface(
         red_f[3][wv] = lxmax; red_f[4][wv] = lymax; red_f[5][wv] = lzmax;
         red_d[0][wv] = litmin; red_d[1][wv] = litmax;
     }
-    for (int k = threadIdx.x; k < SWIN; k += NT) win[k] = make_float2(0.f, 0.f);
+    for (int k = threadIdx.x; k < SWIN; k += NT) win[k] = make_double2(0.0, 0.0);
     __syncthreads();
     if (threadIdx.x == 0 && bounds) {
         float a[6];
@@ -509,9 +572,9 @@ __global__ void __launch_bounds__(NT) k_surface(
     }
     __syncthreads();
     for (int k = threadIdx.x; k < SWIN; k += NT) {
-        const float2 v = win[k];
+        const double2 v = win[k];
         const int b = base + k;
-        if (b < nr && (v.x != 0.f || v.y != 0.f)) glb_add2(row, b, v);
+        if (b < nr && (v.x != 0.0 || v.y != 0.0)) glb_add2(row, b, make_float2((float)v.x, (float)v.y));
     }
 }
 
@@ -528,7 +591,7 @@ __global__ void k_mul_spec(float2* __restrict__ dst, const float2* __restrict__ 
     }
 }
 
-// out[r][j] = scale * y[r][j + nr - 1] for j < nr (window of the exact linear
+// out[r][j] = scale * y[r][j + nr] for j < nr (window of the exact linear
 // convolution); if pad, also zero out[r][nr .. P).
 __global__ void k_window(float2* __restrict__ out, int ostride, const float2* __restrict__ y,
                          int P, int nr, int rows, float scale, int pad) {
@@ -539,7 +602,7 @@ __global__ void k_window(float2* __restrict__ out, int ostride, const float2* __
         const int r = (int)(i / lim), j = (int)(i - (size_t)r * lim);
         float2 v = make_float2(0.f, 0.f);
         if (j < nr) {
-            v = y[(size_t)r * P + j + nr - 1];
+            v = y[(size_t)r * P + j + nr];
             v.x *= scale;
             v.y *= scale;
         }
@@ -561,17 +624,21 @@ __global__ void k_copy_rows(float2* __restrict__ out, int ostride, const float2*
 // ---------------------------------------------------------------------------
 // 5. chirp tables
 // ---------------------------------------------------------------------------
-// Rearranged kernel g[e], e < 2nr-1, of the reference convolutions:
+// Rearranged kernel g of the reference convolutions, placed at e = 1 .. 2nr-1
+// (e' = e - 1) so that the exact linear convolution's range window starts at
+// index nr (a row boundary of the two-pass FFT layout):
 //  kind 0 (convolvePhasorChirpLinear with genCenteredChirpPadded):
-//     g[e] = h[(e + nr/2 - nr + 1) mod 2nr],  h[i] = sinc((i - nr/2) dr / res)
+//     g[e] = h[(e' + nr/2 - nr + 1) mod 2nr],  h[i] = sinc((i - nr/2) dr / res)
 //  kind 1 (convolvePhasorChirp with genChirp):
-//     g[e] = h[(e + 1 - nr) mod nr],  h[i] = sinc((r-rst)/res) + sinc((r-rst-nr dr)/res)
+//     g[e] = h[(e' + 1 - nr) mod nr],  h[i] = sinc((r-rst)/res) + sinc((r-rst-nr dr)/res)
+// out[j] = sum_m x[m] g[j + nr - m] for j < nr: no wrap-around for P >= 2nr.
 __global__ void k_gen_kernel(float2* __restrict__ g, int P, int nr, double dr, double res,
                              double rst, int kind) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= P) return;
+    const int e0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e0 >= P) return;
     float v = 0.f;
-    if (e < 2 * nr - 1) {
+    const int e = e0 - 1;
+    if (e >= 0 && e < 2 * nr - 1) {
         if (kind == 0) {
             const int kc = nr / 2;
             const int m = (int)(((long long)e + kc - nr + 1 + 2LL * nr) % (2LL * nr));
@@ -587,7 +654,7 @@ __global__ void k_gen_kernel(float2* __restrict__ g, int P, int nr, double dr, d
             v = (float)(a + b);
         }
     }
-    g[e] = make_float2(v, 0.f);
+    g[e0] = make_float2(v, 0.f);
 }
 
 __global__ void k_scale(float2* __restrict__ p, int n, float s) {
@@ -742,15 +809,16 @@ __device__ __forceinline__ float rerad_f(int fn, float th) {
 // MODE 2: type == 0 (genRefrPhasor): single bin, summed into one per-shot
 //         trace (out row = per shot, global atomics after the LDS window).
 template <int MODE>
-__global__ void __launch_bounds__(NT) k_target_accum(
+__global__ void __launch_bounds__(NTT) k_target_accum(
     const float* __restrict__ tg, int64_t Tpad, const int* __restrict__ pshot,
     const int* __restrict__ ptgt, const erp_shot* __restrict__ shots,
     const int* __restrict__ Vs, const int64_t* __restrict__ rec_base, Rec rec,
     const Bounds* __restrict__ bounds, const erp_prism* __restrict__ prisms, int np, KPar kp,
-    float2* __restrict__ outA, float2* __restrict__ outB) {
-    extern __shared__ float2 lds[];
-    float2* winA = lds;
-    float2* winB = lds + TWIN;  // MODE 1 only
+    float2* __restrict__ outA, float2* __restrict__ outB, int* __restrict__ pbase) {
+    extern __shared__ double2 lds[];
+    constexpr int WIN = (MODE == 1) ? TWIN / 2 : TWIN;  // MODE 1 holds two windows
+    double2* winA = lds;
+    double2* winB = lds + WIN;  // MODE 1 only
     const int pi = blockIdx.x;
     const int s = pshot[pi], t = ptgt[pi];
     const ShotK sh = load_shot(shots, s);
@@ -775,26 +843,40 @@ __global__ void __launch_bounds__(NT) k_target_accum(
     else rlo = itmin + kp.n_d * tmin;
     int base = (int)floor((rlo - rst) / kp.drd) - 2;
     base = max(base, 0);
+    // upper bound: Itd_max + n * (distance to the farthest AABB corner)
+    bool fits = false;
+    if (MODE == 0 && pbase) {
+        const float fx = fmaxf(fabsf(o2f(bd.xmin) - txf), fabsf(o2f(bd.xmax) - txf));
+        const float fy = fmaxf(fabsf(o2f(bd.ymin) - tyf), fabsf(o2f(bd.ymax) - tyf));
+        const float fz = fmaxf(fabsf(o2f(bd.zmin) - tzf), fabsf(o2f(bd.zmax) - tzf));
+        const double tmax = sqrt((double)fx * fx + (double)fy * fy + (double)fz * fz);
+        const double ithmax = __longlong_as_double((long long)bd.itd_max);
+        const double rhi = ithmax + kp.n_d * tmax * (1.0 + 1e-6) + 1.0;
+        const int top = (int)ceil((rhi - rst) / kp.drd) + 2;
+        fits = (top < base + WIN - 1) || (base + WIN >= nr);
+        if (threadIdx.x == 0) pbase[pi] = fits ? base : -1;
+    }
 
     float2* rowA = (MODE == 2) ? outA + (size_t)s * P : outA + (size_t)pi * P;
     float2* rowB = (MODE == 1) ? outB + (size_t)pi * P : nullptr;
-    // zero the output rows outside the window (MODE 2 rows are pre-zeroed)
-    if (MODE != 2) {
-        for (int k = threadIdx.x; k < P; k += NT) {
-            if (k < base || k >= base + TWIN || k >= nr) {
+    // zero the output rows outside the window (MODE 2 rows are pre-zeroed);
+    // unnecessary when every contribution provably lands inside the window
+    if (MODE != 2 && !fits) {
+        for (int k = threadIdx.x; k < P; k += NTT) {
+            if (k < base || k >= base + WIN || k >= nr) {
                 rowA[k] = make_float2(0.f, 0.f);
                 if (MODE == 1) rowB[k] = make_float2(0.f, 0.f);
             }
         }
     }
-    for (int k = threadIdx.x; k < TWIN; k += NT) {
-        winA[k] = make_float2(0.f, 0.f);
-        if (MODE == 1) winB[k] = make_float2(0.f, 0.f);
+    for (int k = threadIdx.x; k < WIN; k += NTT) {
+        winA[k] = make_double2(0.0, 0.0);
+        if (MODE == 1) winB[k] = make_double2(0.0, 0.0);
     }
     __threadfence();
     __syncthreads();
 
-    for (int k = threadIdx.x; k < V; k += NT) {
+    for (int k = threadIdx.x; k < V; k += NTT) {
         const int64_t r = r0 + k;
         const float fx = rec.geo[r], fy = rec.geo[rec.cap + r], fz = rec.geo[2 * rec.cap + r];
         const float nx = rec.geo[3 * rec.cap + r], ny = rec.geo[4 * rec.cap + r],
@@ -872,7 +954,7 @@ __global__ void __launch_bounds__(NT) k_target_accum(
             const float2 ph = phasor_cycles(-2.0 * rngt * kp.inv_cyc);
             const float2 c = make_float2(ph.x * rc, ph.y * rc);
             const int k0 = b - base;
-            if (k0 >= 0 && k0 < TWIN) lds_add2(winA, k0, c); else glb_add2(rowA, b, c);
+            if (k0 >= 0 && k0 < WIN) lds_add2(winA, k0, c); else glb_add2(rowA, b, c);
             continue;
         }
         // accumulateTarget / radiateTarget amplitude:
@@ -916,9 +998,9 @@ __global__ void __launch_bounds__(NT) k_target_accum(
         const int k0 = b - base;
         const float2 c0 = make_float2(c.x * (1.f - w), c.y * (1.f - w));
         const float2 c1 = make_float2(c.x * w, c.y * w);
-        if (k0 >= 0 && k0 < TWIN) lds_add2(winA, k0, c0); else glb_add2(rowA, b, c0);
+        if (k0 >= 0 && k0 < WIN) lds_add2(winA, k0, c0); else glb_add2(rowA, b, c0);
         if (b + 1 < nr) {
-            if (k0 + 1 >= 0 && k0 + 1 < TWIN) lds_add2(winA, k0 + 1, c1);
+            if (k0 + 1 >= 0 && k0 + 1 < WIN) lds_add2(winA, k0 + 1, c1);
             else glb_add2(rowA, b + 1, c1);
         }
         if (MODE == 1) {
@@ -927,24 +1009,159 @@ __global__ void __launch_bounds__(NT) k_target_accum(
             const float2 d = make_float2(pb.x * ampB, pb.y * ampB);
             const float2 d0 = make_float2(d.x * (1.f - w), d.y * (1.f - w));
             const float2 d1 = make_float2(d.x * w, d.y * w);
-            if (k0 >= 0 && k0 < TWIN) lds_add2(winB, k0, d0); else glb_add2(rowB, b, d0);
+            if (k0 >= 0 && k0 < WIN) lds_add2(winB, k0, d0); else glb_add2(rowB, b, d0);
             if (b + 1 < nr) {
-                if (k0 + 1 >= 0 && k0 + 1 < TWIN) lds_add2(winB, k0 + 1, d1);
+                if (k0 + 1 >= 0 && k0 + 1 < WIN) lds_add2(winB, k0 + 1, d1);
                 else glb_add2(rowB, b + 1, d1);
             }
         }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < TWIN; k += NT) {
+    for (int k = threadIdx.x; k < WIN; k += NTT) {
         const int b = base + k;
         if (b >= nr) break;
         if (MODE == 2) {
-            const float2 v = winA[k];
-            if (v.x != 0.f || v.y != 0.f) glb_add2(rowA, b, v);
+            const double2 v = winA[k];
+            if (v.x != 0.0 || v.y != 0.0) glb_add2(rowA, b, make_float2((float)v.x, (float)v.y));
         } else {
-            rowA[b] = winA[k];
-            if (MODE == 1) rowB[b] = winB[k];
+            rowA[b] = make_float2((float)winA[k].x, (float)winA[k].y);
+            if (MODE == 1) rowB[b] = make_float2((float)winB[k].x, (float)winB[k].y);
         }
+    }
+}
+
+// Plain-mode (type != 0, non-specular) facet x target kernel with
+// instruction-lean arithmetic: Newton-refined FP64 range from an FP32 rsqrt
+// seed, one reciprocal for the product of the four sinc factors of
+// facet_G(Tth,Tph) * facet_G(TargetTh,Tph), Taylor-series sin/cos.  Bins are
+// decided on the approximate range unless it lies within 1e-7 bin of a bin
+// edge, where the exact FP64 sqrt and division of the oracle are repeated, so
+// bin indices stay identical to the FP64 reference.
+__global__ void __launch_bounds__(NTT) k_target_accum_fast(
+    const float* __restrict__ tg, int64_t Tpad, const int* __restrict__ pshot,
+    const int* __restrict__ ptgt, const erp_shot* __restrict__ shots,
+    const int* __restrict__ Vs, const int64_t* __restrict__ rec_base, Rec rec,
+    const Bounds* __restrict__ bounds, const erp_prism* __restrict__ prisms, int np, KPar kp,
+    float2* __restrict__ outA, int* __restrict__ pbase) {
+    extern __shared__ double2 winA[];
+    const int pi = blockIdx.x;
+    const int s = pshot[pi], t = ptgt[pi];
+    const ShotK sh = load_shot(shots, s);
+    const int V = Vs[s];
+    const int64_t r0 = rec_base[s];
+    const float txf = tg[t], tyf = tg[Tpad + t], tzf = tg[2 * Tpad + t];
+    const float tnx = tg[3 * Tpad + t], tny = tg[4 * Tpad + t], tnz = tg[5 * Tpad + t];
+    const double tx = txf, ty = tyf, tz = tzf;
+    const double rst = (double)sh.rst;
+    const int nr = kp.nr, P = kp.P_fft;
+    const bool lossless = kp.mode & ERP_MODE_LOSSLESS;
+
+    // window from the shot bounds (see k_target_accum)
+    const Bounds bd = bounds[s];
+    const float bx = fmaxf(fmaxf(o2f(bd.xmin) - txf, txf - o2f(bd.xmax)), 0.f);
+    const float by = fmaxf(fmaxf(o2f(bd.ymin) - tyf, tyf - o2f(bd.ymax)), 0.f);
+    const float bz = fmaxf(fmaxf(o2f(bd.zmin) - tzf, tzf - o2f(bd.zmax)), 0.f);
+    const double tmin = sqrt((double)bx * bx + (double)by * by + (double)bz * bz);
+    const double itmin = __longlong_as_double((long long)bd.itd_min);
+    int base = (int)floor((itmin + kp.n_d * tmin - rst) / kp.drd) - 2;
+    base = max(base, 0);
+    bool fits = false;
+    if (pbase) {
+        const float fx = fmaxf(fabsf(o2f(bd.xmin) - txf), fabsf(o2f(bd.xmax) - txf));
+        const float fy = fmaxf(fabsf(o2f(bd.ymin) - tyf), fabsf(o2f(bd.ymax) - tyf));
+        const float fz = fmaxf(fabsf(o2f(bd.zmin) - tzf), fabsf(o2f(bd.zmax) - tzf));
+        const double tmax = sqrt((double)fx * fx + (double)fy * fy + (double)fz * fz);
+        const double ithmax = __longlong_as_double((long long)bd.itd_max);
+        const double rhi = ithmax + kp.n_d * tmax * (1.0 + 1e-6) + 1.0;
+        const int top = (int)ceil((rhi - rst) / kp.drd) + 2;
+        fits = (top < base + TWIN - 1) || (base + TWIN >= nr);
+        if (threadIdx.x == 0) pbase[pi] = fits ? base : -1;
+    }
+    float2* rowA = outA + (size_t)pi * P;
+    if (!fits) {
+        for (int k = threadIdx.x; k < P; k += NTT)
+            if (k < base || k >= base + TWIN || k >= nr) rowA[k] = make_float2(0.f, 0.f);
+    }
+    for (int k = threadIdx.x; k < TWIN; k += NTT) winA[k] = make_double2(0.0, 0.0);
+    __threadfence();
+    __syncthreads();
+
+    const double inv_dr = kp.inv_dr, n_d = kp.n_d, two_cyc = 2.0 * kp.inv_cyc;
+    const float a = kp.piLlam;
+    const float gscale = kp.Dmax;
+    const int64_t cap = rec.cap;
+    for (int k = threadIdx.x; k < V; k += NTT) {
+        const int64_t r = r0 + k;
+        const float fx = rec.geo[r], fy = rec.geo[cap + r], fz = rec.geo[2 * cap + r];
+        const double Itd = rec.itd[r];
+        const double dx = tx - (double)fx, dy = ty - (double)fy, dz = tz - (double)fz;
+        const double t2 = fma(dx, dx, fma(dy, dy, dz * dz));
+        const float rs = rsqrtf((float)t2);
+        const double y0 = t2 * (double)rs;
+        const double Ttd = fma(fma(-y0, y0, t2), 0.5 * (double)rs, y0);
+        double rngt = fma(Ttd, n_d, Itd);
+        const double xq = (rngt - rst) * inv_dr;
+        int b;
+        float w;
+        bool valid;
+        if (fabs(xq - rint(xq)) < 1e-7) {
+            // within 1e-7 bin of an edge: the oracle's exact operation sequence
+            const double Te = dist3(dx, dy, dz);
+            rngt = madd_nc(Itd, Te, n_d);
+            valid = bin_of(rngt, rst, kp.drd, nr, &b, &w);
+        } else {
+            valid = (xq > -1.0) && (xq < (double)nr);
+            b = (int)xq;
+            w = (float)(xq - (double)b);
+        }
+        if (!valid) continue;
+        const float nx = rec.geo[3 * cap + r], ny = rec.geo[4 * cap + r], nz = rec.geo[5 * cap + r];
+        const float ux = rec.geo[6 * cap + r], uy = rec.geo[7 * cap + r], uz = rec.geo[8 * cap + r];
+        const float vx = rec.geo[9 * cap + r], vy = rec.geo[10 * cap + r], vz = rec.geo[11 * cap + r];
+        const float Ix = (float)dx * rs, Iy = (float)dy * rs, Iz = (float)dz * rs;
+        const float cT = fmaxf(-1.f, fminf(1.f, -(Ix * nx + Iy * ny + Iz * nz)));
+        const float Iu = Ix * ux + Iy * uy + Iz * uz;
+        const float Iv = Ix * vx + Iy * vy + Iz * vz;
+        const float sTT = cross_norm(Ix, Iy, Iz, tnx, tny, tnz);
+        const float q2 = Iu * Iu + Iv * Iv;
+        const float rinv = (q2 > 0.f) ? rsqrtf(q2) : 0.f;
+        const float cph = (q2 > 0.f) ? Iv * rinv : 1.f;
+        const float sph = Iu * rinv;
+        float n1, d1, n2, d2, n3, d3, n4, d4;
+        sinc_nd(a * Iv, &n1, &d1);
+        sinc_nd(a * Iu, &n2, &d2);
+        sinc_nd(a * sTT * cph, &n3, &d3);
+        sinc_nd(a * sTT * sph, &n4, &d4);
+        const float den = fabsf((d1 * d2) * (d3 * d4));
+        float amp = rec.F[r] * gscale * fabsf((n1 * n2) * (n3 * n4)) *
+                    __builtin_amdgcn_rcpf(den) * rs;
+        if (!lossless) {
+            float expo = 0.f, tot = 0.f;
+            for (int q = 0; q < np; ++q) {
+                const float d = prism_len(fx, fy, fz, Ix, Iy, Iz, prisms[q], (float)Ttd);
+                tot += d;
+                expo += prisms[q].alpha * d;
+            }
+            expo += kp.alpha2 * ((float)Ttd - tot);
+            const float kc = kp.ks * cT;
+            amp *= __expf(-2.f * (expo + kc * kc));
+        }
+        const float2 ph = phasor_half_cycles(rngt * two_cyc);
+        const float2 c = make_float2(ph.x * amp, ph.y * amp);
+        const float2 c0 = make_float2(c.x * (1.f - w), c.y * (1.f - w));
+        const float2 c1 = make_float2(c.x * w, c.y * w);
+        const int k0 = b - base;
+        if (k0 >= 0 && k0 < TWIN) lds_add2(winA, k0, c0); else glb_add2(rowA, b, c0);
+        if (b + 1 < nr) {
+            if (k0 + 1 >= 0 && k0 + 1 < TWIN) lds_add2(winA, k0 + 1, c1);
+            else glb_add2(rowA, b + 1, c1);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < TWIN; k += NTT) {
+        const int b = base + k;
+        if (b >= nr) break;
+        rowA[b] = make_float2((float)winA[k].x, (float)winA[k].y);
     }
 }
 
@@ -1194,6 +1411,7 @@ KPar make_kpar(const erp_params& p, int P_fft) {
     k.sqrt_e1e2 = sqrtf(p.eps_1 / p.eps_2);
     k.sqrt_e2e1 = sqrtf(p.eps_2 / p.eps_1);
     k.drd = (double)p.dr;
+    k.inv_dr = 1.0 / (double)p.dr;
     k.cos_ap = p.cos_ap;
     k.n_d = std::sqrt((double)p.eps_2);
     // phasor(r) = exp(-i (2 PI_F / lam) r) = exp(-i 2 pi r * inv_cyc)
@@ -1402,13 +1620,16 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                                    hipMemcpyHostToDevice, st));
         const erp_shot* dshots = W.shots.as<erp_shot>();
 
+        StageTimer& PT_ = ctx->prof;
         // 1-2: aperture mask, counts, offsets
+        PT_.begin(ERP_STAGE_MASK, st, (double)F * S);
         hipLaunchKernelGGL(k_illum_mask, dim3(nblk, S), dim3(NT), 0, st, mesh->soa.as<float>(),
                            mesh->Fpad, F, dshots, p.cos_ap, dmask, W.cnt.as<int>(), nblk);
         ERP_CHECK_LAUNCH("k_illum_mask");
         hipLaunchKernelGGL(k_scan_blocks, dim3(S), dim3(1024), 0, st, W.cnt.as<int>(),
                            W.off.as<int>(), dV, nblk);
         ERP_CHECK_LAUNCH("k_scan_blocks");
+        PT_.end(st);
         ERP_HIP_TRY(hipMemcpyAsync(hV.data(), dV, sizeof(int) * S, hipMemcpyDeviceToHost, st));
         ERP_HIP_TRY(hipStreamSynchronize(st));
         hbase[0] = 0;
@@ -1435,11 +1656,15 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         zero(W.psurf.as<float2>(), (size_t)S * P, st);
         hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, st, dbounds, S);
         ERP_CHECK_LAUNCH("k_init_bounds");
+        double Vsum = 0;
+        for (int s = 0; s < S; ++s) Vsum += hV[s];
+        PT_.begin(ERP_STAGE_SURFACE, st, Vsum);
         hipLaunchKernelGGL(k_surface, dim3(nblk, S), dim3(NT), 0, st, mesh->soa.as<float>(),
                            mesh->Fpad, F, dshots, dmask, W.off.as<int>(), dV, dbase, nblk, kp, rec,
                            (need_rec && Vtot > 0) ? 1 : 0, do_trace ? 1 : 0, W.psurf.as<float2>(),
                            s0, dbounds);
         ERP_CHECK_LAUNCH("k_surface");
+        PT_.end(st);
 
         if (flags & ERP_RUN_RAW_SURFACE) {
             // debug_surface: the raw surface phasor trace, no range compression
@@ -1456,16 +1681,20 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         float2* refl = W.refl.as<float2>();
         bool have_refl = false;
         if (!spec || (conv && lin)) {
+            PT_.begin(ERP_STAGE_SURFACE_CONV, st, (double)S);
             if (mode & ERP_MODE_DISABLE_SURFACE && !spec) {
                 zero(refl, (size_t)S * nr, st);
             } else {
                 window_conv(ctx, planP, W.psurf.as<float2>(), S, g_surf, refl, nr, st);
             }
+            PT_.end(st);
             have_refl = true;
         } else if (spec && !conv) {
+            PT_.begin(ERP_STAGE_OTHER, st, 0.0);
             hipLaunchKernelGGL(k_refl_direct, dim3((nr + NT - 1) / NT, S), dim3(NT), 0, st, refl,
                                dshots, dV, dbase, rec, kp);
             ERP_CHECK_LAUNCH("k_refl_direct");
+            PT_.end(st);
             have_refl = true;
         }
 
@@ -1475,11 +1704,13 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
             const int Tcap = (int)T;
             int* list1 = W.tlist.as<int>();
             int* list0 = list1 + (size_t)S * Tcap;
+            PT_.begin(ERP_STAGE_TARGET_SELECT, st, (double)T * S);
             hipLaunchKernelGGL(k_target_select, dim3(S), dim3(1024), 0, st,
                                targets->xyzn.as<float>(), Tpad, (int)T,
                                targets->type.as<unsigned char>(), dshots, dV, p.cos_ap, list1,
                                list0, W.tcnt.as<int>(), Tcap);
             ERP_CHECK_LAUNCH("k_target_select");
+            PT_.end(st);
             ERP_HIP_TRY(hipMemcpyAsync(htc.data(), W.tcnt.p, sizeof(int) * 2 * S,
                                        hipMemcpyDeviceToHost, st));
             ERP_HIP_TRY(hipStreamSynchronize(st));
@@ -1491,25 +1722,31 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
             if (npairs > 0) {
                 float2* Sacc = W.shotS.as<float2>();
                 zero(Sacc, (size_t)S * P, st);
-                // chunk size: bounded by 3 (or 4) P-rows per pair
-                const int nbuf = spec ? 4 : 3;
-                int C = (int)std::max<int64_t>(1, ((int64_t)768 << 20) / ((int64_t)P * 8 * nbuf));
-                C = std::min<int>(C, 4096);
+                // plain mode with a suitable size: fused LDS spectral chain
+                const bool fused = !spec && (!planP.two_pass || nr % planP.N1 == 0);
+                const int GS = 16;  // pairs per multiply-accumulate group
+                const int nbuf = fused ? 3 : (spec ? 4 : 3);
+                int C = (int)std::max<int64_t>(1, ((int64_t)1536 << 20) / ((int64_t)P * 8 * nbuf));
+                C = std::min<int>(C, 8192);
                 W.tA.ensure(sizeof(float2) * (size_t)C * P * (nbuf - 1));
                 W.tC.ensure(sizeof(float2) * (size_t)C * P);
                 float2* A = W.tA.as<float2>();
-                float2* Bq = A + (size_t)C * P;                 // y (and MODE 1: B)
+                float2* Bq = A + (size_t)C * P;                 // y / Y' (and MODE 1: B)
                 float2* Bs = spec ? Bq + (size_t)C * P : nullptr;
-                float2* Z = W.tC.as<float2>();
+                float2* Z = W.tC.as<float2>();                  // Z or fused workspace
                 std::vector<int> pshot, ptgt;
                 std::vector<int> hl1((size_t)S * Tcap);
                 ERP_HIP_TRY(hipMemcpyAsync(hl1.data(), list1, sizeof(int) * (size_t)S * Tcap,
                                            hipMemcpyDeviceToHost, st));
                 ERP_HIP_TRY(hipStreamSynchronize(st));
-                W.pairs.ensure(sizeof(int) * (size_t)C * 2 + sizeof(int) * 3 * (size_t)S);
+                const size_t gcap = (size_t)C / GS + S + 2;
+                W.pairs.ensure(sizeof(int) * ((size_t)C * 3 + 2 * gcap + 3 * (size_t)S + 16));
                 int* dps = W.pairs.as<int>();
                 int* dpt = dps + C;
-                int* dseg = dpt + C;
+                int* dpb = dpt + C;                  // per-pair window base (fused)
+                int* dgb = dpb + C;                  // group begin / end
+                int* dge = dgb + gcap;
+                int* dseg = dge + gcap;              // segment shot / begin / end
                 // iterate pairs in (shot, target) order
                 int cs = 0, cj = 0;
                 while (cs < S) {
@@ -1523,8 +1760,8 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                     }
                     const int n = (int)pshot.size();
                     if (n == 0) break;
-                    // segments (contiguous pairs of one shot)
-                    std::vector<int> sg_s, sg_b, sg_e;
+                    // segments (contiguous pairs of one shot) and MAC groups
+                    std::vector<int> sg_s, sg_b, sg_e, gb, ge, gsb, gse;
                     for (int i = 0; i < n; ++i) {
                         if (i == 0 || pshot[i] != pshot[i - 1]) {
                             sg_s.push_back(pshot[i]);
@@ -1535,26 +1772,56 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                         }
                     }
                     const int nseg = (int)sg_s.size();
-                    std::vector<int> up(2 * (size_t)C + 3 * (size_t)S, 0);
+                    for (int k = 0; k < nseg; ++k) {
+                        gsb.push_back((int)gb.size());
+                        for (int i = sg_b[k]; i < sg_e[k]; i += GS) {
+                            gb.push_back(i);
+                            ge.push_back(std::min(i + GS, sg_e[k]));
+                        }
+                        gse.push_back((int)gb.size());
+                    }
+                    const int ngroups = (int)gb.size();
+                    std::vector<int> up((size_t)C * 3 + 2 * gcap + 3 * (size_t)S, 0);
                     std::copy(pshot.begin(), pshot.end(), up.begin());
                     std::copy(ptgt.begin(), ptgt.end(), up.begin() + C);
-                    std::copy(sg_s.begin(), sg_s.end(), up.begin() + 2 * C);
-                    std::copy(sg_b.begin(), sg_b.end(), up.begin() + 2 * C + S);
-                    std::copy(sg_e.begin(), sg_e.end(), up.begin() + 2 * C + 2 * S);
+                    std::copy(gb.begin(), gb.end(), up.begin() + 3 * C);
+                    std::copy(ge.begin(), ge.end(), up.begin() + 3 * C + gcap);
+                    const size_t so = 3 * (size_t)C + 2 * gcap;
+                    std::copy(sg_s.begin(), sg_s.end(), up.begin() + so);
+                    const bool two = planP.two_pass;
+                    const std::vector<int>& segb = (fused && two) ? gsb : sg_b;
+                    const std::vector<int>& sege = (fused && two) ? gse : sg_e;
+                    std::copy(segb.begin(), segb.end(), up.begin() + so + S);
+                    std::copy(sege.begin(), sege.end(), up.begin() + so + 2 * S);
                     ERP_HIP_TRY(hipMemcpyAsync(dps, up.data(), sizeof(int) * up.size(),
                                                hipMemcpyHostToDevice, st));
+                    double triples = 0;
+                    for (int i = 0; i < n; ++i) triples += hV[pshot[i]];
+                    PT_.begin(ERP_STAGE_TARGET_ACCUM, st, triples);
                     if (!spec) {
-                        hipLaunchKernelGGL(k_target_accum<0>, dim3(n), dim3(NT),
-                                           sizeof(float2) * TWIN, st, targets->xyzn.as<float>(),
+                        hipLaunchKernelGGL(k_target_accum_fast, dim3(n), dim3(NTT),
+                                           sizeof(double2) * TWIN, st, targets->xyzn.as<float>(),
                                            Tpad, dps, dpt, dshots, dV, dbase, rec, dbounds, dpr,
-                                           (int)nprisms, kp, A, nullptr);
-                        ERP_CHECK_LAUNCH("k_target_accum<0>");
+                                           (int)nprisms, kp, A, fused ? dpb : nullptr);
+                        ERP_CHECK_LAUNCH("k_target_accum_fast");
                     } else {
-                        hipLaunchKernelGGL(k_target_accum<1>, dim3(n), dim3(NT),
-                                           sizeof(float2) * TWIN * 2, st,
+                        hipLaunchKernelGGL(k_target_accum<1>, dim3(n), dim3(NTT),
+                                           sizeof(double2) * TWIN, st,
                                            targets->xyzn.as<float>(), Tpad, dps, dpt, dshots, dV,
-                                           dbase, rec, dbounds, dpr, (int)nprisms, kp, A, Bs);
+                                           dbase, rec, dbounds, dpr, (int)nprisms, kp, A, Bs,
+                                           nullptr);
                         ERP_CHECK_LAUNCH("k_target_accum<1>");
+                    }
+                    PT_.end(st);
+                    PT_.begin(ERP_STAGE_TARGET_FFT, st, (double)n);
+                    if (fused) {
+                        // outward trace = -A/sqrt(P): scale -1/sqrt(P)
+                        target_spectra_fused(ctx, planP, A, Bq, dpb, TWIN, nr, n, g_targ, Z,
+                                             ngroups, dgb, dge, dseg, dseg + S, dseg + 2 * S,
+                                             nseg, Sacc,
+                                             -1.0f / kp.sqrtP, st);
+                        PT_.end(st);
+                        continue;
                     }
                     // X = FFT(A); y = IFFT(X * g_targ); Ptarg = window(y); Z = FFT(Ptarg)
                     fft_forward(ctx, planP, A, n, st);
@@ -1576,8 +1843,10 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                     hipLaunchKernelGGL(k_accum_spectra, dim3(std::max(1, P / 256), nseg), dim3(256),
                                        0, st, Sacc, Xs, Z, dseg, dseg + S, dseg + 2 * S, P, sc);
                     ERP_CHECK_LAUNCH("k_accum_spectra");
+                    PT_.end(st);
                 }
                 // per shot: c = IFFT(S)/P; Bluestein DFT_L, crop, IDFT_L, decimate
+                PT_.begin(ERP_STAGE_SHOT_FINAL, st, (double)S);
                 fft_inverse(ctx, planP, Sacc, S, st);
                 float2* bu = W.blu.as<float2>();
                 const int M = planM.N;
@@ -1600,6 +1869,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                 hipLaunchKernelGGL(k_blue_out, dim3(grid_for((size_t)nr * S)), dim3(256), 0, st,
                                    refr, nr, bu, M, blue->w.as<float2>(), 1.0f / (float)L, S, dV);
                 ERP_CHECK_LAUNCH("k_blue_out");
+                PT_.end(st);
                 have_refr = true;
             }
 
@@ -1607,6 +1877,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
             int64_t n0 = 0;
             for (int s = 0; s < S; ++s) n0 += htc[2 * s + 1];
             if (n0 > 0) {
+                PT_.begin(ERP_STAGE_OTHER, st, (double)n0);
                 if (!conv) {
                     hipLaunchKernelGGL(k_refr0_direct, dim3((nr + NT - 1) / NT, S), dim3(NT), 0, st,
                                        refr0, targets->xyzn.as<float>(), Tpad, list0,
@@ -1633,9 +1904,10 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                     up.insert(up.end(), pt.begin(), pt.end());
                     ERP_HIP_TRY(hipMemcpyAsync(dps, up.data(), sizeof(int) * up.size(),
                                                hipMemcpyHostToDevice, st));
-                    hipLaunchKernelGGL(k_target_accum<2>, dim3(n), dim3(NT), sizeof(float2) * TWIN,
+                    hipLaunchKernelGGL(k_target_accum<2>, dim3(n), dim3(NTT), sizeof(double2) * TWIN,
                                        st, targets->xyzn.as<float>(), Tpad, dps, dps + n, dshots,
-                                       dV, dbase, rec, dbounds, dpr, (int)nprisms, kp, t0, nullptr);
+                                       dV, dbase, rec, dbounds, dpr, (int)nprisms, kp, t0, nullptr,
+                                       nullptr);
                     ERP_CHECK_LAUNCH("k_target_accum<2>");
                     if (lin) {
                         window_conv(ctx, planP, t0, S, g_targ, refr0, nr, st);
@@ -1650,6 +1922,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                         }
                     }
                 }
+                PT_.end(st);
                 have_refr0 = true;
             }
         }

@@ -170,6 +170,27 @@ int erp_poll(erp_job* job);
 /* Block until the job is done; releases it. */
 int erp_wait(erp_job* job);
 
+/* ---- stage timers (tracing) ------------------------------------------------
+ * Replaces the reference's gettimeofday shot timers (sim.cu:44-58, 1000-1026)
+ * with hipEvent timers recorded on the context stream around every stage of
+ * erp_run / erp_run_dev.  units = work items the stage processed:
+ *   MASK: facet-shot predicate tests, SURFACE: illuminated facet-shot pairs,
+ *   TARGET_ACCUM: facet-target-shot triples, TARGET_FFT: target-shot traces. */
+#define ERP_STAGE_MASK          0  /* k_illum_mask + k_scan_blocks          */
+#define ERP_STAGE_SURFACE       1  /* fused surface kernel                  */
+#define ERP_STAGE_SURFACE_CONV  2  /* surface range compression             */
+#define ERP_STAGE_TARGET_SELECT 3  /* in-aperture target lists              */
+#define ERP_STAGE_TARGET_ACCUM  4  /* facet x target accumulation kernel    */
+#define ERP_STAGE_TARGET_FFT    5  /* per-target spectra + spectral sum     */
+#define ERP_STAGE_SHOT_FINAL    6  /* Bluestein DFT/crop/IDFT + combine     */
+#define ERP_STAGE_OTHER         7  /* type-0 targets, direct sums           */
+#define ERP_NSTAGES             8
+/* on != 0 enables the timers and clears the counters. */
+int erp_profile_enable(erp_ctx* ctx, int on);
+/* Totals since the last enable: summed milliseconds, number of timed
+ * launches and units for one stage. */
+int erp_profile_read(erp_ctx* ctx, int stage, double* ms, int64_t* launches, double* units);
+
 /* Device bytes a run of (F facets, T targets, ns shots, nr bins) needs beyond
  * the resident mesh and targets; used for admission by footprint. */
 int64_t erp_run_footprint(const erp_params* par, int64_t F, int64_t T, int64_t ns);
