@@ -2,8 +2,12 @@
 //
 // Replaces the cuFFT calls of the reference (radar_kernels.cu:532-550,
 // 589-645; elementary_kernels.cu:303-351).  Stockham autosort stages of
-// radix 4 (plus one radix-2 stage for odd log2 N) run in LDS with 256-thread
-// workgroups holding 4096 complex points (32 KB).
+// radix 16 (plus one radix 2, 4 or 8 stage when log2 n is not a multiple of
+// four) run on 256-thread workgroups holding 4096 complex points in LDS
+// (32 KB); each thread keeps 16 points in registers, so a 256-point transform
+// costs two LDS exchanges.  LDS indices go through an XOR swizzle that keeps
+// the Stockham strided writes and the column transposes free of bank
+// conflicts.
 //
 // N <= 4096: one pass, natural order in and out; several transforms share a
 // workgroup when N < 4096.
@@ -15,10 +19,10 @@
 //             order and returns natural order.
 // Pointwise spectral products never look at the order, so a forward/product/
 // inverse chain needs no transpose at all.  Stage twiddles come from a
-// "ladder" table lad[m + j] = exp(-2 pi i j / m) (m = 1, 2, 4, ..., 4096) so
-// the lanes of a stage read consecutive entries; the inter-pass twiddles of
-// each two-pass size live in a table laid out like the data (coalesced, L2
-// resident).  All tables are computed in double and rounded once.
+// "ladder" table lad[m + j] = exp(-2 pi i j / m) (m = 1, 2, 4, ..., 4096);
+// the inter-pass twiddles of each two-pass size live in a table laid out
+// like the data (coalesced, L2 resident).  All tables are computed in double
+// and rounded once.
 #include "erp_internal.h"
 
 namespace {
@@ -26,85 +30,211 @@ namespace {
 constexpr int NT = 256;         // threads per workgroup
 constexpr int PPT = 16;         // points per thread
 constexpr int NPTS = NT * PPT;  // points per workgroup (4096)
-constexpr int LOG_TWN = 18;
+constexpr int TWL = 1024;       // LDS ladder capacity: transforms up to 512 points
+constexpr int LADN = 65536;     // ladder entries: m = 2^0 .. 2^15
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
 
-template <int R> __device__ __forceinline__ void dft_r(float2* v, bool inv);
-
-template <> __device__ __forceinline__ void dft_r<2>(float2* v, bool) {
-    float2 a = v[0], b = v[1];
-    v[0] = make_float2(a.x + b.x, a.y + b.y);
-    v[1] = make_float2(a.x - b.x, a.y - b.y);
+// LDS swizzle (a bijection on [0, 4096) that only moves bits 0-4): the low
+// four index bits are XORed with bits 4-11, then bit 4 with bits 6-8.  Every
+// group of 16 lanes of the Stockham reads/writes and of the column tile
+// load/store then touches 16 distinct 8-byte bank pairs.
+__device__ __forceinline__ int sw(int a) {
+    a ^= ((a >> 4) ^ (a >> 8)) & 15;
+    return a ^ ((((a >> 6) ^ (a >> 7) ^ (a >> 8)) & 1) << 4);
 }
 
-template <> __device__ __forceinline__ void dft_r<4>(float2* v, bool inv) {
-    float2 a = make_float2(v[0].x + v[2].x, v[0].y + v[2].y);
-    float2 b = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
-    float2 c = make_float2(v[1].x + v[3].x, v[1].y + v[3].y);
-    float2 d = make_float2(v[1].x - v[3].x, v[1].y - v[3].y);
-    v[0] = make_float2(a.x + c.x, a.y + c.y);
-    v[2] = make_float2(a.x - c.x, a.y - c.y);
-    if (!inv) {  // -i*d
-        v[1] = make_float2(b.x + d.y, b.y - d.x);
-        v[3] = make_float2(b.x - d.y, b.y + d.x);
-    } else {     // +i*d
-        v[1] = make_float2(b.x - d.y, b.y + d.x);
-        v[3] = make_float2(b.x + d.y, b.y - d.x);
+// Thread index hidden from the optimiser: addresses derived from it are
+// recomputed where they are used instead of being hoisted and kept live
+// across transforms (which would exhaust the register budget).
+__device__ __forceinline__ int opaque_tid() {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
+// v * W with W = (c, -s) forward, (c, +s) inverse
+__device__ __forceinline__ float2 twc(float2 v, float c, float s, bool inv) {
+    const float si = inv ? s : -s;
+    return make_float2(v.x * c - v.y * si, v.x * si + v.y * c);
+}
+
+__device__ __forceinline__ void dft2(float2& a, float2& b) {
+    const float2 t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+__device__ __forceinline__ void dft4(float2& v0, float2& v1, float2& v2, float2& v3, bool inv) {
+    const float2 a = cadd(v0, v2), b = csub(v0, v2), c = cadd(v1, v3), d = csub(v1, v3);
+    v0 = cadd(a, c);
+    v2 = csub(a, c);
+    if (!inv) {  // -i d
+        v1 = make_float2(b.x + d.y, b.y - d.x);
+        v3 = make_float2(b.x - d.y, b.y + d.x);
+    } else {     // +i d
+        v1 = make_float2(b.x - d.y, b.y + d.x);
+        v3 = make_float2(b.x + d.y, b.y - d.x);
     }
 }
 
-// One Stockham stage of radix R over nb transforms of length n stored in LDS
-// at s[b*bstride + i].  Ls = product of the radices of the previous stages.
-// All inputs are read into registers before the barrier, so the stage is
-// in place.
+constexpr float K_C1 = 0.92387953251128674f;  // cos(pi/8)
+constexpr float K_S1 = 0.38268343236508978f;  // sin(pi/8)
+constexpr float K_R2 = 0.70710678118654752f;  // sqrt(1/2)
+
+template <int R> __device__ __forceinline__ void dft_r(float2* v, bool inv);
+
+template <> __device__ __forceinline__ void dft_r<2>(float2* v, bool) { dft2(v[0], v[1]); }
+
+template <> __device__ __forceinline__ void dft_r<4>(float2* v, bool inv) {
+    dft4(v[0], v[1], v[2], v[3], inv);
+}
+
+// 8-point DFT as 2 x 4: n = i2 + 2j (i2 < 2, j < 4), k = k1 + 4 k2
+template <> __device__ __forceinline__ void dft_r<8>(float2* v, bool inv) {
+    float2 a0 = v[0], a1 = v[2], a2 = v[4], a3 = v[6];
+    float2 b0 = v[1], b1 = v[3], b2 = v[5], b3 = v[7];
+    dft4(a0, a1, a2, a3, inv);
+    dft4(b0, b1, b2, b3, inv);
+    b1 = twc(b1, K_R2, K_R2, inv);   // W8^1
+    b2 = twc(b2, 0.f, 1.f, inv);     // W8^2
+    b3 = twc(b3, -K_R2, K_R2, inv);  // W8^3
+    v[0] = cadd(a0, b0); v[4] = csub(a0, b0);
+    v[1] = cadd(a1, b1); v[5] = csub(a1, b1);
+    v[2] = cadd(a2, b2); v[6] = csub(a2, b2);
+    v[3] = cadd(a3, b3); v[7] = csub(a3, b3);
+}
+
+// 16-point DFT as 4 x 4: n = i2 + 4j, k = k1 + 4 k2
+template <> __device__ __forceinline__ void dft_r<16>(float2* v, bool inv) {
+    float2 a[4][4];
+#pragma unroll
+    for (int i2 = 0; i2 < 4; ++i2) {
+        a[i2][0] = v[i2]; a[i2][1] = v[i2 + 4]; a[i2][2] = v[i2 + 8]; a[i2][3] = v[i2 + 12];
+        dft4(a[i2][0], a[i2][1], a[i2][2], a[i2][3], inv);
+    }
+    // W16^(i2 k1)
+    a[1][1] = twc(a[1][1], K_C1, K_S1, inv);    // 1
+    a[1][2] = twc(a[1][2], K_R2, K_R2, inv);    // 2
+    a[1][3] = twc(a[1][3], K_S1, K_C1, inv);    // 3
+    a[2][1] = twc(a[2][1], K_R2, K_R2, inv);    // 2
+    a[2][2] = twc(a[2][2], 0.f, 1.f, inv);      // 4
+    a[2][3] = twc(a[2][3], -K_R2, K_R2, inv);   // 6
+    a[3][1] = twc(a[3][1], K_S1, K_C1, inv);    // 3
+    a[3][2] = twc(a[3][2], -K_R2, K_R2, inv);   // 6
+    a[3][3] = twc(a[3][3], -K_C1, -K_S1, inv);  // 9
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        float2 t0 = a[0][k1], t1 = a[1][k1], t2 = a[2][k1], t3 = a[3][k1];
+        dft4(t0, t1, t2, t3, inv);
+        v[k1] = t0; v[k1 + 4] = t1; v[k1 + 8] = t2; v[k1 + 12] = t3;
+    }
+}
+
+// Multiply v[i] by the stage twiddle W_m^(i k) (conjugated for the inverse),
+// i < R, m = Ls R.  Powers 1, 2, 4 and 8 come from the ladder table, the
+// others are formed as products just before use, which keeps few twiddles
+// live at once.
 template <int R>
-__device__ __forceinline__ void stockham_stage(float2* s, int n, int Ls, int log2LsR,
-                                               int nb, int bstride, bool inv,
-                                               const float2* __restrict__ tw) {
+__device__ __forceinline__ void apply_twiddles(float2* v, const float2* __restrict__ lad, int m,
+                                               int k, bool inv) {
+    const float sg = inv ? -1.f : 1.f;
+    auto ld = [&](int e) {
+        const float2 w = lad[m + e * k];
+        return make_float2(w.x, sg * w.y);
+    };
+    const float2 w1 = ld(1);
+    v[1] = cmul(v[1], w1);
+    if (R == 2) return;
+    const float2 w2 = ld(2);
+    v[2] = cmul(v[2], w2);
+    if (R == 4) {
+        v[3] = cmul(v[3], ld(3));
+        return;
+    }
+    const float2 w3 = cmul(w1, w2);
+    v[3] = cmul(v[3], w3);
+    const float2 w4 = ld(4);
+    v[4] = cmul(v[4], w4);
+    v[5] = cmul(v[5], cmul(w4, w1));
+    v[6] = cmul(v[6], cmul(w4, w2));
+    v[7] = cmul(v[7], cmul(w4, w3));
+    if (R == 8) return;
+    const float2 w8 = ld(8);
+    v[8] = cmul(v[8], w8);
+    v[9] = cmul(v[9], cmul(w8, w1));
+    v[10] = cmul(v[10], cmul(w8, w2));
+    v[11] = cmul(v[11], cmul(w8, w3));
+    const float2 w12 = cmul(w8, w4);
+    v[12] = cmul(v[12], w12);
+    v[13] = cmul(v[13], cmul(w12, w1));
+    v[14] = cmul(v[14], cmul(w12, w2));
+    v[15] = cmul(v[15], cmul(w12, w3));
+}
+
+// One Stockham stage of radix R over nb transforms of length n stored at
+// logical LDS index b*n + i.  Ls = product of the previous radices.  All
+// inputs are read into registers before the barrier, so the stage is in place.
+template <int R, int NTB = NT>
+__device__ __forceinline__ void stockham_stage(float2* s, int n, int Ls, int nb, bool inv,
+                                               const float2* __restrict__ lad) {
     constexpr int Q = PPT / R;
     const int nR = n / R;
     const int nbf = nb * nR;
     float2 v[Q][R];
+    const int tid = opaque_tid();
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        const int j = threadIdx.x + q * NT;
+        const int j = tid + q * NTB;
         if (j < nbf) {
             const int b = j / nR, jj = j - b * nR;
-            const float2* src = s + b * bstride + jj;
+            const int src = b * n + jj;
 #pragma unroll
-            for (int i = 0; i < R; ++i) v[q][i] = src[i * nR];
+            for (int i = 0; i < R; ++i) v[q][i] = s[sw(src + i * nR)];
         }
     }
     __syncthreads();
+    const int tid2 = opaque_tid();
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        const int j = threadIdx.x + q * NT;
+        const int j = tid2 + q * NTB;
         if (j < nbf) {
             const int b = j / nR, jj = j - b * nR;
             const int k = jj & (Ls - 1);
-            if (Ls > 1) {
-#pragma unroll
-                for (int i = 1; i < R; ++i) {
-                    float2 w = tw[(1 << log2LsR) + i * k];
-                    if (inv) w.y = -w.y;
-                    v[q][i] = cmul(v[q][i], w);
-                }
-            }
+            if (Ls > 1) apply_twiddles<R>(v[q], lad, Ls * R, k, inv);
             dft_r<R>(v[q], inv);
-            float2* dst = s + b * bstride + (jj - k) * R + k;
+            const int dst = b * n + (jj - k) * R + k;
 #pragma unroll
-            for (int i = 0; i < R; ++i) dst[i * Ls] = v[q][i];
+            for (int i = 0; i < R; ++i) s[sw(dst + i * Ls)] = v[q][i];
         }
     }
     __syncthreads();
 }
 
+// Complete FFT of nb transforms of length n (power of two, nb*n <= 16 NTB),
+// logical layout b*n + i, by a workgroup of NTB threads.  Callers sync
+// before the call.
+template <int NTB = NT>
+__device__ __forceinline__ void lds_fft(float2* s, int n, int log2n, int nb, bool inv,
+                        const float2* __restrict__ lad) {
+    int Ls = 1;
+    const int r = log2n & 3;
+    if (r == 1) { stockham_stage<2, NTB>(s, n, Ls, nb, inv, lad); Ls <<= 1; }
+    if (r == 2) { stockham_stage<4, NTB>(s, n, Ls, nb, inv, lad); Ls <<= 2; }
+    if (r == 3) { stockham_stage<8, NTB>(s, n, Ls, nb, inv, lad); Ls <<= 3; }
+    for (int rem = log2n - r; rem > 0; rem -= 4) {
+        stockham_stage<16, NTB>(s, n, Ls, nb, inv, lad);
+        Ls <<= 4;
+    }
+}
+
 // Copy ladder entries [0, 2n) (the twiddles of every stage of a length-n
 // transform) into LDS; callers sync before the first stage.
-constexpr int TWL = 1024;  // LDS ladder capacity: transforms up to 512 points
 __device__ __forceinline__ const float2* stage_ladder(float2* twl, const float2* __restrict__ tw,
                                                       int n) {
     if (2 * n > TWL) return tw;
@@ -112,24 +242,10 @@ __device__ __forceinline__ const float2* stage_ladder(float2* twl, const float2*
     return twl;
 }
 
-// Complete FFT of nb transforms of length n (power of two, nb*n <= NPTS).
-__device__ void lds_fft(float2* s, int n, int log2n, int nb, int bstride, bool inv,
-                        const float2* __restrict__ tw) {
-    int Ls = 1, lg = 0, rem = log2n;
-    if (rem & 1) {
-        stockham_stage<2>(s, n, Ls, lg + 1, nb, bstride, inv, tw);
-        Ls <<= 1; lg += 1; rem -= 1;
-    }
-    while (rem > 0) {
-        stockham_stage<4>(s, n, Ls, lg + 2, nb, bstride, inv, tw);
-        Ls <<= 2; lg += 2; rem -= 2;
-    }
-}
-
-// ladder table: lad[m + j] = exp(-2 pi i j / m) for m = 2^0 .. 2^12, j < m
+// ladder table: lad[m + j] = exp(-2 pi i j / m) for m = 2^0 .. 2^15, j < m
 __global__ void k_twiddle_init(float2* lad) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < 2 * NPTS) {
+    if (t < LADN) {
         if (t == 0) {
             lad[0] = make_float2(1.f, 0.f);
             return;
@@ -167,35 +283,41 @@ __global__ void __launch_bounds__(NT) k_fft_small(float2* __restrict__ data, int
     float2* g = data + (size_t)row0 * n;
     const int tot = rows * n;
     for (int i = threadIdx.x; i < NPTS; i += NT)
-        s[i] = (i < tot) ? g[i] : make_float2(0.f, 0.f);
+        s[sw(i)] = (i < tot) ? g[i] : make_float2(0.f, 0.f);
     __syncthreads();
-    lds_fft(s, n, log2n, nb, n, inv != 0, tw);
-    for (int i = threadIdx.x; i < tot; i += NT) g[i] = s[i];
+    lds_fft(s, n, log2n, nb, inv != 0, tw);
+    for (int i = threadIdx.x; i < tot; i += NT) g[i] = s[sw(i)];
 }
 
-// Column pass of the two-pass transform.  Workgroup = CT adjacent columns x
-// N2 rows of one batch row; LDS column stride N2+1 avoids bank conflicts.
+// Column tile load: CT = NPTS/N2 adjacent columns of N2 rows go to LDS
+// logical index c*N2 + r.
+__device__ __forceinline__ void load_cols(float2* s, const float2* __restrict__ g, int N1, int N2,
+                                          int c0) {
+    const int CT = NPTS / N2;
+    for (int idx = threadIdx.x; idx < NPTS; idx += NT) {
+        const int r = idx / CT, c = idx - r * CT;
+        s[sw(c * N2 + r)] = g[(size_t)r * N1 + c0 + c];
+    }
+}
+
+// Column pass of the two-pass transform.
 // fwd: FFT then multiply element (k2, n1) by W_N^(n1*k2).  inv: IFFT only.
 __global__ void __launch_bounds__(NT) k_fft_cols(float2* __restrict__ data, int N1,
-                                                 int N2, int log2N2, int log2N, int inv,
+                                                 int N2, int log2N2, int inv,
                                                  const float2* __restrict__ tw,
                                                  const float2* __restrict__ it) {
-    extern __shared__ float2 s[];
+    __shared__ float2 s[NPTS];
+    __shared__ float2 twl[TWL];
     const int CT = NPTS / N2;
-    const int bstride = N2 + 1;
     const int c0 = blockIdx.x * CT;
     float2* g = data + (size_t)blockIdx.y * N1 * N2;
-    __shared__ float2 twl[TWL];
     const float2* tws = stage_ladder(twl, tw, N2);
-    for (int idx = threadIdx.x; idx < NPTS; idx += NT) {
-        const int r = idx / CT, c = idx - r * CT;
-        s[c * bstride + r] = g[(size_t)r * N1 + c0 + c];
-    }
+    load_cols(s, g, N1, N2, c0);
     __syncthreads();
-    lds_fft(s, N2, log2N2, CT, bstride, inv != 0, tws);
+    lds_fft(s, N2, log2N2, CT, inv != 0, tws);
     for (int idx = threadIdx.x; idx < NPTS; idx += NT) {
         const int r = idx / CT, c = idx - r * CT;
-        float2 v = s[c * bstride + r];
+        float2 v = s[sw(c * N2 + r)];
         if (!inv) v = cmul(v, it[(size_t)r * N1 + c0 + c]);
         g[(size_t)r * N1 + c0 + c] = v;
     }
@@ -204,29 +326,24 @@ __global__ void __launch_bounds__(NT) k_fft_cols(float2* __restrict__ data, int 
 // Row pass of the two-pass transform.  Workgroup = RT rows of length N1.
 // fwd: FFT only.  inv: IFFT then multiply element (k2, n1) by W_N^(-n1*k2).
 __global__ void __launch_bounds__(NT) k_fft_rows(float2* __restrict__ data, int N1,
-                                                 int log2N1, int N2, int log2N, int inv,
+                                                 int log2N1, int N2, int inv,
                                                  const float2* __restrict__ tw,
                                                  const float2* __restrict__ it) {
     __shared__ float2 s[NPTS];
+    __shared__ float2 twl[TWL];
     const int RT = NPTS / N1;
     const int r0 = blockIdx.x * RT;
     float2* g = data + (size_t)blockIdx.y * N1 * N2 + (size_t)r0 * N1;
-    __shared__ float2 twl[TWL];
     const float2* tws = stage_ladder(twl, tw, N1);
-    for (int i = threadIdx.x; i < NPTS; i += NT) s[i] = g[i];
+    for (int i = threadIdx.x; i < NPTS; i += NT) s[sw(i)] = g[i];
     __syncthreads();
-    lds_fft(s, N1, log2N1, RT, N1, inv != 0, tws);
+    lds_fft(s, N1, log2N1, RT, inv != 0, tws);
     for (int i = threadIdx.x; i < NPTS; i += NT) {
-        float2 v = s[i];
-        if (inv) {
-            float2 w = it[(size_t)r0 * N1 + i];
-            w.y = -w.y;
-            v = cmul(v, w);
-        }
+        float2 v = s[sw(i)];
+        if (inv) v = cmul(v, conjf2(it[(size_t)r0 * N1 + i]));
         g[i] = v;
     }
 }
-
 
 // ---------------------------------------------------------------------------
 // Fused per-target spectral chain.  For every (shot, target) pair p with
@@ -244,95 +361,83 @@ __global__ void __launch_bounds__(NT) k_fft_rows(float2* __restrict__ data, int 
 //               pairs into a partial spectrum
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(NT) k_tfft_cols_fwd(float2* __restrict__ data, int N1, int N2,
-                                                      int log2N2, int log2N,
-                                                      const int* __restrict__ pbase, int win,
-                                                      int nr, const float2* __restrict__ tw,
+                                                      int log2N2, const int* __restrict__ pbase,
+                                                      int win, int nr,
+                                                      const float2* __restrict__ tw,
                                                       const float2* __restrict__ it) {
-    extern __shared__ float2 s[];
+    __shared__ float2 s[NPTS];
+    __shared__ float2 twl[TWL];
     const int CT = NPTS / N2;
-    const int bstride = N2 + 1;
     const int c0 = blockIdx.x * CT;
     const int base = pbase[blockIdx.y];
     float2* g = data + (size_t)blockIdx.y * N1 * N2;
-    __shared__ float2 twl[TWL];
     const float2* tws = stage_ladder(twl, tw, N2);
     for (int idx = threadIdx.x; idx < NPTS; idx += NT) {
         const int r = idx / CT, c = idx - r * CT;
         const int n = r * N1 + c0 + c;
         const bool live = (n < nr) && (base < 0 || (n >= base && n < base + win));
-        s[c * bstride + r] = live ? g[n] : make_float2(0.f, 0.f);
+        s[sw(c * N2 + r)] = live ? g[n] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    lds_fft(s, N2, log2N2, CT, bstride, false, tws);
+    lds_fft(s, N2, log2N2, CT, false, tws);
     for (int idx = threadIdx.x; idx < NPTS; idx += NT) {
         const int r = idx / CT, c = idx - r * CT;
-        g[(size_t)r * N1 + c0 + c] = cmul(s[c * bstride + r], it[(size_t)r * N1 + c0 + c]);
+        g[(size_t)r * N1 + c0 + c] = cmul(s[sw(c * N2 + r)], it[(size_t)r * N1 + c0 + c]);
     }
 }
 
 __global__ void __launch_bounds__(NT) k_tfft_rows_conv(const float2* __restrict__ src,
                                                        float2* __restrict__ dst, int N1,
-                                                       int log2N1, int N2, int log2N,
+                                                       int log2N1, int N2,
                                                        const float2* __restrict__ G,
                                                        const float2* __restrict__ tw,
                                                        const float2* __restrict__ it) {
     __shared__ float2 s[NPTS];
+    __shared__ float2 twl[TWL];
     const int RT = NPTS / N1;
     const int r0 = blockIdx.x * RT;
     const size_t off = (size_t)blockIdx.y * N1 * N2 + (size_t)r0 * N1;
-    __shared__ float2 twl[TWL];
     const float2* tws = stage_ladder(twl, tw, N1);
-    for (int i = threadIdx.x; i < NPTS; i += NT) s[i] = src[off + i];
+    for (int i = threadIdx.x; i < NPTS; i += NT) s[sw(i)] = src[off + i];
     __syncthreads();
-    lds_fft(s, N1, log2N1, RT, N1, false, tws);
+    lds_fft(s, N1, log2N1, RT, false, tws);
     const float2* Gr = G + (size_t)r0 * N1;
-    for (int i = threadIdx.x; i < NPTS; i += NT) s[i] = cmul(s[i], Gr[i]);
+    for (int i = threadIdx.x; i < NPTS; i += NT) s[sw(i)] = cmul(s[sw(i)], Gr[i]);
     __syncthreads();
-    lds_fft(s, N1, log2N1, RT, N1, true, tws);
-    for (int i = threadIdx.x; i < NPTS; i += NT) {
-        float2 w = it[(size_t)r0 * N1 + i];
-        w.y = -w.y;
-        dst[off + i] = cmul(s[i], w);
-    }
+    lds_fft(s, N1, log2N1, RT, true, tws);
+    for (int i = threadIdx.x; i < NPTS; i += NT)
+        dst[off + i] = cmul(s[sw(i)], conjf2(it[(size_t)r0 * N1 + i]));
 }
 
 __global__ void __launch_bounds__(NT) k_tfft_cols_shift(float2* __restrict__ data, int N1, int N2,
-                                                        int log2N2, int log2N, int shift,
+                                                        int log2N2, int shift,
                                                         const float2* __restrict__ tw,
                                                         const float2* __restrict__ it) {
-    extern __shared__ float2 s[];
+    __shared__ float2 s[NPTS];
+    __shared__ float2 twl[TWL];
     const int CT = NPTS / N2;
-    const int bstride = N2 + 1;
     const int c0 = blockIdx.x * CT;
     float2* g = data + (size_t)blockIdx.y * N1 * N2;
-    __shared__ float2 twl[TWL];
     const float2* tws = stage_ladder(twl, tw, N2);
-    for (int idx = threadIdx.x; idx < NPTS; idx += NT) {
-        const int r = idx / CT, c = idx - r * CT;
-        s[c * bstride + r] = g[(size_t)r * N1 + c0 + c];
-    }
+    load_cols(s, g, N1, N2, c0);
     __syncthreads();
-    lds_fft(s, N2, log2N2, CT, bstride, true, tws);
-    // z row r = y row r + shift for r < shift (shift = nr / N1), zero above
+    lds_fft(s, N2, log2N2, CT, true, tws);
+    // z row r = y row r + shift for r < shift (shift = nr / N1 <= N2 / 2), zero above
     float2 v[PPT];
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
         const int idx = threadIdx.x + q * NT;
         const int c = idx / N2, r = idx - c * N2;
-        v[q] = (r < shift) ? s[c * bstride + r + shift] : make_float2(0.f, 0.f);
+        v[q] = (r < shift) ? s[sw(c * N2 + r + shift)] : make_float2(0.f, 0.f);
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < PPT; ++q) {
-        const int idx = threadIdx.x + q * NT;
-        const int c = idx / N2, r = idx - c * N2;
-        s[c * bstride + r] = v[q];
-    }
+    for (int q = 0; q < PPT; ++q) s[sw(threadIdx.x + q * NT)] = v[q];
     __syncthreads();
-    lds_fft(s, N2, log2N2, CT, bstride, false, tws);
+    lds_fft(s, N2, log2N2, CT, false, tws);
     for (int idx = threadIdx.x; idx < NPTS; idx += NT) {
         const int r = idx / CT, c = idx - r * CT;
-        g[(size_t)r * N1 + c0 + c] = cmul(s[c * bstride + r], it[(size_t)r * N1 + c0 + c]);
+        g[(size_t)r * N1 + c0 + c] = cmul(s[sw(c * N2 + r)], it[(size_t)r * N1 + c0 + c]);
     }
 }
 
@@ -344,29 +449,29 @@ __global__ void __launch_bounds__(NT) k_tfft_rows_mac(const float2* __restrict__
                                                       float2* __restrict__ partial,
                                                       const float2* __restrict__ tw) {
     __shared__ float2 s[NPTS];
+    __shared__ float2 twl[TWL];
     const int RT = NPTS / N1;
     const int r0 = blockIdx.x * RT;
     const int grp = blockIdx.y;
     const size_t P = (size_t)N1 * N2;
-    float2 acc[PPT], xr[PPT];
-    __shared__ float2 twl[TWL];
     const float2* tws = stage_ladder(twl, tw, N1);
+    float2 acc[PPT], xr[PPT];
 #pragma unroll
     for (int q = 0; q < PPT; ++q) acc[q] = make_float2(0.f, 0.f);
     for (int p = gbeg[grp]; p < gend[grp]; ++p) {
         const size_t off = (size_t)p * P + (size_t)r0 * N1;
-        for (int i = threadIdx.x; i < NPTS; i += NT) s[i] = X[off + i];
+        for (int i = threadIdx.x; i < NPTS; i += NT) s[sw(i)] = X[off + i];
         __syncthreads();
-        lds_fft(s, N1, log2N1, RT, N1, false, tws);
+        lds_fft(s, N1, log2N1, RT, false, tws);
 #pragma unroll
-        for (int q = 0; q < PPT; ++q) xr[q] = s[threadIdx.x + q * NT];
+        for (int q = 0; q < PPT; ++q) xr[q] = s[sw(threadIdx.x + q * NT)];
         __syncthreads();
-        for (int i = threadIdx.x; i < NPTS; i += NT) s[i] = Z[off + i];
+        for (int i = threadIdx.x; i < NPTS; i += NT) s[sw(i)] = Z[off + i];
         __syncthreads();
-        lds_fft(s, N1, log2N1, RT, N1, false, tws);
+        lds_fft(s, N1, log2N1, RT, false, tws);
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
-            const float2 z = s[threadIdx.x + q * NT];
+            const float2 z = s[sw(threadIdx.x + q * NT)];
             acc[q].x += xr[q].x * z.x - xr[q].y * z.y;
             acc[q].y += xr[q].x * z.y + xr[q].y * z.x;
         }
@@ -390,46 +495,49 @@ __global__ void __launch_bounds__(NT) k_tfft_small(const float2* __restrict__ A,
     const float2* a = A + (size_t)p * P;
     for (int n = threadIdx.x; n < P; n += NT) {
         const bool live = (n < nr) && (base < 0 || (n >= base && n < base + win));
-        s[n] = live ? a[n] : make_float2(0.f, 0.f);
+        s[sw(n)] = live ? a[n] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    lds_fft(s, P, log2P, 1, P, false, tw);
+    lds_fft(s, P, log2P, 1, false, tw);
     float2 xr[PPT];
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
         const int n = threadIdx.x + q * NT;
         if (n < P) {
-            xr[q] = s[n];
-            s[n] = cmul(xr[q], G[n]);
+            xr[q] = s[sw(n)];
+            s[sw(n)] = cmul(xr[q], G[n]);
         }
     }
     __syncthreads();
-    lds_fft(s, P, log2P, 1, P, true, tw);
+    lds_fft(s, P, log2P, 1, true, tw);
     float2 v[PPT];
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
         const int n = threadIdx.x + q * NT;
-        if (n < P) v[q] = (n < nr) ? s[n + nr] : make_float2(0.f, 0.f);
+        if (n < P) v[q] = (n < nr) ? s[sw(n + nr)] : make_float2(0.f, 0.f);
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
         const int n = threadIdx.x + q * NT;
-        if (n < P) s[n] = v[q];
+        if (n < P) s[sw(n)] = v[q];
     }
     __syncthreads();
-    lds_fft(s, P, log2P, 1, P, false, tw);
+    lds_fft(s, P, log2P, 1, false, tw);
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
         const int n = threadIdx.x + q * NT;
-        if (n < P) prod[(size_t)p * P + n] = cmul(xr[q], s[n]);
+        if (n < P) prod[(size_t)p * P + n] = cmul(xr[q], s[sw(n)]);
     }
 }
 
-// S[shot] += scale * sum of rows [beg, end) of src (one segment per shot)
+// S[shot] += scale * sum of rows [beg, end) of src (one segment per shot).
+// pN1 > 0: src rows are in natural order and S in the permuted order of a
+// two-pass N = pN1 * pN2 transform (frequency k at (k mod pN2) pN1 + k / pN2).
 __global__ void k_sum_rows(float2* __restrict__ S, const float2* __restrict__ src,
                            const int* __restrict__ seg_shot, const int* __restrict__ seg_beg,
-                           const int* __restrict__ seg_end, int P, float scale) {
+                           const int* __restrict__ seg_end, int P, float scale, int pN1,
+                           int pN2) {
     const int sg = blockIdx.y;
     const int sh = seg_shot[sg], b = seg_beg[sg], e = seg_end[sg];
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < P; k += gridDim.x * blockDim.x) {
@@ -439,10 +547,180 @@ __global__ void k_sum_rows(float2* __restrict__ S, const float2* __restrict__ sr
             acc.x += v.x;
             acc.y += v.y;
         }
-        float2* d = S + (size_t)sh * P + k;
+        const int kd = pN1 > 0 ? (k % pN2) * pN1 + k / pN2 : k;
+        float2* d = S + (size_t)sh * P + kd;
         d->x += scale * acc.x;
         d->y += scale * acc.y;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Whole per-pair chain in one workgroup for P = 2 nr, nr = 16 NTB (the nr
+// points fill 16 registers per thread and nr * 8 bytes of LDS).  With
+// m(j) = exp(-i pi j / nr), a P-point transform of x supported on [0, nr)
+// splits into two nr-point ones:
+//     X[2k] = DFT_nr(x)[k],   X[2k+1] = DFT_nr(x m)[k];
+// and the upper half of an (unnormalised) inverse P-point transform is
+//     y[j + nr] = IDFT_nr(Y_even)[j] - conj(m(j)) IDFT_nr(Y_odd)[j].
+// Both the inward trace a and the cropped z = y[nr + j] live on [0, nr), so
+//     z = IDFT(E G_e) - conj(m) IDFT(O G_o),   E = DFT(a), O = DFT(a m)
+//     out[2k] = E[k] DFT(z)[k],                out[2k+1] = O[k] DFT(z m)[k]
+// without HBM round trips of the intermediate transforms.  E and O are
+// transformed twice (once for z, once for the product) and DFT(z) waits in
+// the output row, so one register array at most is live across a transform.
+// Gn holds the chirp spectrum in natural order as (G[2k], G[2k+1]) pairs;
+// out rows are in natural order.
+// ---------------------------------------------------------------------------
+template <int NTB>
+__device__ __forceinline__ void load_window(float2* s, const float2* __restrict__ a, int base,
+                                            int win, const float2* __restrict__ mod) {
+    const int tid = opaque_tid();
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+        const int i = tid + q * NTB;
+        const bool live = base < 0 || (i >= base && i < base + win);
+        float2 v = live ? a[i] : make_float2(0.f, 0.f);
+        if (mod) v = cmul(v, mod[i]);
+        s[sw(i)] = v;
+    }
+    __syncthreads();
+}
+
+template <int NTB>
+__global__ void __launch_bounds__(NTB) k_tpair_half(const float2* __restrict__ A, int P,
+                                                    const int* __restrict__ pbase, int win,
+                                                    const float4* __restrict__ Gn,
+                                                    float4* __restrict__ out,
+                                                    const float2* __restrict__ lad) {
+    constexpr int n = NTB * PPT;
+    constexpr int log2n = NTB == 256 ? 12 : (NTB == 512 ? 13 : 14);
+    __shared__ float2 s[n];
+    const int p = blockIdx.x;
+    const int base = pbase[p];
+    const float2* a = A + (size_t)p * P;
+    const float2* mod = lad + 2 * n;  // m(j) = lad[2n + j]
+    float2* o2 = reinterpret_cast<float2*>(out + (size_t)p * n);
+    float2 r1[PPT];
+    // Eight transforms, one code copy.  Phase: input -> transform -> use
+    //   0: a          -> E   -> s = E G_e
+    //   1: s          -> u   -> r1 = u              (inverse)
+    //   2: a m        -> O   -> s = O G_o
+    //   3: s          -> v   -> r1 = u - conj(m) v = z   (inverse)
+    //   4: z          -> Ze  -> even output slots = Ze
+    //   5: z m        -> Zo  -> r1 = Zo
+    //   6: a          -> E   -> even slots = E Ze
+    //   7: a m        -> O   -> odd slots = O Zo
+#pragma unroll 1
+    for (int ph = 0; ph < 8; ++ph) {
+        if (ph == 0 || ph == 2 || ph == 6 || ph == 7) {
+            load_window<NTB>(s, a, base, win, (ph == 2 || ph == 7) ? mod : nullptr);
+        } else if (ph == 4 || ph == 5) {
+            const int tid = opaque_tid();
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int i = tid + q * NTB;
+                s[sw(i)] = ph == 4 ? r1[q] : cmul(r1[q], mod[i]);
+            }
+            __syncthreads();
+        }
+        lds_fft<NTB>(s, n, log2n, 1, ph == 1 || ph == 3, lad);
+        const int tid = opaque_tid();
+        if (ph == 0 || ph == 2) {
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int i = tid + q * NTB;
+                const float4 g = Gn[i];
+                s[sw(i)] = cmul(s[sw(i)], ph == 0 ? make_float2(g.x, g.y) : make_float2(g.z, g.w));
+            }
+        } else if (ph == 1 || ph == 5) {
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) r1[q] = s[sw(tid + q * NTB)];
+        } else if (ph == 3) {
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int i = tid + q * NTB;
+                r1[q] = csub(r1[q], cmul(conjf2(mod[i]), s[sw(i)]));
+            }
+        } else if (ph == 4) {
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int i = tid + q * NTB;
+                o2[2 * i] = s[sw(i)];
+            }
+        } else if (ph == 6) {
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int i = tid + q * NTB;
+                o2[2 * i] = cmul(s[sw(i)], o2[2 * i]);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int i = tid + q * NTB;
+                o2[2 * i + 1] = cmul(s[sw(i)], r1[q]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Two-level deterministic sum of per-pair rows (natural order, P complex).
+// Level 1: partial[slice] = sum of the rows of one slice (at most SLICE rows
+// of one shot), two complex values per thread, eight rows in flight.
+constexpr int SLICE = 32;
+__global__ void __launch_bounds__(256) k_sum_slices(float4* __restrict__ partial,
+                                                    const float4* __restrict__ rows,
+                                                    const int* __restrict__ sl_beg,
+                                                    const int* __restrict__ sl_end, int P2) {
+    const int k = blockIdx.x * 256 + threadIdx.x;  // float4 column, P2 = P / 2
+    if (k >= P2) return;
+    const int b = sl_beg[blockIdx.y], e = sl_end[blockIdx.y];
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    int r = b;
+    for (; r + 8 <= e; r += 8) {
+        float4 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = rows[(size_t)(r + u) * P2 + k];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            acc.x += v[u].x; acc.y += v[u].y; acc.z += v[u].z; acc.w += v[u].w;
+        }
+    }
+    for (; r < e; ++r) {
+        const float4 v = rows[(size_t)r * P2 + k];
+        acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
+    }
+    partial[(size_t)blockIdx.y * P2 + k] = acc;
+}
+
+// Level 2: S[shot] += scale * sum of the slices [beg, end) of one shot,
+// written in the permuted order of a two-pass transform when pN1 > 0.
+__global__ void __launch_bounds__(256) k_sum_final(float2* __restrict__ S,
+                                                   const float2* __restrict__ partial,
+                                                   const int* __restrict__ seg_shot,
+                                                   const int* __restrict__ seg_beg,
+                                                   const int* __restrict__ seg_end, int P,
+                                                   float scale, int pN1, int pN2) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= P) return;
+    const int sg = blockIdx.y;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int r = seg_beg[sg]; r < seg_end[sg]; ++r) {
+        const float2 v = partial[(size_t)r * P + k];
+        acc.x += v.x;
+        acc.y += v.y;
+    }
+    const int kd = pN1 > 0 ? (k % pN2) * pN1 + k / pN2 : k;
+    float2* d = S + (size_t)seg_shot[sg] * P + kd;
+    d->x += scale * acc.x;
+    d->y += scale * acc.y;
+}
+
+// natural-order copy of a two-pass spectrum: dst[k] = src[(k mod N2) N1 + k / N2]
+__global__ void k_unpermute(float2* __restrict__ dst, const float2* __restrict__ src, int N1,
+                            int N2) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < N1 * N2) dst[k] = src[(k % N2) * N1 + k / N2];
 }
 
 }  // namespace
@@ -465,8 +743,8 @@ FftPlan fft_plan(int N) {
 }
 
 void fft_init_twiddles(erp_ctx* ctx) {
-    ctx->twiddle.ensure(sizeof(float2) * (size_t)(2 * NPTS));
-    hipLaunchKernelGGL(k_twiddle_init, dim3(2 * NPTS / 256), dim3(256), 0, ctx->stream,
+    ctx->twiddle.ensure(sizeof(float2) * (size_t)LADN);
+    hipLaunchKernelGGL(k_twiddle_init, dim3(LADN / 256), dim3(256), 0, ctx->stream,
                        ctx->twiddle.as<float2>());
     ERP_CHECK_LAUNCH("k_twiddle_init");
 }
@@ -497,22 +775,21 @@ static void run_fft(erp_ctx* ctx, const FftPlan& p, float2* data, int batch, boo
         return;
     }
     const int CT = NPTS / p.N2, RT = NPTS / p.N1;
-    const size_t col_lds = sizeof(float2) * (size_t)CT * (p.N2 + 1);
     const float2* it = inter_table(ctx, p, s);
     dim3 gcol(p.N1 / CT, batch), grow(p.N2 / RT, batch);
     if (!inv) {
-        hipLaunchKernelGGL(k_fft_cols, gcol, dim3(NT), col_lds, s, data, p.N1, p.N2,
-                           ilog2(p.N2), p.log2N, 0, tw, it);
+        hipLaunchKernelGGL(k_fft_cols, gcol, dim3(NT), 0, s, data, p.N1, p.N2, ilog2(p.N2), 0,
+                           tw, it);
         ERP_CHECK_LAUNCH("k_fft_cols");
-        hipLaunchKernelGGL(k_fft_rows, grow, dim3(NT), 0, s, data, p.N1, ilog2(p.N1), p.N2,
-                           p.log2N, 0, tw, it);
+        hipLaunchKernelGGL(k_fft_rows, grow, dim3(NT), 0, s, data, p.N1, ilog2(p.N1), p.N2, 0,
+                           tw, it);
         ERP_CHECK_LAUNCH("k_fft_rows");
     } else {
-        hipLaunchKernelGGL(k_fft_rows, grow, dim3(NT), 0, s, data, p.N1, ilog2(p.N1), p.N2,
-                           p.log2N, 1, tw, it);
+        hipLaunchKernelGGL(k_fft_rows, grow, dim3(NT), 0, s, data, p.N1, ilog2(p.N1), p.N2, 1,
+                           tw, it);
         ERP_CHECK_LAUNCH("k_fft_rows");
-        hipLaunchKernelGGL(k_fft_cols, gcol, dim3(NT), col_lds, s, data, p.N1, p.N2,
-                           ilog2(p.N2), p.log2N, 1, tw, it);
+        hipLaunchKernelGGL(k_fft_cols, gcol, dim3(NT), 0, s, data, p.N1, p.N2, ilog2(p.N2), 1,
+                           tw, it);
         ERP_CHECK_LAUNCH("k_fft_cols");
     }
 }
@@ -526,7 +803,7 @@ void fft_inverse(erp_ctx* ctx, const FftPlan& p, float2* data, int batch, hipStr
 }
 
 // Host driver of the fused chain for n pairs (see the kernel comment).  A and
-// Y hold n rows of P; partial/prod workspace holds max(ngroups, n) rows.
+// Y hold n rows of P; the work buffer holds max(ngroups, n) rows.
 void target_spectra_fused(erp_ctx* ctx, const FftPlan& p, float2* A, float2* Y,
                           const int* pbase, int win, int nr, int n, const float2* G,
                           float2* work, int ngroups, const int* gbeg, const int* gend,
@@ -540,28 +817,84 @@ void target_spectra_fused(erp_ctx* ctx, const FftPlan& p, float2* A, float2* Y,
         ERP_CHECK_LAUNCH("k_tfft_small");
         // groups are unused here: sum the per-pair products per shot segment
         hipLaunchKernelGGL(k_sum_rows, dim3(std::max(1, P / 256), nseg), dim3(256), 0, st, S,
-                           work, gseg_shot, gseg_beg, gseg_end, P, scale);
+                           work, gseg_shot, gseg_beg, gseg_end, P, scale, 0, 0);
         ERP_CHECK_LAUNCH("k_sum_rows");
         return;
     }
     const int CT = NPTS / p.N2, RT = NPTS / p.N1;
-    const size_t col_lds = sizeof(float2) * (size_t)CT * (p.N2 + 1);
     const int l1 = ilog2(p.N1), l2 = ilog2(p.N2);
     if (nr % p.N1 != 0) throw ErpError{ERP_E_INVALID, "fused target chain needs nr % N1 == 0"};
     const float2* it = inter_table(ctx, p, st);
-    hipLaunchKernelGGL(k_tfft_cols_fwd, dim3(p.N1 / CT, n), dim3(NT), col_lds, st, A, p.N1, p.N2,
-                       l2, p.log2N, pbase, win, nr, tw, it);
+    hipLaunchKernelGGL(k_tfft_cols_fwd, dim3(p.N1 / CT, n), dim3(NT), 0, st, A, p.N1, p.N2, l2,
+                       pbase, win, nr, tw, it);
     ERP_CHECK_LAUNCH("k_tfft_cols_fwd");
     hipLaunchKernelGGL(k_tfft_rows_conv, dim3(p.N2 / RT, n), dim3(NT), 0, st, A, Y, p.N1, l1,
-                       p.N2, p.log2N, G, tw, it);
+                       p.N2, G, tw, it);
     ERP_CHECK_LAUNCH("k_tfft_rows_conv");
-    hipLaunchKernelGGL(k_tfft_cols_shift, dim3(p.N1 / CT, n), dim3(NT), col_lds, st, Y, p.N1,
-                       p.N2, l2, p.log2N, nr / p.N1, tw, it);
+    hipLaunchKernelGGL(k_tfft_cols_shift, dim3(p.N1 / CT, n), dim3(NT), 0, st, Y, p.N1, p.N2, l2,
+                       nr / p.N1, tw, it);
     ERP_CHECK_LAUNCH("k_tfft_cols_shift");
     hipLaunchKernelGGL(k_tfft_rows_mac, dim3(p.N2 / RT, ngroups), dim3(NT), 0, st, A, Y, p.N1, l1,
                        p.N2, gbeg, gend, work, tw);
     ERP_CHECK_LAUNCH("k_tfft_rows_mac");
     hipLaunchKernelGGL(k_sum_rows, dim3(std::max(1, P / 256), nseg), dim3(256), 0, st, S, work,
-                       gseg_shot, gseg_beg, gseg_end, P, scale);
+                       gseg_shot, gseg_beg, gseg_end, P, scale, 0, 0);
     ERP_CHECK_LAUNCH("k_sum_rows");
+}
+
+int target_half_slice() { return SLICE; }
+
+bool target_half_supported(const FftPlan& p, int nr) {
+    return p.N == 2 * nr && (nr == 4096 || nr == 8192 || nr == 16384);
+}
+
+void fft_natural_order(const FftPlan& p, const float2* src, float2* dst, hipStream_t st) {
+    if (!p.two_pass) {
+        ERP_HIP_TRY(hipMemcpyAsync(dst, src, sizeof(float2) * (size_t)p.N,
+                                   hipMemcpyDeviceToDevice, st));
+        return;
+    }
+    hipLaunchKernelGGL(k_unpermute, dim3((p.N + 255) / 256), dim3(256), 0, st, dst, src, p.N1,
+                       p.N2);
+    ERP_CHECK_LAUNCH("k_unpermute");
+}
+
+// Host driver of the one-workgroup-per-pair chain (see k_tpair_half): n
+// pairs of A (rows of P) -> per-pair products in work (natural order) ->
+// slice sums in partial (nslices rows; slices [sl_beg, sl_end) of pairs of
+// one shot, at most target_half_slice() pairs each) -> per-shot sums of the
+// slices [seg_beg, seg_end) into S, in the order fft_inverse expects.
+void target_spectra_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const int* pbase,
+                         int win, int nr, int n, const float2* Gnat, float2* work,
+                         float2* partial, const int* sl_beg, const int* sl_end, int nslices,
+                         const int* seg_shot, const int* seg_beg, const int* seg_end, int nseg,
+                         float2* S, float scale, hipStream_t st) {
+    if (!target_half_supported(p, nr))
+        throw ErpError{ERP_E_INVALID, "per-pair LDS chain needs P = 2 nr, nr in {4096, 8192, 16384}"};
+    const float2* lad = ctx->twiddle.as<float2>();
+    const float4* G4 = reinterpret_cast<const float4*>(Gnat);
+    float4* o4 = reinterpret_cast<float4*>(work);
+    switch (nr) {
+        case 4096:
+            hipLaunchKernelGGL(k_tpair_half<256>, dim3(n), dim3(256), 0, st, A, p.N, pbase, win,
+                               G4, o4, lad);
+            break;
+        case 8192:
+            hipLaunchKernelGGL(k_tpair_half<512>, dim3(n), dim3(512), 0, st, A, p.N, pbase, win,
+                               G4, o4, lad);
+            break;
+        default:
+            hipLaunchKernelGGL(k_tpair_half<1024>, dim3(n), dim3(1024), 0, st, A, p.N, pbase,
+                               win, G4, o4, lad);
+            break;
+    }
+    ERP_CHECK_LAUNCH("k_tpair_half");
+    const int P2 = p.N / 2;
+    hipLaunchKernelGGL(k_sum_slices, dim3((P2 + 255) / 256, nslices), dim3(256), 0, st,
+                       reinterpret_cast<float4*>(partial), o4, sl_beg, sl_end, P2);
+    ERP_CHECK_LAUNCH("k_sum_slices");
+    hipLaunchKernelGGL(k_sum_final, dim3((p.N + 255) / 256, nseg), dim3(256), 0, st, S, partial,
+                       seg_shot, seg_beg, seg_end, p.N, scale, p.two_pass ? p.N1 : 0,
+                       p.two_pass ? p.N2 : 0);
+    ERP_CHECK_LAUNCH("k_sum_final");
 }

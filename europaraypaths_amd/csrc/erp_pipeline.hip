@@ -1427,27 +1427,47 @@ KPar make_kpar(const erp_params& p, int P_fft) {
     return k;
 }
 
+// Range-compression kernel spectrum, cached per context.  natural = true
+// returns it in natural frequency order (the per-pair LDS chain), otherwise
+// in the order fft_forward produces.  Only kind-1 entries (one per receive
+// window start) are evicted, so kind-0 pointers held by a run stay valid.
 const float2* get_chirp(erp_ctx* ctx, int nr, float dr, float res, int kind, float rst, int P,
-                        const FftPlan& plan, hipStream_t st) {
+                        const FftPlan& plan, hipStream_t st, bool natural = false) {
     ChirpKey key{nr, dr, res, kind, kind == 1 ? rst : 0.f};
-    for (ChirpSpec* c : ctx->chirps)
-        if (c->key == key) return c->spec.as<float2>();
-    ChirpSpec* c = new ChirpSpec();
-    c->key = key;
-    c->spec.ensure(sizeof(float2) * (size_t)P);
-    hipLaunchKernelGGL(k_gen_kernel, dim3((P + 255) / 256), dim3(256), 0, st,
-                       c->spec.as<float2>(), P, nr, (double)dr, (double)res, (double)rst, kind);
-    ERP_CHECK_LAUNCH("k_gen_kernel");
-    fft_forward(ctx, plan, c->spec.as<float2>(), 1, st);
-    hipLaunchKernelGGL(k_scale, dim3((P + 255) / 256), dim3(256), 0, st, c->spec.as<float2>(), P,
-                       1.0f / (float)P);
-    ERP_CHECK_LAUNCH("k_scale");
-    if (ctx->chirps.size() > 16) {
-        delete ctx->chirps.front();
-        ctx->chirps.erase(ctx->chirps.begin());
+    ChirpSpec* c = nullptr;
+    for (ChirpSpec* e : ctx->chirps)
+        if (e->key == key) c = e;
+    if (!c) {
+        c = new ChirpSpec();
+        c->key = key;
+        c->spec.ensure(sizeof(float2) * (size_t)P);
+        hipLaunchKernelGGL(k_gen_kernel, dim3((P + 255) / 256), dim3(256), 0, st,
+                           c->spec.as<float2>(), P, nr, (double)dr, (double)res, (double)rst,
+                           kind);
+        ERP_CHECK_LAUNCH("k_gen_kernel");
+        fft_forward(ctx, plan, c->spec.as<float2>(), 1, st);
+        hipLaunchKernelGGL(k_scale, dim3((P + 255) / 256), dim3(256), 0, st,
+                           c->spec.as<float2>(), P, 1.0f / (float)P);
+        ERP_CHECK_LAUNCH("k_scale");
+        int n1 = 0;
+        for (ChirpSpec* e : ctx->chirps) n1 += e->key.kind == 1;
+        if (n1 > 16) {
+            for (size_t i = 0; i < ctx->chirps.size(); ++i)
+                if (ctx->chirps[i]->key.kind == 1) {
+                    ERP_HIP_TRY(hipStreamSynchronize(st));
+                    delete ctx->chirps[i];
+                    ctx->chirps.erase(ctx->chirps.begin() + (long)i);
+                    break;
+                }
+        }
+        ctx->chirps.push_back(c);
     }
-    ctx->chirps.push_back(c);
-    return c->spec.as<float2>();
+    if (!natural) return c->spec.as<float2>();
+    if (!c->nat.p) {
+        c->nat.ensure(sizeof(float2) * (size_t)P);
+        fft_natural_order(plan, c->spec.as<float2>(), c->nat.as<float2>(), st);
+    }
+    return c->nat.as<float2>();
 }
 
 BlueTables* get_blue(erp_ctx* ctx, int L, float smpl, float B, hipStream_t st) {
@@ -1568,6 +1588,10 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     if (!spec || (conv && lin)) g_surf = get_chirp(ctx, nr, p.dr, p.rng_res, 0, 0.f, P, planP, st);
     const float2* g_targ = nullptr;
     if (T > 0 && conv && lin) g_targ = get_chirp(ctx, nr, p.dr, res_t, 0, 0.f, P, planP, st);
+    // plain mode: per-pair spectral chain in one workgroup when P = 2 nr fits LDS
+    const bool half = !spec && T > 0 && conv && lin && target_half_supported(planP, nr);
+    const float2* g_targ_nat =
+        half ? get_chirp(ctx, nr, p.dr, res_t, 0, 0.f, P, planP, st, true) : nullptr;
     BlueTables* blue = nullptr;
     FftPlan planM;
     if (any_typed) {
@@ -1723,12 +1747,16 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                 float2* Sacc = W.shotS.as<float2>();
                 zero(Sacc, (size_t)S * P, st);
                 // plain mode with a suitable size: fused LDS spectral chain
-                const bool fused = !spec && (!planP.two_pass || nr % planP.N1 == 0);
-                const int GS = 16;  // pairs per multiply-accumulate group
-                const int nbuf = fused ? 3 : (spec ? 4 : 3);
+                const bool fused = !spec && !half && (!planP.two_pass || nr % planP.N1 == 0);
+                // pairs per multiply-accumulate group (fused) or per summed slice (half)
+                const int GS = half ? target_half_slice() : 16;
+                const int nbuf = half ? 2 : (fused ? 3 : (spec ? 4 : 3));
                 int C = (int)std::max<int64_t>(1, ((int64_t)1536 << 20) / ((int64_t)P * 8 * nbuf));
                 C = std::min<int>(C, 8192);
-                W.tA.ensure(sizeof(float2) * (size_t)C * P * (nbuf - 1));
+                const size_t gcap = (size_t)C / GS + S + 2;
+                // half: A rows, then one partial row per slice
+                W.tA.ensure(sizeof(float2) * (size_t)P *
+                            ((size_t)C * (nbuf - 1) + (half ? gcap : 0)));
                 W.tC.ensure(sizeof(float2) * (size_t)C * P);
                 float2* A = W.tA.as<float2>();
                 float2* Bq = A + (size_t)C * P;                 // y / Y' (and MODE 1: B)
@@ -1739,7 +1767,6 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                 ERP_HIP_TRY(hipMemcpyAsync(hl1.data(), list1, sizeof(int) * (size_t)S * Tcap,
                                            hipMemcpyDeviceToHost, st));
                 ERP_HIP_TRY(hipStreamSynchronize(st));
-                const size_t gcap = (size_t)C / GS + S + 2;
                 W.pairs.ensure(sizeof(int) * ((size_t)C * 3 + 2 * gcap + 3 * (size_t)S + 16));
                 int* dps = W.pairs.as<int>();
                 int* dpt = dps + C;
@@ -1789,8 +1816,9 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                     const size_t so = 3 * (size_t)C + 2 * gcap;
                     std::copy(sg_s.begin(), sg_s.end(), up.begin() + so);
                     const bool two = planP.two_pass;
-                    const std::vector<int>& segb = (fused && two) ? gsb : sg_b;
-                    const std::vector<int>& sege = (fused && two) ? gse : sg_e;
+                    const bool by_group = (fused && two) || half;
+                    const std::vector<int>& segb = by_group ? gsb : sg_b;
+                    const std::vector<int>& sege = by_group ? gse : sg_e;
                     std::copy(segb.begin(), segb.end(), up.begin() + so + S);
                     std::copy(sege.begin(), sege.end(), up.begin() + so + 2 * S);
                     ERP_HIP_TRY(hipMemcpyAsync(dps, up.data(), sizeof(int) * up.size(),
@@ -1802,7 +1830,8 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                         hipLaunchKernelGGL(k_target_accum_fast, dim3(n), dim3(NTT),
                                            sizeof(double2) * TWIN, st, targets->xyzn.as<float>(),
                                            Tpad, dps, dpt, dshots, dV, dbase, rec, dbounds, dpr,
-                                           (int)nprisms, kp, A, fused ? dpb : nullptr);
+                                           (int)nprisms, kp, A,
+                                           (fused || half) ? dpb : nullptr);
                         ERP_CHECK_LAUNCH("k_target_accum_fast");
                     } else {
                         hipLaunchKernelGGL(k_target_accum<1>, dim3(n), dim3(NTT),
@@ -1814,6 +1843,15 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                     }
                     PT_.end(st);
                     PT_.begin(ERP_STAGE_TARGET_FFT, st, (double)n);
+                    if (half) {
+                        // outward trace = -A/sqrt(P): scale -1/sqrt(P)
+                        target_spectra_half(ctx, planP, A, dpb, TWIN, nr, n, g_targ_nat, Z,
+                                            A + (size_t)C * P, dgb, dge, ngroups, dseg,
+                                            dseg + S, dseg + 2 * S, nseg, Sacc,
+                                            -1.0f / kp.sqrtP, st);
+                        PT_.end(st);
+                        continue;
+                    }
                     if (fused) {
                         // outward trace = -A/sqrt(P): scale -1/sqrt(P)
                         target_spectra_fused(ctx, planP, A, Bq, dpb, TWIN, nr, n, g_targ, Z,

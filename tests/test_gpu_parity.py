@@ -99,6 +99,20 @@ def test_targets_plain_non_pow2(gpu_ctx):
     compare_traces(g, o)
 
 
+@pytest.mark.parametrize("nr", [2048, 4096, 8192, 16384, 12288, 5000])
+def test_targets_plain_range_sizes(gpu_ctx, nr):
+    """Every spectral-chain variant of plain-mode targets: one workgroup per
+    pair (P <= 4096), the per-pair LDS chain (P = 2 nr, nr 4096..16384), the
+    four-kernel two-pass chain (nr 12288: P = 32768, nr a multiple of the
+    256-point rows) and the unfused path (nr 5000)."""
+    print(f"targets plain nr={nr}", flush=True)
+    par, soa, shots = scene(rough=0.3, nr=nr, h=3000.0, rst=2600.0, ns=2)
+    tg = _targets(par, n=3)
+    g, gv, o, ov = run_both(gpu_ctx, par, soa, shots, targets=tg)
+    np.testing.assert_array_equal(gv, ov)
+    compare_traces(g, o)
+
+
 def test_targets_lossless_and_prisms(gpu_ctx):
     par, soa, shots = scene(rough=0.3, nr=1024, h=3000.0, rst=2600.0, lossless=True)
     tg = _targets(par, n=3)
