@@ -11,6 +11,12 @@ Prints ONE JSON line: throughput in facet-shot pairs/s (pairs = illuminated
 facet x shot, counted by the kernels), the roofline of the dominant kernel
 from live hipEvent timers, and the FP32 CPU restatement of the reference path
 timed on a bounded sample of the same workload on the host cores.
+
+Before the warm-up a 256 MiB device-to-device copy runs once (outside the
+timed region): under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE calibrate
+the counters against a transfer of known size.  `traffic` is filled from the
+committed counter measurement in profiles/traffic.json when it matches the
+kernel and workload.
 """
 from __future__ import annotations
 
@@ -30,13 +36,25 @@ METRIC = ("facet·shot pairs/s/GPU at ≤1e-3 rad phase err (served vs plain); "
           "p50/p99 latency")
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-# Algorithmic operations per facet-target-shot triple of the target kernel
-# (FP32 + FP64 adds, multiplies, divides, square roots and one per
-# transcendental call of the formula), census in DESIGN.md.
-TRIPLE_FLOPS = 121.0
+# Algorithmic operations per facet x target x shot triple of the target
+# accumulation kernel: adds, multiplies, divides, square roots and one per
+# transcendental call of the per-triple formula (98, itemised in DESIGN.md).
+TRIPLE_FLOPS = 98.0
 # Algorithmic HBM bytes per facet-shot pair of the surface path: the 12
 # float facet attributes streamed once per shot.
 PAIR_BYTES = 48.0
+CALIB_BYTES = 256 << 20
+
+
+def target_pair_flops(P: int) -> float:
+    """Algorithmic operations of one (shot, target) spectral chain of size P:
+    three P-point complex FFTs at 5 P log2 P, the two pointwise complex
+    products (6 P each) and the accumulation (2 P)."""
+    return 3 * 5 * P * np.log2(P) + 14 * P
+
+
+KERNELS = {"target_accum": "k_target_accum_fast", "target_fft": "k_tpair_half",
+           "surface": "k_surface"}
 
 
 def hip():
@@ -54,6 +72,75 @@ def dev_alloc(h, nbytes):
     if rc != 0:
         raise RuntimeError(f"hipMalloc failed: {rc}")
     return p.value
+
+
+def traffic_for(kernel, workload, units_per_launch):
+    """HBM bytes per launch from the committed rocprofv3 counter run
+    (profiles/traffic.json: calibrated FETCH_SIZE + WRITE_SIZE per unit of
+    work), or None when no measurement matches the kernel and workload."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = t.get(kernel)
+    if not e or e.get("workload") != workload:
+        return None
+    return e["bytes_per_unit"] * units_per_launch
+
+
+def _counter_rows(directory):
+    import csv
+    import glob
+    rows = []
+    for path in glob.glob(os.path.join(directory, "**", "*counter_collection.csv"),
+                          recursive=True):
+        with open(path) as f:
+            rows.extend(csv.DictReader(f))
+    return rows
+
+
+def ingest_pmc(fetch_dir, write_dir, bench_json, out_path):
+    """Turn two rocprofv3 --pmc runs of this script (FETCH_SIZE, WRITE_SIZE;
+    csv output) plus that run's JSON line into profiles/traffic.json: HBM
+    bytes per unit of work per kernel, each counter scaled by what it read for
+    the 256 MiB calibration copy."""
+    with open(bench_json) as f:
+        b = json.loads([l for l in f if l.startswith("{")][-1])
+    sums = {}
+    calib = {}
+    for d, cname in ((fetch_dir, "FETCH_SIZE"), (write_dir, "WRITE_SIZE")):
+        for r in _counter_rows(d):
+            if r.get("Counter_Name") != cname:
+                continue
+            v = float(r["Counter_Value"])
+            k = r["Kernel_Name"]
+            if "copyBuffer" in k:
+                calib[cname] = max(calib.get(cname, 0.0), v)
+            for stage, kern in KERNELS.items():
+                if kern in k:
+                    sums.setdefault(kern, {"FETCH_SIZE": 0.0, "WRITE_SIZE": 0.0, "n": 0,
+                                           "stage": stage})
+                    sums[kern][cname] += v
+                    sums[kern]["n"] += cname == "FETCH_SIZE"
+    if "FETCH_SIZE" not in calib or "WRITE_SIZE" not in calib:
+        raise SystemExit("calibration copy not found in the counter runs")
+    kib = CALIB_BYTES / 1024.0
+    sf, sw = kib / calib["FETCH_SIZE"], kib / calib["WRITE_SIZE"]
+    out = {"_calibration": {"copy_bytes": CALIB_BYTES, "fetch_kib_read": calib["FETCH_SIZE"],
+                            "write_kib_read": calib["WRITE_SIZE"], "fetch_scale": sf,
+                            "write_scale": sw}}
+    for kern, e in sums.items():
+        units = b["stage_units"][e["stage"]]
+        byts = (e["FETCH_SIZE"] * sf + e["WRITE_SIZE"] * sw) * 1024.0
+        out[kern] = {"workload": b["config"]["workload"], "dispatches": e["n"],
+                     "fetch_bytes": e["FETCH_SIZE"] * sf * 1024.0,
+                     "write_bytes": e["WRITE_SIZE"] * sw * 1024.0,
+                     "units": units, "bytes_per_unit": byts / units if units else None}
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out, indent=1))
 
 
 def build_workload(name, ns):
@@ -95,10 +182,15 @@ def main():
     ap.add_argument("--shots-per-step", type=int, default=50)
     ap.add_argument("--workload", default="fbm_two_layer")
     ap.add_argument("--track-shots", type=int, default=2000)
-    ap.add_argument("--cpu-shots", type=int, default=2)
+    ap.add_argument("--cpu-shots", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--flags", type=int, default=0)
+    ap.add_argument("--ingest-pmc", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "BENCH_JSON"),
+                    help="write profiles/traffic.json from two rocprofv3 --pmc runs")
     a = ap.parse_args()
+    if a.ingest_pmc:
+        ingest_pmc(*a.ingest_pmc, os.path.join(ROOT, "profiles", "traffic.json"))
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the path shards by shot range with no "
               "exchange between GPUs, and this build runs on a single MI355X; "
@@ -124,6 +216,13 @@ def main():
     B = a.shots_per_step
     out_dev = dev_alloc(h, B * nr * 8)
     nv_dev = dev_alloc(h, B * 4)
+    # counter calibration: one copy kernel of known size, outside timing
+    c0, c1 = dev_alloc(h, CALIB_BYTES), dev_alloc(h, CALIB_BYTES)
+    h.hipMemset(ctypes.c_void_p(c0), 0, ctypes.c_size_t(CALIB_BYTES))
+    h.hipMemcpy(ctypes.c_void_p(c1), ctypes.c_void_p(c0), ctypes.c_size_t(CALIB_BYTES), 3)
+    h.hipDeviceSynchronize()
+    h.hipFree(ctypes.c_void_p(c0))
+    h.hipFree(ctypes.c_void_p(c1))
 
     def batch(i):
         s0 = (i * B) % len(shots)
@@ -152,17 +251,31 @@ def main():
     value = pairs / dt
 
     stage_ms = {k: v[0] for k, v in prof.items()}
-    dom = max(stage_ms, key=stage_ms.get)
-    ms, launches, units = prof[dom]
-    if dom == "target_accum":
-        achieved = (units * TRIPLE_FLOPS) / (ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None}
-    else:
-        achieved = (units * PAIR_BYTES) / (ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None}
-    roof.update({"kernel": dom, "avg_launch_ms": ms / max(launches, 1), "launches": launches})
+    P_fft = 1 << int(np.ceil(np.log2(2 * nr - 1)))
+
+    def roof_for(stage):
+        ms, launches, units = prof[stage]
+        if ms <= 0 or launches == 0:
+            return None
+        if stage == "target_accum":
+            per, bound, unit, peak = units * TRIPLE_FLOPS / 1e12, "mfma", "TFLOP/s", FP32_PEAK_TFLOPS
+        elif stage == "target_fft":
+            per, bound, unit, peak = (units * target_pair_flops(P_fft) / 1e12, "mfma", "TFLOP/s",
+                                      FP32_PEAK_TFLOPS)
+        else:
+            per, bound, unit, peak = units * PAIR_BYTES / 1e9, "hbm", "GB/s", HBM_PEAK_GBS
+        achieved = per / (ms * 1e-3)
+        r = {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+             "frac": achieved / peak, "traffic": traffic_for(KERNELS[stage], a.workload,
+                                                             units / launches),
+             "kernel": KERNELS[stage], "avg_launch_ms": ms / launches, "launches": launches,
+             "units_per_launch": units / launches}
+        return r
+
+    cands = [k for k in ("target_accum", "target_fft", "surface") if prof[k][1] > 0]
+    dom = max(cands, key=lambda k: stage_ms[k])
+    roof = roof_for(dom)
+    others = {k: roof_for(k) for k in cands if k != dom}
 
     res = {
         "metric": METRIC, "value": value, "unit": "facet·shot pairs/s", "n_gpus": 1,
@@ -178,7 +291,10 @@ def main():
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)),
                        "p99": 1e3 * float(np.percentile(lat, 99))},
         "stage_ms": stage_ms,
+        "stage_launches": {k: v[1] for k, v in prof.items()},
+        "stage_units": {k: v[2] for k, v in prof.items()},
         "roofline": roof,
+        "roofline_other_kernels": others,
     }
     if not a.no_cpu:
         try:

@@ -17,8 +17,12 @@ __version__ = "0.1.0"
 
 def __getattr__(name):
     # the native layer is imported lazily so that CPU-only tooling works
-    # without the shared library
+    # without the shared library; the submodule `simulate` is replaced by
+    # its function of the same name at package level
     if name in ("simulate", "Context", "Simulation"):
-        from . import simulate as _s
-        return getattr(_s, name)
+        import importlib
+        mod = importlib.import_module(".simulate", __name__)
+        for n in ("simulate", "Context", "Simulation"):
+            globals()[n] = getattr(mod, n)
+        return globals()[name]
     raise AttributeError(name)

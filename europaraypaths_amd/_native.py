@@ -25,7 +25,7 @@ EXPORTS = ("erp_ctx_create", "erp_ctx_destroy", "erp_last_error", "erp_mesh_uplo
            "erp_profile_enable", "erp_profile_read")
 
 STAGES = ("mask", "surface", "surface_conv", "target_select", "target_accum", "target_fft",
-          "shot_final", "other")
+          "shot_final", "other", "target_sum")
 
 
 class ErpError(RuntimeError):

@@ -860,15 +860,10 @@ void fft_natural_order(const FftPlan& p, const float2* src, float2* dst, hipStre
 }
 
 // Host driver of the one-workgroup-per-pair chain (see k_tpair_half): n
-// pairs of A (rows of P) -> per-pair products in work (natural order) ->
-// slice sums in partial (nslices rows; slices [sl_beg, sl_end) of pairs of
-// one shot, at most target_half_slice() pairs each) -> per-shot sums of the
-// slices [seg_beg, seg_end) into S, in the order fft_inverse expects.
-void target_spectra_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const int* pbase,
-                         int win, int nr, int n, const float2* Gnat, float2* work,
-                         float2* partial, const int* sl_beg, const int* sl_end, int nslices,
-                         const int* seg_shot, const int* seg_beg, const int* seg_end, int nseg,
-                         float2* S, float scale, hipStream_t st) {
+// pairs of A (rows of P) -> per-pair products in work (natural order).
+void target_pairs_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const int* pbase,
+                       int win, int nr, int n, const float2* Gnat, float2* work,
+                       hipStream_t st) {
     if (!target_half_supported(p, nr))
         throw ErpError{ERP_E_INVALID, "per-pair LDS chain needs P = 2 nr, nr in {4096, 8192, 16384}"};
     const float2* lad = ctx->twiddle.as<float2>();
@@ -889,9 +884,19 @@ void target_spectra_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const 
             break;
     }
     ERP_CHECK_LAUNCH("k_tpair_half");
+}
+
+// Per-shot sums of the per-pair rows of work: slice sums into partial
+// (nslices rows; slice i = pairs [sl_beg[i], sl_end[i]) of one shot, at most
+// target_half_slice() pairs), then the slices [seg_beg, seg_end) of each
+// shot into S, scaled and in the order fft_inverse expects.
+void target_sum_half(const FftPlan& p, const float2* work, float2* partial, const int* sl_beg,
+                     const int* sl_end, int nslices, const int* seg_shot, const int* seg_beg,
+                     const int* seg_end, int nseg, float2* S, float scale, hipStream_t st) {
     const int P2 = p.N / 2;
     hipLaunchKernelGGL(k_sum_slices, dim3((P2 + 255) / 256, nslices), dim3(256), 0, st,
-                       reinterpret_cast<float4*>(partial), o4, sl_beg, sl_end, P2);
+                       reinterpret_cast<float4*>(partial), reinterpret_cast<const float4*>(work),
+                       sl_beg, sl_end, P2);
     ERP_CHECK_LAUNCH("k_sum_slices");
     hipLaunchKernelGGL(k_sum_final, dim3((p.N + 255) / 256, nseg), dim3(256), 0, st, S, partial,
                        seg_shot, seg_beg, seg_end, p.N, scale, p.two_pass ? p.N1 : 0,

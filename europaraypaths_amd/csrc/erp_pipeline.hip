@@ -1555,6 +1555,9 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     const bool spec = mode & ERP_MODE_SPECULAR;
     const int nr = p.nr;
     if (nr <= 0 || nr > 32767) throw ErpError{ERP_E_INVALID, "nr must be in [1, 32767]"};
+    if (flags & ~(uint32_t)ERP_RUN_RAW_SURFACE)
+        throw ErpError{ERP_E_UNSUPPORTED,
+                       "run flags DETERMINISTIC / PHASE_F32 / PHASE_DF are not implemented"};
     if (!spec && !(conv && lin))
         throw ErpError{ERP_E_UNSUPPORTED,
                        "non-specular runs need convolution and convolution_linear "
@@ -1845,10 +1848,11 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                     PT_.begin(ERP_STAGE_TARGET_FFT, st, (double)n);
                     if (half) {
                         // outward trace = -A/sqrt(P): scale -1/sqrt(P)
-                        target_spectra_half(ctx, planP, A, dpb, TWIN, nr, n, g_targ_nat, Z,
-                                            A + (size_t)C * P, dgb, dge, ngroups, dseg,
-                                            dseg + S, dseg + 2 * S, nseg, Sacc,
-                                            -1.0f / kp.sqrtP, st);
+                        target_pairs_half(ctx, planP, A, dpb, TWIN, nr, n, g_targ_nat, Z, st);
+                        PT_.end(st);
+                        PT_.begin(ERP_STAGE_TARGET_SUM, st, (double)n);
+                        target_sum_half(planP, Z, A + (size_t)C * P, dgb, dge, ngroups, dseg,
+                                        dseg + S, dseg + 2 * S, nseg, Sacc, -1.0f / kp.sqrtP, st);
                         PT_.end(st);
                         continue;
                     }

@@ -181,10 +181,11 @@ int erp_wait(erp_job* job);
 #define ERP_STAGE_SURFACE_CONV  2  /* surface range compression             */
 #define ERP_STAGE_TARGET_SELECT 3  /* in-aperture target lists              */
 #define ERP_STAGE_TARGET_ACCUM  4  /* facet x target accumulation kernel    */
-#define ERP_STAGE_TARGET_FFT    5  /* per-target spectra + spectral sum     */
+#define ERP_STAGE_TARGET_FFT    5  /* per-target spectral chain             */
 #define ERP_STAGE_SHOT_FINAL    6  /* Bluestein DFT/crop/IDFT + combine     */
 #define ERP_STAGE_OTHER         7  /* type-0 targets, direct sums           */
-#define ERP_NSTAGES             8
+#define ERP_STAGE_TARGET_SUM    8  /* per-shot sums of per-target spectra   */
+#define ERP_NSTAGES             9
 /* on != 0 enables the timers and clears the counters. */
 int erp_profile_enable(erp_ctx* ctx, int on);
 /* Totals since the last enable: summed milliseconds, number of timed

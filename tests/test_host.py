@@ -249,3 +249,19 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, cwd=ROOT)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_package_level_entry_points():
+    import europaraypaths_amd as e
+    assert callable(e.simulate) and e.simulate.__name__ == "simulate"
+    assert e.Context is _native.Context
+    assert callable(e.gen_params) and callable(e.compile_rdrgrm)
+
+
+def test_cli_rejects_missing_output_directory(tmp_path):
+    """sim.cu:97 / file_io.cu:266-281: the output directory must exist; the
+    drop-in CLI reports it and exits with status 1 before touching a GPU."""
+    r = subprocess.run([sys.executable, "-m", "europaraypaths_amd.sim", "p.json", "f.fct",
+                        "t.txt", str(tmp_path / "missing")], capture_output=True, text=True,
+                       cwd=ROOT)
+    assert r.returncode == 1 and "Directory not found" in r.stderr
