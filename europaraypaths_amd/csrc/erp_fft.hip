@@ -568,8 +568,10 @@ __global__ void k_sum_rows(float2* __restrict__ S, const float2* __restrict__ sr
 // without HBM round trips of the intermediate transforms.  E and O are
 // transformed twice (once for z, once for the product) and DFT(z) waits in
 // the output row, so one register array at most is live across a transform.
-// Gn holds the chirp spectrum in natural order as (G[2k], G[2k+1]) pairs;
-// out rows are in natural order.
+// Gn holds the chirp spectrum in natural order as (G[2k], G[2k+1]) pairs.
+// Each output row (P complex) holds two planes: out[k] = product at
+// frequency 2k, out[n + k] = product at frequency 2k + 1, so every store is
+// a full, contiguous line.
 // ---------------------------------------------------------------------------
 template <int NTB>
 __device__ __forceinline__ void load_window(float2* s, const float2* __restrict__ a, int base,
@@ -590,7 +592,7 @@ template <int NTB>
 __global__ void __launch_bounds__(NTB) k_tpair_half(const float2* __restrict__ A, int P,
                                                     const int* __restrict__ pbase, int win,
                                                     const float4* __restrict__ Gn,
-                                                    float4* __restrict__ out,
+                                                    float2* __restrict__ out,
                                                     const float2* __restrict__ lad) {
     constexpr int n = NTB * PPT;
     constexpr int log2n = NTB == 256 ? 12 : (NTB == 512 ? 13 : 14);
@@ -599,17 +601,17 @@ __global__ void __launch_bounds__(NTB) k_tpair_half(const float2* __restrict__ A
     const int base = pbase[p];
     const float2* a = A + (size_t)p * P;
     const float2* mod = lad + 2 * n;  // m(j) = lad[2n + j]
-    float2* o2 = reinterpret_cast<float2*>(out + (size_t)p * n);
+    float2* o2 = out + (size_t)p * P;  // even plane [0, n), odd plane [n, 2n)
     float2 r1[PPT];
     // Eight transforms, one code copy.  Phase: input -> transform -> use
     //   0: a          -> E   -> s = E G_e
     //   1: s          -> u   -> r1 = u              (inverse)
     //   2: a m        -> O   -> s = O G_o
     //   3: s          -> v   -> r1 = u - conj(m) v = z   (inverse)
-    //   4: z          -> Ze  -> even output slots = Ze
+    //   4: z          -> Ze  -> even plane = Ze
     //   5: z m        -> Zo  -> r1 = Zo
-    //   6: a          -> E   -> even slots = E Ze
-    //   7: a m        -> O   -> odd slots = O Zo
+    //   6: a          -> E   -> even plane = E Ze
+    //   7: a m        -> O   -> odd plane = O Zo
 #pragma unroll 1
     for (int ph = 0; ph < 8; ++ph) {
         if (ph == 0 || ph == 2 || ph == 6 || ph == 7) {
@@ -645,50 +647,56 @@ __global__ void __launch_bounds__(NTB) k_tpair_half(const float2* __restrict__ A
 #pragma unroll
             for (int q = 0; q < PPT; ++q) {
                 const int i = tid + q * NTB;
-                o2[2 * i] = s[sw(i)];
+                o2[i] = s[sw(i)];
             }
         } else if (ph == 6) {
 #pragma unroll
             for (int q = 0; q < PPT; ++q) {
                 const int i = tid + q * NTB;
-                o2[2 * i] = cmul(s[sw(i)], o2[2 * i]);
+                o2[i] = cmul(s[sw(i)], o2[i]);
             }
         } else {
 #pragma unroll
             for (int q = 0; q < PPT; ++q) {
                 const int i = tid + q * NTB;
-                o2[2 * i + 1] = cmul(s[sw(i)], r1[q]);
+                o2[n + i] = cmul(s[sw(i)], r1[q]);
             }
         }
         __syncthreads();
     }
 }
 
-// Two-level deterministic sum of per-pair rows (natural order, P complex).
+// Two-level deterministic sum of per-pair rows (P complex each, as the two
+// planes written by k_tpair_half: [0, P/2) even, [P/2, P) odd frequencies).
 // Level 1: partial[slice] = sum of the rows of one slice (at most SLICE rows
-// of one shot), two complex values per thread, eight rows in flight.
+// of one shot) in natural order, two frequencies per thread, eight rows in
+// flight.
 constexpr int SLICE = 32;
 __global__ void __launch_bounds__(256) k_sum_slices(float4* __restrict__ partial,
-                                                    const float4* __restrict__ rows,
+                                                    const float2* __restrict__ rows,
                                                     const int* __restrict__ sl_beg,
                                                     const int* __restrict__ sl_end, int P2) {
-    const int k = blockIdx.x * 256 + threadIdx.x;  // float4 column, P2 = P / 2
+    const int k = blockIdx.x * 256 + threadIdx.x;  // frequency pair (2k, 2k+1), P2 = P / 2
     if (k >= P2) return;
     const int b = sl_beg[blockIdx.y], e = sl_end[blockIdx.y];
+    const size_t P = 2 * (size_t)P2;
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     int r = b;
     for (; r + 8 <= e; r += 8) {
-        float4 v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = rows[(size_t)(r + u) * P2 + k];
+        float2 ve[8], vo[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            acc.x += v[u].x; acc.y += v[u].y; acc.z += v[u].z; acc.w += v[u].w;
+            ve[u] = rows[(size_t)(r + u) * P + k];
+            vo[u] = rows[(size_t)(r + u) * P + P2 + k];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            acc.x += ve[u].x; acc.y += ve[u].y; acc.z += vo[u].x; acc.w += vo[u].y;
         }
     }
     for (; r < e; ++r) {
-        const float4 v = rows[(size_t)r * P2 + k];
-        acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
+        const float2 ve = rows[(size_t)r * P + k], vo = rows[(size_t)r * P + P2 + k];
+        acc.x += ve.x; acc.y += ve.y; acc.z += vo.x; acc.w += vo.y;
     }
     partial[(size_t)blockIdx.y * P2 + k] = acc;
 }
@@ -860,7 +868,7 @@ void fft_natural_order(const FftPlan& p, const float2* src, float2* dst, hipStre
 }
 
 // Host driver of the one-workgroup-per-pair chain (see k_tpair_half): n
-// pairs of A (rows of P) -> per-pair products in work (natural order).
+// pairs of A (rows of P) -> per-pair products in work (even/odd planes).
 void target_pairs_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const int* pbase,
                        int win, int nr, int n, const float2* Gnat, float2* work,
                        hipStream_t st) {
@@ -868,7 +876,7 @@ void target_pairs_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const in
         throw ErpError{ERP_E_INVALID, "per-pair LDS chain needs P = 2 nr, nr in {4096, 8192, 16384}"};
     const float2* lad = ctx->twiddle.as<float2>();
     const float4* G4 = reinterpret_cast<const float4*>(Gnat);
-    float4* o4 = reinterpret_cast<float4*>(work);
+    float2* o4 = work;
     switch (nr) {
         case 4096:
             hipLaunchKernelGGL(k_tpair_half<256>, dim3(n), dim3(256), 0, st, A, p.N, pbase, win,
@@ -895,8 +903,7 @@ void target_sum_half(const FftPlan& p, const float2* work, float2* partial, cons
                      const int* seg_end, int nseg, float2* S, float scale, hipStream_t st) {
     const int P2 = p.N / 2;
     hipLaunchKernelGGL(k_sum_slices, dim3((P2 + 255) / 256, nslices), dim3(256), 0, st,
-                       reinterpret_cast<float4*>(partial), reinterpret_cast<const float4*>(work),
-                       sl_beg, sl_end, P2);
+                       reinterpret_cast<float4*>(partial), work, sl_beg, sl_end, P2);
     ERP_CHECK_LAUNCH("k_sum_slices");
     hipLaunchKernelGGL(k_sum_final, dim3((p.N + 255) / 256, nseg), dim3(256), 0, st, S, partial,
                        seg_shot, seg_beg, seg_end, p.N, scale, p.two_pass ? p.N1 : 0,

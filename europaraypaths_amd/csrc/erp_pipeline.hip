@@ -344,6 +344,10 @@ struct Rec {
     float* F;       // cap: sqrt(P n Grefr G1 [fRfrC]) lam^2 / (K Itd)
     float* fRfrC;   // cap
     float* ang;     // 3 x cap: Ith, Iph, Rth
+    // the same facet record packed for k_target_accum_fast, 4 float4 per
+    // record (four wide loads instead of fourteen narrow ones):
+    // (x, y, z, F) (nx, ny, nz, ux) (uy, uz, vx, vy) (vz, 0, Itd lo, Itd hi)
+    float4* q;
     int64_t cap;
 };
 
@@ -480,9 +484,20 @@ __global__ void __launch_bounds__(NT) k_surface(
             // target amplitude factor: sqrt(P n Grefr G1 fRfrC) lam^2/(K Itd)
             const float fac = lossless ? 1.f : fRfrC;
             // NoConv specular keeps fRe here for reflRadarSignal's direct sum
-            rec.F[r] = (spec && !(kp.mode & ERP_MODE_CONVOLUTION))
-                           ? fRe
-                           : sqrtf(kp.P * kp.nf * Grefr * G1 * fac) * kp.lam * kp.lam * kp.invK / Itdf;
+            const float Fr = (spec && !(kp.mode & ERP_MODE_CONVOLUTION))
+                                 ? fRe
+                                 : sqrtf(kp.P * kp.nf * Grefr * G1 * fac) * kp.lam * kp.lam *
+                                       kp.invK / Itdf;
+            rec.F[r] = Fr;
+            if (rec.q) {
+                const long long ib = __double_as_longlong(Itd);
+                float4* q = rec.q + 4 * r;
+                q[0] = make_float4(g[0], g[1], g[2], Fr);
+                q[1] = make_float4(g[3], g[4], g[5], g[6]);
+                q[2] = make_float4(g[7], g[8], g[9], g[10]);
+                q[3] = make_float4(g[11], 0.f, __int_as_float((int)(ib & 0xffffffffll)),
+                                   __int_as_float((int)(ib >> 32)));
+            }
             rec.fRfrC[r] = fRfrC;
             rec.ang[r] = Ith;
             rec.ang[rec.cap + r] = Iph;
@@ -1089,11 +1104,12 @@ __global__ void __launch_bounds__(NTT) k_target_accum_fast(
     const double inv_dr = kp.inv_dr, n_d = kp.n_d, two_cyc = 2.0 * kp.inv_cyc;
     const float a = kp.piLlam;
     const float gscale = kp.Dmax;
-    const int64_t cap = rec.cap;
     for (int k = threadIdx.x; k < V; k += NTT) {
         const int64_t r = r0 + k;
-        const float fx = rec.geo[r], fy = rec.geo[cap + r], fz = rec.geo[2 * cap + r];
-        const double Itd = rec.itd[r];
+        const float4 p0 = rec.q[4 * r], p1 = rec.q[4 * r + 1], p2 = rec.q[4 * r + 2],
+                     p3 = rec.q[4 * r + 3];
+        const float fx = p0.x, fy = p0.y, fz = p0.z;
+        const double Itd = __hiloint2double(__float_as_int(p3.w), __float_as_int(p3.z));
         const double dx = tx - (double)fx, dy = ty - (double)fy, dz = tz - (double)fz;
         const double t2 = fma(dx, dx, fma(dy, dy, dz * dz));
         const float rs = rsqrtf((float)t2);
@@ -1115,9 +1131,9 @@ __global__ void __launch_bounds__(NTT) k_target_accum_fast(
             w = (float)(xq - (double)b);
         }
         if (!valid) continue;
-        const float nx = rec.geo[3 * cap + r], ny = rec.geo[4 * cap + r], nz = rec.geo[5 * cap + r];
-        const float ux = rec.geo[6 * cap + r], uy = rec.geo[7 * cap + r], uz = rec.geo[8 * cap + r];
-        const float vx = rec.geo[9 * cap + r], vy = rec.geo[10 * cap + r], vz = rec.geo[11 * cap + r];
+        const float nx = p1.x, ny = p1.y, nz = p1.z;
+        const float ux = p1.w, uy = p2.x, uz = p2.y;
+        const float vx = p2.z, vy = p2.w, vz = p3.x;
         const float Ix = (float)dx * rs, Iy = (float)dy * rs, Iz = (float)dz * rs;
         const float cT = fmaxf(-1.f, fminf(1.f, -(Ix * nx + Iy * ny + Iz * nz)));
         const float Iu = Ix * ux + Iy * uy + Iz * uz;
@@ -1133,7 +1149,7 @@ __global__ void __launch_bounds__(NTT) k_target_accum_fast(
         sinc_nd(a * sTT * cph, &n3, &d3);
         sinc_nd(a * sTT * sph, &n4, &d4);
         const float den = fabsf((d1 * d2) * (d3 * d4));
-        float amp = rec.F[r] * gscale * fabsf((n1 * n2) * (n3 * n4)) *
+        float amp = p0.w * gscale * fabsf((n1 * n2) * (n3 * n4)) *
                     __builtin_amdgcn_rcpf(den) * rs;
         if (!lossless) {
             float expo = 0.f, tot = 0.f;
@@ -1667,7 +1683,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         Rec rec{};
         if (need_rec && Vtot > 0) {
             const size_t cap = (size_t)Vtot;
-            const size_t bytes = cap * (12 * 4 + 8 + 4 + 4 + 3 * 4) + 256 * 6;
+            const size_t bytes = cap * (12 * 4 + 8 + 4 + 4 + 3 * 4 + 64) + 256 * 7;
             W.rec.ensure(bytes);
             char* b = W.rec.as<char>();
             auto take = [&](size_t n) { char* r = b; b += (n + 255) & ~(size_t)255; return r; };
@@ -1676,6 +1692,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
             rec.F = (float*)take(cap * 4);
             rec.fRfrC = (float*)take(cap * 4);
             rec.ang = (float*)take(cap * 3 * 4);
+            rec.q = (float4*)take(cap * 64);
             rec.cap = (int64_t)cap;
         }
 
