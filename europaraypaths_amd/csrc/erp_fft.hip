@@ -562,20 +562,63 @@ __global__ void k_sum_rows(float2* __restrict__ S, const float2* __restrict__ sr
 //     X[2k] = DFT_nr(x)[k],   X[2k+1] = DFT_nr(x m)[k];
 // and the upper half of an (unnormalised) inverse P-point transform is
 //     y[j + nr] = IDFT_nr(Y_even)[j] - conj(m(j)) IDFT_nr(Y_odd)[j].
-// Both the inward trace a and the cropped z = y[nr + j] live on [0, nr), so
-//     z = IDFT(E G_e) - conj(m) IDFT(O G_o),   E = DFT(a), O = DFT(a m)
-//     out[2k] = E[k] DFT(z)[k],                out[2k+1] = O[k] DFT(z m)[k]
-// without HBM round trips of the intermediate transforms.  E and O are
-// transformed twice (once for z, once for the product) and DFT(z) waits in
-// the output row, so one register array at most is live across a transform.
+// Both the inward trace a and the cropped z = y[nr + j] live on [0, nr).
+// With E = DFT(a), O = DFT(a m), u = IDFT(E G_e), v = IDFT(O G_o) and
+// z = u - conj(m) v, and since DFT(IDFT(x)) = nr x and |m| = 1:
+//     Ze = DFT(z)   = nr E G_e - T1,   T1 = DFT(conj(m) v)
+//     Zo = DFT(z m) = T2 - nr O G_o,   T2 = DFT(m u)
+//     out[2k] = E Ze,  out[2k+1] = O Zo.
+// Seven nr-point transforms per pair (E twice), no HBM round trip of an
+// intermediate, and at most one register array held across a transform.
 // Gn holds the chirp spectrum in natural order as (G[2k], G[2k+1]) pairs.
 // Each output row (P complex) holds two planes: out[k] = product at
-// frequency 2k, out[n + k] = product at frequency 2k + 1, so every store is
-// a full, contiguous line.
+// frequency 2k, out[n + k] = product at frequency 2k + 1.
 // ---------------------------------------------------------------------------
+
+// nr-point transform held in registers: x[q] = element tid + q NTB in and
+// out (natural order).  The first stage reads its inputs from registers and
+// the last stage leaves its outputs there, so only the middle stages go
+// through LDS.  Starts with a barrier (earlier readers of s are done).
 template <int NTB>
-__device__ __forceinline__ void load_window(float2* s, const float2* __restrict__ a, int base,
-                                            int win, const float2* __restrict__ mod) {
+__device__ __forceinline__ void fft_regs(float2* s, float2 (&x)[PPT], bool inv,
+                                         const float2* __restrict__ lad) {
+    constexpr int n = NTB * PPT;
+    constexpr int L = NTB == 256 ? 12 : (NTB == 512 ? 13 : 14);
+    constexpr int r0 = L & 3;
+    constexpr int R0 = r0 == 0 ? 16 : (1 << r0);
+    constexpr int Q0 = PPT / R0;
+    constexpr int MID = (L - (r0 == 0 ? 4 : r0)) / 4 - 1;
+    int tid = opaque_tid();
+    __syncthreads();
+    // first stage, Ls = 1 (no twiddles): input j + i n/R0 is x[q + Q0 i]
+#pragma unroll
+    for (int q = 0; q < Q0; ++q) {
+        float2 v[R0];
+#pragma unroll
+        for (int i = 0; i < R0; ++i) v[i] = x[q + Q0 * i];
+        dft_r<R0>(v, inv);
+        const int j = tid + q * NTB;
+#pragma unroll
+        for (int i = 0; i < R0; ++i) s[sw(j * R0 + i)] = v[i];
+    }
+    __syncthreads();
+    int Ls = R0;
+#pragma unroll
+    for (int st = 0; st < MID; ++st) {
+        stockham_stage<16, NTB>(s, n, Ls, 1, inv, lad);
+        Ls *= 16;
+    }
+    // last stage, Ls = NTB: thread tid owns outputs tid + i NTB
+    tid = opaque_tid();
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) x[i] = s[sw(tid + i * NTB)];
+    apply_twiddles<16>(x, lad, n, tid, inv);
+    dft_r<16>(x, inv);
+}
+
+template <int NTB>
+__device__ __forceinline__ void load_window(float2 (&x)[PPT], const float2* __restrict__ a,
+                                            int base, int win, const float2* __restrict__ mod) {
     const int tid = opaque_tid();
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
@@ -583,9 +626,8 @@ __device__ __forceinline__ void load_window(float2* s, const float2* __restrict_
         const bool live = base < 0 || (i >= base && i < base + win);
         float2 v = live ? a[i] : make_float2(0.f, 0.f);
         if (mod) v = cmul(v, mod[i]);
-        s[sw(i)] = v;
+        x[q] = v;
     }
-    __syncthreads();
 }
 
 template <int NTB>
@@ -595,74 +637,50 @@ __global__ void __launch_bounds__(NTB) k_tpair_half(const float2* __restrict__ A
                                                     float2* __restrict__ out,
                                                     const float2* __restrict__ lad) {
     constexpr int n = NTB * PPT;
-    constexpr int log2n = NTB == 256 ? 12 : (NTB == 512 ? 13 : 14);
     __shared__ float2 s[n];
     const int p = blockIdx.x;
     const int base = pbase[p];
     const float2* a = A + (size_t)p * P;
     const float2* mod = lad + 2 * n;  // m(j) = lad[2n + j]
     float2* o2 = out + (size_t)p * P;  // even plane [0, n), odd plane [n, 2n)
-    float2 r1[PPT];
-    // Eight transforms, one code copy.  Phase: input -> transform -> use
-    //   0: a          -> E   -> s = E G_e
-    //   1: s          -> u   -> r1 = u              (inverse)
-    //   2: a m        -> O   -> s = O G_o
-    //   3: s          -> v   -> r1 = u - conj(m) v = z   (inverse)
-    //   4: z          -> Ze  -> even plane = Ze
-    //   5: z m        -> Zo  -> r1 = Zo
-    //   6: a          -> E   -> even plane = E Ze
-    //   7: a m        -> O   -> odd plane = O Zo
+    const float fn = (float)n;
+    float2 x[PPT], h[PPT];
+    // Phase: input -> transform -> use
+    //   0: a        -> E   -> x = E G_e
+    //   1: x        -> u   -> x = m u                 (inverse)
+    //   2: x        -> T2  -> h = T2
+    //   3: a m      -> O   -> odd plane = O (T2 - n O G_o); x = O G_o
+    //   4: x        -> v   -> x = conj(m) v           (inverse)
+    //   5: x        -> T1  -> h = T1
+    //   6: a        -> E   -> even plane = E (n E G_e - T1)
 #pragma unroll 1
-    for (int ph = 0; ph < 8; ++ph) {
-        if (ph == 0 || ph == 2 || ph == 6 || ph == 7) {
-            load_window<NTB>(s, a, base, win, (ph == 2 || ph == 7) ? mod : nullptr);
-        } else if (ph == 4 || ph == 5) {
-            const int tid = opaque_tid();
-#pragma unroll
-            for (int q = 0; q < PPT; ++q) {
-                const int i = tid + q * NTB;
-                s[sw(i)] = ph == 4 ? r1[q] : cmul(r1[q], mod[i]);
-            }
-            __syncthreads();
-        }
-        lds_fft<NTB>(s, n, log2n, 1, ph == 1 || ph == 3, lad);
+    for (int ph = 0; ph < 7; ++ph) {
+        if (ph == 0 || ph == 3 || ph == 6) load_window<NTB>(x, a, base, win, ph == 3 ? mod : nullptr);
+        fft_regs<NTB>(s, x, ph == 1 || ph == 4, lad);
         const int tid = opaque_tid();
-        if (ph == 0 || ph == 2) {
 #pragma unroll
-            for (int q = 0; q < PPT; ++q) {
-                const int i = tid + q * NTB;
+        for (int q = 0; q < PPT; ++q) {
+            const int i = tid + q * NTB;
+            if (ph == 0) {
                 const float4 g = Gn[i];
-                s[sw(i)] = cmul(s[sw(i)], ph == 0 ? make_float2(g.x, g.y) : make_float2(g.z, g.w));
-            }
-        } else if (ph == 1 || ph == 5) {
-#pragma unroll
-            for (int q = 0; q < PPT; ++q) r1[q] = s[sw(tid + q * NTB)];
-        } else if (ph == 3) {
-#pragma unroll
-            for (int q = 0; q < PPT; ++q) {
-                const int i = tid + q * NTB;
-                r1[q] = csub(r1[q], cmul(conjf2(mod[i]), s[sw(i)]));
-            }
-        } else if (ph == 4) {
-#pragma unroll
-            for (int q = 0; q < PPT; ++q) {
-                const int i = tid + q * NTB;
-                o2[i] = s[sw(i)];
-            }
-        } else if (ph == 6) {
-#pragma unroll
-            for (int q = 0; q < PPT; ++q) {
-                const int i = tid + q * NTB;
-                o2[i] = cmul(s[sw(i)], o2[i]);
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < PPT; ++q) {
-                const int i = tid + q * NTB;
-                o2[n + i] = cmul(s[sw(i)], r1[q]);
+                x[q] = cmul(x[q], make_float2(g.x, g.y));
+            } else if (ph == 1) {
+                x[q] = cmul(x[q], mod[i]);
+            } else if (ph == 2 || ph == 5) {
+                h[q] = x[q];
+            } else if (ph == 3) {
+                const float4 g = Gn[i];
+                const float2 og = cmul(x[q], make_float2(g.z, g.w));
+                o2[n + i] = cmul(x[q], make_float2(h[q].x - fn * og.x, h[q].y - fn * og.y));
+                x[q] = og;
+            } else if (ph == 4) {
+                x[q] = cmul(x[q], conjf2(mod[i]));
+            } else {
+                const float4 g = Gn[i];
+                const float2 eg = cmul(x[q], make_float2(g.x, g.y));
+                o2[i] = cmul(x[q], make_float2(fn * eg.x - h[q].x, fn * eg.y - h[q].y));
             }
         }
-        __syncthreads();
     }
 }
 
