@@ -348,7 +348,7 @@ __global__ void __launch_bounds__(NT) k_fft_rows(float2* __restrict__ data, int 
 // ---------------------------------------------------------------------------
 // Fused per-target spectral chain.  For every (shot, target) pair p with
 // inward trace A_p (rows of length P, nonzero only in the bin window
-// [base_p, base_p + win) unless base_p < 0) it produces
+// [base_p, base_p + win_p) unless base_p < 0) it produces
 //     X_p = FFT(A_p),  y_p = IFFT(X_p G),  z_p[n] = y_p[n + nr] (n < nr),
 //     Z_p = FFT(z_p)
 // and sums X_p Z_p over the pairs of each shot.  Two-pass sizes use four
@@ -362,14 +362,14 @@ __global__ void __launch_bounds__(NT) k_fft_rows(float2* __restrict__ data, int 
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(NT) k_tfft_cols_fwd(float2* __restrict__ data, int N1, int N2,
                                                       int log2N2, const int* __restrict__ pbase,
-                                                      int win, int nr,
+                                                      const int* __restrict__ pwin, int nr,
                                                       const float2* __restrict__ tw,
                                                       const float2* __restrict__ it) {
     __shared__ float2 s[NPTS];
     __shared__ float2 twl[TWL];
     const int CT = NPTS / N2;
     const int c0 = blockIdx.x * CT;
-    const int base = pbase[blockIdx.y];
+    const int base = pbase[blockIdx.y], win = pwin[blockIdx.y];
     float2* g = data + (size_t)blockIdx.y * N1 * N2;
     const float2* tws = stage_ladder(twl, tw, N2);
     for (int idx = threadIdx.x; idx < NPTS; idx += NT) {
@@ -485,13 +485,14 @@ __global__ void __launch_bounds__(NT) k_tfft_rows_mac(const float2* __restrict__
 // Single-pass sizes (P <= 4096): the whole chain for one pair in one
 // workgroup; writes the product X_p Z_p.
 __global__ void __launch_bounds__(NT) k_tfft_small(const float2* __restrict__ A, int P, int log2P,
-                                                   const int* __restrict__ pbase, int win, int nr,
+                                                   const int* __restrict__ pbase,
+                                                   const int* __restrict__ pwin, int nr,
                                                    const float2* __restrict__ G,
                                                    float2* __restrict__ prod,
                                                    const float2* __restrict__ tw) {
     __shared__ float2 s[NPTS];
     const int p = blockIdx.x;
-    const int base = pbase[p];
+    const int base = pbase[p], win = pwin[p];
     const float2* a = A + (size_t)p * P;
     for (int n = threadIdx.x; n < P; n += NT) {
         const bool live = (n < nr) && (base < 0 || (n >= base && n < base + win));
@@ -575,12 +576,19 @@ __global__ void k_sum_rows(float2* __restrict__ S, const float2* __restrict__ sr
 // frequency 2k, out[n + k] = product at frequency 2k + 1.
 // ---------------------------------------------------------------------------
 
-// nr-point transform held in registers: x[q] = element tid + q NTB in and
-// out (natural order).  The first stage reads its inputs from registers and
-// the last stage leaves its outputs there, so only the middle stages go
-// through LDS.  Starts with a barrier (earlier readers of s are done).
+// LDS index with one float2 of padding per 16: for every access pattern of
+// fft_regs the padded index is a per-thread base plus a compile-time
+// constant, so the addresses cost no per-access arithmetic, and each group
+// of 16 lanes still touches 16 distinct 8-byte bank pairs.
+__device__ __forceinline__ int pad16(int a) { return a + (a >> 4); }
+
+// Forward nr-point transform held in registers: x[q] = element tid + q NTB
+// in and out (natural order); the inverse is conj(fft(conj(x))).  The first
+// stage reads its inputs from registers and the last stage leaves its
+// outputs there, so only the middle stages go through LDS.  Starts with a
+// barrier (earlier readers of s are done).  s holds n + n/16 float2.
 template <int NTB>
-__device__ __forceinline__ void fft_regs(float2* s, float2 (&x)[PPT], bool inv,
+__device__ __forceinline__ void fft_regs(float2* s, float2 (&x)[PPT],
                                          const float2* __restrict__ lad) {
     constexpr int n = NTB * PPT;
     constexpr int L = NTB == 256 ? 12 : (NTB == 512 ? 13 : 14);
@@ -590,30 +598,52 @@ __device__ __forceinline__ void fft_regs(float2* s, float2 (&x)[PPT], bool inv,
     constexpr int MID = (L - (r0 == 0 ? 4 : r0)) / 4 - 1;
     int tid = opaque_tid();
     __syncthreads();
-    // first stage, Ls = 1 (no twiddles): input j + i n/R0 is x[q + Q0 i]
+    // first stage, Ls = 1 (no twiddles): input j + i n/R0 is x[q + Q0 i];
+    // output j R0 + i
 #pragma unroll
     for (int q = 0; q < Q0; ++q) {
         float2 v[R0];
 #pragma unroll
         for (int i = 0; i < R0; ++i) v[i] = x[q + Q0 * i];
-        dft_r<R0>(v, inv);
+        dft_r<R0>(v, false);
         const int j = tid + q * NTB;
+        float2* d = s + pad16(j * R0);
 #pragma unroll
-        for (int i = 0; i < R0; ++i) s[sw(j * R0 + i)] = v[i];
+        for (int i = 0; i < R0; ++i) d[i] = v[i];  // j R0 + i stays in one 16-block
     }
     __syncthreads();
+    // middle radix-16 stages: input tid + i n/16, output (tid - k) 16 + k + i Ls
     int Ls = R0;
 #pragma unroll
     for (int st = 0; st < MID; ++st) {
-        stockham_stage<16, NTB>(s, n, Ls, 1, inv, lad);
+        tid = opaque_tid();
+        float2 v[16];
+        const float2* src = s + pad16(tid);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = src[i * (NTB + NTB / 16)];
+        const int k = tid & (Ls - 1);
+        apply_twiddles<16>(v, lad, Ls * 16, k, false);
+        dft_r<16>(v, false);
+        __syncthreads();
+        const int dst = (tid - k) * 16 + k;
+        if (Ls >= 16) {
+            float2* d = s + pad16(dst);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) d[i * (Ls + Ls / 16)] = v[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) s[pad16(dst + i * Ls)] = v[i];
+        }
+        __syncthreads();
         Ls *= 16;
     }
     // last stage, Ls = NTB: thread tid owns outputs tid + i NTB
     tid = opaque_tid();
+    const float2* src = s + pad16(tid);
 #pragma unroll
-    for (int i = 0; i < PPT; ++i) x[i] = s[sw(tid + i * NTB)];
-    apply_twiddles<16>(x, lad, n, tid, inv);
-    dft_r<16>(x, inv);
+    for (int i = 0; i < PPT; ++i) x[i] = src[i * (NTB + NTB / 16)];
+    apply_twiddles<16>(x, lad, n, tid, false);
+    dft_r<16>(x, false);
 }
 
 template <int NTB>
@@ -632,49 +662,51 @@ __device__ __forceinline__ void load_window(float2 (&x)[PPT], const float2* __re
 
 template <int NTB>
 __global__ void __launch_bounds__(NTB) k_tpair_half(const float2* __restrict__ A, int P,
-                                                    const int* __restrict__ pbase, int win,
+                                                    const int* __restrict__ pbase,
+                                                    const int* __restrict__ pwin,
                                                     const float4* __restrict__ Gn,
                                                     float2* __restrict__ out,
                                                     const float2* __restrict__ lad) {
     constexpr int n = NTB * PPT;
-    __shared__ float2 s[n];
+    __shared__ float2 s[n + n / 16];
     const int p = blockIdx.x;
-    const int base = pbase[p];
+    const int base = pbase[p], win = pwin[p];
     const float2* a = A + (size_t)p * P;
     const float2* mod = lad + 2 * n;  // m(j) = lad[2n + j]
     float2* o2 = out + (size_t)p * P;  // even plane [0, n), odd plane [n, 2n)
     const float fn = (float)n;
     float2 x[PPT], h[PPT];
-    // Phase: input -> transform -> use
-    //   0: a        -> E   -> x = E G_e
-    //   1: x        -> u   -> x = m u                 (inverse)
-    //   2: x        -> T2  -> h = T2
-    //   3: a m      -> O   -> odd plane = O (T2 - n O G_o); x = O G_o
-    //   4: x        -> v   -> x = conj(m) v           (inverse)
-    //   5: x        -> T1  -> h = T1
-    //   6: a        -> E   -> even plane = E (n E G_e - T1)
+    // Phase: input -> forward transform -> use; an inverse transform is
+    // fft(conj(.)) with the conjugates folded into the neighbouring steps
+    //   0: a          -> E        -> x = conj(E G_e)
+    //   1: x          -> conj(u)  -> x = m u
+    //   2: x          -> T2       -> h = T2
+    //   3: a m        -> O        -> odd plane = O (T2 - n O G_o); x = conj(O G_o)
+    //   4: x          -> conj(v)  -> x = conj(m) v
+    //   5: x          -> T1       -> h = T1
+    //   6: a          -> E        -> even plane = E (n E G_e - T1)
 #pragma unroll 1
     for (int ph = 0; ph < 7; ++ph) {
         if (ph == 0 || ph == 3 || ph == 6) load_window<NTB>(x, a, base, win, ph == 3 ? mod : nullptr);
-        fft_regs<NTB>(s, x, ph == 1 || ph == 4, lad);
+        fft_regs<NTB>(s, x, lad);
         const int tid = opaque_tid();
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
             const int i = tid + q * NTB;
             if (ph == 0) {
                 const float4 g = Gn[i];
-                x[q] = cmul(x[q], make_float2(g.x, g.y));
+                x[q] = conjf2(cmul(x[q], make_float2(g.x, g.y)));
             } else if (ph == 1) {
-                x[q] = cmul(x[q], mod[i]);
+                x[q] = cmul(conjf2(x[q]), mod[i]);
             } else if (ph == 2 || ph == 5) {
                 h[q] = x[q];
             } else if (ph == 3) {
                 const float4 g = Gn[i];
                 const float2 og = cmul(x[q], make_float2(g.z, g.w));
                 o2[n + i] = cmul(x[q], make_float2(h[q].x - fn * og.x, h[q].y - fn * og.y));
-                x[q] = og;
+                x[q] = conjf2(og);
             } else if (ph == 4) {
-                x[q] = cmul(x[q], conjf2(mod[i]));
+                x[q] = conjf2(cmul(x[q], mod[i]));
             } else {
                 const float4 g = Gn[i];
                 const float2 eg = cmul(x[q], make_float2(g.x, g.y));
@@ -831,14 +863,14 @@ void fft_inverse(erp_ctx* ctx, const FftPlan& p, float2* data, int batch, hipStr
 // Host driver of the fused chain for n pairs (see the kernel comment).  A and
 // Y hold n rows of P; the work buffer holds max(ngroups, n) rows.
 void target_spectra_fused(erp_ctx* ctx, const FftPlan& p, float2* A, float2* Y,
-                          const int* pbase, int win, int nr, int n, const float2* G,
+                          const int* pbase, const int* pwin, int nr, int n, const float2* G,
                           float2* work, int ngroups, const int* gbeg, const int* gend,
                           const int* gseg_shot, const int* gseg_beg, const int* gseg_end,
                           int nseg, float2* S, float scale, hipStream_t st) {
     const float2* tw = ctx->twiddle.as<float2>();
     const int P = p.N;
     if (!p.two_pass) {
-        hipLaunchKernelGGL(k_tfft_small, dim3(n), dim3(NT), 0, st, A, P, p.log2N, pbase, win, nr,
+        hipLaunchKernelGGL(k_tfft_small, dim3(n), dim3(NT), 0, st, A, P, p.log2N, pbase, pwin, nr,
                            G, work, tw);
         ERP_CHECK_LAUNCH("k_tfft_small");
         // groups are unused here: sum the per-pair products per shot segment
@@ -852,7 +884,7 @@ void target_spectra_fused(erp_ctx* ctx, const FftPlan& p, float2* A, float2* Y,
     if (nr % p.N1 != 0) throw ErpError{ERP_E_INVALID, "fused target chain needs nr % N1 == 0"};
     const float2* it = inter_table(ctx, p, st);
     hipLaunchKernelGGL(k_tfft_cols_fwd, dim3(p.N1 / CT, n), dim3(NT), 0, st, A, p.N1, p.N2, l2,
-                       pbase, win, nr, tw, it);
+                       pbase, pwin, nr, tw, it);
     ERP_CHECK_LAUNCH("k_tfft_cols_fwd");
     hipLaunchKernelGGL(k_tfft_rows_conv, dim3(p.N2 / RT, n), dim3(NT), 0, st, A, Y, p.N1, l1,
                        p.N2, G, tw, it);
@@ -888,7 +920,7 @@ void fft_natural_order(const FftPlan& p, const float2* src, float2* dst, hipStre
 // Host driver of the one-workgroup-per-pair chain (see k_tpair_half): n
 // pairs of A (rows of P) -> per-pair products in work (even/odd planes).
 void target_pairs_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const int* pbase,
-                       int win, int nr, int n, const float2* Gnat, float2* work,
+                       const int* pwin, int nr, int n, const float2* Gnat, float2* work,
                        hipStream_t st) {
     if (!target_half_supported(p, nr))
         throw ErpError{ERP_E_INVALID, "per-pair LDS chain needs P = 2 nr, nr in {4096, 8192, 16384}"};
@@ -897,16 +929,16 @@ void target_pairs_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const in
     float2* o4 = work;
     switch (nr) {
         case 4096:
-            hipLaunchKernelGGL(k_tpair_half<256>, dim3(n), dim3(256), 0, st, A, p.N, pbase, win,
+            hipLaunchKernelGGL(k_tpair_half<256>, dim3(n), dim3(256), 0, st, A, p.N, pbase, pwin,
                                G4, o4, lad);
             break;
         case 8192:
-            hipLaunchKernelGGL(k_tpair_half<512>, dim3(n), dim3(512), 0, st, A, p.N, pbase, win,
+            hipLaunchKernelGGL(k_tpair_half<512>, dim3(n), dim3(512), 0, st, A, p.N, pbase, pwin,
                                G4, o4, lad);
             break;
         default:
             hipLaunchKernelGGL(k_tpair_half<1024>, dim3(n), dim3(1024), 0, st, A, p.N, pbase,
-                               win, G4, o4, lad);
+                               pwin, G4, o4, lad);
             break;
     }
     ERP_CHECK_LAUNCH("k_tpair_half");

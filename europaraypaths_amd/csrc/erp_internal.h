@@ -193,13 +193,14 @@ void fft_init_twiddles(erp_ctx* ctx);
 bool target_half_supported(const FftPlan& p, int nr);
 int target_half_slice();
 void target_pairs_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const int* pbase,
-                       int win, int nr, int n, const float2* Gnat, float2* work, hipStream_t st);
+                       const int* pwin, int nr, int n, const float2* Gnat, float2* work,
+                       hipStream_t st);
 void target_sum_half(const FftPlan& p, const float2* work, float2* partial, const int* sl_beg,
                      const int* sl_end, int nslices, const int* seg_shot, const int* seg_beg,
                      const int* seg_end, int nseg, float2* S, float scale, hipStream_t st);// natural-order copy of a forward spectrum produced by fft_forward
 void fft_natural_order(const FftPlan& p, const float2* src, float2* dst, hipStream_t st);
 void target_spectra_fused(erp_ctx* ctx, const FftPlan& p, float2* A, float2* Y,
-                          const int* pbase, int win, int nr, int n, const float2* G,
+                          const int* pbase, const int* pwin, int nr, int n, const float2* G,
                           float2* work, int ngroups, const int* gbeg, const int* gend,
                           const int* gseg_shot, const int* gseg_beg, const int* gseg_end,
                           int nseg, float2* S, float scale, hipStream_t st);

@@ -49,6 +49,8 @@ constexpr int MASKW = FB / 64; // mask words per block (32)
 constexpr int SWIN = 2048;     // surface LDS window (bins)
 constexpr int TWIN = 4096;     // target LDS window (bins)
 constexpr int NTT = 512;       // threads per workgroup of the target kernels
+constexpr int TWF = 8192;      // bin window of k_target_accum_fast (FP64, 128 KB of LDS)
+constexpr int NTF = 1024;      // threads of k_target_accum_fast (one workgroup per CU)
 // Bin windows accumulate in FP64: LDS atomic adds of doubles (ds_add_f64) run
 // at more than twice the rate of FP32 ones (ds_add_f32) on gfx950 and remove
 // FP32 accumulation error from the bins.
@@ -1076,13 +1078,13 @@ __global__ void __launch_bounds__(NTT) k_target_accum(
 // decided on the approximate range unless it lies within 1e-7 bin of a bin
 // edge, where the exact FP64 sqrt and division of the oracle are repeated, so
 // bin indices stay identical to the FP64 reference.
-__global__ void __launch_bounds__(NTT) k_target_accum_fast(
+__global__ void __launch_bounds__(NTF) k_target_accum_fast(
     const float* __restrict__ tg, int64_t Tpad, const int* __restrict__ pshot,
     const int* __restrict__ ptgt, const erp_shot* __restrict__ shots,
     const int* __restrict__ Vs, const int64_t* __restrict__ rec_base, Rec rec,
     const Bounds* __restrict__ bounds, const erp_prism* __restrict__ prisms, int np, KPar kp,
-    float2* __restrict__ outA, int* __restrict__ pbase) {
-    extern __shared__ double2 winA[];
+    float2* __restrict__ outA, int* __restrict__ pbase, int* __restrict__ pwin) {
+    __shared__ double2 winA[TWF];
     const int pi = blockIdx.x;
     const int s = pshot[pi], t = ptgt[pi];
     const ShotK sh = load_shot(shots, s);
@@ -1105,6 +1107,7 @@ __global__ void __launch_bounds__(NTT) k_target_accum_fast(
     int base = (int)floor((itmin + kp.n_d * tmin - rst) / kp.drd) - 2;
     base = max(base, 0);
     bool fits = false;
+    int len = TWF;
     if (pbase) {
         const float fx = fmaxf(fabsf(o2f(bd.xmin) - txf), fabsf(o2f(bd.xmax) - txf));
         const float fy = fmaxf(fabsf(o2f(bd.ymin) - tyf), fabsf(o2f(bd.ymax) - tyf));
@@ -1112,16 +1115,23 @@ __global__ void __launch_bounds__(NTT) k_target_accum_fast(
         const double tmax = sqrt((double)fx * fx + (double)fy * fy + (double)fz * fz);
         const double ithmax = __longlong_as_double((long long)bd.itd_max);
         const double rhi = ithmax + kp.n_d * tmax * (1.0 + 1e-6) + 1.0;
-        const int top = (int)ceil((rhi - rst) / kp.drd) + 2;
-        fits = (top < base + TWIN - 1) || (base + TWIN >= nr);
-        if (threadIdx.x == 0) pbase[pi] = fits ? base : -1;
+        // every bin written (b and b + 1) is below top
+        const int top = min((int)ceil((rhi - rst) / kp.drd) + 2, nr);
+        fits = top - base <= TWF;
+        // live window: [base, base + len), len a multiple of 64 (only it is
+        // zeroed, flushed and read by the spectral chain)
+        if (fits) len = max(0, min(TWF, (top - base + 63) & ~63));
+        if (threadIdx.x == 0) {
+            pbase[pi] = fits ? base : -1;
+            pwin[pi] = len;
+        }
     }
     float2* rowA = outA + (size_t)pi * P;
     if (!fits) {
-        for (int k = threadIdx.x; k < P; k += NTT)
-            if (k < base || k >= base + TWIN || k >= nr) rowA[k] = make_float2(0.f, 0.f);
+        for (int k = threadIdx.x; k < P; k += NTF)
+            if (k < base || k >= base + TWF || k >= nr) rowA[k] = make_float2(0.f, 0.f);
     }
-    for (int k = threadIdx.x; k < TWIN; k += NTT) winA[k] = make_double2(0.0, 0.0);
+    for (int k = threadIdx.x; k < len; k += NTF) winA[k] = make_double2(0.0, 0.0);
     __threadfence();
     __syncthreads();
 
@@ -1129,7 +1139,7 @@ __global__ void __launch_bounds__(NTT) k_target_accum_fast(
     const bool tn_up = tnx == 0.f && tny == 0.f && tnz == 1.f;
     const float a = kp.piLlam;
     const float gscale = kp.Dmax;
-    for (int k = threadIdx.x; k < V; k += NTT) {
+    for (int k = threadIdx.x; k < V; k += NTF) {
         const int64_t r = r0 + k;
         const float4 p0 = rec.q[4 * r], p1 = rec.q[4 * r + 1], p2 = rec.q[4 * r + 2],
                      p3 = rec.q[4 * r + 3];
@@ -1193,14 +1203,14 @@ __global__ void __launch_bounds__(NTT) k_target_accum_fast(
         const float2 c0 = make_float2(c.x * (1.f - w), c.y * (1.f - w));
         const float2 c1 = make_float2(c.x * w, c.y * w);
         const int k0 = b - base;
-        if (k0 >= 0 && k0 < TWIN) lds_add2(winA, k0, c0); else glb_add2(rowA, b, c0);
+        if (k0 >= 0 && k0 < len) lds_add2(winA, k0, c0); else glb_add2(rowA, b, c0);
         if (b + 1 < nr) {
-            if (k0 + 1 >= 0 && k0 + 1 < TWIN) lds_add2(winA, k0 + 1, c1);
+            if (k0 + 1 >= 0 && k0 + 1 < len) lds_add2(winA, k0 + 1, c1);
             else glb_add2(rowA, b + 1, c1);
         }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < TWIN; k += NTT) {
+    for (int k = threadIdx.x; k < len; k += NTF) {
         const int b = base + k;
         if (b >= nr) break;
         rowA[b] = make_float2((float)winA[k].x, (float)winA[k].y);
@@ -1813,13 +1823,14 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                 ERP_HIP_TRY(hipMemcpyAsync(hl1.data(), list1, sizeof(int) * (size_t)S * Tcap,
                                            hipMemcpyDeviceToHost, st));
                 ERP_HIP_TRY(hipStreamSynchronize(st));
-                W.pairs.ensure(sizeof(int) * ((size_t)C * 3 + 2 * gcap + 3 * (size_t)S + 16));
+                W.pairs.ensure(sizeof(int) * ((size_t)C * 4 + 2 * gcap + 3 * (size_t)S + 16));
                 int* dps = W.pairs.as<int>();
                 int* dpt = dps + C;
                 int* dpb = dpt + C;                  // per-pair window base (fused)
                 int* dgb = dpb + C;                  // group begin / end
                 int* dge = dgb + gcap;
                 int* dseg = dge + gcap;              // segment shot / begin / end
+                int* dpw = dseg + 3 * S;             // per-pair window length (fused)
                 // iterate pairs in (shot, target) order
                 int cs = 0, cj = 0;
                 while (cs < S) {
@@ -1873,11 +1884,10 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                     for (int i = 0; i < n; ++i) triples += hV[pshot[i]];
                     PT_.begin(ERP_STAGE_TARGET_ACCUM, st, triples);
                     if (!spec) {
-                        hipLaunchKernelGGL(k_target_accum_fast, dim3(n), dim3(NTT),
-                                           sizeof(double2) * TWIN, st, targets->xyzn.as<float>(),
-                                           Tpad, dps, dpt, dshots, dV, dbase, rec, dbounds, dpr,
-                                           (int)nprisms, kp, A,
-                                           (fused || half) ? dpb : nullptr);
+                        hipLaunchKernelGGL(k_target_accum_fast, dim3(n), dim3(NTF), 0, st,
+                                           targets->xyzn.as<float>(), Tpad, dps, dpt, dshots, dV,
+                                           dbase, rec, dbounds, dpr, (int)nprisms, kp, A,
+                                           (fused || half) ? dpb : nullptr, dpw);
                         ERP_CHECK_LAUNCH("k_target_accum_fast");
                     } else {
                         hipLaunchKernelGGL(k_target_accum<1>, dim3(n), dim3(NTT),
@@ -1891,7 +1901,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                     PT_.begin(ERP_STAGE_TARGET_FFT, st, (double)n);
                     if (half) {
                         // outward trace = -A/sqrt(P): scale -1/sqrt(P)
-                        target_pairs_half(ctx, planP, A, dpb, TWIN, nr, n, g_targ_nat, Z, st);
+                        target_pairs_half(ctx, planP, A, dpb, dpw, nr, n, g_targ_nat, Z, st);
                         PT_.end(st);
                         PT_.begin(ERP_STAGE_TARGET_SUM, st, (double)n);
                         target_sum_half(planP, Z, A + (size_t)C * P, dgb, dge, ngroups, dseg,
@@ -1901,7 +1911,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                     }
                     if (fused) {
                         // outward trace = -A/sqrt(P): scale -1/sqrt(P)
-                        target_spectra_fused(ctx, planP, A, Bq, dpb, TWIN, nr, n, g_targ, Z,
+                        target_spectra_fused(ctx, planP, A, Bq, dpb, dpw, nr, n, g_targ, Z,
                                              ngroups, dgb, dge, dseg, dseg + S, dseg + 2 * S,
                                              nseg, Sacc,
                                              -1.0f / kp.sqrtP, st);
