@@ -49,8 +49,8 @@ constexpr int MASKW = FB / 64; // mask words per block (32)
 constexpr int SWIN = 2048;     // surface LDS window (bins)
 constexpr int TWIN = 4096;     // target LDS window (bins)
 constexpr int NTT = 512;       // threads per workgroup of the target kernels
-constexpr int TWF = 8192;      // bin window of k_target_accum_fast (FP64, 128 KB of LDS)
-constexpr int NTF = 1024;      // threads of k_target_accum_fast (one workgroup per CU)
+constexpr int TWF = 4096;      // bin window of k_target_accum_fast (FP64, 64 KB of LDS)
+constexpr int NTF = 512;       // threads of k_target_accum_fast (two workgroups per CU)
 // Bin windows accumulate in FP64: LDS atomic adds of doubles (ds_add_f64) run
 // at more than twice the rate of FP32 ones (ds_add_f32) on gfx950 and remove
 // FP32 accumulation error from the bins.
@@ -1070,6 +1070,95 @@ __global__ void __launch_bounds__(NTT) k_target_accum(
     }
 }
 
+// One facet x target term of the plain-mode target kernel: range bin, split
+// weight and complex amplitude (compTargetRays + compRefrEnergyIn +
+// accumulateTarget, geometry_kernels.cu:272-350,520-554, radar_kernels.cu:
+// 767-830).  q points at the facet's packed record (4 float4).
+struct Tri {
+    int b;
+    float w;
+    float2 c;
+    bool valid;
+};
+
+__device__ __forceinline__ Tri tri_eval(const float4* __restrict__ q, double tx, double ty,
+                                        double tz, float tnx, float tny, float tnz, bool tn_up,
+                                        double rst, double inv_dr, double n_d, double two_cyc,
+                                        int nr, bool lossless,
+                                        const erp_prism* __restrict__ prisms, int np,
+                                        const KPar& kp) {
+    Tri o;
+    const float4 p0 = q[0], p1 = q[1], p2 = q[2], p3 = q[3];
+    const float fx = p0.x, fy = p0.y, fz = p0.z;
+    const double Itd = __hiloint2double(__float_as_int(p3.w), __float_as_int(p3.z));
+    const double dx = tx - (double)fx, dy = ty - (double)fy, dz = tz - (double)fz;
+    const double t2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const float rs = rsqrtf((float)t2);
+    const double y0 = t2 * (double)rs;
+    const double Ttd = fma(fma(-y0, y0, t2), 0.5 * (double)rs, y0);
+    double rngt = fma(Ttd, n_d, Itd);
+    const double xq = (rngt - rst) * inv_dr;
+    if (fabs(xq - rint(xq)) < 1e-7) {
+        // within 1e-7 bin of an edge: the oracle's exact operation sequence
+        const double Te = dist3(dx, dy, dz);
+        rngt = madd_nc(Itd, Te, n_d);
+        o.valid = bin_of(rngt, rst, kp.drd, nr, &o.b, &o.w);
+    } else {
+        o.valid = (xq > -1.0) && (xq < (double)nr);
+        o.b = (int)xq;
+        o.w = (float)(xq - (double)o.b);
+    }
+    const float nx = p1.x, ny = p1.y, nz = p1.z;
+    const float ux = p1.w, uy = p2.x, uz = p2.y;
+    const float vx = p2.z, vy = p2.w, vz = p3.x;
+    const float Ix = (float)dx * rs, Iy = (float)dy * rs, Iz = (float)dz * rs;
+    const float cT = fmaxf(-1.f, fminf(1.f, -(Ix * nx + Iy * ny + Iz * nz)));
+    const float Iu = Ix * ux + Iy * uy + Iz * uz;
+    const float Iv = Ix * vx + Iy * vy + Iz * vz;
+    // |I x tn|; for the common tn = (0, 0, 1) this is |(Iy, -Ix, 0)|
+    const float sTT = tn_up ? sqrtf(Ix * Ix + Iy * Iy) : cross_norm(Ix, Iy, Iz, tnx, tny, tnz);
+    const float q2 = Iu * Iu + Iv * Iv;
+    const float rinv = (q2 > 0.f) ? rsqrtf(q2) : 0.f;
+    const float cph = (q2 > 0.f) ? Iv * rinv : 1.f;
+    const float sph = Iu * rinv;
+    const float a = kp.piLlam;
+    float n1, d1, n2, d2, n3, d3, n4, d4;
+    sinc_nd_hw(a * Iv, &n1, &d1);
+    sinc_nd_hw(a * Iu, &n2, &d2);
+    sinc_nd_hw(a * sTT * cph, &n3, &d3);
+    sinc_nd_hw(a * sTT * sph, &n4, &d4);
+    const float den = fabsf((d1 * d2) * (d3 * d4));
+    float amp = p0.w * kp.Dmax * fabsf((n1 * n2) * (n3 * n4)) * __builtin_amdgcn_rcpf(den) * rs;
+    if (!lossless) {
+        float expo = 0.f, tot = 0.f;
+        for (int k = 0; k < np; ++k) {
+            const float d = prism_len(fx, fy, fz, Ix, Iy, Iz, prisms[k], (float)Ttd);
+            tot += d;
+            expo += prisms[k].alpha * d;
+        }
+        expo += kp.alpha2 * ((float)Ttd - tot);
+        const float kc = kp.ks * cT;
+        amp *= __expf(-2.f * (expo + kc * kc));
+    }
+    const float2 ph = phasor_half_cycles_hw(rngt * two_cyc);
+    o.c = make_float2(ph.x * amp, ph.y * amp);
+    return o;
+}
+
+// two-bin split of one term into the LDS window [base, base + len) or, past
+// it, the global row
+__device__ __forceinline__ void tri_add(const Tri& t, int base, int len, int nr,
+                                        double2* winA, float2* rowA) {
+    const float2 c0 = make_float2(t.c.x * (1.f - t.w), t.c.y * (1.f - t.w));
+    const float2 c1 = make_float2(t.c.x * t.w, t.c.y * t.w);
+    const int k0 = t.b - base;
+    if (k0 >= 0 && k0 < len) lds_add2(winA, k0, c0); else glb_add2(rowA, t.b, c0);
+    if (t.b + 1 < nr) {
+        if (k0 + 1 >= 0 && k0 + 1 < len) lds_add2(winA, k0 + 1, c1);
+        else glb_add2(rowA, t.b + 1, c1);
+    }
+}
+
 // Plain-mode (type != 0, non-specular) facet x target kernel with
 // instruction-lean arithmetic: Newton-refined FP64 range from an FP32 rsqrt
 // seed, one reciprocal for the product of the four sinc factors of
@@ -1137,77 +1226,19 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
 
     const double inv_dr = kp.inv_dr, n_d = kp.n_d, two_cyc = 2.0 * kp.inv_cyc;
     const bool tn_up = tnx == 0.f && tny == 0.f && tnz == 1.f;
-    const float a = kp.piLlam;
-    const float gscale = kp.Dmax;
-    for (int k = threadIdx.x; k < V; k += NTF) {
-        const int64_t r = r0 + k;
-        const float4 p0 = rec.q[4 * r], p1 = rec.q[4 * r + 1], p2 = rec.q[4 * r + 2],
-                     p3 = rec.q[4 * r + 3];
-        const float fx = p0.x, fy = p0.y, fz = p0.z;
-        const double Itd = __hiloint2double(__float_as_int(p3.w), __float_as_int(p3.z));
-        const double dx = tx - (double)fx, dy = ty - (double)fy, dz = tz - (double)fz;
-        const double t2 = fma(dx, dx, fma(dy, dy, dz * dz));
-        const float rs = rsqrtf((float)t2);
-        const double y0 = t2 * (double)rs;
-        const double Ttd = fma(fma(-y0, y0, t2), 0.5 * (double)rs, y0);
-        double rngt = fma(Ttd, n_d, Itd);
-        const double xq = (rngt - rst) * inv_dr;
-        int b;
-        float w;
-        bool valid;
-        if (fabs(xq - rint(xq)) < 1e-7) {
-            // within 1e-7 bin of an edge: the oracle's exact operation sequence
-            const double Te = dist3(dx, dy, dz);
-            rngt = madd_nc(Itd, Te, n_d);
-            valid = bin_of(rngt, rst, kp.drd, nr, &b, &w);
-        } else {
-            valid = (xq > -1.0) && (xq < (double)nr);
-            b = (int)xq;
-            w = (float)(xq - (double)b);
-        }
-        if (!valid) continue;
-        const float nx = p1.x, ny = p1.y, nz = p1.z;
-        const float ux = p1.w, uy = p2.x, uz = p2.y;
-        const float vx = p2.z, vy = p2.w, vz = p3.x;
-        const float Ix = (float)dx * rs, Iy = (float)dy * rs, Iz = (float)dz * rs;
-        const float cT = fmaxf(-1.f, fminf(1.f, -(Ix * nx + Iy * ny + Iz * nz)));
-        const float Iu = Ix * ux + Iy * uy + Iz * uz;
-        const float Iv = Ix * vx + Iy * vy + Iz * vz;
-        // |I x tn|; for the common tn = (0, 0, 1) this is |(Iy, -Ix, 0)|
-        const float sTT = tn_up ? sqrtf(Ix * Ix + Iy * Iy) : cross_norm(Ix, Iy, Iz, tnx, tny, tnz);
-        const float q2 = Iu * Iu + Iv * Iv;
-        const float rinv = (q2 > 0.f) ? rsqrtf(q2) : 0.f;
-        const float cph = (q2 > 0.f) ? Iv * rinv : 1.f;
-        const float sph = Iu * rinv;
-        float n1, d1, n2, d2, n3, d3, n4, d4;
-        sinc_nd_hw(a * Iv, &n1, &d1);
-        sinc_nd_hw(a * Iu, &n2, &d2);
-        sinc_nd_hw(a * sTT * cph, &n3, &d3);
-        sinc_nd_hw(a * sTT * sph, &n4, &d4);
-        const float den = fabsf((d1 * d2) * (d3 * d4));
-        float amp = p0.w * gscale * fabsf((n1 * n2) * (n3 * n4)) *
-                    __builtin_amdgcn_rcpf(den) * rs;
-        if (!lossless) {
-            float expo = 0.f, tot = 0.f;
-            for (int q = 0; q < np; ++q) {
-                const float d = prism_len(fx, fy, fz, Ix, Iy, Iz, prisms[q], (float)Ttd);
-                tot += d;
-                expo += prisms[q].alpha * d;
-            }
-            expo += kp.alpha2 * ((float)Ttd - tot);
-            const float kc = kp.ks * cT;
-            amp *= __expf(-2.f * (expo + kc * kc));
-        }
-        const float2 ph = phasor_half_cycles_hw(rngt * two_cyc);
-        const float2 c = make_float2(ph.x * amp, ph.y * amp);
-        const float2 c0 = make_float2(c.x * (1.f - w), c.y * (1.f - w));
-        const float2 c1 = make_float2(c.x * w, c.y * w);
-        const int k0 = b - base;
-        if (k0 >= 0 && k0 < len) lds_add2(winA, k0, c0); else glb_add2(rowA, b, c0);
-        if (b + 1 < nr) {
-            if (k0 + 1 >= 0 && k0 + 1 < len) lds_add2(winA, k0 + 1, c1);
-            else glb_add2(rowA, b + 1, c1);
-        }
+    // two facets per thread and iteration: two independent dependency chains
+    // for the scheduler to interleave
+    for (int k = threadIdx.x; k < V; k += 2 * NTF) {
+        const bool two = k + NTF < V;
+        const Tri ta = tri_eval(rec.q + 4 * (r0 + k), tx, ty, tz, tnx, tny, tnz, tn_up, rst,
+                                inv_dr, n_d, two_cyc, nr, lossless, prisms, np, kp);
+        Tri tb;
+        tb.valid = false;
+        if (two)
+            tb = tri_eval(rec.q + 4 * (r0 + k + NTF), tx, ty, tz, tnx, tny, tnz, tn_up, rst,
+                          inv_dr, n_d, two_cyc, nr, lossless, prisms, np, kp);
+        if (ta.valid) tri_add(ta, base, len, nr, winA, rowA);
+        if (tb.valid) tri_add(tb, base, len, nr, winA, rowA);
     }
     __syncthreads();
     for (int k = threadIdx.x; k < len; k += NTF) {
