@@ -5,6 +5,9 @@
 // policy (sim.cu:60-70) with status codes and a per-context error string.
 #include "erp_internal.h"
 
+#include <algorithm>
+#include <cmath>
+
 #include <cstring>
 #include <exception>
 #include <new>
@@ -185,6 +188,20 @@ int erp_mesh_upload(erp_ctx* ctx, const float* soa, int64_t F, erp_mesh** out) {
                                            hipMemcpyHostToDevice, ctx->stream));
         }
         ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        bool ok = true;
+        for (int64_t i = 0; i < F && ok; ++i) {
+            float g[12];
+            for (int a = 0; a < 12; ++a) g[a] = soa[(size_t)a * F + i];
+            const double wx = (double)g[7] * g[11] - (double)g[8] * g[10];
+            const double wy = (double)g[8] * g[9] - (double)g[6] * g[11];
+            const double wz = (double)g[6] * g[10] - (double)g[7] * g[9];
+            const double ep = std::max(std::fabs(g[3] - wx), std::max(std::fabs(g[4] - wy),
+                                                                      std::fabs(g[5] - wz)));
+            const double em = std::max(std::fabs(g[3] + wx), std::max(std::fabs(g[4] + wy),
+                                                                      std::fabs(g[5] + wz)));
+            ok = std::min(ep, em) <= 1e-5;
+        }
+        m->frame_ok = ok;
     });
     if (rc != ERP_OK) {
         m->soa.release();

@@ -79,6 +79,9 @@ struct erp_mesh {
     int64_t F;
     DevBuf soa;  // 12 x F float32 (stride Fpad)
     int64_t Fpad;
+    // every facet normal equals +-(u x v) to 1e-5: kernels may derive n from
+    // the tangents instead of loading it
+    bool frame_ok = false;
 };
 
 struct erp_targets {

@@ -64,6 +64,7 @@ struct KPar {
     float fs_lam;      // fs/lam (facetReradiation)
     float fs2_lam;     // fs*fs/lam
     float nf;          // sqrt(eps_2) in float (accumulateTarget's n)
+    int ortho;         // every facet frame (u, v, n) orthonormal (see sc_angles)
     float sqrtP;
     float invK;        // 1/(4*3.14159)^2
     float sqrt_e1e2;   // sqrt(eps_1/eps_2)
@@ -138,11 +139,35 @@ __device__ __forceinline__ float2 phasor_cycles(double cyc) {
     return make_float2(c, s);
 }
 
-// |a x b| (sine of the angle between unit vectors)
-__device__ __forceinline__ float cross_norm(float ax, float ay, float az, float bx, float by,
-                                            float bz) {
-    const float cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
-    return sqrtf(cx * cx + cy * cy + cz * cz);
+// Cosine and sine of the angle between -I and a stored normal m as the
+// reference forms them (geometry_kernels.cu:103-140): c = clamp(-I.m, -1, 1),
+// s = sin(acos(c)) = sqrt((1 - c)(1 + c)), with I = (ix, iy, iz) in FP64.
+// Near c = 1 the sine depends on |m| (stored normals are unit length only to
+// float rounding) at the 1e-4 level, which FP32 products cannot resolve.
+__device__ __forceinline__ float2 cos_sin_ref(double ix, double iy, double iz, float mx, float my,
+                                              float mz) {
+    double c = -(ix * mx + iy * my + iz * mz);
+    c = fmin(1.0, fmax(-1.0, c));
+    return make_float2((float)c, sqrtf((float)((1.0 - c) * (1.0 + c))));
+}
+
+// (I.u, I.v) in FP64.  Needed for a general (non-orthonormal) facet frame,
+// where the azimuth atan2(I.u, I.v) is ill-conditioned near u x v and FP32
+// dot products lose it.
+__device__ __forceinline__ float2 uv_dots_exact(double ix, double iy, double iz, float ux,
+                                                float uy, float uz, float vx, float vy,
+                                                float vz) {
+    return make_float2((float)(ix * ux + iy * uy + iz * uz), (float)(ix * vx + iy * vy + iz * vz));
+}
+
+// sin(th) cos(ph) and sin(th) sin(ph) for ph = atan2(I.u, I.v)
+// (compIncidentRays), given s = sin(th); ph = 0 when both projections
+// vanish, as atan2(0, 0) = 0.
+__device__ __forceinline__ float2 sc_angles(float Iu, float Iv, float s) {
+    const float r2 = Iu * Iu + Iv * Iv;
+    if (!(r2 > 0.f)) return make_float2(s, 0.f);
+    const float ri = rsqrtf(r2);
+    return make_float2(s * Iv * ri, s * Iu * ri);
 }
 
 // elementary_kernels.cu:50-53
@@ -369,10 +394,13 @@ struct Rec {
     float* F;       // cap: sqrt(P n Grefr G1 [fRfrC]) lam^2 / (K Itd)
     float* fRfrC;   // cap
     float* ang;     // 3 x cap: Ith, Iph, Rth
-    // the same facet record packed for k_target_accum_fast, 4 float4 per
-    // record (four wide loads instead of fourteen narrow ones):
-    // (x, y, z, F) (nx, ny, nz, ux) (uy, uz, vx, vy) (vz, 0, Itd lo, Itd hi)
-    float4* q;
+    // the facet record packed for k_target_accum_fast (52 bytes, four wide
+    // loads): qa = (x, y, z, F), qb = (nx, ny, nz, ux), qc = (uy, uz, vx, vy),
+    // qd = vz; Itd is recomputed from the source position
+    float4* qa;
+    float4* qb;
+    float4* qc;
+    float* qd;
     int64_t cap;
 };
 
@@ -447,15 +475,18 @@ __global__ void __launch_bounds__(NT) k_surface(
         const double dx = (double)g[0] - sh.sx, dy = (double)g[1] - sh.sy,
                      dz = (double)g[2] - sh.sz;
         const double Itd = dist3(dx, dy, dz);
-        const float inv = (float)(1.0 / Itd);
-        const float Ix = (float)dx * inv, Iy = (float)dy * inv, Iz = (float)dz * inv;
-        float cI = -(Ix * g[3] + Iy * g[4] + Iz * g[5]);
-        cI = fmaxf(-1.f, fminf(1.f, cI));
-        const float Iu = Ix * g[6] + Iy * g[7] + Iz * g[8];
-        const float Iv = Ix * g[9] + Iy * g[10] + Iz * g[11];
-        // sin(Ith) = |I x n|: well conditioned at every angle (acos of the FP32
-        // cosine loses ~1e-7/sin(Ith) relative precision near nadir)
-        const float sI = cross_norm(Ix, Iy, Iz, g[3], g[4], g[5]);
+        const double invd = 1.0 / Itd;
+        const double ix = dx * invd, iy = dy * invd, iz = dz * invd;
+        const float Ix = (float)ix, Iy = (float)iy, Iz = (float)iz;
+        const float2 csI = cos_sin_ref(ix, iy, iz, g[3], g[4], g[5]);
+        const float cI = csI.x, sI = csI.y;
+        float Iu = Ix * g[6] + Iy * g[7] + Iz * g[8];
+        float Iv = Ix * g[9] + Iy * g[10] + Iz * g[11];
+        if (!kp.ortho) {
+            const float2 e = uv_dots_exact(ix, iy, iz, g[6], g[7], g[8], g[9], g[10], g[11]);
+            Iu = e.x;
+            Iv = e.y;
+        }
         const float Ith = atan2f(sI, cI);
         const float Iph = atan2f(Iu, Iv);
         // compRefractedRays (snellsLaw)
@@ -480,13 +511,14 @@ __global__ void __launch_bounds__(NT) k_surface(
         // facet_G(Ith, Iph) (radar_kernels.cu:658-677)
         // facet_G(Ith, Iph): sin(Ith)cos(Iph) = I.v and sin(Ith)sin(Iph) = I.u for the
         // orthonormal facet basis (u, v, n)
-        const float s1 = sinc_f(kp.piLlam * Iv), s2 = sinc_f(kp.piLlam * Iu);
+        const float2 scI = sc_angles(Iu, Iv, sI);
+        const float s1 = sinc_f(kp.piLlam * scI.x), s2 = sinc_f(kp.piLlam * scI.y);
         const float G1 = (s1 * s2) * (s1 * s2) * kp.Dmax;
         if (spec) {
             // facetReradiation(Itd, 2 Ith, -Iph) * radarEq(P, Grefl, fs, lam, Itd, V);
             // sin(2 Ith) cos(-Iph) = 2 cos(Ith) I.v, sin(2 Ith) sin(-Iph) = -2 cos(Ith) I.u
-            const float r1 = sinc_f(kp.fs_lam * 2.f * cI * Iv),
-                        r2 = sinc_f(kp.fs_lam * -2.f * cI * Iu);
+            const float r1 = sinc_f(kp.fs_lam * 2.f * cI * scI.x),
+                        r2 = sinc_f(kp.fs_lam * -2.f * cI * scI.y);
             const float rr = kp.fs2_lam * r1 * r2;
             const float Gl = sh.grefl;
             const double Itd2 = Itd * Itd;
@@ -514,14 +546,11 @@ __global__ void __launch_bounds__(NT) k_surface(
                                  : sqrtf(kp.P * kp.nf * Grefr * G1 * fac) * kp.lam * kp.lam *
                                        kp.invK / Itdf;
             rec.F[r] = Fr;
-            if (rec.q) {
-                const long long ib = __double_as_longlong(Itd);
-                float4* q = rec.q + 4 * r;
-                q[0] = make_float4(g[0], g[1], g[2], Fr);
-                q[1] = make_float4(g[3], g[4], g[5], g[6]);
-                q[2] = make_float4(g[7], g[8], g[9], g[10]);
-                q[3] = make_float4(g[11], 0.f, __int_as_float((int)(ib & 0xffffffffll)),
-                                   __int_as_float((int)(ib >> 32)));
+            if (rec.qa) {
+                rec.qa[r] = make_float4(g[0], g[1], g[2], Fr);
+                rec.qb[r] = make_float4(g[3], g[4], g[5], g[6]);
+                rec.qc[r] = make_float4(g[7], g[8], g[9], g[10]);
+                rec.qd[r] = g[11];
             }
             rec.fRfrC[r] = fRfrC;
             rec.ang[r] = Ith;
@@ -941,12 +970,19 @@ __global__ void __launch_bounds__(NTT) k_target_accum(
         float w;
         if (!bin_of(rngt, rst, kp.drd, nr, &b, &w)) continue;
         const float Ttdf = (float)Ttd;
-        const float inv = (float)(1.0 / Ttd);
-        const float Ix = (float)dx * inv, Iy = (float)dy * inv, Iz = (float)dz * inv;
-        const float cT = fmaxf(-1.f, fminf(1.f, -(Ix * nx + Iy * ny + Iz * nz)));
-        const float cTT = fmaxf(-1.f, fminf(1.f, -(Ix * tnx + Iy * tny + Iz * tnz)));
-        const float Iu = Ix * ux + Iy * uy + Iz * uz;
-        const float Iv = Ix * vx + Iy * vy + Iz * vz;
+        const double invd = 1.0 / Ttd;
+        const double ix = dx * invd, iy = dy * invd, iz = dz * invd;
+        const float Ix = (float)ix, Iy = (float)iy, Iz = (float)iz;
+        const float2 csT = cos_sin_ref(ix, iy, iz, nx, ny, nz);
+        const float2 csTT = cos_sin_ref(ix, iy, iz, tnx, tny, tnz);
+        const float cT = csT.x, sT = csT.y, cTT = csTT.x, sTT = csTT.y;
+        float Iu = Ix * ux + Iy * uy + Iz * uz;
+        float Iv = Ix * vx + Iy * vy + Iz * vz;
+        if (!kp.ortho) {
+            const float2 e = uv_dots_exact(ix, iy, iz, ux, uy, uz, vx, vy, vz);
+            Iu = e.x;
+            Iv = e.y;
+        }
         // attenuation along the forced ray (prisms + background alpha2)
         float expo = 0.f;
         {
@@ -967,7 +1003,7 @@ __global__ void __launch_bounds__(NTT) k_target_accum(
             if (kp.mode & ERP_MODE_SPECULAR) {
                 const float Ith = rec.ang[r], Iph = rec.ang[rec.cap + r],
                             Rth = rec.ang[2 * rec.cap + r];
-                const float Tth = atan2f(cross_norm(Ix, Iy, Iz, nx, ny, nz), cT);
+                const float Tth = atan2f(sT, cT);
                 const float Tph = atan2f(Iu, Iv);
                 const float dth = Rth - Tth, dph = Iph - Tph;
                 const float si = sinf(dth);
@@ -984,7 +1020,7 @@ __global__ void __launch_bounds__(NTT) k_target_accum(
                 EO *= vo * vo;
                 (void)Ith;
             }
-            const float TTh = atan2f(cross_norm(Ix, Iy, Iz, tnx, tny, tnz), cTT);
+            const float TTh = atan2f(sTT, cTT);
             const double sl = Itd + Ttd;
             const double sl2 = sl * sl;
             const float req = (float)((double)V * ((double)(kp.P * sh.grefr * sh.grefr * kp.fs *
@@ -999,12 +1035,12 @@ __global__ void __launch_bounds__(NTT) k_target_accum(
         }
         // accumulateTarget / radiateTarget amplitude:
         // sqrt(Pray n) = F_f * sqrt(G(Tth,Tph) G(TargetTh,Tph)) / Ttd * sqrt(EI-part)
-        // sin(Tth)cos(Tph) = I.v, sin(Tth)sin(Tph) = I.u; sin(TargetTh) = |I x tn|
+        // (cos Tph, sin Tph) = (I.v, I.u) / |(I.u, I.v)|
         const float rp = sqrtf(Iu * Iu + Iv * Iv);
         const float cph = (rp > 0.f) ? Iv / rp : 1.f;
         const float sph = (rp > 0.f) ? Iu / rp : 0.f;
-        const float sTT = cross_norm(Ix, Iy, Iz, tnx, tny, tnz);
-        const float s1 = sinc_f(kp.piLlam * Iv), s2 = sinc_f(kp.piLlam * Iu);
+        const float2 scT = make_float2(sT * cph, sT * sph);
+        const float s1 = sinc_f(kp.piLlam * scT.x), s2 = sinc_f(kp.piLlam * scT.y);
         const float s3 = sinc_f(kp.piLlam * sTT * cph), s4 = sinc_f(kp.piLlam * sTT * sph);
         const float gg = fabsf(s1 * s2) * fabsf(s3 * s4) * kp.Dmax;
         float amp = rec.F[r] * gg / Ttdf;
@@ -1015,7 +1051,7 @@ __global__ void __launch_bounds__(NTT) k_target_accum(
             if (MODE == 1) {
                 // specular EI/EO facet reradiation factors (geometry_kernels.cu:530-539,565-572)
                 const float Iph = rec.ang[rec.cap + r], Rth = rec.ang[2 * rec.cap + r];
-                const float Tth = atan2f(cross_norm(Ix, Iy, Iz, nx, ny, nz), cT);
+                const float Tth = atan2f(sT, cT);
                 const float Tph = atan2f(Iu, Iv);
                 const float si = sinf(Rth - Tth), dph = Iph - Tph;
                 const float a1 = sinc_f(kp.fs_lam * si * cosf(dph)),
@@ -1073,7 +1109,7 @@ __global__ void __launch_bounds__(NTT) k_target_accum(
 // One facet x target term of the plain-mode target kernel: range bin, split
 // weight and complex amplitude (compTargetRays + compRefrEnergyIn +
 // accumulateTarget, geometry_kernels.cu:272-350,520-554, radar_kernels.cu:
-// 767-830).  q points at the facet's packed record (4 float4).
+// 767-830) for facet record r.
 struct Tri {
     int b;
     float w;
@@ -1081,16 +1117,22 @@ struct Tri {
     bool valid;
 };
 
-__device__ __forceinline__ Tri tri_eval(const float4* __restrict__ q, double tx, double ty,
-                                        double tz, float tnx, float tny, float tnz, bool tn_up,
-                                        double rst, double inv_dr, double n_d, double two_cyc,
-                                        int nr, bool lossless,
-                                        const erp_prism* __restrict__ prisms, int np,
-                                        const KPar& kp) {
+__device__ __forceinline__ Tri tri_eval(const Rec& rec, int64_t r, bool frame_ok, double sx,
+                                        double sy, double sz, double tx, double ty, double tz,
+                                        float tnx, float tny, float tnz, bool tn_up, double rst,
+                                        double inv_dr, double n_d, double two_cyc, int nr,
+                                        bool lossless, const erp_prism* __restrict__ prisms,
+                                        int np, const KPar& kp) {
     Tri o;
-    const float4 p0 = q[0], p1 = q[1], p2 = q[2], p3 = q[3];
-    const float fx = p0.x, fy = p0.y, fz = p0.z;
-    const double Itd = __hiloint2double(__float_as_int(p3.w), __float_as_int(p3.z));
+    const float4 qa = rec.qa[r], qb = rec.qb[r], qc = rec.qc[r];
+    const float qd = rec.qd[r];
+    const float fx = qa.x, fy = qa.y, fz = qa.z;
+    // Itd = |facet - source| in FP64 (Newton step from an FP32 rsqrt seed)
+    const double ex = (double)fx - sx, ey = (double)fy - sy, ez = (double)fz - sz;
+    const double e2 = fma(ex, ex, fma(ey, ey, ez * ez));
+    const double er = (double)rsqrtf((float)e2);
+    const double e0 = e2 * er;
+    const double Itd = fma(fma(-e0, e0, e2), 0.5 * er, e0);
     const double dx = tx - (double)fx, dy = ty - (double)fy, dz = tz - (double)fz;
     const double t2 = fma(dx, dx, fma(dy, dy, dz * dz));
     const float rs = rsqrtf((float)t2);
@@ -1101,34 +1143,47 @@ __device__ __forceinline__ Tri tri_eval(const float4* __restrict__ q, double tx,
     if (fabs(xq - rint(xq)) < 1e-7) {
         // within 1e-7 bin of an edge: the oracle's exact operation sequence
         const double Te = dist3(dx, dy, dz);
-        rngt = madd_nc(Itd, Te, n_d);
+        rngt = madd_nc(dist3(ex, ey, ez), Te, n_d);
         o.valid = bin_of(rngt, rst, kp.drd, nr, &o.b, &o.w);
     } else {
         o.valid = (xq > -1.0) && (xq < (double)nr);
         o.b = (int)xq;
         o.w = (float)(xq - (double)o.b);
     }
-    const float nx = p1.x, ny = p1.y, nz = p1.z;
-    const float ux = p1.w, uy = p2.x, uz = p2.y;
-    const float vx = p2.z, vy = p2.w, vz = p3.x;
-    const float Ix = (float)dx * rs, Iy = (float)dy * rs, Iz = (float)dz * rs;
-    const float cT = fmaxf(-1.f, fminf(1.f, -(Ix * nx + Iy * ny + Iz * nz)));
-    const float Iu = Ix * ux + Iy * uy + Iz * uz;
-    const float Iv = Ix * vx + Iy * vy + Iz * vz;
-    // |I x tn|; for the common tn = (0, 0, 1) this is |(Iy, -Ix, 0)|
-    const float sTT = tn_up ? sqrtf(Ix * Ix + Iy * Iy) : cross_norm(Ix, Iy, Iz, tnx, tny, tnz);
+    const float nx = qb.x, ny = qb.y, nz = qb.z;
+    const float ux = qb.w, uy = qc.x, uz = qc.y;
+    const float vx = qc.z, vy = qc.w, vz = qd;
+    // unit vector facet -> target in FP64: 1/Ttd by one Newton step from rs
+    const double rd = (double)rs;
+    const double invd = rd * fma(-0.5 * t2 * rd, rd, 1.5);
+    const double ix = dx * invd, iy = dy * invd, iz = dz * invd;
+    const float Ix = (float)ix, Iy = (float)iy, Iz = (float)iz;
+    const float2 csT = cos_sin_ref(ix, iy, iz, nx, ny, nz);
+    const float cT = csT.x, sT = csT.y;
+    float Iu = Ix * ux + Iy * uy + Iz * uz;
+    float Iv = Ix * vx + Iy * vy + Iz * vz;
+    if (!frame_ok) {
+        const float2 e = uv_dots_exact(ix, iy, iz, ux, uy, uz, vx, vy, vz);
+        Iu = e.x;
+        Iv = e.y;
+    }
+    // sin(TargetTh); tn = (0, 0, 1) is the common case
+    const float sTT = tn_up ? sqrtf(fmaxf(0.f, (float)((1.0 + iz) * (1.0 - iz))))
+                            : cos_sin_ref(ix, iy, iz, tnx, tny, tnz).y;
     const float q2 = Iu * Iu + Iv * Iv;
     const float rinv = (q2 > 0.f) ? rsqrtf(q2) : 0.f;
     const float cph = (q2 > 0.f) ? Iv * rinv : 1.f;
     const float sph = Iu * rinv;
     const float a = kp.piLlam;
     float n1, d1, n2, d2, n3, d3, n4, d4;
-    sinc_nd_hw(a * Iv, &n1, &d1);
-    sinc_nd_hw(a * Iu, &n2, &d2);
+    // sin(Tth) (cos Tph, sin Tph)
+    const float scx = sT * cph, scy = sT * sph;
+    sinc_nd_hw(a * scx, &n1, &d1);
+    sinc_nd_hw(a * scy, &n2, &d2);
     sinc_nd_hw(a * sTT * cph, &n3, &d3);
     sinc_nd_hw(a * sTT * sph, &n4, &d4);
     const float den = fabsf((d1 * d2) * (d3 * d4));
-    float amp = p0.w * kp.Dmax * fabsf((n1 * n2) * (n3 * n4)) * __builtin_amdgcn_rcpf(den) * rs;
+    float amp = qa.w * kp.Dmax * fabsf((n1 * n2) * (n3 * n4)) * __builtin_amdgcn_rcpf(den) * rs;
     if (!lossless) {
         float expo = 0.f, tot = 0.f;
         for (int k = 0; k < np; ++k) {
@@ -1172,7 +1227,7 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
     const int* __restrict__ ptgt, const erp_shot* __restrict__ shots,
     const int* __restrict__ Vs, const int64_t* __restrict__ rec_base, Rec rec,
     const Bounds* __restrict__ bounds, const erp_prism* __restrict__ prisms, int np, KPar kp,
-    float2* __restrict__ outA, int* __restrict__ pbase, int* __restrict__ pwin) {
+    float2* __restrict__ outA, int* __restrict__ pbase, int* __restrict__ pwin, int frame_ok) {
     __shared__ double2 winA[TWF];
     const int pi = blockIdx.x;
     const int s = pshot[pi], t = ptgt[pi];
@@ -1230,13 +1285,14 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
     // for the scheduler to interleave
     for (int k = threadIdx.x; k < V; k += 2 * NTF) {
         const bool two = k + NTF < V;
-        const Tri ta = tri_eval(rec.q + 4 * (r0 + k), tx, ty, tz, tnx, tny, tnz, tn_up, rst,
-                                inv_dr, n_d, two_cyc, nr, lossless, prisms, np, kp);
+        const Tri ta = tri_eval(rec, r0 + k, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx, tny,
+                                tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr, lossless, prisms, np,
+                                kp);
         Tri tb;
         tb.valid = false;
         if (two)
-            tb = tri_eval(rec.q + 4 * (r0 + k + NTF), tx, ty, tz, tnx, tny, tnz, tn_up, rst,
-                          inv_dr, n_d, two_cyc, nr, lossless, prisms, np, kp);
+            tb = tri_eval(rec, r0 + k + NTF, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx, tny,
+                          tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr, lossless, prisms, np, kp);
         if (ta.valid) tri_add(ta, base, len, nr, winA, rowA);
         if (tb.valid) tri_add(tb, base, len, nr, winA, rowA);
     }
@@ -1408,9 +1464,11 @@ __global__ void __launch_bounds__(NT) k_refr0_direct(
                 const double Itd = rec.itd[rr];
                 const double dx = tx - fx, dy = ty - fy, dz = tz - fz;
                 const double Ttd = dist3(dx, dy, dz);
-                const float inv = (float)(1.0 / Ttd);
-                const float Ix = (float)dx * inv, Iy = (float)dy * inv, Iz = (float)dz * inv;
-                const float cT = fmaxf(-1.f, fminf(1.f, -(Ix * nx + Iy * ny + Iz * nz)));
+                const double invd = 1.0 / Ttd;
+                const double ix = dx * invd, iy = dy * invd, iz = dz * invd;
+                const float Ix = (float)ix, Iy = (float)iy, Iz = (float)iz;
+                const float2 csT = cos_sin_ref(ix, iy, iz, nx, ny, nz);
+                const float cT = csT.x;
                 float expo = 0.f, tot = 0.f;
                 for (int p = 0; p < np; ++p) {
                     const float d = prism_len(fx, fy, fz, Ix, Iy, Iz, prisms[p], (float)Ttd);
@@ -1423,12 +1481,21 @@ __global__ void __launch_bounds__(NT) k_refr0_direct(
                 float EI = expf(-4.f * expo) * fRfrC * expf(-4.f * kc * kc);
                 float EO = EI;
                 {
-                    const float Iu = Ix * rec.geo[6 * rec.cap + rr] + Iy * rec.geo[7 * rec.cap + rr] +
-                                     Iz * rec.geo[8 * rec.cap + rr];
-                    const float Iv = Ix * rec.geo[9 * rec.cap + rr] + Iy * rec.geo[10 * rec.cap + rr] +
-                                     Iz * rec.geo[11 * rec.cap + rr];
+                    float Iu = Ix * rec.geo[6 * rec.cap + rr] + Iy * rec.geo[7 * rec.cap + rr] +
+                               Iz * rec.geo[8 * rec.cap + rr];
+                    float Iv = Ix * rec.geo[9 * rec.cap + rr] + Iy * rec.geo[10 * rec.cap + rr] +
+                               Iz * rec.geo[11 * rec.cap + rr];
+                    const float sT = csT.y;
+                    if (!kp.ortho) {
+                        const float2 e = uv_dots_exact(
+                            ix, iy, iz, rec.geo[6 * rec.cap + rr], rec.geo[7 * rec.cap + rr],
+                            rec.geo[8 * rec.cap + rr], rec.geo[9 * rec.cap + rr],
+                            rec.geo[10 * rec.cap + rr], rec.geo[11 * rec.cap + rr]);
+                        Iu = e.x;
+                        Iv = e.y;
+                    }
                     const float Iph = rec.ang[rec.cap + rr], Rth = rec.ang[2 * rec.cap + rr];
-                    const float Tth = atan2f(cross_norm(Ix, Iy, Iz, nx, ny, nz), cT);
+                    const float Tth = atan2f(sT, cT);
                     const float Tph = atan2f(Iu, Iv);
                     const float si = sinf(Rth - Tth), dph = Iph - Tph;
                     const float a1 = sinc_f(kp.fs_lam * si * cosf(dph)),
@@ -1450,7 +1517,7 @@ __global__ void __launch_bounds__(NT) k_refr0_direct(
                                                        (ERP_FOURPI5_SQ * sl2 * sl2)));
                 // refrRadarSignal passes d_Tth to rerad_funct (sim.cu:930-934)
                 rc_s[threadIdx.x] =
-                    rerad_f(kp.rerad, atan2f(cross_norm(Ix, Iy, Iz, nx, ny, nz), cT)) * EI * EO * req;
+                    rerad_f(kp.rerad, atan2f(csT.y, cT)) * EI * EO * req;
                 rng_s[threadIdx.x] = madd_nc(sl - Ttd, Ttd, (double)(kp.c / kp.c2));
             }
             __syncthreads();
@@ -1485,6 +1552,7 @@ KPar make_kpar(const erp_params& p, int P_fft) {
     k.nu1 = p.nu_1; k.nu2 = p.nu_2; k.alpha1 = p.alpha1; k.alpha2 = p.alpha2;
     k.rng_res = p.rng_res; k.dr = p.dr; k.c = p.c; k.c1 = p.c_1; k.c2 = p.c_2;
     k.piLlam = (ERP_PI_F * p.fs) / p.lam;
+    k.ortho = 0;
     k.Dmax = (4.f * ERP_PI_F * p.fs * p.fs) / (p.lam * p.lam);
     k.fs_lam = p.fs / p.lam;
     k.fs2_lam = p.fs * p.fs / p.lam;
@@ -1662,7 +1730,8 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
 
     const int P = next_pow2(2 * (int64_t)nr - 1);
     const FftPlan planP = fft_plan(P);
-    const KPar kp = make_kpar(p, P);
+    KPar kp = make_kpar(p, P);
+    kp.ortho = mesh->frame_ok ? 1 : 0;
     const int L = 2 * nr - 1;
     const int nblk = (int)((F + FB - 1) / FB);
     const bool need_rec = (T > 0) || (spec && !conv);
@@ -1750,7 +1819,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         Rec rec{};
         if (need_rec && Vtot > 0) {
             const size_t cap = (size_t)Vtot;
-            const size_t bytes = cap * (12 * 4 + 8 + 4 + 4 + 3 * 4 + 64) + 256 * 7;
+            const size_t bytes = cap * (12 * 4 + 8 + 4 + 4 + 3 * 4 + 52) + 256 * 10;
             W.rec.ensure(bytes);
             char* b = W.rec.as<char>();
             auto take = [&](size_t n) { char* r = b; b += (n + 255) & ~(size_t)255; return r; };
@@ -1759,7 +1828,10 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
             rec.F = (float*)take(cap * 4);
             rec.fRfrC = (float*)take(cap * 4);
             rec.ang = (float*)take(cap * 3 * 4);
-            rec.q = (float4*)take(cap * 64);
+            rec.qa = (float4*)take(cap * 16);
+            rec.qb = (float4*)take(cap * 16);
+            rec.qc = (float4*)take(cap * 16);
+            rec.qd = (float*)take(cap * 4);
             rec.cap = (int64_t)cap;
         }
 
@@ -1918,7 +1990,8 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                         hipLaunchKernelGGL(k_target_accum_fast, dim3(n), dim3(NTF), 0, st,
                                            targets->xyzn.as<float>(), Tpad, dps, dpt, dshots, dV,
                                            dbase, rec, dbounds, dpr, (int)nprisms, kp, A,
-                                           (fused || half) ? dpb : nullptr, dpw);
+                                           (fused || half) ? dpb : nullptr, dpw,
+                                           mesh->frame_ok ? 1 : 0);
                         ERP_CHECK_LAUNCH("k_target_accum_fast");
                     } else {
                         hipLaunchKernelGGL(k_target_accum<1>, dim3(n), dim3(NTT),

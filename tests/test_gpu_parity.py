@@ -113,6 +113,19 @@ def test_targets_plain_range_sizes(gpu_ctx, nr):
     compare_traces(g, o)
 
 
+def test_targets_plain_general_facet_frames(gpu_ctx):
+    """Facet normals that are not +-(u x v) (a general mesh file): the target
+    kernel must use the stored normal, not the one derived from u and v."""
+    par, soa, shots = scene(rough=0.3, nr=1024, h=3000.0, rst=2600.0)
+    soa = soa.copy()
+    n = soa[3:6] + 0.05 * soa[6:9]
+    soa[3:6] = (n / np.linalg.norm(n, axis=0)).astype(np.float32)
+    tg = _targets(par, n=3)
+    g, gv, o, ov = run_both(gpu_ctx, par, soa, shots, targets=tg)
+    np.testing.assert_array_equal(gv, ov)
+    compare_traces(g, o)
+
+
 def test_targets_lossless_and_prisms(gpu_ctx):
     par, soa, shots = scene(rough=0.3, nr=1024, h=3000.0, rst=2600.0, lossless=True)
     tg = _targets(par, n=3)
