@@ -22,7 +22,7 @@ _lock = threading.Lock()
 EXPORTS = ("erp_ctx_create", "erp_ctx_destroy", "erp_last_error", "erp_mesh_upload",
            "erp_mesh_release", "erp_targets_upload", "erp_targets_release", "erp_run",
            "erp_run_dev", "erp_submit", "erp_poll", "erp_wait", "erp_run_footprint",
-           "erp_profile_enable", "erp_profile_read")
+           "erp_profile_enable", "erp_profile_read", "erp_serve_slots")
 
 STAGES = ("mask", "surface", "surface_conv", "target_select", "target_accum", "target_fft",
           "shot_final", "other", "target_sum")
@@ -72,6 +72,8 @@ def lib() -> ctypes.CDLL:
         L.erp_poll.restype = i32
         L.erp_wait.argtypes = [vp]
         L.erp_wait.restype = i32
+        L.erp_serve_slots.argtypes = [vp, i32]
+        L.erp_serve_slots.restype = i32
         L.erp_run_footprint.argtypes = [P(ErpParams), i64, i64, i64]
         L.erp_run_footprint.restype = i64
         L.erp_profile_enable.argtypes = [vp, i32]
@@ -231,6 +233,10 @@ class Context:
                                      _ptr(shots), ns, _ptr(prisms), npr, _ptr(out),
                                      _ptr(nvalid), flags, ctypes.byref(h)))
         return Job(self, h, out, nvalid, (shots, prisms, params))
+
+    def serve_slots(self, n: int):
+        """Start worker slots for submit() up to n (each: thread, stream, buffers)."""
+        self._check(lib().erp_serve_slots(self.handle, n))
 
     def profile(self, on: bool = True):
         """Enable (and clear) or disable the per-stage hipEvent timers."""

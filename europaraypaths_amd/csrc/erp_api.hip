@@ -93,20 +93,148 @@ void StageTimer::reset() {
     }
 }
 
+// One asynchronous job: a copy of the call's host arguments and its status.
 struct erp_job {
-    erp_ctx* ctx;
-    hipStream_t stream;
-    hipEvent_t done;
-    WorkBufs work;
-    DevBuf out;
-    DevBuf nvalid;
-    float* host_out;
-    int32_t* host_nvalid;
-    int64_t n_out;
-    int64_t ns;
-    int status;
+    erp_ctx* ctx = nullptr;
+    erp_params par{};
+    const erp_mesh* mesh = nullptr;
+    const erp_targets* targets = nullptr;
+    std::vector<erp_shot> shots;
+    std::vector<erp_prism> prisms;
+    uint32_t flags = 0;
+    float* host_out = nullptr;
+    int32_t* host_nvalid = nullptr;
+    std::mutex mu;
+    std::condition_variable cv;
+    bool done = false;
+    int status = ERP_OK;
     std::string err;
 };
+
+// A worker slot: one host thread, one stream and work buffers that persist
+// from job to job, so that a warm slot allocates nothing.
+struct ErpSlot {
+    std::thread th;
+    hipStream_t stream = nullptr;
+    WorkBufs work;
+    DevBuf out, nvalid;
+};
+
+namespace {
+
+void release_work(WorkBufs& w) {
+    DevBuf* bufs[] = {&w.cnt, &w.off, &w.V, &w.Vd, &w.tcnt, &w.tlist, &w.tcount, &w.rec,
+                      &w.bounds, &w.psurf, &w.srow, &w.refl, &w.shotS, &w.blu, &w.tA, &w.tB, &w.tC,
+                      &w.pairs, &w.t0trace, &w.shots, &w.prisms, &w.out, &w.nvalid};
+    for (DevBuf* b : bufs) b->release();
+}
+
+// Runs one job on a slot: the pipeline on the slot's stream, the copies back
+// to the caller's host buffers, then a wait on the stream.
+void run_job(erp_ctx* ctx, ErpSlot* s, erp_job* j) {
+    const size_t nout = (size_t)j->shots.size() * j->par.nr;
+    const int64_t ns = (int64_t)j->shots.size();
+    s->out.ensure(sizeof(float2) * std::max<size_t>(nout, 1));
+    s->nvalid.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(ns, 1));
+    erp_run_impl(ctx, &j->par, j->mesh, j->targets, j->shots.data(), ns,
+                 j->prisms.empty() ? nullptr : j->prisms.data(), (int64_t)j->prisms.size(),
+                 s->out.as<float2>(), s->nvalid.as<int32_t>(), j->flags, s->stream, s->work,
+                 nullptr);
+    if (nout)
+        ERP_HIP_TRY(hipMemcpyAsync(j->host_out, s->out.p, sizeof(float2) * nout,
+                                   hipMemcpyDeviceToHost, s->stream));
+    if (j->host_nvalid && ns)
+        ERP_HIP_TRY(hipMemcpyAsync(j->host_nvalid, s->nvalid.p, sizeof(int32_t) * ns,
+                                   hipMemcpyDeviceToHost, s->stream));
+    ERP_HIP_TRY(hipStreamSynchronize(s->stream));
+}
+
+void slot_main(erp_ctx* ctx, ErpSlot* s) {
+    hipSetDevice(ctx->device);
+    for (;;) {
+        erp_job* j = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(ctx->qmu);
+            ctx->qcv.wait(lk, [&] { return ctx->stopping || !ctx->queue.empty(); });
+            if (ctx->queue.empty()) return;  // stopping with nothing queued
+            j = ctx->queue.front();
+            ctx->queue.pop_front();
+            // counted before the job can take pointers into the table caches
+            ctx->active.fetch_add(1);
+        }
+        int rc = ERP_OK;
+        std::string err;
+        try {
+            run_job(ctx, s, j);
+        } catch (const ErpError& e) {
+            rc = e.code;
+            err = e.msg;
+        } catch (const std::bad_alloc&) {
+            rc = ERP_E_NOMEM;
+            err = "host allocation failed";
+        } catch (const std::exception& e) {
+            rc = ERP_E_INVALID;
+            err = e.what();
+        }
+        if (rc != ERP_OK) hipStreamSynchronize(s->stream);  // drain what was queued
+        ctx->active.fetch_sub(1);
+        {
+            std::lock_guard<std::mutex> lk(j->mu);
+            j->status = rc;
+            j->err = err;
+            j->done = true;
+        }
+        j->cv.notify_all();
+    }
+}
+
+// Starts worker slots up to n (never fewer than exist).
+void start_slots(erp_ctx* ctx, int n) {
+    std::lock_guard<std::mutex> lk(ctx->qmu);
+    while ((int)ctx->slots.size() < n) {
+        ErpSlot* s = new ErpSlot();
+        hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete s;
+            throw ErpError{ERP_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)};
+        }
+        s->th = std::thread(slot_main, ctx, s);
+        ctx->slots.push_back(s);
+    }
+}
+
+void stop_slots(erp_ctx* ctx) {
+    {
+        std::lock_guard<std::mutex> lk(ctx->qmu);
+        ctx->stopping = true;
+        // jobs still queued at destruction never run
+        for (erp_job* j : ctx->queue) {
+            std::lock_guard<std::mutex> jl(j->mu);
+            j->status = ERP_E_INVALID;
+            j->err = "context destroyed before the job ran";
+            j->ctx = nullptr;
+            j->done = true;
+            j->cv.notify_all();
+        }
+        ctx->queue.clear();
+    }
+    ctx->qcv.notify_all();
+    for (ErpSlot* s : ctx->slots) {
+        if (s->th.joinable()) s->th.join();
+        release_work(s->work);
+        s->out.release();
+        s->nvalid.release();
+        if (s->stream) hipStreamDestroy(s->stream);
+        delete s;
+    }
+    ctx->slots.clear();
+}
+
+constexpr int kDefaultSlots = 4;
+constexpr int kMaxSlots = 16;
+
+}  // namespace
+
 
 extern "C" {
 
@@ -141,16 +269,13 @@ int erp_ctx_create(int device, erp_ctx** out) {
 void erp_ctx_destroy(erp_ctx* ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
+    stop_slots(ctx);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     for (auto* c : ctx->chirps) delete c;
     for (auto* b : ctx->blues) delete b;
     ctx->chirps.clear();
     ctx->blues.clear();
-    WorkBufs& w = ctx->work;
-    DevBuf* bufs[] = {&w.cnt, &w.off, &w.V, &w.Vd, &w.tcnt, &w.tlist, &w.tcount, &w.rec,
-                      &w.bounds, &w.psurf, &w.refl, &w.shotS, &w.blu, &w.tA, &w.tB, &w.tC,
-                      &w.pairs, &w.t0trace, &w.shots, &w.prisms, &w.out, &w.nvalid};
-    for (DevBuf* b : bufs) b->release();
+    release_work(ctx->work);
     ctx->twiddle.release();
     for (auto& t : ctx->inter) {
         t.second->release();
@@ -277,7 +402,7 @@ int erp_run_dev(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
             throw ErpError{ERP_E_INVALID, "mesh/targets belong to another context"};
         set_device(ctx);
         erp_run_impl(ctx, par, mesh, targets, shots, ns, prisms, nprisms, (float2*)out_dev,
-                     (int32_t*)nvalid_dev, flags, ctx->stream, ctx->work);
+                     (int32_t*)nvalid_dev, flags, ctx->stream, ctx->work, &ctx->prof);
         ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
         ctx->prof.resolve();
     });
@@ -305,7 +430,7 @@ int erp_run(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         DevBuf& ob = ctx->work.tB;  // output staging
         ob.ensure(sizeof(float2) * std::max<size_t>(nout, 1));
         erp_run_impl(ctx, par, mesh, targets, shots, ns, prisms, nprisms, ob.as<float2>(),
-                     ctx->work.nvalid.as<int32_t>(), flags, ctx->stream, ctx->work);
+                     ctx->work.nvalid.as<int32_t>(), flags, ctx->stream, ctx->work, &ctx->prof);
         if (nout)
             ERP_HIP_TRY(hipMemcpyAsync(out, ob.p, sizeof(float2) * nout, hipMemcpyDeviceToHost,
                                        ctx->stream));
@@ -317,59 +442,46 @@ int erp_run(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     });
 }
 
+int erp_serve_slots(erp_ctx* ctx, int n) {
+    if (!ctx || n < 1 || n > kMaxSlots) return ERP_E_INVALID;
+    return guarded(ctx, [&] {
+        set_device(ctx);
+        start_slots(ctx, n);
+    });
+}
+
 int erp_submit(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                const erp_targets* targets, const erp_shot* shots, int64_t ns,
                const erp_prism* prisms, int64_t nprisms, float* out, int32_t* nvalid,
                uint32_t flags, erp_job** job) {
     if (!ctx || !job) return ERP_E_INVALID;
     *job = nullptr;
-    erp_job* j = new (std::nothrow) erp_job();
-    if (!j) return ERP_E_NOMEM;
-    j->ctx = ctx;
-    j->stream = nullptr;
-    j->done = nullptr;
+    erp_job* j = nullptr;
     int rc = guarded(ctx, [&] {
         validate_params(par);
         if (!mesh || !shots || ns < 0 || !out || nprisms < 0 || (nprisms > 0 && !prisms))
             throw ErpError{ERP_E_INVALID, "invalid argument"};
+        if (mesh->ctx != ctx || (targets && targets->ctx != ctx))
+            throw ErpError{ERP_E_INVALID, "mesh/targets belong to another context"};
         set_device(ctx);
-        ERP_HIP_TRY(hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking));
-        ERP_HIP_TRY(hipEventCreateWithFlags(&j->done, hipEventDisableTiming));
+        if (ctx->slots.empty()) start_slots(ctx, kDefaultSlots);
+        j = new erp_job();
+        j->ctx = ctx;
+        j->par = *par;
+        j->mesh = mesh;
+        j->targets = targets;
+        j->shots.assign(shots, shots + ns);
+        if (nprisms) j->prisms.assign(prisms, prisms + nprisms);
+        j->flags = flags;
+        j->host_out = out;
+        j->host_nvalid = nvalid;
         {
-            // chirp/Bluestein caches are shared: build them under the lock
-            std::lock_guard<std::mutex> lk(ctx->mu);
-            const size_t nout = (size_t)ns * par->nr;
-            j->out.ensure(sizeof(float2) * std::max<size_t>(nout, 1));
-            j->nvalid.ensure(sizeof(int32_t) * (size_t)std::max<int64_t>(ns, 1));
-            // make sure context-level tables exist before the job's stream uses them
-            ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
-            const bool prof_on = ctx->prof.on;
-            ctx->prof.on = false;  // stage timers cover synchronous runs only
-            try {
-                erp_run_impl(ctx, par, mesh, targets, shots, ns, prisms, nprisms,
-                             j->out.as<float2>(), j->nvalid.as<int32_t>(), flags, j->stream,
-                             j->work);
-            } catch (...) {
-                ctx->prof.on = prof_on;
-                throw;
-            }
-            ctx->prof.on = prof_on;
-            j->host_out = out;
-            j->host_nvalid = nvalid;
-            j->n_out = (int64_t)nout;
-            j->ns = ns;
-            if (nout)
-                ERP_HIP_TRY(hipMemcpyAsync(out, j->out.p, sizeof(float2) * nout,
-                                           hipMemcpyDeviceToHost, j->stream));
-            if (nvalid && ns)
-                ERP_HIP_TRY(hipMemcpyAsync(nvalid, j->nvalid.p, sizeof(int32_t) * ns,
-                                           hipMemcpyDeviceToHost, j->stream));
-            ERP_HIP_TRY(hipEventRecord(j->done, j->stream));
+            std::lock_guard<std::mutex> lk(ctx->qmu);
+            ctx->queue.push_back(j);
         }
+        ctx->qcv.notify_one();
     });
     if (rc != ERP_OK) {
-        if (j->done) hipEventDestroy(j->done);
-        if (j->stream) hipStreamDestroy(j->stream);
         delete j;
         return rc;
     }
@@ -377,37 +489,29 @@ int erp_submit(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     return ERP_OK;
 }
 
-static void job_free(erp_job* j) {
-    hipSetDevice(j->ctx->device);
-    WorkBufs& w = j->work;
-    DevBuf* bufs[] = {&w.cnt, &w.off, &w.V, &w.Vd, &w.tcnt, &w.tlist, &w.tcount, &w.rec,
-                      &w.bounds, &w.psurf, &w.refl, &w.shotS, &w.blu, &w.tA, &w.tB, &w.tC,
-                      &w.pairs, &w.t0trace, &w.shots, &w.prisms, &w.out, &w.nvalid};
-    for (DevBuf* b : bufs) b->release();
-    j->out.release();
-    j->nvalid.release();
-    if (j->done) hipEventDestroy(j->done);
-    if (j->stream) hipStreamDestroy(j->stream);
+static int job_finish(erp_job* j) {
+    const int rc = j->status;
+    if (rc != ERP_OK && j->ctx) j->ctx->last_error = j->err;
     delete j;
+    return rc;
 }
 
 int erp_poll(erp_job* j) {
     if (!j) return ERP_E_INVALID;
-    hipError_t e = hipEventQuery(j->done);
-    if (e == hipErrorNotReady) return ERP_E_NOTREADY;
-    int rc = (e == hipSuccess) ? ERP_OK : ERP_E_HIP;
-    if (rc != ERP_OK) j->ctx->last_error = hipGetErrorString(e);
-    job_free(j);
-    return rc;
+    {
+        std::lock_guard<std::mutex> lk(j->mu);
+        if (!j->done) return ERP_E_NOTREADY;
+    }
+    return job_finish(j);
 }
 
 int erp_wait(erp_job* j) {
     if (!j) return ERP_E_INVALID;
-    hipError_t e = hipEventSynchronize(j->done);
-    int rc = (e == hipSuccess) ? ERP_OK : ERP_E_HIP;
-    if (rc != ERP_OK) j->ctx->last_error = hipGetErrorString(e);
-    job_free(j);
-    return rc;
+    {
+        std::unique_lock<std::mutex> lk(j->mu);
+        j->cv.wait(lk, [&] { return j->done; });
+    }
+    return job_finish(j);
 }
 
 int erp_profile_enable(erp_ctx* ctx, int on) {

@@ -809,6 +809,7 @@ void fft_init_twiddles(erp_ctx* ctx) {
 
 // inter-pass twiddle table of a two-pass plan (cached per size in the context)
 static const float2* inter_table(erp_ctx* ctx, const FftPlan& p, hipStream_t s) {
+    std::lock_guard<std::recursive_mutex> lk(ctx->cache_mu);
     for (auto& t : ctx->inter)
         if (t.first == p.N) return t.second->as<float2>();
     DevBuf* b = new DevBuf();
@@ -816,6 +817,7 @@ static const float2* inter_table(erp_ctx* ctx, const FftPlan& p, hipStream_t s) 
     hipLaunchKernelGGL(k_inter_init, dim3((p.N + 255) / 256), dim3(256), 0, s, b->as<float2>(),
                        p.N1, p.N);
     ERP_CHECK_LAUNCH("k_inter_init");
+    ERP_HIP_TRY(hipStreamSynchronize(s));  // usable from other streams once published
     ctx->inter.push_back({p.N, b});
     return b->as<float2>();
 }

@@ -6,7 +6,11 @@
 #include <stdint.h>
 #include <string>
 #include <vector>
+#include <atomic>
+#include <condition_variable>
+#include <deque>
 #include <mutex>
+#include <thread>
 
 #include "../../include/erp.h"
 
@@ -130,6 +134,7 @@ struct WorkBufs {
     DevBuf rec;        // compacted facet records
     DevBuf bounds;     // per-shot bounds of illuminated facets
     DevBuf psurf;      // S x P complex: surface phasor trace (padded)
+    DevBuf srow;       // S x nr FP64 / fixed-point surface bins + S per-shot maxima
     DevBuf refl;       // S x nr complex
     DevBuf shotS;      // S x P complex: per-shot spectral accumulator
     DevBuf blu;        // S x M complex: Bluestein workspace
@@ -163,18 +168,31 @@ struct StageTimer {
     int cur = -1;
 };
 
+struct ErpSlot;  // asynchronous job worker (erp_api.hip)
+
 struct erp_ctx {
     int device = 0;
     StageTimer prof;
     hipStream_t stream = nullptr;
     std::string last_error;
-    DevBuf twiddle;  // ladder table of the LDS FFT stages (8192 complex64)
+    DevBuf twiddle;  // ladder table of the LDS FFT stages (65536 complex64)
     std::vector<std::pair<int, DevBuf*>> inter;  // inter-pass twiddles per size
     std::vector<ChirpSpec*> chirps;
     std::vector<BlueTables*> blues;
     WorkBufs work;
-    std::mutex mu;
+    std::mutex mu;        // serialises the synchronous entry points
+    // guards inter / chirps / blues (shared by all streams); recursive because
+    // building a chirp or Bluestein table runs FFTs that look up inter tables
+    std::recursive_mutex cache_mu;
     int num_cu = 0;
+    // asynchronous jobs: worker slots, each with its own thread, stream and
+    // work buffers, fed from one FIFO
+    std::vector<ErpSlot*> slots;
+    std::deque<erp_job*> queue;
+    std::mutex qmu;
+    std::condition_variable qcv;
+    bool stopping = false;
+    std::atomic<int> active{0};  // jobs running in slots
 };
 
 // ---------------------------------------------------------------------------
@@ -211,11 +229,12 @@ void target_spectra_fused(erp_ctx* ctx, const FftPlan& p, float2* A, float2* Y,
 // ---------------------------------------------------------------------------
 // pipeline (erp_pipeline.hip)
 // ---------------------------------------------------------------------------
+// timer: stage timers to record into, or nullptr (asynchronous jobs)
 void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                   const erp_targets* targets, const erp_shot* shots, int64_t ns,
                   const erp_prism* prisms, int64_t nprisms, float2* out_dev,
                   int32_t* nvalid_dev, uint32_t flags, hipStream_t stream,
-                  WorkBufs& work);
+                  WorkBufs& work, StageTimer* timer);
 int64_t erp_footprint_impl(const erp_params* par, int64_t F, int64_t T, int64_t ns);
 
 inline int next_pow2(int64_t n) {

@@ -412,13 +412,48 @@ __device__ __forceinline__ void glb_add2(float2* row, int idx, float2 v) {
     atomicAdd(&row[idx].x, v.x);
     atomicAdd(&row[idx].y, v.y);
 }
+__device__ __forceinline__ void glb_add2d(double2* row, int idx, double x, double y) {
+    atomicAdd(&row[idx].x, x);
+    atomicAdd(&row[idx].y, y);
+}
+
+// Deterministic accumulation: every term is rounded once to a signed 64-bit
+// fixed-point value v * 2^E and summed with integer atomics, which are exact
+// and so independent of the order in which threads and workgroups arrive.
+// E is chosen from a bound M >= |component| of every term and the number of
+// terms n that can reach one bin: |sum| * 2^E <= 2 n M 2^E <= 2^61.
+__device__ __forceinline__ double det_scale(double n, double M) {
+    if (!(M > 0.0)) return 0.0;
+    int e;
+    frexp(2.0 * n * M, &e);  // 2 n M < 2^e
+    return ldexp(1.0, 61 - e);
+}
+__device__ __forceinline__ unsigned long long det_fix(float v, double scale) {
+    return (unsigned long long)__double2ll_rn((double)v * scale);
+}
+__device__ __forceinline__ void lds_add2_fix(unsigned long long* win, int idx, float2 v,
+                                             double scale) {
+    atomicAdd(&win[2 * idx], det_fix(v.x, scale));
+    atomicAdd(&win[2 * idx + 1], det_fix(v.y, scale));
+}
+__device__ __forceinline__ void glb_add2_fix(unsigned long long* row, int idx, float2 v,
+                                             double scale) {
+    atomicAdd(&row[2 * idx], det_fix(v.x, scale));
+    atomicAdd(&row[2 * idx + 1], det_fix(v.y, scale));
+}
+__device__ __forceinline__ double det_unfix(unsigned long long u, double inv_scale) {
+    return (double)(long long)u * inv_scale;
+}
 
 __global__ void __launch_bounds__(NT) k_surface(
     const float* __restrict__ mesh, int64_t Fpad, int64_t F, const erp_shot* __restrict__ shots,
     const unsigned long long* __restrict__ mask, const int* __restrict__ off,
     const int* __restrict__ Vs, const int64_t* __restrict__ rec_base, int nblk, KPar kp,
-    Rec rec, int write_rec, int do_trace, float2* __restrict__ psurf, int64_t shot0,
-    Bounds* __restrict__ bounds) {
+    Rec rec, int write_rec, int do_trace, double2* __restrict__ srow, int64_t shot0,
+    Bounds* __restrict__ bounds, int det, unsigned* __restrict__ smax) {
+    // det 0: FP64 bins (LDS window, FP64 atomics into srow)
+    // det 1: first deterministic pass, largest |component| per shot into smax
+    // det 2: second deterministic pass, fixed-point bins (srow holds int64)
     const int blk = blockIdx.x, s = blockIdx.y;
     const unsigned long long* mw = mask + ((size_t)s * nblk + blk) * MASKW;
     __shared__ unsigned long long m_s[MASKW];
@@ -587,6 +622,14 @@ __global__ void __launch_bounds__(NT) k_surface(
         bmin = min(bmin, b);
     }
 
+    if (det == 1 && do_trace) {
+        float m = 0.f;
+#pragma unroll
+        for (int q = 0; q < FPT; ++q)
+            if (bq[q] >= 0) m = fmaxf(m, fmaxf(fabsf(cq[q].x), fabsf(cq[q].y)));
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if (lane == 0 && m > 0.f) atomicMax(&smax[s], __float_as_uint(m));
+    }
     // block reductions: window base and bounds
     for (int o = 32; o > 0; o >>= 1) {
         bmin = min(bmin, __shfl_xor(bmin, o));
@@ -620,11 +663,42 @@ __global__ void __launch_bounds__(NT) k_surface(
         atomicMin(&bd->itd_min, (unsigned long long)__double_as_longlong(d0));
         atomicMax(&bd->itd_max, (unsigned long long)__double_as_longlong(d1));
     }
-    if (!do_trace) return;
+    if (!do_trace || det == 1) return;
     const int base = wmin_s;
     if (base == 0x7FFFFFFF) return;
-    float2* row = psurf + (size_t)s * kp.P_fft;
     const int nr = kp.nr;
+    double2* row = srow + (size_t)s * nr;
+    if (det == 2) {
+        // the bound covers every term of the shot: 2 terms per facet, V facets
+        const double scale = det_scale((double)V, (double)__uint_as_float(smax[s]));
+        unsigned long long* wu = reinterpret_cast<unsigned long long*>(win);
+        unsigned long long* ru = reinterpret_cast<unsigned long long*>(row);
+#pragma unroll
+        for (int q = 0; q < FPT; ++q) {
+            const int b = bq[q];
+            if (b < 0) continue;
+            const float w = wq[q];
+            const float2 c = cq[q];
+            const float2 c0 = make_float2(c.x * (1.f - w), c.y * (1.f - w));
+            const int k0 = b - base;
+            if (k0 < SWIN) lds_add2_fix(wu, k0, c0, scale); else glb_add2_fix(ru, b, c0, scale);
+            if (!spec && b + 1 < nr) {
+                const float2 c1 = make_float2(c.x * w, c.y * w);
+                if (k0 + 1 < SWIN) lds_add2_fix(wu, k0 + 1, c1, scale);
+                else glb_add2_fix(ru, b + 1, c1, scale);
+            }
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < SWIN; k += NT) {
+            const int b = base + k;
+            const unsigned long long x = wu[2 * k], y = wu[2 * k + 1];
+            if (b < nr && (x | y)) {
+                atomicAdd(&ru[2 * b], x);
+                atomicAdd(&ru[2 * b + 1], y);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < FPT; ++q) {
         const int b = bq[q];
@@ -633,17 +707,43 @@ __global__ void __launch_bounds__(NT) k_surface(
         const float2 c = cq[q];
         const float2 c0 = make_float2(c.x * (1.f - w), c.y * (1.f - w));
         const int k0 = b - base;
-        if (k0 < SWIN) lds_add2(win, k0, c0); else glb_add2(row, b, c0);
+        if (k0 < SWIN) lds_add2(win, k0, c0); else glb_add2d(row, b, c0.x, c0.y);
         if (!spec && b + 1 < nr) {
             const float2 c1 = make_float2(c.x * w, c.y * w);
-            if (k0 + 1 < SWIN) lds_add2(win, k0 + 1, c1); else glb_add2(row, b + 1, c1);
+            if (k0 + 1 < SWIN) lds_add2(win, k0 + 1, c1); else glb_add2d(row, b + 1, c1.x, c1.y);
         }
     }
     __syncthreads();
     for (int k = threadIdx.x; k < SWIN; k += NT) {
         const double2 v = win[k];
         const int b = base + k;
-        if (b < nr && (v.x != 0.0 || v.y != 0.0)) glb_add2(row, b, make_float2((float)v.x, (float)v.y));
+        if (b < nr && (v.x != 0.0 || v.y != 0.0)) glb_add2d(row, b, v.x, v.y);
+    }
+}
+
+// surface rows (FP64 or fixed point, stride nr) -> FP32 rows of the FFT
+// buffer (stride P); bins nr..P-1 are left as they are (zero)
+__global__ void k_surf_rows(float2* __restrict__ psurf, int P, const double2* __restrict__ srow,
+                            int nr, int S, int det, const unsigned* __restrict__ smax,
+                            const int* __restrict__ Vs) {
+    const int s = blockIdx.y;
+    if (s >= S) return;
+    const double2* r = srow + (size_t)s * nr;
+    double inv = 0.0;
+    if (det) {
+        const double sc = det_scale((double)Vs[s], (double)__uint_as_float(smax[s]));
+        inv = sc > 0.0 ? 1.0 / sc : 0.0;
+    }
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nr; b += gridDim.x * blockDim.x) {
+        const double2 v = r[b];
+        float2 o;
+        if (det) {
+            o = make_float2((float)det_unfix(__double_as_longlong(v.x), inv),
+                            (float)det_unfix(__double_as_longlong(v.y), inv));
+        } else {
+            o = make_float2((float)v.x, (float)v.y);
+        }
+        psurf[(size_t)s * P + b] = o;
     }
 }
 
@@ -1222,6 +1322,7 @@ __device__ __forceinline__ void tri_add(const Tri& t, int base, int len, int nr,
 // decided on the approximate range unless it lies within 1e-7 bin of a bin
 // edge, where the exact FP64 sqrt and division of the oracle are repeated, so
 // bin indices stay identical to the FP64 reference.
+template <bool DET>
 __global__ void __launch_bounds__(NTF) k_target_accum_fast(
     const float* __restrict__ tg, int64_t Tpad, const int* __restrict__ pshot,
     const int* __restrict__ ptgt, const erp_shot* __restrict__ shots,
@@ -1252,7 +1353,8 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
     base = max(base, 0);
     bool fits = false;
     int len = TWF;
-    if (pbase) {
+    int top = nr;
+    if (pbase || DET) {
         const float fx = fmaxf(fabsf(o2f(bd.xmin) - txf), fabsf(o2f(bd.xmax) - txf));
         const float fy = fmaxf(fabsf(o2f(bd.ymin) - tyf), fabsf(o2f(bd.ymax) - tyf));
         const float fz = fmaxf(fabsf(o2f(bd.zmin) - tzf), fabsf(o2f(bd.zmax) - tzf));
@@ -1260,47 +1362,108 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
         const double ithmax = __longlong_as_double((long long)bd.itd_max);
         const double rhi = ithmax + kp.n_d * tmax * (1.0 + 1e-6) + 1.0;
         // every bin written (b and b + 1) is below top
-        const int top = min((int)ceil((rhi - rst) / kp.drd) + 2, nr);
+        top = min((int)ceil((rhi - rst) / kp.drd) + 2, nr);
         fits = top - base <= TWF;
         // live window: [base, base + len), len a multiple of 64 (only it is
         // zeroed, flushed and read by the spectral chain)
         if (fits) len = max(0, min(TWF, (top - base + 63) & ~63));
-        if (threadIdx.x == 0) {
-            pbase[pi] = fits ? base : -1;
-            pwin[pi] = len;
-        }
+    }
+    if (pbase && threadIdx.x == 0) {
+        pbase[pi] = fits ? base : -1;
+        pwin[pi] = len;
     }
     float2* rowA = outA + (size_t)pi * P;
-    if (!fits) {
+    if (!fits || (DET && !pbase)) {
+        // bins outside the window(s) [base, base + len) or, deterministic,
+        // [base, top): zero, then reached only by out-of-window atomics
+        const int hi = fits ? base + len : (DET ? max(top, base) : base + TWF);
         for (int k = threadIdx.x; k < P; k += NTF)
-            if (k < base || k >= base + TWF || k >= nr) rowA[k] = make_float2(0.f, 0.f);
+            if (k < base || k >= hi || k >= nr) rowA[k] = make_float2(0.f, 0.f);
     }
-    for (int k = threadIdx.x; k < len; k += NTF) winA[k] = make_double2(0.0, 0.0);
-    __threadfence();
-    __syncthreads();
 
     const double inv_dr = kp.inv_dr, n_d = kp.n_d, two_cyc = 2.0 * kp.inv_cyc;
     const bool tn_up = tnx == 0.f && tny == 0.f && tnz == 1.f;
-    // two facets per thread and iteration: two independent dependency chains
-    // for the scheduler to interleave
-    for (int k = threadIdx.x; k < V; k += 2 * NTF) {
-        const bool two = k + NTF < V;
-        const Tri ta = tri_eval(rec, r0 + k, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx, tny,
-                                tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr, lossless, prisms, np,
-                                kp);
-        Tri tb;
-        tb.valid = false;
-        if (two)
-            tb = tri_eval(rec, r0 + k + NTF, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx, tny,
-                          tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr, lossless, prisms, np, kp);
-        if (ta.valid) tri_add(ta, base, len, nr, winA, rowA);
-        if (tb.valid) tri_add(tb, base, len, nr, winA, rowA);
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < len; k += NTF) {
-        const int b = base + k;
-        if (b >= nr) break;
-        rowA[b] = make_float2((float)winA[k].x, (float)winA[k].y);
+
+    if constexpr (DET) {
+        // bound of every term of this pair: |component| <= F Dmax / Ttd (the
+        // four sinc factors, the loss factor and the two-bin weights are <= 1)
+        __shared__ float red[NTF / 64];
+        float m = 0.f;
+        for (int k = threadIdx.x; k < V; k += NTF) {
+            const float4 qa = rec.qa[r0 + k];
+            const double dx = tx - (double)qa.x, dy = ty - (double)qa.y, dz = tz - (double)qa.z;
+            const double t2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            m = fmaxf(m, qa.w * rsqrtf((float)t2));
+        }
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+        __syncthreads();
+        m = red[0];
+        for (int w = 1; w < NTF / 64; ++w) m = fmaxf(m, red[w]);
+        // 2x margin for the rounding of the sinc quotients
+        const double scale = det_scale((double)V, 2.0 * (double)m * (double)kp.Dmax);
+        const double inv = scale > 0.0 ? 1.0 / scale : 0.0;
+        unsigned long long* wu = reinterpret_cast<unsigned long long*>(winA);
+        // one LDS window of TWF bins at a time over [base, top)
+        for (int wb = base; wb < top || wb == base; wb += TWF) {
+            const int wl = fits ? len : min(TWF, max(0, top - wb));
+            for (int k = threadIdx.x; k < wl; k += NTF) winA[k] = make_double2(0.0, 0.0);
+            __syncthreads();
+            for (int k = threadIdx.x; k < V; k += NTF) {
+                const Tri ta = tri_eval(rec, r0 + k, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz,
+                                        tnx, tny, tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr,
+                                        lossless, prisms, np, kp);
+                if (!ta.valid) continue;
+                const float2 c0 = make_float2(ta.c.x * (1.f - ta.w), ta.c.y * (1.f - ta.w));
+                const float2 c1 = make_float2(ta.c.x * ta.w, ta.c.y * ta.w);
+                const int k0 = ta.b - wb;
+                const bool first = wb == base, last = wb + TWF >= top;
+                if (k0 >= 0 && k0 < wl) lds_add2_fix(wu, k0, c0, scale);
+                else if ((ta.b < base && first) || (ta.b >= top && last)) glb_add2(rowA, ta.b, c0);
+                if (ta.b + 1 < nr) {
+                    if (k0 + 1 >= 0 && k0 + 1 < wl) lds_add2_fix(wu, k0 + 1, c1, scale);
+                    else if ((ta.b + 1 < base && first) || (ta.b + 1 >= top && last))
+                        glb_add2(rowA, ta.b + 1, c1);
+                }
+            }
+            __syncthreads();
+            for (int k = threadIdx.x; k < wl; k += NTF) {
+                const int b = wb + k;
+                if (b >= nr) break;
+                rowA[b] = make_float2((float)det_unfix(wu[2 * k], inv),
+                                      (float)det_unfix(wu[2 * k + 1], inv));
+            }
+            __syncthreads();
+            if (fits) break;
+        }
+        return;
+    } else {
+        for (int k = threadIdx.x; k < len; k += NTF) winA[k] = make_double2(0.0, 0.0);
+        __threadfence();
+        __syncthreads();
+
+        // two facets per thread and iteration: two independent dependency
+        // chains for the scheduler to interleave
+        for (int k = threadIdx.x; k < V; k += 2 * NTF) {
+            const bool two = k + NTF < V;
+            const Tri ta = tri_eval(rec, r0 + k, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx,
+                                    tny, tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr, lossless,
+                                    prisms, np, kp);
+            Tri tb;
+            tb.valid = false;
+            if (two)
+                tb = tri_eval(rec, r0 + k + NTF, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx,
+                              tny, tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr, lossless, prisms,
+                              np, kp);
+            if (ta.valid) tri_add(ta, base, len, nr, winA, rowA);
+            if (tb.valid) tri_add(tb, base, len, nr, winA, rowA);
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < len; k += NTF) {
+            const int b = base + k;
+            if (b >= nr) break;
+            rowA[b] = make_float2((float)winA[k].x, (float)winA[k].y);
+        }
     }
 }
 
@@ -1585,6 +1748,7 @@ KPar make_kpar(const erp_params& p, int P_fft) {
 const float2* get_chirp(erp_ctx* ctx, int nr, float dr, float res, int kind, float rst, int P,
                         const FftPlan& plan, hipStream_t st, bool natural = false) {
     ChirpKey key{nr, dr, res, kind, kind == 1 ? rst : 0.f};
+    std::lock_guard<std::recursive_mutex> lk(ctx->cache_mu);
     ChirpSpec* c = nullptr;
     for (ChirpSpec* e : ctx->chirps)
         if (e->key == key) c = e;
@@ -1600,9 +1764,12 @@ const float2* get_chirp(erp_ctx* ctx, int nr, float dr, float res, int kind, flo
         hipLaunchKernelGGL(k_scale, dim3((P + 255) / 256), dim3(256), 0, st,
                            c->spec.as<float2>(), P, 1.0f / (float)P);
         ERP_CHECK_LAUNCH("k_scale");
+        // other streams may use the new spectrum as soon as the lock is released
+        ERP_HIP_TRY(hipStreamSynchronize(st));
         int n1 = 0;
         for (ChirpSpec* e : ctx->chirps) n1 += e->key.kind == 1;
-        if (n1 > 16) {
+        // evict only while no asynchronous job can hold a pointer into the cache
+        if (n1 > 16 && ctx->active.load() == 0) {
             for (size_t i = 0; i < ctx->chirps.size(); ++i)
                 if (ctx->chirps[i]->key.kind == 1) {
                     ERP_HIP_TRY(hipStreamSynchronize(st));
@@ -1617,11 +1784,13 @@ const float2* get_chirp(erp_ctx* ctx, int nr, float dr, float res, int kind, flo
     if (!c->nat.p) {
         c->nat.ensure(sizeof(float2) * (size_t)P);
         fft_natural_order(plan, c->spec.as<float2>(), c->nat.as<float2>(), st);
+        ERP_HIP_TRY(hipStreamSynchronize(st));
     }
     return c->nat.as<float2>();
 }
 
 BlueTables* get_blue(erp_ctx* ctx, int L, float smpl, float B, hipStream_t st) {
+    std::lock_guard<std::recursive_mutex> lk(ctx->cache_mu);
     for (BlueTables* b : ctx->blues)
         if (b->L == L && b->smpl == smpl && b->B == B) return b;
     BlueTables* b = new BlueTables();
@@ -1641,11 +1810,13 @@ BlueTables* get_blue(erp_ctx* ctx, int L, float smpl, float B, hipStream_t st) {
     hipLaunchKernelGGL(k_scale, dim3((b->M + 255) / 256), dim3(256), 0, st, b->Bw.as<float2>(),
                        b->M, 1.0f / (float)b->M);
     ERP_CHECK_LAUNCH("k_scale");
-    if (ctx->blues.size() > 8) {
+    if (ctx->blues.size() > 8 && ctx->active.load() == 0) {
+        ERP_HIP_TRY(hipStreamSynchronize(st));
         delete ctx->blues.front();
         ctx->blues.erase(ctx->blues.begin());
     }
     ctx->blues.push_back(b);
+    ERP_HIP_TRY(hipStreamSynchronize(st));
     return b;
 }
 
@@ -1678,27 +1849,40 @@ struct BatchPlan {
 
 }  // namespace
 
+// Device bytes of one run, following the allocations of erp_run_impl: the
+// work buffers of one batch of up to 32 shots (facet records counted for
+// every facet illuminated by every shot of the batch, an upper bound) plus
+// the ns x nr output and counts.
 int64_t erp_footprint_impl(const erp_params* par, int64_t F, int64_t T, int64_t ns) {
     const int64_t nr = par->nr;
     const int64_t P = next_pow2(2 * nr - 1);
     const int64_t M = next_pow2(2 * (2 * nr - 1) - 1);
-    const int64_t S = std::min<int64_t>(ns, 32);
+    const int64_t S = std::max<int64_t>(1, std::min<int64_t>(ns, 32));
     const int64_t nblk = (F + FB - 1) / FB;
+    const bool spec = par->mode & ERP_MODE_SPECULAR;
     int64_t b = 0;
-    b += S * nblk * (MASKW * 8 + 8);          // mask + cnt + off
-    b += S * std::min<int64_t>(F, 1 << 20) * 76;  // facet records (typical cap)
-    b += S * P * 8 * 3 + S * M * 8 + S * nr * 8 * 3;
+    b += S * nblk * (MASKW * 8 + 8);          // mask + counts + offsets
+    b += S * F * 128;                         // facet records (128 B per facet-shot)
+    b += S * P * 8 * 3 + S * M * 8;           // surface, spectral sum, type-0 rows; Bluestein
+    b += S * nr * (16 + 8 + 16) + S * 4;      // FP64 surface bins, reflected, refracted
     if (T > 0) {
-        b += S * T * 8;                       // target lists
-        b += 3 * 256 * P * 8;                 // target chunk buffers
+        const int64_t nbuf = spec ? 4 : 3;
+        int64_t C = std::max<int64_t>(1, ((int64_t)1536 << 20) / (P * 8 * nbuf));
+        C = std::min<int64_t>(C, 8192);
+        const int64_t gcap = C / 16 + S + 2;
+        b += S * T * 8 + S * 8;               // target lists and counts
+        b += P * 8 * (C * nbuf + gcap);       // per-target chunk rows and partial sums
+        b += 4 * (C * 4 + 2 * gcap + 3 * S + 16);
     }
+    b += ns * nr * 8 + ns * 4;                // the radargram and the counts
     return b;
 }
 
 void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                   const erp_targets* targets, const erp_shot* shots, int64_t ns,
                   const erp_prism* prisms, int64_t nprisms, float2* out_dev,
-                  int32_t* nvalid_dev, uint32_t flags, hipStream_t st, WorkBufs& W) {
+                  int32_t* nvalid_dev, uint32_t flags, hipStream_t st, WorkBufs& W,
+                  StageTimer* timer) {
     const erp_params& p = *par;
     const uint32_t mode = p.mode;
     const bool conv = mode & ERP_MODE_CONVOLUTION;
@@ -1706,9 +1890,12 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     const bool spec = mode & ERP_MODE_SPECULAR;
     const int nr = p.nr;
     if (nr <= 0 || nr > 32767) throw ErpError{ERP_E_INVALID, "nr must be in [1, 32767]"};
-    if (flags & ~(uint32_t)ERP_RUN_RAW_SURFACE)
+    if (flags & ~(uint32_t)(ERP_RUN_RAW_SURFACE | ERP_RUN_DETERMINISTIC))
+        throw ErpError{ERP_E_UNSUPPORTED, "run flags PHASE_F32 / PHASE_DF are not implemented"};
+    const bool det = flags & ERP_RUN_DETERMINISTIC;
+    if (det && spec)
         throw ErpError{ERP_E_UNSUPPORTED,
-                       "run flags DETERMINISTIC / PHASE_F32 / PHASE_DF are not implemented"};
+                       "ERP_RUN_DETERMINISTIC covers the non-specular (convolution) method"};
     if (!spec && !(conv && lin))
         throw ErpError{ERP_E_UNSUPPORTED,
                        "non-specular runs need convolution and convolution_linear "
@@ -1772,6 +1959,10 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     W.bounds.ensure(sizeof(Bounds) * (size_t)Smax);
     W.tcount.ensure(sizeof(int64_t) * (size_t)Smax);  // rec_base
     W.psurf.ensure(sizeof(float2) * (size_t)Smax * P);
+    const size_t srow_f2 = (size_t)Smax * nr * 2 + ((size_t)Smax + 1) / 2;  // in float2 units
+    W.srow.ensure(sizeof(float2) * srow_f2);
+    double2* srow = W.srow.as<double2>();
+    unsigned* smax = reinterpret_cast<unsigned*>(srow + (size_t)Smax * nr);
     W.refl.ensure(sizeof(float2) * (size_t)Smax * nr);
     W.out.ensure(sizeof(float2) * (size_t)Smax * nr * 2);  // refr + refr0
     if (any_typed) {
@@ -1799,7 +1990,8 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                                    hipMemcpyHostToDevice, st));
         const erp_shot* dshots = W.shots.as<erp_shot>();
 
-        StageTimer& PT_ = ctx->prof;
+        StageTimer none_;
+        StageTimer& PT_ = timer ? *timer : none_;
         // 1-2: aperture mask, counts, offsets
         PT_.begin(ERP_STAGE_MASK, st, (double)F * S);
         hipLaunchKernelGGL(k_illum_mask, dim3(nblk, S), dim3(NT), 0, st, mesh->soa.as<float>(),
@@ -1842,11 +2034,25 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         double Vsum = 0;
         for (int s = 0; s < S; ++s) Vsum += hV[s];
         PT_.begin(ERP_STAGE_SURFACE, st, Vsum);
+        if (do_trace) zero(W.srow.as<float2>(), srow_f2, st);
+        const int write = (need_rec && Vtot > 0) ? 1 : 0;
         hipLaunchKernelGGL(k_surface, dim3(nblk, S), dim3(NT), 0, st, mesh->soa.as<float>(),
                            mesh->Fpad, F, dshots, dmask, W.off.as<int>(), dV, dbase, nblk, kp, rec,
-                           (need_rec && Vtot > 0) ? 1 : 0, do_trace ? 1 : 0, W.psurf.as<float2>(),
-                           s0, dbounds);
+                           write, do_trace ? 1 : 0, srow, s0, dbounds, det ? 1 : 0, smax);
         ERP_CHECK_LAUNCH("k_surface");
+        if (det && do_trace) {
+            // second pass: the same terms as fixed-point values scaled by the
+            // per-shot maxima of the first
+            hipLaunchKernelGGL(k_surface, dim3(nblk, S), dim3(NT), 0, st, mesh->soa.as<float>(),
+                               mesh->Fpad, F, dshots, dmask, W.off.as<int>(), dV, dbase, nblk, kp,
+                               rec, 0, 1, srow, s0, (Bounds*)nullptr, 2, smax);
+            ERP_CHECK_LAUNCH("k_surface");
+        }
+        if (do_trace) {
+            hipLaunchKernelGGL(k_surf_rows, dim3((nr + 255) / 256, S), dim3(256), 0, st,
+                               W.psurf.as<float2>(), P, srow, nr, S, det ? 1 : 0, smax, dV);
+            ERP_CHECK_LAUNCH("k_surf_rows");
+        }
         PT_.end(st);
 
         if (flags & ERP_RUN_RAW_SURFACE) {
@@ -1987,7 +2193,8 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                     for (int i = 0; i < n; ++i) triples += hV[pshot[i]];
                     PT_.begin(ERP_STAGE_TARGET_ACCUM, st, triples);
                     if (!spec) {
-                        hipLaunchKernelGGL(k_target_accum_fast, dim3(n), dim3(NTF), 0, st,
+                        auto kern = det ? k_target_accum_fast<true> : k_target_accum_fast<false>;
+                        hipLaunchKernelGGL(kern, dim3(n), dim3(NTF), 0, st,
                                            targets->xyzn.as<float>(), Tpad, dps, dpt, dshots, dV,
                                            dbase, rec, dbounds, dpr, (int)nprisms, kp, A,
                                            (fused || half) ? dpb : nullptr, dpw,

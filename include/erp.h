@@ -18,7 +18,7 @@
  *   erp_run / erp_run_dev              - the shot loop sim.cu:615-1008 and every
  *                                        kernel it launches, for ns shots.
  *   erp_submit / erp_poll / erp_wait   - asynchronous form of erp_run used by
- *                                        the serving loop (no reference
+ *   erp_serve_slots                      the serving loop (no reference
  *                                        counterpart: the reference is
  *                                        synchronous, sim.cu:992-998).
  *   erp_last_error                     - replaces checkCUDAError's exit()
@@ -57,7 +57,13 @@ extern "C" {
 #define ERP_MODE_DISABLE_SURFACE 0x20u /* "disable_surface"    */
 
 /* ---- run flags (erp_run flags argument) -------------------------------- */
-#define ERP_RUN_DETERMINISTIC 0x01u /* fixed-order reductions: bitwise repeatable */
+#define ERP_RUN_DETERMINISTIC 0x01u /* bitwise repeatable bins: each term rounded
+                                       once to 64-bit fixed point (scaled from a
+                                       per-shot / per-target bound) and summed
+                                       with integer atomics; non-specular method
+                                       only.  Replaces the order-dependent float
+                                       atomicAdd binning of radar_kernels.cu:
+                                       754-758 and 821-825                      */
 #define ERP_RUN_PHASE_F32     0x02u /* precision study: ranges and phase in FP32 */
 #define ERP_RUN_PHASE_DF      0x04u /* precision study: phase in double-float    */
 #define ERP_RUN_RAW_SURFACE   0x08u /* output the raw surface phasor trace before
@@ -158,13 +164,23 @@ int erp_run_dev(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                 const erp_prism* prisms, int64_t nprisms,
                 void* out_dev, void* nvalid_dev, uint32_t flags);
 
-/* Asynchronous form: queue a job on its own stream and return at once.  The
- * host buffers `shots`, `prisms` are copied before return; `out`/`nvalid`
- * must stay valid until erp_wait/erp_poll reports completion. */
+/* Asynchronous form of erp_run: queue a job and return at once.  Jobs run
+ * in worker slots (erp_serve_slots; 4 are started at the first submit), each
+ * a host thread with its own stream and work buffers, so jobs submitted
+ * while others run overlap on the device.  `par`, `shots` and `prisms` are
+ * copied before return; `mesh`, `targets`, `out` and `nvalid` must stay
+ * valid until erp_wait/erp_poll reports completion.  Every job must be
+ * finished with erp_wait or erp_poll before erp_ctx_destroy.  Replaces the
+ * reference's one-shot-at-a-time process loop (sim.cu:615-1008) for callers
+ * that keep several simulations in flight. */
 int erp_submit(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                const erp_targets* targets, const erp_shot* shots, int64_t ns,
                const erp_prism* prisms, int64_t nprisms,
                float* out, int32_t* nvalid, uint32_t flags, erp_job** job);
+/* Worker slots for erp_submit: starts slots up to n (1..16); never stops
+ * any.  Each slot keeps its work buffers between jobs.  No counterpart in
+ * the reference, which runs one simulation per process. */
+int erp_serve_slots(erp_ctx* ctx, int n);
 /* ERP_OK when done (job resources released), ERP_E_NOTREADY while running. */
 int erp_poll(erp_job* job);
 /* Block until the job is done; releases it. */

@@ -9,7 +9,8 @@ import numpy as np
 import pytest
 
 from europaraypaths_amd import surfaces, workloads
-from europaraypaths_amd._abi import (ERP_RUN_RAW_SURFACE, PRISM_DTYPE)
+from europaraypaths_amd._abi import (ERP_RUN_DETERMINISTIC, ERP_RUN_RAW_SURFACE,
+                                    PRISM_DTYPE)
 from europaraypaths_amd.params import make_shots, parse_parameters, prism_alpha
 from oracle import oracle
 from tests.helpers import compare_traces
@@ -111,6 +112,43 @@ def test_targets_plain_range_sizes(gpu_ctx, nr):
     g, gv, o, ov = run_both(gpu_ctx, par, soa, shots, targets=tg)
     np.testing.assert_array_equal(gv, ov)
     compare_traces(g, o)
+
+
+@pytest.mark.parametrize("nr", [1024, 4096, 5000])
+def test_deterministic_mode_matches_oracle_and_repeats_bitwise(gpu_ctx, nr):
+    """ERP_RUN_DETERMINISTIC: fixed-point bins summed with integer atomics.
+    Same tolerance against the FP64 oracle as the default mode, and two runs
+    give the same bits.  nr 1024: one workgroup per pair; 4096: the per-pair
+    LDS chain; 5000: the unfused chain that reads whole rows."""
+    print(f"deterministic nr={nr}", flush=True)
+    par, soa, shots = scene(rough=0.3, nr=nr, h=3000.0, rst=2600.0, ns=2)
+    tg = _targets(par, n=3)
+    g, gv, o, ov = run_both(gpu_ctx, par, soa, shots, targets=tg, flags=ERP_RUN_DETERMINISTIC)
+    np.testing.assert_array_equal(gv, ov)
+    compare_traces(g, o)
+    m = gpu_ctx.upload_mesh(soa)
+    t = gpu_ctx.upload_targets(*tg)
+    g2, _ = gpu_ctx.run(par.to_erp(), m, t, shots, flags=ERP_RUN_DETERMINISTIC)
+    assert np.array_equal(g.view(np.uint32), g2.view(np.uint32))
+
+
+def test_wide_range_spread_both_modes(gpu_ctx):
+    """A 30 km wide mesh seen at 85 deg aperture from 3 km: one target's
+    ranges span about 6200 bins, more than one LDS window of the target
+    kernel, and surface blocks spill past their LDS window.  Default mode
+    (window plus atomics in the trace row) and deterministic mode (two
+    fixed-point windows) against the FP64 oracle."""
+    par, soa, shots = scene(nx=300, ny=40, fs=100.0, rough=0.3, nr=16384, h=3000.0,
+                            rst=2600.0, aperture=85.0, ns=2)
+    tg = _targets(par, n=3)
+    o, ov = oracle.run(par.to_erp(), soa, tg, shots)
+    m = gpu_ctx.upload_mesh(soa)
+    t = gpu_ctx.upload_targets(*tg)
+    for flags in (0, ERP_RUN_DETERMINISTIC):
+        print(f"wide spread flags={flags}", flush=True)
+        g, gv = gpu_ctx.run(par.to_erp(), m, t, shots, flags=flags)
+        np.testing.assert_array_equal(gv, ov)
+        compare_traces(g, o)
 
 
 def test_targets_plain_general_facet_frames(gpu_ctx):
