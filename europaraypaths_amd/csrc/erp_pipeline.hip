@@ -431,10 +431,21 @@ __device__ __forceinline__ double det_scale(double n, double M) {
 __device__ __forceinline__ unsigned long long det_fix(float v, double scale) {
     return (unsigned long long)__double2ll_rn((double)v * scale);
 }
-__device__ __forceinline__ void lds_add2_fix(unsigned long long* win, int idx, float2 v,
-                                             double scale) {
-    atomicAdd(&win[2 * idx], det_fix(v.x, scale));
-    atomicAdd(&win[2 * idx + 1], det_fix(v.y, scale));
+// LDS windows keep real and imaginary parts in separate arrays: 8-byte
+// elements at an 8-byte stride spread a wave's bins over every bank pair
+// (an interleaved 16-byte complex uses only every other pair)
+struct WinS {
+    double* x;
+    double* y;
+};
+__device__ __forceinline__ void lds_add2s(WinS w, int idx, float2 v) {
+    atomicAdd(&w.x[idx], (double)v.x);
+    atomicAdd(&w.y[idx], (double)v.y);
+}
+__device__ __forceinline__ void lds_add2_fix(unsigned long long* wx, unsigned long long* wy,
+                                             int idx, float2 v, double scale) {
+    atomicAdd(&wx[idx], det_fix(v.x, scale));
+    atomicAdd(&wy[idx], det_fix(v.y, scale));
 }
 __device__ __forceinline__ void glb_add2_fix(unsigned long long* row, int idx, float2 v,
                                              double scale) {
@@ -458,7 +469,7 @@ __global__ void __launch_bounds__(NT) k_surface(
     const unsigned long long* mw = mask + ((size_t)s * nblk + blk) * MASKW;
     __shared__ unsigned long long m_s[MASKW];
     __shared__ int pre_s[MASKW + 1];
-    __shared__ double2 win[SWIN];
+    __shared__ double swin[2 * SWIN];
     __shared__ int wmin_s;
     __shared__ float red_f[6][WPB];
     __shared__ double red_d[2][WPB];
@@ -644,7 +655,7 @@ __global__ void __launch_bounds__(NT) k_surface(
         red_f[3][wv] = lxmax; red_f[4][wv] = lymax; red_f[5][wv] = lzmax;
         red_d[0][wv] = litmin; red_d[1][wv] = litmax;
     }
-    for (int k = threadIdx.x; k < SWIN; k += NT) win[k] = make_double2(0.0, 0.0);
+    for (int k = threadIdx.x; k < 2 * SWIN; k += NT) swin[k] = 0.0;
     __syncthreads();
     if (threadIdx.x == 0 && bounds) {
         float a[6];
@@ -671,7 +682,8 @@ __global__ void __launch_bounds__(NT) k_surface(
     if (det == 2) {
         // the bound covers every term of the shot: 2 terms per facet, V facets
         const double scale = det_scale((double)V, (double)__uint_as_float(smax[s]));
-        unsigned long long* wu = reinterpret_cast<unsigned long long*>(win);
+        unsigned long long* wux = reinterpret_cast<unsigned long long*>(swin);
+        unsigned long long* wuy = wux + SWIN;
         unsigned long long* ru = reinterpret_cast<unsigned long long*>(row);
 #pragma unroll
         for (int q = 0; q < FPT; ++q) {
@@ -681,17 +693,18 @@ __global__ void __launch_bounds__(NT) k_surface(
             const float2 c = cq[q];
             const float2 c0 = make_float2(c.x * (1.f - w), c.y * (1.f - w));
             const int k0 = b - base;
-            if (k0 < SWIN) lds_add2_fix(wu, k0, c0, scale); else glb_add2_fix(ru, b, c0, scale);
+            if (k0 < SWIN) lds_add2_fix(wux, wuy, k0, c0, scale);
+            else glb_add2_fix(ru, b, c0, scale);
             if (!spec && b + 1 < nr) {
                 const float2 c1 = make_float2(c.x * w, c.y * w);
-                if (k0 + 1 < SWIN) lds_add2_fix(wu, k0 + 1, c1, scale);
+                if (k0 + 1 < SWIN) lds_add2_fix(wux, wuy, k0 + 1, c1, scale);
                 else glb_add2_fix(ru, b + 1, c1, scale);
             }
         }
         __syncthreads();
         for (int k = threadIdx.x; k < SWIN; k += NT) {
             const int b = base + k;
-            const unsigned long long x = wu[2 * k], y = wu[2 * k + 1];
+            const unsigned long long x = wux[k], y = wuy[k];
             if (b < nr && (x | y)) {
                 atomicAdd(&ru[2 * b], x);
                 atomicAdd(&ru[2 * b + 1], y);
@@ -707,17 +720,18 @@ __global__ void __launch_bounds__(NT) k_surface(
         const float2 c = cq[q];
         const float2 c0 = make_float2(c.x * (1.f - w), c.y * (1.f - w));
         const int k0 = b - base;
-        if (k0 < SWIN) lds_add2(win, k0, c0); else glb_add2d(row, b, c0.x, c0.y);
+        const WinS W{swin, swin + SWIN};
+        if (k0 < SWIN) lds_add2s(W, k0, c0); else glb_add2d(row, b, c0.x, c0.y);
         if (!spec && b + 1 < nr) {
             const float2 c1 = make_float2(c.x * w, c.y * w);
-            if (k0 + 1 < SWIN) lds_add2(win, k0 + 1, c1); else glb_add2d(row, b + 1, c1.x, c1.y);
+            if (k0 + 1 < SWIN) lds_add2s(W, k0 + 1, c1); else glb_add2d(row, b + 1, c1.x, c1.y);
         }
     }
     __syncthreads();
     for (int k = threadIdx.x; k < SWIN; k += NT) {
-        const double2 v = win[k];
+        const double vx = swin[k], vy = swin[SWIN + k];
         const int b = base + k;
-        if (b < nr && (v.x != 0.0 || v.y != 0.0)) glb_add2d(row, b, v.x, v.y);
+        if (b < nr && (vx != 0.0 || vy != 0.0)) glb_add2d(row, b, vx, vy);
     }
 }
 
@@ -1303,13 +1317,13 @@ __device__ __forceinline__ Tri tri_eval(const Rec& rec, int64_t r, bool frame_ok
 // two-bin split of one term into the LDS window [base, base + len) or, past
 // it, the global row
 __device__ __forceinline__ void tri_add(const Tri& t, int base, int len, int nr,
-                                        double2* winA, float2* rowA) {
+                                        WinS winA, float2* rowA) {
     const float2 c0 = make_float2(t.c.x * (1.f - t.w), t.c.y * (1.f - t.w));
     const float2 c1 = make_float2(t.c.x * t.w, t.c.y * t.w);
     const int k0 = t.b - base;
-    if (k0 >= 0 && k0 < len) lds_add2(winA, k0, c0); else glb_add2(rowA, t.b, c0);
+    if (k0 >= 0 && k0 < len) lds_add2s(winA, k0, c0); else glb_add2(rowA, t.b, c0);
     if (t.b + 1 < nr) {
-        if (k0 + 1 >= 0 && k0 + 1 < len) lds_add2(winA, k0 + 1, c1);
+        if (k0 + 1 >= 0 && k0 + 1 < len) lds_add2s(winA, k0 + 1, c1);
         else glb_add2(rowA, t.b + 1, c1);
     }
 }
@@ -1329,7 +1343,8 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
     const int* __restrict__ Vs, const int64_t* __restrict__ rec_base, Rec rec,
     const Bounds* __restrict__ bounds, const erp_prism* __restrict__ prisms, int np, KPar kp,
     float2* __restrict__ outA, int* __restrict__ pbase, int* __restrict__ pwin, int frame_ok) {
-    __shared__ double2 winA[TWF];
+    __shared__ double winS[2 * TWF];
+    const WinS winA{winS, winS + TWF};
     const int pi = blockIdx.x;
     const int s = pshot[pi], t = ptgt[pi];
     const ShotK sh = load_shot(shots, s);
@@ -1403,11 +1418,15 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
         // 2x margin for the rounding of the sinc quotients
         const double scale = det_scale((double)V, 2.0 * (double)m * (double)kp.Dmax);
         const double inv = scale > 0.0 ? 1.0 / scale : 0.0;
-        unsigned long long* wu = reinterpret_cast<unsigned long long*>(winA);
+        unsigned long long* wux = reinterpret_cast<unsigned long long*>(winS);
+        unsigned long long* wuy = wux + TWF;
         // one LDS window of TWF bins at a time over [base, top)
         for (int wb = base; wb < top || wb == base; wb += TWF) {
             const int wl = fits ? len : min(TWF, max(0, top - wb));
-            for (int k = threadIdx.x; k < wl; k += NTF) winA[k] = make_double2(0.0, 0.0);
+            for (int k = threadIdx.x; k < wl; k += NTF) {
+                winS[k] = 0.0;
+                winS[TWF + k] = 0.0;
+            }
             __syncthreads();
             for (int k = threadIdx.x; k < V; k += NTF) {
                 const Tri ta = tri_eval(rec, r0 + k, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz,
@@ -1418,10 +1437,10 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
                 const float2 c1 = make_float2(ta.c.x * ta.w, ta.c.y * ta.w);
                 const int k0 = ta.b - wb;
                 const bool first = wb == base, last = wb + TWF >= top;
-                if (k0 >= 0 && k0 < wl) lds_add2_fix(wu, k0, c0, scale);
+                if (k0 >= 0 && k0 < wl) lds_add2_fix(wux, wuy, k0, c0, scale);
                 else if ((ta.b < base && first) || (ta.b >= top && last)) glb_add2(rowA, ta.b, c0);
                 if (ta.b + 1 < nr) {
-                    if (k0 + 1 >= 0 && k0 + 1 < wl) lds_add2_fix(wu, k0 + 1, c1, scale);
+                    if (k0 + 1 >= 0 && k0 + 1 < wl) lds_add2_fix(wux, wuy, k0 + 1, c1, scale);
                     else if ((ta.b + 1 < base && first) || (ta.b + 1 >= top && last))
                         glb_add2(rowA, ta.b + 1, c1);
                 }
@@ -1430,15 +1449,18 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
             for (int k = threadIdx.x; k < wl; k += NTF) {
                 const int b = wb + k;
                 if (b >= nr) break;
-                rowA[b] = make_float2((float)det_unfix(wu[2 * k], inv),
-                                      (float)det_unfix(wu[2 * k + 1], inv));
+                rowA[b] = make_float2((float)det_unfix(wux[k], inv),
+                                      (float)det_unfix(wuy[k], inv));
             }
             __syncthreads();
             if (fits) break;
         }
         return;
     } else {
-        for (int k = threadIdx.x; k < len; k += NTF) winA[k] = make_double2(0.0, 0.0);
+        for (int k = threadIdx.x; k < len; k += NTF) {
+            winS[k] = 0.0;
+            winS[TWF + k] = 0.0;
+        }
         __threadfence();
         __syncthreads();
 
@@ -1462,7 +1484,7 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
         for (int k = threadIdx.x; k < len; k += NTF) {
             const int b = base + k;
             if (b >= nr) break;
-            rowA[b] = make_float2((float)winA[k].x, (float)winA[k].y);
+            rowA[b] = make_float2((float)winS[k], (float)winS[TWF + k]);
         }
     }
 }
