@@ -21,6 +21,8 @@ ERP_RUN_DETERMINISTIC = 0x01
 ERP_RUN_PHASE_F32 = 0x02
 ERP_RUN_PHASE_DF = 0x04
 ERP_RUN_RAW_SURFACE = 0x08
+ERP_RUN_ACC_F32 = 0x10
+ERP_RUN_ACC_COMP = 0x20
 
 _PARAM_FLOATS = ("lam", "k", "rng_res", "P", "fs", "ks", "eps_1", "eps_2", "c",
                  "c_1", "c_2", "nu_1", "nu_2", "alpha1", "alpha2", "aperture",

@@ -1231,6 +1231,65 @@ struct Tri {
     bool valid;
 };
 
+// Arithmetic of the range, bin and phase of a triple (precision study):
+// FP64 (the product), FP32 (the reference's own arithmetic,
+// radar_kernels.cu:681-686,811-813) or double-float (pairs of floats).
+constexpr int PH_F64 = 0, PH_F32 = 1, PH_DF = 2;
+// Accumulation of the bin window: FP64 (the product), plain FP32, or FP32
+// with the rounding error of every addition recovered (TwoSum on the value
+// an atomic returns) and accumulated beside it.
+constexpr int ACC_F64 = 0, ACC_F32 = 1, ACC_COMP = 2;
+
+// double-float: an unevaluated sum hi + lo of two floats (Dekker, Knuth).
+// Contraction is off inside: an FMA formed from a product and a later sum
+// would use the exact product where the error term assumes the rounded one.
+struct DF {
+    float hi, lo;
+};
+__device__ __forceinline__ DF df_two_sum(float a, float b) {
+#pragma clang fp contract(off)
+    const float s = a + b;
+    const float bb = s - a;
+    return DF{s, (a - (s - bb)) + (b - bb)};
+}
+__device__ __forceinline__ DF df_fast(float a, float b) {
+#pragma clang fp contract(off)
+    const float s = a + b;
+    return DF{s, b - (s - a)};
+}
+__device__ __forceinline__ DF df_of(double x) {
+    const float h = (float)x;
+    return DF{h, (float)(x - (double)h)};
+}
+__device__ __forceinline__ DF df_add(DF a, DF b) {
+#pragma clang fp contract(off)
+    const DF s = df_two_sum(a.hi, b.hi);
+    return df_fast(s.hi, s.lo + (a.lo + b.lo));
+}
+__device__ __forceinline__ DF df_neg(DF a) { return DF{-a.hi, -a.lo}; }
+__device__ __forceinline__ DF df_mul(DF a, DF b) {
+#pragma clang fp contract(off)
+    const float p = a.hi * b.hi;
+    const float e = fmaf(a.hi, b.hi, -p);
+    return df_fast(p, e + (a.hi * b.lo + a.lo * b.hi));
+}
+__device__ __forceinline__ DF df_sqrt(DF a) {
+#pragma clang fp contract(off)
+    if (!(a.hi > 0.f)) return DF{0.f, 0.f};
+    const float x = sqrtf(a.hi);
+    const float p = x * x;
+    const float e = fmaf(x, x, -p);
+    return df_fast(x, ((a.hi - p) - e + a.lo) / (2.f * x));
+}
+// |a - b| for float a and double-float b components
+__device__ __forceinline__ DF df_dist(float ax, float ay, float az, DF bx, DF by, DF bz) {
+    const DF dx = df_add(DF{ax, 0.f}, df_neg(bx));
+    const DF dy = df_add(DF{ay, 0.f}, df_neg(by));
+    const DF dz = df_add(DF{az, 0.f}, df_neg(bz));
+    return df_sqrt(df_add(df_add(df_mul(dx, dx), df_mul(dy, dy)), df_mul(dz, dz)));
+}
+
+template <int PH = PH_F64>
 __device__ __forceinline__ Tri tri_eval(const Rec& rec, int64_t r, bool frame_ok, double sx,
                                         double sy, double sz, double tx, double ty, double tz,
                                         float tnx, float tny, float tnz, bool tn_up, double rst,
@@ -1252,17 +1311,52 @@ __device__ __forceinline__ Tri tri_eval(const Rec& rec, int64_t r, bool frame_ok
     const float rs = rsqrtf((float)t2);
     const double y0 = t2 * (double)rs;
     const double Ttd = fma(fma(-y0, y0, t2), 0.5 * (double)rs, y0);
-    double rngt = fma(Ttd, n_d, Itd);
-    const double xq = (rngt - rst) * inv_dr;
-    if (fabs(xq - rint(xq)) < 1e-7) {
-        // within 1e-7 bin of an edge: the oracle's exact operation sequence
-        const double Te = dist3(dx, dy, dz);
-        rngt = madd_nc(dist3(ex, ey, ez), Te, n_d);
-        o.valid = bin_of(rngt, rst, kp.drd, nr, &o.b, &o.w);
+    double rngt = 0.0;
+    float2 ph_st = make_float2(1.f, 0.f);
+    if constexpr (PH == PH_F64) {
+        rngt = fma(Ttd, n_d, Itd);
+        const double xq = (rngt - rst) * inv_dr;
+        if (fabs(xq - rint(xq)) < 1e-7) {
+            // within 1e-7 bin of an edge: the oracle's exact operation sequence
+            const double Te = dist3(dx, dy, dz);
+            rngt = madd_nc(dist3(ex, ey, ez), Te, n_d);
+            o.valid = bin_of(rngt, rst, kp.drd, nr, &o.b, &o.w);
+        } else {
+            o.valid = (xq > -1.0) && (xq < (double)nr);
+            o.b = (int)xq;
+            o.w = (float)(xq - (double)o.b);
+        }
+    } else if constexpr (PH == PH_F32) {
+        // Itd, Ttd, rngt, the bin coordinate and the phase in FP32 as in
+        // compIncidentRays / compTargetRays / accumulateTarget / phasor
+        const float ax = fx - (float)sx, ay = fy - (float)sy, az = fz - (float)sz;
+        const float bx = (float)tx - fx, by = (float)ty - fy, bz = (float)tz - fz;
+        const float I32 = sqrtf(ax * ax + ay * ay + az * az);
+        const float T32 = sqrtf(bx * bx + by * by + bz * bz);
+        const float r32 = I32 + T32 * kp.nf;
+        const float x32 = (r32 - (float)rst) / kp.dr;
+        o.valid = (x32 > -1.f) && (x32 < (float)nr);
+        o.b = (int)x32;
+        o.w = x32 - (float)o.b;
+        float sn, cs;
+        sincosf(-(2.0f * 3.14159265f / kp.lam) * r32, &sn, &cs);
+        ph_st = make_float2(cs, sn);
     } else {
+        // double-float ranges and phase from the FP32 facet and target
+        // coordinates and the shot position
+        const DF Id = df_dist(fx, fy, fz, df_of(sx), df_of(sy), df_of(sz));
+        const DF Td = df_dist(fx, fy, fz, df_of(tx), df_of(ty), df_of(tz));
+        const DF rd2 = df_add(Id, df_mul(Td, df_of(n_d)));
+        const DF xd = df_mul(df_add(rd2, df_neg(df_of(rst))), df_of(inv_dr));
+        const double xq = (double)xd.hi + (double)xd.lo;
         o.valid = (xq > -1.0) && (xq < (double)nr);
         o.b = (int)xq;
         o.w = (float)(xq - (double)o.b);
+        // exp(-i pi t), t = rngt * two_cyc half cycles, reduced in double-float
+        const DF t = df_mul(rd2, df_of(0.5 * two_cyc));
+        const float rr = rintf(t.hi);
+        const float fr = (t.hi - rr) + t.lo;
+        ph_st = make_float2(__builtin_amdgcn_cosf(fr), -__builtin_amdgcn_sinf(fr));
     }
     const float nx = qb.x, ny = qb.y, nz = qb.z;
     const float ux = qb.w, uy = qc.x, uz = qc.y;
@@ -1309,21 +1403,78 @@ __device__ __forceinline__ Tri tri_eval(const Rec& rec, int64_t r, bool frame_ok
         const float kc = kp.ks * cT;
         amp *= __expf(-2.f * (expo + kc * kc));
     }
-    const float2 ph = phasor_half_cycles_hw(rngt * two_cyc);
+    const float2 ph = (PH == PH_F64) ? phasor_half_cycles_hw(rngt * two_cyc) : ph_st;
     o.c = make_float2(ph.x * amp, ph.y * amp);
     return o;
 }
 
 // two-bin split of one term into the LDS window [base, base + len) or, past
 // it, the global row
+// FP32 window add; ACC_COMP also accumulates the rounding error of the
+// addition the atomic performed (TwoSum of the returned old value and v)
+template <int ACC>
+__device__ __forceinline__ void lds_add_f32(float* w, int idx, float v, int stride2) {
+    if constexpr (ACC == ACC_COMP) {
+        const float old = atomicAdd(&w[idx], v);
+        const float s = old + v;
+        const float bb = s - old;
+        const float e = (old - (s - bb)) + (v - bb);
+        atomicAdd(&w[stride2 + idx], e);
+    } else {
+        atomicAdd(&w[idx], v);
+    }
+}
+template <int ACC>
+__device__ __forceinline__ void win_add(WinS w, int idx, float2 v) {
+    if constexpr (ACC == ACC_F64) {
+        lds_add2s(w, idx, v);
+    } else {
+        // FP32 layout inside the same LDS: x at [0, n), y at [n, 2n),
+        // compensations at [2n, 4n) (n = the window's FP64 capacity)
+        float* f = reinterpret_cast<float*>(w.x);
+        const int n = (int)(w.y - w.x);
+        lds_add_f32<ACC>(f, idx, v.x, 2 * n);
+        lds_add_f32<ACC>(f + n, idx, v.y, 2 * n);
+    }
+}
+template <int ACC>
+__device__ __forceinline__ void win_zero(WinS w, int idx) {
+    if constexpr (ACC == ACC_F64) {
+        w.x[idx] = 0.0;
+        w.y[idx] = 0.0;
+    } else {
+        float* f = reinterpret_cast<float*>(w.x);
+        const int n = (int)(w.y - w.x);
+        f[idx] = 0.f;
+        f[n + idx] = 0.f;
+        if constexpr (ACC == ACC_COMP) {
+            f[2 * n + idx] = 0.f;
+            f[3 * n + idx] = 0.f;
+        }
+    }
+}
+template <int ACC>
+__device__ __forceinline__ float2 win_get(WinS w, int idx) {
+    if constexpr (ACC == ACC_F64) {
+        return make_float2((float)w.x[idx], (float)w.y[idx]);
+    } else {
+        const float* f = reinterpret_cast<const float*>(w.x);
+        const int n = (int)(w.y - w.x);
+        if constexpr (ACC == ACC_COMP)
+            return make_float2(f[idx] + f[2 * n + idx], f[n + idx] + f[3 * n + idx]);
+        return make_float2(f[idx], f[n + idx]);
+    }
+}
+
+template <int ACC = ACC_F64>
 __device__ __forceinline__ void tri_add(const Tri& t, int base, int len, int nr,
                                         WinS winA, float2* rowA) {
     const float2 c0 = make_float2(t.c.x * (1.f - t.w), t.c.y * (1.f - t.w));
     const float2 c1 = make_float2(t.c.x * t.w, t.c.y * t.w);
     const int k0 = t.b - base;
-    if (k0 >= 0 && k0 < len) lds_add2s(winA, k0, c0); else glb_add2(rowA, t.b, c0);
+    if (k0 >= 0 && k0 < len) win_add<ACC>(winA, k0, c0); else glb_add2(rowA, t.b, c0);
     if (t.b + 1 < nr) {
-        if (k0 + 1 >= 0 && k0 + 1 < len) lds_add2s(winA, k0 + 1, c1);
+        if (k0 + 1 >= 0 && k0 + 1 < len) win_add<ACC>(winA, k0 + 1, c1);
         else glb_add2(rowA, t.b + 1, c1);
     }
 }
@@ -1336,7 +1487,7 @@ __device__ __forceinline__ void tri_add(const Tri& t, int base, int len, int nr,
 // decided on the approximate range unless it lies within 1e-7 bin of a bin
 // edge, where the exact FP64 sqrt and division of the oracle are repeated, so
 // bin indices stay identical to the FP64 reference.
-template <bool DET>
+template <bool DET, int PH = PH_F64, int ACC = ACC_F64>
 __global__ void __launch_bounds__(NTF) k_target_accum_fast(
     const float* __restrict__ tg, int64_t Tpad, const int* __restrict__ pshot,
     const int* __restrict__ ptgt, const erp_shot* __restrict__ shots,
@@ -1457,10 +1608,7 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
         }
         return;
     } else {
-        for (int k = threadIdx.x; k < len; k += NTF) {
-            winS[k] = 0.0;
-            winS[TWF + k] = 0.0;
-        }
+        for (int k = threadIdx.x; k < len; k += NTF) win_zero<ACC>(winA, k);
         __threadfence();
         __syncthreads();
 
@@ -1468,23 +1616,23 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
         // chains for the scheduler to interleave
         for (int k = threadIdx.x; k < V; k += 2 * NTF) {
             const bool two = k + NTF < V;
-            const Tri ta = tri_eval(rec, r0 + k, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx,
-                                    tny, tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr, lossless,
-                                    prisms, np, kp);
+            const Tri ta = tri_eval<PH>(rec, r0 + k, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz,
+                                        tnx, tny, tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr,
+                                        lossless, prisms, np, kp);
             Tri tb;
             tb.valid = false;
             if (two)
-                tb = tri_eval(rec, r0 + k + NTF, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx,
-                              tny, tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr, lossless, prisms,
-                              np, kp);
-            if (ta.valid) tri_add(ta, base, len, nr, winA, rowA);
-            if (tb.valid) tri_add(tb, base, len, nr, winA, rowA);
+                tb = tri_eval<PH>(rec, r0 + k + NTF, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz,
+                                  tnx, tny, tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr, lossless,
+                                  prisms, np, kp);
+            if (ta.valid) tri_add<ACC>(ta, base, len, nr, winA, rowA);
+            if (tb.valid) tri_add<ACC>(tb, base, len, nr, winA, rowA);
         }
         __syncthreads();
         for (int k = threadIdx.x; k < len; k += NTF) {
             const int b = base + k;
             if (b >= nr) break;
-            rowA[b] = make_float2((float)winS[k], (float)winS[TWF + k]);
+            rowA[b] = win_get<ACC>(winA, k);
         }
     }
 }
@@ -1871,6 +2019,23 @@ struct BatchPlan {
 
 }  // namespace
 
+using TargetAccumFn = decltype(&k_target_accum_fast<false>);
+
+TargetAccumFn pick_target_accum(bool det, int ph, int acc) {
+    if (det) return k_target_accum_fast<true>;
+    switch (ph * 3 + acc) {
+    case PH_F64 * 3 + ACC_F32: return k_target_accum_fast<false, PH_F64, ACC_F32>;
+    case PH_F64 * 3 + ACC_COMP: return k_target_accum_fast<false, PH_F64, ACC_COMP>;
+    case PH_F32 * 3 + ACC_F64: return k_target_accum_fast<false, PH_F32, ACC_F64>;
+    case PH_F32 * 3 + ACC_F32: return k_target_accum_fast<false, PH_F32, ACC_F32>;
+    case PH_F32 * 3 + ACC_COMP: return k_target_accum_fast<false, PH_F32, ACC_COMP>;
+    case PH_DF * 3 + ACC_F64: return k_target_accum_fast<false, PH_DF, ACC_F64>;
+    case PH_DF * 3 + ACC_F32: return k_target_accum_fast<false, PH_DF, ACC_F32>;
+    case PH_DF * 3 + ACC_COMP: return k_target_accum_fast<false, PH_DF, ACC_COMP>;
+    default: return k_target_accum_fast<false>;
+    }
+}
+
 // Device bytes of one run, following the allocations of erp_run_impl: the
 // work buffers of one batch of up to 32 shots (facet records counted for
 // every facet illuminated by every shot of the batch, an upper bound) plus
@@ -1912,12 +2077,25 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     const bool spec = mode & ERP_MODE_SPECULAR;
     const int nr = p.nr;
     if (nr <= 0 || nr > 32767) throw ErpError{ERP_E_INVALID, "nr must be in [1, 32767]"};
-    if (flags & ~(uint32_t)(ERP_RUN_RAW_SURFACE | ERP_RUN_DETERMINISTIC))
-        throw ErpError{ERP_E_UNSUPPORTED, "run flags PHASE_F32 / PHASE_DF are not implemented"};
+    const uint32_t known = ERP_RUN_RAW_SURFACE | ERP_RUN_DETERMINISTIC | ERP_RUN_PHASE_F32 |
+                           ERP_RUN_PHASE_DF | ERP_RUN_ACC_F32 | ERP_RUN_ACC_COMP;
+    if (flags & ~known) throw ErpError{ERP_E_INVALID, "unknown run flags"};
     const bool det = flags & ERP_RUN_DETERMINISTIC;
     if (det && spec)
         throw ErpError{ERP_E_UNSUPPORTED,
                        "ERP_RUN_DETERMINISTIC covers the non-specular (convolution) method"};
+    // precision-study arithmetic of the target accumulation
+    const int ph = (flags & ERP_RUN_PHASE_F32) ? PH_F32 : (flags & ERP_RUN_PHASE_DF) ? PH_DF : PH_F64;
+    const int acc = (flags & ERP_RUN_ACC_F32) ? ACC_F32 : (flags & ERP_RUN_ACC_COMP) ? ACC_COMP
+                                                                                 : ACC_F64;
+    if ((flags & ERP_RUN_PHASE_F32) && (flags & ERP_RUN_PHASE_DF))
+        throw ErpError{ERP_E_INVALID, "PHASE_F32 and PHASE_DF are exclusive"};
+    if ((flags & ERP_RUN_ACC_F32) && (flags & ERP_RUN_ACC_COMP))
+        throw ErpError{ERP_E_INVALID, "ACC_F32 and ACC_COMP are exclusive"};
+    if ((ph != PH_F64 || acc != ACC_F64) && (det || spec))
+        throw ErpError{ERP_E_UNSUPPORTED,
+                       "the precision-study flags apply to the non-specular target "
+                       "accumulation without ERP_RUN_DETERMINISTIC"};
     if (!spec && !(conv && lin))
         throw ErpError{ERP_E_UNSUPPORTED,
                        "non-specular runs need convolution and convolution_linear "
@@ -2215,7 +2393,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                     for (int i = 0; i < n; ++i) triples += hV[pshot[i]];
                     PT_.begin(ERP_STAGE_TARGET_ACCUM, st, triples);
                     if (!spec) {
-                        auto kern = det ? k_target_accum_fast<true> : k_target_accum_fast<false>;
+                        auto kern = pick_target_accum(det, ph, acc);
                         hipLaunchKernelGGL(kern, dim3(n), dim3(NTF), 0, st,
                                            targets->xyzn.as<float>(), Tpad, dps, dpt, dshots, dV,
                                            dbase, rec, dbounds, dpr, (int)nprisms, kp, A,

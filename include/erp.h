@@ -64,11 +64,20 @@ extern "C" {
                                        only.  Replaces the order-dependent float
                                        atomicAdd binning of radar_kernels.cu:
                                        754-758 and 821-825                      */
-#define ERP_RUN_PHASE_F32     0x02u /* precision study: ranges and phase in FP32 */
-#define ERP_RUN_PHASE_DF      0x04u /* precision study: phase in double-float    */
+/* Precision study of the subsurface-target accumulation (non-specular
+ * method, not with ERP_RUN_DETERMINISTIC); the surface echo is unaffected.
+ * PHASE_*: arithmetic of Itd, Ttd, rngt = Itd + n Ttd, the bin coordinate
+ * and the phase: FP32 as the reference computes them (radar_kernels.cu:
+ * 681-686, 811-813) or double-float; the default is FP64.  ACC_*: the bin
+ * window sums: plain FP32, or FP32 with the rounding error of every
+ * addition recovered by TwoSum and summed beside it; the default is FP64. */
+#define ERP_RUN_PHASE_F32     0x02u
+#define ERP_RUN_PHASE_DF      0x04u
 #define ERP_RUN_RAW_SURFACE   0x08u /* output the raw surface phasor trace before
                                        range compression (the Psurf dump of
                                        debug_surface, sim.cu:782-788)           */
+#define ERP_RUN_ACC_F32       0x10u
+#define ERP_RUN_ACC_COMP      0x20u
 
 /*
  * Simulation parameters after the derivations of parseSimulationParameters
