@@ -1300,12 +1300,8 @@ __device__ __forceinline__ Tri tri_eval(const Rec& rec, int64_t r, bool frame_ok
     const float4 qa = rec.qa[r], qb = rec.qb[r], qc = rec.qc[r];
     const float qd = rec.qd[r];
     const float fx = qa.x, fy = qa.y, fz = qa.z;
-    // Itd = |facet - source| in FP64 (Newton step from an FP32 rsqrt seed)
-    const double ex = (double)fx - sx, ey = (double)fy - sy, ez = (double)fz - sz;
-    const double e2 = fma(ex, ex, fma(ey, ey, ez * ez));
-    const double er = (double)rsqrtf((float)e2);
-    const double e0 = e2 * er;
-    const double Itd = fma(fma(-e0, e0, e2), 0.5 * er, e0);
+    // Itd = |facet - source|: the surface kernel's exact FP64 value
+    const double Itd = rec.itd[r];
     const double dx = tx - (double)fx, dy = ty - (double)fy, dz = tz - (double)fz;
     const double t2 = fma(dx, dx, fma(dy, dy, dz * dz));
     const float rs = rsqrtf((float)t2);
@@ -1316,15 +1312,15 @@ __device__ __forceinline__ Tri tri_eval(const Rec& rec, int64_t r, bool frame_ok
     if constexpr (PH == PH_F64) {
         rngt = fma(Ttd, n_d, Itd);
         const double xq = (rngt - rst) * inv_dr;
-        if (fabs(xq - rint(xq)) < 1e-7) {
-            // within 1e-7 bin of an edge: the oracle's exact operation sequence
-            const double Te = dist3(dx, dy, dz);
-            rngt = madd_nc(dist3(ex, ey, ez), Te, n_d);
+        o.b = (int)xq;
+        o.w = (float)(xq - (double)o.b);
+        if (o.w < 1e-7f || o.w > 1.f - 1e-7f) {
+            // within about 1e-7 bin of an edge (every negative coordinate
+            // included): the oracle's exact operation sequence
+            rngt = madd_nc(Itd, dist3(dx, dy, dz), n_d);
             o.valid = bin_of(rngt, rst, kp.drd, nr, &o.b, &o.w);
         } else {
             o.valid = (xq > -1.0) && (xq < (double)nr);
-            o.b = (int)xq;
-            o.w = (float)(xq - (double)o.b);
         }
     } else if constexpr (PH == PH_F32) {
         // Itd, Ttd, rngt, the bin coordinate and the phase in FP32 as in
