@@ -187,7 +187,7 @@ def max_phase_err(a, b, floor_db=40.0):
     return worst
 
 
-def serve_bench(a):
+def serve_measure(a):
     """Served vs plain on the mixed-size job stream.
 
     plain : each job alone, one after another, as `simulate` runs it (mesh
@@ -244,6 +244,28 @@ def serve_bench(a):
         print(f"# rate {r}: {sm['pairs_per_s']:.4g} pairs/s ({sm['served_vs_plain']:.2f}x plain), "
               f"p50 {sm['latency_ms']['p50']:.1f} ms p99 {sm['latency_ms']['p99']:.1f} ms, "
               f"hits {sm['cache_hits']}, phase diff {ph:.2e}", file=sys.stderr, flush=True)
+    ctx.close()
+    plain = {"pairs": plain_pairs, "seconds": sum(plain_t), "pairs_per_s": plain_rate,
+             "job_ms": {"p50": 1e3 * float(np.percentile(plain_t, 50)),
+                        "p99": 1e3 * float(np.percentile(plain_t, 99))}}
+    return jobs, plain, runs
+
+
+def serve_summary(a, jobs, plain, runs):
+    """The compact served-vs-plain record of the default bench line."""
+    return {"workload": (f"mixed_job_stream: {len(jobs)} jobs, seed {a.serve_seed}, "
+                         f"shots x {a.serve_ns_scale}, {a.serve_slots} slots"),
+            "plain_pairs_per_s": plain["pairs_per_s"],
+            "served_pairs_per_s": runs[-1]["pairs_per_s"],
+            "served_vs_plain": runs[-1]["served_vs_plain"],
+            "latency_ms": {("saturated" if r["rate_jobs_per_s"] is None
+                            else f"{r['rate_jobs_per_s']:g}/s"):
+                           {"p50": r["latency_ms"]["p50"], "p99": r["latency_ms"]["p99"],
+                            "pairs_per_s": r["pairs_per_s"]} for r in runs}}
+
+
+def serve_bench(a):
+    jobs, plain, runs = serve_measure(a)
     sat = runs[-1]
     res = {
         "metric": METRIC, "value": sat["pairs_per_s"], "unit": "facet·shot pairs/s",
@@ -252,14 +274,11 @@ def serve_bench(a):
         "config": {"workload": "mixed_job_stream", "jobs": len(jobs), "seed": a.serve_seed,
                    "ns_scale": a.serve_ns_scale, "slots": a.serve_slots,
                    "budget_bytes": a.serve_budget},
-        "plain": {"pairs": plain_pairs, "seconds": sum(plain_t), "pairs_per_s": plain_rate,
-                  "job_ms": {"p50": 1e3 * float(np.percentile(plain_t, 50)),
-                             "p99": 1e3 * float(np.percentile(plain_t, 99))}},
+        "plain": plain,
         "served": runs,
         "served_vs_plain": sat["served_vs_plain"],
     }
     print(json.dumps(res))
-    ctx.close()
 
 
 def main():
@@ -284,6 +303,8 @@ def main():
     ap.add_argument("--serve-rates", default="20,50,100")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-budget", type=float, default=250e9)
+    ap.add_argument("--no-serve", action="store_true",
+                    help="leave the served-vs-plain record out of the default line")
     a = ap.parse_args()
     if a.ingest_pmc:
         ingest_pmc(*a.ingest_pmc, os.path.join(ROOT, "profiles", "traffic.json"))
@@ -396,6 +417,15 @@ def main():
         "roofline": roof,
         "roofline_other_kernels": others,
     }
+    if not a.no_serve:
+        mesh.release()
+        if targets is not None:
+            targets.release()
+        ctx.close()
+        try:
+            res["serving"] = serve_summary(a, *serve_measure(a))
+        except Exception as ex:  # noqa: BLE001
+            res["serving"] = {"error": str(ex)}
     if not a.no_cpu:
         try:
             res["cpu_baseline"] = cpu_baseline(p, soa, tg, shots, a.cpu_shots)
