@@ -1289,22 +1289,40 @@ __device__ __forceinline__ DF df_dist(float ax, float ay, float az, DF bx, DF by
     return df_sqrt(df_add(df_add(df_mul(dx, dx), df_mul(dy, dy)), df_mul(dz, dz)));
 }
 
+// one facet record in registers
+struct RecV {
+    float4 qa, qb, qc;
+    float qd;
+    double itd;
+};
+__device__ __forceinline__ RecV load_rec(const Rec& rec, int64_t r) {
+    RecV v;
+    v.qa = rec.qa[r];
+    v.qb = rec.qb[r];
+    v.qc = rec.qc[r];
+    v.qd = rec.qd[r];
+    v.itd = rec.itd[r];
+    return v;
+}
+
 template <int PH = PH_F64>
-__device__ __forceinline__ Tri tri_eval(const Rec& rec, int64_t r, bool frame_ok, double sx,
+__device__ __forceinline__ Tri tri_eval(const RecV& rv, bool frame_ok, double sx,
                                         double sy, double sz, double tx, double ty, double tz,
                                         float tnx, float tny, float tnz, bool tn_up, double rst,
                                         double inv_dr, double n_d, double two_cyc, int nr,
                                         bool lossless, const erp_prism* __restrict__ prisms,
                                         int np, const KPar& kp) {
     Tri o;
-    const float4 qa = rec.qa[r], qb = rec.qb[r], qc = rec.qc[r];
-    const float qd = rec.qd[r];
+    const float4 qa = rv.qa, qb = rv.qb, qc = rv.qc;
+    const float qd = rv.qd;
     const float fx = qa.x, fy = qa.y, fz = qa.z;
     // Itd = |facet - source|: the surface kernel's exact FP64 value
-    const double Itd = rec.itd[r];
+    const double Itd = rv.itd;
     const double dx = tx - (double)fx, dy = ty - (double)fy, dz = tz - (double)fz;
     const double t2 = fma(dx, dx, fma(dy, dy, dz * dz));
-    const float rs = rsqrtf((float)t2);
+    // hardware reciprocal square root: t2 is a squared facet-target
+    // distance, far from the denormal range rsqrtf guards against
+    const float rs = __builtin_amdgcn_rsqf((float)t2);
     const double y0 = t2 * (double)rs;
     const double Ttd = fma(fma(-y0, y0, t2), 0.5 * (double)rs, y0);
     double rngt = 0.0;
@@ -1555,7 +1573,7 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
             const float4 qa = rec.qa[r0 + k];
             const double dx = tx - (double)qa.x, dy = ty - (double)qa.y, dz = tz - (double)qa.z;
             const double t2 = fma(dx, dx, fma(dy, dy, dz * dz));
-            m = fmaxf(m, qa.w * rsqrtf((float)t2));
+            m = fmaxf(m, qa.w * __builtin_amdgcn_rsqf((float)t2));
         }
         for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
         if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
@@ -1576,7 +1594,7 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
             }
             __syncthreads();
             for (int k = threadIdx.x; k < V; k += NTF) {
-                const Tri ta = tri_eval(rec, r0 + k, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz,
+                const Tri ta = tri_eval(load_rec(rec, r0 + k), frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz,
                                         tnx, tny, tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr,
                                         lossless, prisms, np, kp);
                 if (!ta.valid) continue;
@@ -1608,19 +1626,26 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
         __threadfence();
         __syncthreads();
 
-        // two facets per thread and iteration: two independent dependency
-        // chains for the scheduler to interleave
+        // two facets per thread and iteration (two independent dependency
+        // chains), with the next iteration's two records loaded before this
+        // iteration's arithmetic so that their latency overlaps it
+        RecV na{}, nb{};
+        if (threadIdx.x < V) na = load_rec(rec, r0 + threadIdx.x);
+        if (threadIdx.x + NTF < V) nb = load_rec(rec, r0 + threadIdx.x + NTF);
         for (int k = threadIdx.x; k < V; k += 2 * NTF) {
+            const RecV ca = na, cb = nb;
             const bool two = k + NTF < V;
-            const Tri ta = tri_eval<PH>(rec, r0 + k, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz,
-                                        tnx, tny, tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr,
-                                        lossless, prisms, np, kp);
+            const int kn = k + 2 * NTF;
+            if (kn < V) na = load_rec(rec, r0 + kn);
+            if (kn + NTF < V) nb = load_rec(rec, r0 + kn + NTF);
+            const Tri ta = tri_eval<PH>(ca, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx, tny,
+                                        tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr, lossless,
+                                        prisms, np, kp);
             Tri tb;
             tb.valid = false;
             if (two)
-                tb = tri_eval<PH>(rec, r0 + k + NTF, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz,
-                                  tnx, tny, tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr, lossless,
-                                  prisms, np, kp);
+                tb = tri_eval<PH>(cb, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx, tny, tnz,
+                                  tn_up, rst, inv_dr, n_d, two_cyc, nr, lossless, prisms, np, kp);
             if (ta.valid) tri_add<ACC>(ta, base, len, nr, winA, rowA);
             if (tb.valid) tri_add<ACC>(tb, base, len, nr, winA, rowA);
         }
