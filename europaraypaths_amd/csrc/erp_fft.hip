@@ -569,8 +569,10 @@ __global__ void k_sum_rows(float2* __restrict__ S, const float2* __restrict__ sr
 //     Ze = DFT(z)   = nr E G_e - T1,   T1 = DFT(conj(m) v)
 //     Zo = DFT(z m) = T2 - nr O G_o,   T2 = DFT(m u)
 //     out[2k] = E Ze,  out[2k+1] = O Zo.
-// Seven nr-point transforms per pair (E twice), no HBM round trip of an
-// intermediate, and at most one register array held across a transform.
+// Six nr-point transforms per pair: E is parked in the even half of the
+// pair's output row (which only the last step writes) instead of being
+// transformed again, and at most one register array is held across a
+// transform.
 // Gn holds the chirp spectrum in natural order as (G[2k], G[2k+1]) pairs.
 // Each output row (P complex) holds two planes: out[k] = product at
 // frequency 2k, out[n + k] = product at frequency 2k + 1.
@@ -678,23 +680,26 @@ __global__ void __launch_bounds__(NTB) k_tpair_half(const float2* __restrict__ A
     float2 x[PPT], h[PPT];
     // Phase: input -> forward transform -> use; an inverse transform is
     // fft(conj(.)) with the conjugates folded into the neighbouring steps
-    //   0: a          -> E        -> x = conj(E G_e)
+    //   0: a          -> E        -> even plane = E (kept there); x = conj(E G_e)
     //   1: x          -> conj(u)  -> x = m u
     //   2: x          -> T2       -> h = T2
     //   3: a m        -> O        -> odd plane = O (T2 - n O G_o); x = conj(O G_o)
     //   4: x          -> conj(v)  -> x = conj(m) v
     //   5: x          -> T1       -> h = T1
-    //   6: a          -> E        -> even plane = E (n E G_e - T1)
+    //   6: E read back from the even plane -> even plane = E (n E G_e - T1)
+    // Each thread reads back only the elements it wrote, so phase 6 needs no
+    // barrier and saves the seventh transform.
 #pragma unroll 1
     for (int ph = 0; ph < 7; ++ph) {
-        if (ph == 0 || ph == 3 || ph == 6) load_window<NTB>(x, a, base, win, ph == 3 ? mod : nullptr);
-        fft_regs<NTB>(s, x, lad);
+        if (ph == 0 || ph == 3) load_window<NTB>(x, a, base, win, ph == 3 ? mod : nullptr);
+        if (ph != 6) fft_regs<NTB>(s, x, lad);
         const int tid = opaque_tid();
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
             const int i = tid + q * NTB;
             if (ph == 0) {
                 const float4 g = Gn[i];
+                o2[i] = x[q];
                 x[q] = conjf2(cmul(x[q], make_float2(g.x, g.y)));
             } else if (ph == 1) {
                 x[q] = cmul(conjf2(x[q]), mod[i]);
@@ -709,8 +714,9 @@ __global__ void __launch_bounds__(NTB) k_tpair_half(const float2* __restrict__ A
                 x[q] = conjf2(cmul(x[q], mod[i]));
             } else {
                 const float4 g = Gn[i];
-                const float2 eg = cmul(x[q], make_float2(g.x, g.y));
-                o2[i] = cmul(x[q], make_float2(fn * eg.x - h[q].x, fn * eg.y - h[q].y));
+                const float2 e = o2[i];
+                const float2 eg = cmul(e, make_float2(g.x, g.y));
+                o2[i] = cmul(e, make_float2(fn * eg.x - h[q].x, fn * eg.y - h[q].y));
             }
         }
     }
