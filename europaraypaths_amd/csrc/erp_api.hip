@@ -179,12 +179,14 @@ void slot_main(erp_ctx* ctx, ErpSlot* s) {
         if (rc != ERP_OK) hipStreamSynchronize(s->stream);  // drain what was queued
         ctx->active.fetch_sub(1);
         {
+            // notify under the lock: once it is released the caller may
+            // free the job (erp_wait / erp_poll)
             std::lock_guard<std::mutex> lk(j->mu);
             j->status = rc;
             j->err = err;
             j->done = true;
+            j->cv.notify_all();
         }
-        j->cv.notify_all();
     }
 }
 
