@@ -384,6 +384,48 @@ def test_specular_direct_sum_matches_convolution_path():
     assert corr > 0.999, corr
 
 
+def test_type0_target_direct_sum_closed_forms():
+    """Type-0 (reradiating) target below a flat surface, specular NoConv
+    (genRefrPhasor / refrRadarSignal, radar_kernels.cu:161-299,351-398), the
+    refracted trace alone:
+    * its peak sits at the bin of rngt = Itd + Ttd c1/c2 = h + n d;
+    * its phase there is +4 pi rngt / lam (the direct sum's sign);
+    * rerad_funct 1 and 2 are boxcars of 1 and 0.5 deg in the facet's
+      transmission angle, so with every lit term nearly in phase the peak
+      amplitude ratio equals the ratio of the facet counts inside the two
+      cones, counted here from the geometry alone."""
+    h, d, fs, n = 3000.0, 250.0, 1.0, 40
+    soa = workloads.flat_soa(n, n, fs, -fs * n / 2, -fs * n / 2)
+    tx, ty = 0.3, 0.2
+    xyzn = np.zeros((6, 1), np.float32)
+    xyzn[0], xyzn[1], xyzn[2], xyzn[5] = tx, ty, -d, 1.0
+    tg = (xyzn, np.zeros(1, np.uint8))
+    rho = np.hypot(soa[0].astype(np.float64) - tx, soa[1].astype(np.float64) - ty)
+    ang = np.degrees(np.arctan2(rho, d))
+    counts = {}
+    for rf, lim in ((1, 1.0), (2, 0.5)):
+        assert np.abs(ang - lim).min() > 1e-3      # no facet on a cone edge
+        counts[rf] = int((ang < lim).sum())
+    n2 = math.sqrt(3.15)
+    peaks = {}
+    for rf in (1, 2):
+        p = workloads.make_params("REASON_HF", "planetary_ice", nr=256, smpl=48e6,
+                                  rst=h + n2 * d - 600.0, aperture=60.0, fs=fs, ns=1, sx0=0.0,
+                                  sdx=0.0, sy=0.0, sz=h, specular=True, convolution=False,
+                                  convolution_linear=False, lossless=True, rerad_funct=rf)
+        par = parse_parameters(p)
+        _, _, parts = oracle.run(par.to_erp(), soa, tg, make_shots(par), parts=True)
+        refr = parts[0, 2]
+        k = int(np.argmax(np.abs(refr)))
+        rngt = h + n2 * d
+        assert k == int(round((rngt - float(par.rst)) / float(par.dr)))
+        want = np.angle(np.exp(4j * np.pi / float(par.lam) * rngt))
+        assert abs(np.angle(refr[k] * np.exp(-1j * want))) < 0.03
+        peaks[rf] = abs(refr[k])
+    assert counts[1] > counts[2] > 4
+    assert abs(peaks[1] / peaks[2] / (counts[1] / counts[2]) - 1) < 2e-3
+
+
 def test_fp32_mirror_tracks_fp64():
     """The FP32 mirror of the reference arithmetic stays near the FP64 oracle
     on a small scene (the per-bin gap is the reference's own FP32 error)."""
