@@ -426,6 +426,35 @@ def test_type0_target_direct_sum_closed_forms():
     assert abs(peaks[1] / peaks[2] / (counts[1] / counts[2]) - 1) < 2e-3
 
 
+def test_type0_reradiation_functions_3_and_4():
+    """rerad_funct 3 compares the angle times pi/180 with 3 (the reference's
+    degree/radian slip, radar_kernels.cu:178-181), so it passes every facet:
+    the same trace as rerad_funct 0.  rerad_funct 4 weights each facet by
+    exp(-theta^2 / 2 sigma^2), sigma = 1 deg: over a patch small enough that
+    every term is nearly in phase, the peak ratio to rerad_funct 0 is the mean
+    of those weights, computed here from the geometry."""
+    h, d, fs, n = 3000.0, 250.0, 1.0, 9
+    soa = workloads.flat_soa(n, n, fs, -fs * n / 2, -fs * n / 2)
+    xyzn = np.zeros((6, 1), np.float32)
+    xyzn[2], xyzn[5] = -d, 1.0
+    tg = (xyzn, np.zeros(1, np.uint8))
+    n2 = math.sqrt(3.15)
+    out = {}
+    for rf in (0, 3, 4):
+        p = workloads.make_params("REASON_HF", "planetary_ice", nr=256, smpl=48e6,
+                                  rst=h + n2 * d - 600.0, aperture=60.0, fs=fs, ns=1, sx0=0.0,
+                                  sdx=0.0, sy=0.0, sz=h, specular=True, convolution=False,
+                                  convolution_linear=False, lossless=True, rerad_funct=rf)
+        par = parse_parameters(p)
+        _, _, parts = oracle.run(par.to_erp(), soa, tg, make_shots(par), parts=True)
+        out[rf] = parts[0, 2]
+    np.testing.assert_array_equal(out[3], out[0])
+    theta = np.arctan2(np.hypot(soa[0].astype(np.float64), soa[1].astype(np.float64)), d)
+    w = np.exp(-0.5 * theta ** 2 / np.radians(1.0) ** 2)
+    k = int(np.argmax(np.abs(out[0])))
+    assert abs(abs(out[4][k]) / abs(out[0][k]) / w.mean() - 1) < 5e-3
+
+
 def test_fp32_mirror_tracks_fp64():
     """The FP32 mirror of the reference arithmetic stays near the FP64 oracle
     on a small scene (the per-bin gap is the reference's own FP32 error)."""
