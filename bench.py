@@ -38,7 +38,9 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 # Algorithmic operations per facet x target x shot triple of the target
 # accumulation kernel: adds, multiplies, divides, square roots and one per
-# transcendental call of the per-triple formula (98, itemised in DESIGN.md).
+# transcendental call of the per-triple formula (98: facet-target vector and
+# range 11, bin 3, direction and angles 39, four sincs 18, gains, loss and
+# amplitude 13, phase and phasor 5, two-bin split 9).
 TRIPLE_FLOPS = 98.0
 # Algorithmic HBM bytes per facet-shot pair of the surface path: the 12
 # float facet attributes streamed once per shot.

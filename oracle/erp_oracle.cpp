@@ -1,7 +1,7 @@
 /*
  * erp_oracle.cpp — CPU restatement of the EuropaRaypaths per-shot path.
  *
- * TEST INFRASTRUCTURE ONLY: this is the checker.  The product
+ * TEST INFRASTRUCTURE ONLY: the oracle the tests compare with.  The product
  * (europaraypaths_amd/) never links, loads or calls it; only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg do.
  *

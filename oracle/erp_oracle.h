@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (europaraypaths_amd/)
  * links, loads or calls this library; only tests/, __graft_entry__.smoke()
- * and bench.py's cpu_baseline leg do, as the checker.
+ * and bench.py's cpu_baseline leg do, as the reference they compare with.
  *
  * precision = 64: every step in IEEE double (the parity oracle).
  * precision = 32: the reference's FP32 arithmetic restated on the CPU (the

@@ -1,7 +1,7 @@
 """ctypes wrapper of liberp_oracle.so — TEST INFRASTRUCTURE ONLY.
 
-The CPU restatement of the reference path (erp_oracle.cpp) used as the
-checker by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+The CPU restatement of the reference path (erp_oracle.cpp) that tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg compare against.
 The product package never imports this module.
 """
 from __future__ import annotations
