@@ -219,3 +219,19 @@ def test_gaussian_layer_subset(gpu_ctx):
     g, gv, o, ov = run_both(gpu_ctx, par, soa, shots[:1], targets=tg)
     np.testing.assert_array_equal(gv, ov)
     compare_traces(g, o)
+
+
+def test_fbm_two_layer_full_scale_shot(gpu_ctx):
+    """One shot of the benchmark workload at full size (2048 x 2048 fBm
+    facets, 84 050 targets in two layers, 16 384 bins, 60 MHz; about 9.8e4
+    lit facets and 2.2e3 targets in the aperture) against the FP64 oracle
+    with the same tolerance as the small cases."""
+    p, soa, tg, shots = workloads.fbm_two_layer(ns=2000)
+    par = parse_parameters(p)
+    s1 = shots[1000:1001]
+    print("fbm_two_layer shot 1000", flush=True)
+    g, gv, o, ov = run_both(gpu_ctx, par, soa, s1, targets=tg)
+    np.testing.assert_array_equal(gv, ov)
+    assert gv[0] > 50000
+    w = compare_traces(g, o)
+    print(f"worst errors {w}", flush=True)
