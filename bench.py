@@ -189,6 +189,32 @@ def max_phase_err(a, b, floor_db=40.0):
     return worst
 
 
+def ingest_measure(ctx, workload, soa):
+    """Mesh ingest on the device for the fBm workload: erp_mesh_upload of the
+    12-float SoA built on the host against erp_mesh_from_heights of the
+    height grid (8 B per facet over PCIe, normals built on the device).
+    Best of three, host normal computation timed separately."""
+    if workload not in ("fbm_two_layer", "fbm_surface_only"):
+        return None
+    from europaraypaths_amd.surfaces import fbm_heights, terrain_soa
+    grid, fs = 2048, 50.0
+    z = fbm_heights(grid, grid, fs, 10.0, 0.8, seed=2)
+    t0 = time.perf_counter()
+    host = terrain_soa(grid, grid, fs, -fs * grid / 2, -fs * grid / 2, z)
+    t_host = time.perf_counter() - t0
+    up, dev = [], []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ctx.upload_mesh(host).release()
+        up.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        ctx.mesh_from_heights(z, fs, -fs * grid / 2, -fs * grid / 2).release()
+        dev.append(time.perf_counter() - t0)
+    return {"facets": int(host.shape[1]), "host_normals_s": t_host,
+            "soa_upload_s": min(up), "heights_on_device_s": min(dev),
+            "same_soa": bool(np.array_equal(host, soa))}
+
+
 def serve_measure(a):
     """Served vs plain on the mixed-size job stream.
 
@@ -419,6 +445,10 @@ def main():
         "roofline": roof,
         "roofline_other_kernels": others,
     }
+    try:
+        res["mesh_ingest"] = ingest_measure(ctx, a.workload, soa)
+    except Exception as ex:  # noqa: BLE001
+        res["mesh_ingest"] = {"error": str(ex)}
     if not a.no_serve:
         mesh.release()
         if targets is not None:

@@ -22,7 +22,8 @@ _lock = threading.Lock()
 EXPORTS = ("erp_ctx_create", "erp_ctx_destroy", "erp_last_error", "erp_mesh_upload",
            "erp_mesh_release", "erp_targets_upload", "erp_targets_release", "erp_run",
            "erp_run_dev", "erp_submit", "erp_poll", "erp_wait", "erp_run_footprint",
-           "erp_profile_enable", "erp_profile_read", "erp_serve_slots")
+           "erp_profile_enable", "erp_profile_read", "erp_serve_slots",
+           "erp_mesh_from_heights", "erp_mesh_download")
 
 STAGES = ("mask", "surface", "surface_conv", "target_select", "target_accum", "target_fft",
           "shot_final", "other", "target_sum")
@@ -55,6 +56,11 @@ def lib() -> ctypes.CDLL:
         L.erp_last_error.restype = ctypes.c_char_p
         L.erp_mesh_upload.argtypes = [vp, vp, i64, P(vp)]
         L.erp_mesh_upload.restype = i32
+        L.erp_mesh_from_heights.argtypes = [vp, vp, i64, i64, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_double, P(vp)]
+        L.erp_mesh_from_heights.restype = i32
+        L.erp_mesh_download.argtypes = [vp, vp]
+        L.erp_mesh_download.restype = i32
         L.erp_mesh_release.argtypes = [vp]
         L.erp_mesh_release.restype = None
         L.erp_targets_upload.argtypes = [vp, vp, vp, i64, P(vp)]
@@ -89,17 +95,33 @@ def _ptr(a: np.ndarray | None):
 
 
 class Mesh:
-    """A facet mesh resident in HBM (erp_mesh_upload)."""
+    """A facet mesh resident in HBM (erp_mesh_upload / erp_mesh_from_heights)."""
 
-    def __init__(self, ctx: "Context", soa: np.ndarray):
-        soa = np.ascontiguousarray(soa, dtype=np.float32)
-        if soa.ndim != 2 or soa.shape[0] != 12:
-            raise ValueError("mesh must be a (12, F) float32 SoA array")
+    def __init__(self, ctx: "Context", soa: np.ndarray | None, heights=None):
         self.ctx = ctx
-        self.F = soa.shape[1]
         h = ctypes.c_void_p()
-        ctx._check(lib().erp_mesh_upload(ctx.handle, _ptr(soa), self.F, ctypes.byref(h)))
+        if heights is None:
+            soa = np.ascontiguousarray(soa, dtype=np.float32)
+            if soa.ndim != 2 or soa.shape[0] != 12:
+                raise ValueError("mesh must be a (12, F) float32 SoA array")
+            self.F = soa.shape[1]
+            ctx._check(lib().erp_mesh_upload(ctx.handle, _ptr(soa), self.F, ctypes.byref(h)))
+        else:
+            z, fs, ox, oy = heights
+            z = np.ascontiguousarray(z, dtype=np.float64)
+            if z.ndim != 2:
+                raise ValueError("heights must be a (ny, nx) array")
+            ny, nx = z.shape
+            self.F = nx * ny
+            ctx._check(lib().erp_mesh_from_heights(ctx.handle, _ptr(z), nx, ny, float(fs),
+                                                   float(ox), float(oy), ctypes.byref(h)))
         self.handle = h
+
+    def download(self) -> np.ndarray:
+        """The resident mesh as the (12, F) float32 SoA (erp_mesh_download)."""
+        out = np.zeros((12, self.F), np.float32)
+        self.ctx._check(lib().erp_mesh_download(self.handle, _ptr(out)))
+        return out
 
     def release(self):
         if getattr(self, "handle", None):
@@ -182,6 +204,11 @@ class Context:
 
     def upload_mesh(self, soa) -> Mesh:
         return Mesh(self, soa)
+
+    def mesh_from_heights(self, heights, fs, ox, oy) -> Mesh:
+        """Build a mesh on the device from a (ny, nx) height grid with spacing
+        fs and origin (ox, oy): the mesh surfaces.terrain_soa would give."""
+        return Mesh(self, None, heights=(heights, fs, ox, oy))
 
     def upload_targets(self, xyzn, ttype) -> Targets:
         return Targets(self, xyzn, ttype)

@@ -339,6 +339,62 @@ int erp_mesh_upload(erp_ctx* ctx, const float* soa, int64_t F, erp_mesh** out) {
     return ERP_OK;
 }
 
+int erp_mesh_from_heights(erp_ctx* ctx, const double* heights, int64_t nx, int64_t ny,
+                          double fs, double ox, double oy, erp_mesh** out) {
+    if (!ctx || !out || nx < 0 || ny < 0 || (nx * ny > 0 && !heights)) return ERP_E_INVALID;
+    *out = nullptr;
+    erp_mesh* m = new (std::nothrow) erp_mesh();
+    if (!m) return ERP_E_NOMEM;
+    int rc = guarded(ctx, [&] {
+        if (!(fs > 0.0)) throw ErpError{ERP_E_INVALID, "fs must be > 0"};
+        if (nx > (int64_t)1 << 20 || ny > (int64_t)1 << 20 || nx * ny > (int64_t)1 << 31)
+            throw ErpError{ERP_E_INVALID, "height grid too large"};
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        set_device(ctx);
+        const int64_t F = nx * ny;
+        m->ctx = ctx;
+        m->device = ctx->device;
+        m->F = F;
+        m->Fpad = ((F + 63) / 64) * 64;
+        m->soa.ensure(sizeof(float) * 12 * (size_t)std::max<int64_t>(m->Fpad, 64));
+        DevBuf z;
+        z.ensure(sizeof(double) * (size_t)std::max<int64_t>(F, 1));
+        if (F > 0) {
+            ERP_HIP_TRY(hipMemcpyAsync(z.p, heights, sizeof(double) * F, hipMemcpyHostToDevice,
+                                       ctx->stream));
+            mesh_from_heights(z.as<double>(), nx, ny, fs, ox, oy, m->soa.as<float>(), m->Fpad,
+                              ctx->stream);
+        }
+        ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        z.release();
+        // tangents and normal form an orthonormal frame by construction
+        m->frame_ok = true;
+    });
+    if (rc != ERP_OK) {
+        m->soa.release();
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return ERP_OK;
+}
+
+int erp_mesh_download(const erp_mesh* m, float* soa) {
+    if (!m || (m->F > 0 && !soa)) return ERP_E_INVALID;
+    erp_ctx* ctx = m->ctx;
+    return guarded(ctx, [&] {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        set_device(ctx);
+        for (int a = 0; a < 12; ++a)
+            if (m->F > 0)
+                ERP_HIP_TRY(hipMemcpyAsync(soa + (size_t)a * m->F,
+                                           m->soa.as<float>() + (size_t)a * m->Fpad,
+                                           sizeof(float) * m->F, hipMemcpyDeviceToHost,
+                                           ctx->stream));
+        ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 void erp_mesh_release(erp_mesh* m) {
     if (!m) return;
     hipSetDevice(m->device);

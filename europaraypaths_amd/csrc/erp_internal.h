@@ -236,6 +236,9 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                   int32_t* nvalid_dev, uint32_t flags, hipStream_t stream,
                   WorkBufs& work, StageTimer* timer);
 int64_t erp_footprint_impl(const erp_params* par, int64_t F, int64_t T, int64_t ns);
+// 12 x Fpad SoA of a height grid (see erp_mesh_from_heights)
+void mesh_from_heights(const double* heights, int64_t nx, int64_t ny, double fs, double ox,
+                       double oy, float* soa, int64_t Fpad, hipStream_t st);
 
 inline int next_pow2(int64_t n) {
     int p = 1;

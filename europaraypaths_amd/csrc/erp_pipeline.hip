@@ -2040,6 +2040,54 @@ struct BatchPlan {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// mesh from a height grid: positions, central-difference normals and
+// cross-product tangents in FP64 with the operation order of gen_normals
+// (simple_surfaces.py:3-39: numpy's elementwise arithmetic, np.cross,
+// np.linalg.norm over three components), rounded once to float
+// ---------------------------------------------------------------------------
+__global__ void k_mesh_from_heights(const double* __restrict__ z, int64_t nx, int64_t ny,
+                                    double fs, double ox, double oy, float* __restrict__ soa,
+                                    int64_t Fpad) {
+#pragma clang fp contract(off)
+    const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nx * ny) return;
+    const int64_t j = f / nx, i = f - j * nx;
+    const double two_fs = 2.0 * fs;
+    // d/dx at column c (interior columns only)
+    auto ddx = [&](int64_t c) { return (z[j * nx + c + 1] - z[j * nx + c - 1]) / two_fs; };
+    auto ddy = [&](int64_t r) { return (z[(r + 1) * nx + i] - z[(r - 1) * nx + i]) / two_fs; };
+    double dx = 0.0, dy = 0.0;
+    if (nx > 2) dx = ddx(i == 0 ? 1 : (i == nx - 1 ? nx - 2 : i));
+    if (ny > 2) dy = ddy(j == 0 ? 1 : (j == ny - 1 ? ny - 2 : j));
+    const double norms = sqrt(dx * dx + dy * dy + 1.0);
+    const double n0 = -dx / norms, n1 = -dy / norms, n2 = 1.0 / norms;
+    double r0 = 1.0, r1 = 0.0, r2 = 0.0;
+    if (fabs(n0) > 0.9) {
+        r0 = 0.0;
+        r1 = 1.0;
+    }
+    double u0 = r1 * n2 - r2 * n1, u1 = r2 * n0 - r0 * n2, u2 = r0 * n1 - r1 * n0;
+    const double un = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
+    u0 = u0 / un;
+    u1 = u1 / un;
+    u2 = u2 / un;
+    const double v0 = n1 * u2 - n2 * u1, v1 = n2 * u0 - n0 * u2, v2 = n0 * u1 - n1 * u0;
+    const double vals[12] = {(double)i * fs + ox, (double)j * fs + oy, z[f], n0, n1, n2,
+                             u0, u1, u2, v0, v1, v2};
+#pragma unroll
+    for (int a = 0; a < 12; ++a) soa[(size_t)a * Fpad + f] = (float)vals[a];
+}
+
+void mesh_from_heights(const double* heights, int64_t nx, int64_t ny, double fs, double ox,
+                       double oy, float* soa, int64_t Fpad, hipStream_t st) {
+    const int64_t F = nx * ny;
+    if (F == 0) return;
+    hipLaunchKernelGGL(k_mesh_from_heights, dim3((unsigned)((F + 255) / 256)), dim3(256), 0, st,
+                       heights, nx, ny, fs, ox, oy, soa, Fpad);
+    ERP_CHECK_LAUNCH("k_mesh_from_heights");
+}
+
 using TargetAccumFn = decltype(&k_target_accum_fast<false>);
 
 TargetAccumFn pick_target_accum(bool det, int ph, int acc) {

@@ -147,6 +147,19 @@ const char* erp_last_error(const erp_ctx* ctx);
  * sim.cu:228-239).  The mesh stays resident in HBM until released. */
 int erp_mesh_upload(erp_ctx* ctx, const float* soa, int64_t F, erp_mesh** out);
 void erp_mesh_release(erp_mesh* mesh);
+/* Build a resident mesh on the device from a height grid: heights holds
+ * ny rows of nx doubles (x fastest), facet (i, j) sits at (ox + i fs,
+ * oy + j fs, heights[j nx + i]).  Normals come from central differences
+ * and the tangents from cross products exactly as gen_normals computes them
+ * (simple_surfaces.py:3-39), facets in make_surface's order
+ * (simple_surfaces.py:147-181, no mask): the same mesh as erp_mesh_upload
+ * of that SoA, with 8 bytes per facet crossing PCIe instead of 48 and no
+ * host-side normal computation. */
+int erp_mesh_from_heights(erp_ctx* ctx, const double* heights, int64_t nx, int64_t ny,
+                          double fs, double ox, double oy, erp_mesh** out);
+/* Copy a resident mesh back as the 12 x F float32 SoA of erp_mesh_upload
+ * (soa must hold 12 F floats). */
+int erp_mesh_download(const erp_mesh* mesh, float* soa);
 
 /* Upload T targets: xyzn holds 6 consecutive float arrays x, y, z, nx, ny, nz
  * of length T; type holds T target types (file_io.cu:365-426). */

@@ -235,3 +235,23 @@ def test_fbm_two_layer_full_scale_shot(gpu_ctx):
     assert gv[0] > 50000
     w = compare_traces(g, o)
     print(f"worst errors {w}", flush=True)
+
+
+@pytest.mark.parametrize("shape", [(40, 48), (1, 7), (2, 2), (3, 300)])
+def test_mesh_from_heights_matches_host_soa(gpu_ctx, shape):
+    """erp_mesh_from_heights builds on the device the SoA that
+    surfaces.terrain_soa builds on the host (gen_normals, simple_surfaces.py:
+    3-39): identical float32 values, including steep slopes where the
+    reference vector switches to y."""
+    ny, nx = shape
+    fs, ox, oy = 12.5, -300.0, 125.0
+    z = surfaces.gaussian_heights(nx, ny, fs, 4.0, 20.0, seed=5) if nx > 1 and ny > 1 \
+        else np.linspace(0.0, 30.0, nx * ny).reshape(ny, nx)
+    z[ny // 2, :] += np.linspace(-60.0, 60.0, nx) * 20.0  # slopes with |n_x| > 0.9
+    host = surfaces.terrain_soa(nx, ny, fs, ox, oy, z)
+    m = gpu_ctx.mesh_from_heights(z, fs, ox, oy)
+    try:
+        dev = m.download()
+    finally:
+        m.release()
+    np.testing.assert_array_equal(dev, host)
