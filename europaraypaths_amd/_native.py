@@ -23,7 +23,7 @@ EXPORTS = ("erp_ctx_create", "erp_ctx_destroy", "erp_last_error", "erp_mesh_uplo
            "erp_mesh_release", "erp_targets_upload", "erp_targets_release", "erp_run",
            "erp_run_dev", "erp_submit", "erp_poll", "erp_wait", "erp_run_footprint",
            "erp_profile_enable", "erp_profile_read", "erp_serve_slots",
-           "erp_mesh_from_heights", "erp_mesh_download")
+           "erp_mesh_from_heights", "erp_mesh_download", "erp_focus")
 
 STAGES = ("mask", "surface", "surface_conv", "target_select", "target_accum", "target_fft",
           "shot_final", "other", "target_sum")
@@ -61,6 +61,8 @@ def lib() -> ctypes.CDLL:
         L.erp_mesh_from_heights.restype = i32
         L.erp_mesh_download.argtypes = [vp, vp]
         L.erp_mesh_download.restype = i32
+        L.erp_focus.argtypes = [vp, vp, i64, i64, i64, i64, vp, vp, vp, vp, vp, i64, vp]
+        L.erp_focus.restype = i32
         L.erp_mesh_release.argtypes = [vp]
         L.erp_mesh_release.restype = None
         L.erp_targets_upload.argtypes = [vp, vp, vp, i64, P(vp)]
@@ -260,6 +262,22 @@ class Context:
                                      _ptr(shots), ns, _ptr(prisms), npr, _ptr(out),
                                      _ptr(nvalid), flags, ctypes.byref(h)))
         return Job(self, h, out, nvalid, (shots, prisms, params))
+
+    def focus(self, rdr, row0: int, nrows: int, hs, nsx, offs, rows, filt) -> np.ndarray:
+        """erp_focus: focused rows row0 .. row0 + nrows - 1 of the complex64
+        (Nr, Na) radargram -> (nrows, Na) complex64."""
+        rdr = np.ascontiguousarray(rdr, dtype=np.complex64)
+        Nr, Na = rdr.shape
+        hs = np.ascontiguousarray(hs, dtype=np.int32)
+        nsx = np.ascontiguousarray(nsx, dtype=np.int32)
+        offs = np.ascontiguousarray(offs, dtype=np.int64)
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        filt = np.ascontiguousarray(filt, dtype=np.complex64)
+        out = np.zeros((nrows, Na), np.complex64)
+        self._check(lib().erp_focus(self.handle, _ptr(rdr), Nr, Na, row0, nrows, _ptr(hs),
+                                    _ptr(nsx), _ptr(offs), _ptr(rows), _ptr(filt), rows.size,
+                                    _ptr(out)))
+        return out
 
     def serve_slots(self, n: int):
         """Start worker slots for submit() up to n (each: thread, stream, buffers)."""

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include <climits>
 #include <cstring>
 #include <exception>
 #include <new>
@@ -587,6 +588,61 @@ int erp_profile_read(erp_ctx* ctx, int stage, double* ms, int64_t* launches, dou
     if (launches) *launches = ctx->prof.launches[stage];
     if (units) *units = ctx->prof.units[stage];
     return ERP_OK;
+}
+
+int erp_focus(erp_ctx* ctx, const float* rdr, int64_t Nr, int64_t Na, int64_t row0,
+              int64_t nrows, const int32_t* row_hs, const int32_t* row_nsx,
+              const int64_t* row_off, const int32_t* rows, const float* filt, int64_t ntab,
+              float* out) {
+    if (!ctx) return ERP_E_INVALID;
+    return guarded(ctx, [&] {
+        if (Nr <= 0 || Na <= 0 || nrows < 0 || row0 < 0 || row0 + nrows > Nr || ntab < 0 ||
+            Nr > INT32_MAX || Na > INT32_MAX || Nr * Na > ((int64_t)1 << 34))
+            throw ErpError{ERP_E_INVALID, "invalid radargram or row range"};
+        if (nrows == 0) return;
+        if (!rdr || !out || !row_hs || !row_nsx || !row_off || (ntab > 0 && (!rows || !filt)))
+            throw ErpError{ERP_E_INVALID, "NULL argument"};
+        for (int64_t q = 0; q < nrows; ++q) {
+            const int64_t hs = row_hs[q], nsx = row_nsx[q], off = row_off[q];
+            if (hs < 0 || nsx < 0 || 2 * hs > nsx || off < 0 || off + nsx > ntab ||
+                nsx > INT32_MAX / 2)
+                throw ErpError{ERP_E_INVALID, "row " + std::to_string(row0 + q) +
+                                                  ": aperture does not fit the tables"};
+        }
+        for (int64_t t = 0; t < ntab; ++t)
+            if (rows[t] < 0 || rows[t] >= Nr)
+                throw ErpError{ERP_E_INVALID, "aperture row index out of the radargram"};
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        set_device(ctx);
+        DevBuf d_rdr, d_hs, d_nsx, d_off, d_rows, d_filt, d_out;
+        d_rdr.ensure(sizeof(float2) * (size_t)(Nr * Na));
+        d_hs.ensure(sizeof(int) * (size_t)nrows);
+        d_nsx.ensure(sizeof(int) * (size_t)nrows);
+        d_off.ensure(sizeof(int64_t) * (size_t)nrows);
+        d_rows.ensure(sizeof(int) * (size_t)std::max<int64_t>(ntab, 1));
+        d_filt.ensure(sizeof(float2) * (size_t)std::max<int64_t>(ntab, 1));
+        d_out.ensure(sizeof(float2) * (size_t)(nrows * Na));
+        hipStream_t st = ctx->stream;
+        ERP_HIP_TRY(hipMemcpyAsync(d_rdr.p, rdr, sizeof(float2) * Nr * Na,
+                                   hipMemcpyHostToDevice, st));
+        ERP_HIP_TRY(hipMemcpyAsync(d_hs.p, row_hs, sizeof(int) * nrows, hipMemcpyHostToDevice, st));
+        ERP_HIP_TRY(hipMemcpyAsync(d_nsx.p, row_nsx, sizeof(int) * nrows, hipMemcpyHostToDevice,
+                                   st));
+        ERP_HIP_TRY(hipMemcpyAsync(d_off.p, row_off, sizeof(int64_t) * nrows,
+                                   hipMemcpyHostToDevice, st));
+        if (ntab > 0) {
+            ERP_HIP_TRY(hipMemcpyAsync(d_rows.p, rows, sizeof(int) * ntab, hipMemcpyHostToDevice,
+                                       st));
+            ERP_HIP_TRY(hipMemcpyAsync(d_filt.p, filt, sizeof(float2) * ntab,
+                                       hipMemcpyHostToDevice, st));
+        }
+        focus_rows(d_rdr.as<float2>(), (int)Nr, (int)Na, (int)row0, (int)nrows, d_hs.as<int>(),
+                   d_nsx.as<int>(), d_off.as<int64_t>(), d_rows.as<int>(), d_filt.as<float2>(),
+                   d_out.as<float2>(), st);
+        ERP_HIP_TRY(hipMemcpyAsync(out, d_out.p, sizeof(float2) * nrows * Na,
+                                   hipMemcpyDeviceToHost, st));
+        ERP_HIP_TRY(hipStreamSynchronize(st));
+    });
 }
 
 int64_t erp_run_footprint(const erp_params* par, int64_t F, int64_t T, int64_t ns) {

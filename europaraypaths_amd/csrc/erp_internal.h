@@ -236,6 +236,10 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                   int32_t* nvalid_dev, uint32_t flags, hipStream_t stream,
                   WorkBufs& work, StageTimer* timer);
 int64_t erp_footprint_impl(const erp_params* par, int64_t F, int64_t T, int64_t ns);
+// focused rows row0 .. row0 + nrows - 1 of an Nr x Na radargram (erp_focus.hip)
+void focus_rows(const float2* rdr, int Nr, int Na, int row0, int nrows, const int* row_hs,
+                const int* row_nsx, const int64_t* row_off, const int* rows, const float2* filt,
+                float2* out, hipStream_t st);
 // 12 x Fpad SoA of a height grid (see erp_mesh_from_heights)
 void mesh_from_heights(const double* heights, int64_t nx, int64_t ny, double fs, double ox,
                        double oy, float* soa, int64_t Fpad, hipStream_t st);

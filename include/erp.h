@@ -230,6 +230,24 @@ int erp_profile_enable(erp_ctx* ctx, int on);
  * launches and units for one stage. */
 int erp_profile_read(erp_ctx* ctx, int stage, double* ms, int64_t* launches, double* units);
 
+/* ---- SAR focusing (simple_focusing.py:310-404) ------------------------------
+ * Focus range bins row0 .. row0 + nrows - 1 of an Nr x Na complex radargram
+ * (rdr: Nr * Na * 2 floats, row-major, as compile_rdrgrm lays it out).  For
+ * row q the caller gives the half aperture row_hs[q] and aperture length
+ * row_nsx[q] (2 row_hs[q] <= row_nsx[q]) in traces, and at row_off[q] in the
+ * ntab-long tables the radargram row each aperture sample reads
+ * (rows[], in [0, Nr): the sample's shifted range bin modulo Nr) and the
+ * matched filter (filt[], complex).  out receives nrows * Na * 2 floats:
+ *     out[q][j] = sum_{m < w} rdr[rows[off + m]][az_st + m] filt[off + ((n - m) mod L)]
+ * with az_st = max(0, j - hs), w = min(Na, j + hs) - az_st, L = 2 nsx,
+ * n = (j - az_st) + (L - w) / 2 and filt zero past nsx: the sample of the
+ * reference's per-pixel FFT product that it keeps (focus_rdrgrm's inner loop,
+ * simple_focusing.py:357-394). */
+int erp_focus(erp_ctx* ctx, const float* rdr, int64_t Nr, int64_t Na, int64_t row0,
+              int64_t nrows, const int32_t* row_hs, const int32_t* row_nsx,
+              const int64_t* row_off, const int32_t* rows, const float* filt, int64_t ntab,
+              float* out);
+
 /* Device bytes a run of (F facets, T targets, ns shots, nr bins) needs beyond
  * the resident mesh and targets; used for admission by footprint. */
 int64_t erp_run_footprint(const erp_params* par, int64_t F, int64_t T, int64_t ns);
