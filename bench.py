@@ -215,6 +215,37 @@ def ingest_measure(ctx, workload, soa):
             "same_soa": bool(np.array_equal(host, soa))}
 
 
+FOCUS_PAR = {"rx_window_offset_m": 24.75e3, "rx_sample_rate": 60e6, "aperture": 3.0,
+             "spacing": 25.0, "altitude": 25e3, "eps_2": 3.15, "frequency": 60e6}
+
+
+def focus_measure(ctx):
+    """focus_rdrgrm on the device for two radargram shapes with the CHAOS
+    example's focusing parameters (examples/CHAOS/code/params.py: 60 MHz,
+    3 deg aperture, 25 m trace spacing, 25 km altitude): its own 150 x 2000
+    radargram and a 1024 x 8000 one (rx window 5120 m).  Wall time of the
+    whole call (host tables, copies both ways, kernel), best of three."""
+    from europaraypaths_amd.focusing import focus_rdrgrm, tables
+    rec = {}
+    for nr, na in ((150, 2000), (1024, 8000)):
+        par = dict(FOCUS_PAR, rx_window_m=nr * (299792458 / 60e6))
+        rng = np.random.default_rng(5)
+        rdr = (rng.standard_normal((nr, na)) + 1j * rng.standard_normal((nr, na))).astype(
+            np.complex64)
+        nsx = tables(par, 0, nr - 1, nr)[1]
+        macs = float(na) * float(nsx.sum())
+        focus_rdrgrm(rdr, par, ctx=ctx)
+        wall = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            focus_rdrgrm(rdr, par, ctx=ctx)
+            wall.append(time.perf_counter() - t0)
+        rec[f"{nr}x{na}"] = {"pixels": nr * na, "aperture_samples_mean": float(nsx.mean()),
+                             "complex_macs": macs, "wall_s": min(wall),
+                             "pixels_per_s": nr * na / min(wall)}
+    return rec
+
+
 def serve_measure(a):
     """Served vs plain on the mixed-size job stream.
 
@@ -449,6 +480,10 @@ def main():
         res["mesh_ingest"] = ingest_measure(ctx, a.workload, soa)
     except Exception as ex:  # noqa: BLE001
         res["mesh_ingest"] = {"error": str(ex)}
+    try:
+        res["focusing"] = focus_measure(ctx)
+    except Exception as ex:  # noqa: BLE001
+        res["focusing"] = {"error": str(ex)}
     if not a.no_serve:
         mesh.release()
         if targets is not None:

@@ -24,16 +24,25 @@ def slantrange_to_rangebin(sltrng, params):
     return ((sltrng - params["rx_window_offset_m"]) // dm).astype(int)
 
 
-def refraction_angle(xoff, h, d, eps_2, iters=80):
+def refraction_angle(xoff, h, d, eps_2, bisect=12, newton=8):
     """Root th_r in [0, 20 deg] of h tan(asin(sqrt(eps_2) sin th)) +
     d tan(th) - |xoff| (refracted_angle, simple_focusing.py:79-83), for every
-    element of xoff at once, by bisection.  Raises ValueError where the
-    bracket holds no sign change, as the reference's brentq does."""
+    element of xoff (h and d scalars or arrays of the same shape) at once:
+    bisection steps to get close, then Newton steps kept inside the
+    bracket.  Raises ValueError where the bracket holds no sign change, as
+    the reference's brentq does."""
     x = np.abs(np.asarray(xoff, dtype=np.float64))
+    h = np.broadcast_to(np.asarray(h, dtype=np.float64), x.shape)
+    d = np.broadcast_to(np.asarray(d, dtype=np.float64), x.shape)
     n2 = np.sqrt(eps_2)
 
     def f(th):
         return h * np.tan(np.arcsin(n2 * np.sin(th))) + d * np.tan(th) - x
+
+    def df(th):
+        s = n2 * np.sin(th)
+        c = np.sqrt(1.0 - s * s)
+        return h * (n2 * np.cos(th) / c) / (c * c) + d / np.cos(th) ** 2
 
     lo = np.zeros_like(x)
     hi = np.full_like(x, np.radians(20))
@@ -41,7 +50,7 @@ def refraction_angle(xoff, h, d, eps_2, iters=80):
     if np.any(np.sign(flo) * np.sign(fhi) > 0):
         raise ValueError("f(a) and f(b) must have different signs")
     root0 = flo == 0.0
-    for _ in range(iters):
+    for _ in range(bisect):
         mid = 0.5 * (lo + hi)
         fm = f(mid)
         left = np.sign(fm) == np.sign(flo)
@@ -49,6 +58,10 @@ def refraction_angle(xoff, h, d, eps_2, iters=80):
         flo = np.where(left, fm, flo)
         hi = np.where(left, hi, mid)
     th = 0.5 * (lo + hi)
+    for _ in range(newton):
+        step = f(th) / df(th)
+        nxt = th - step
+        th = np.where((nxt > lo) & (nxt < hi), nxt, 0.5 * (lo + hi))
     return np.where(root0, 0.0, th)
 
 
@@ -82,6 +95,30 @@ def row_tables(params, i, Nr):
     return apt_smpl, apt_hsmpl, sltrng_rb, mth_filt
 
 
+def tables(params, lo, hi, Nr):
+    """row_tables of bins lo .. hi packed for erp_focus: half apertures,
+    aperture lengths, offsets, radargram rows (shifted bins modulo Nr) and
+    matched filters, with one refraction solve over every sample."""
+    i = np.arange(lo, hi + 1)
+    rng = params["rx_window_m"] * (i / Nr) + params["rx_window_offset_m"]
+    apt_m = rng * np.sin(np.radians(params["aperture"]))
+    nsx = (2 * apt_m / params["spacing"]).astype(np.int64)
+    hs = (apt_m / params["spacing"]).astype(np.int64)
+    offs = np.concatenate([[0], np.cumsum(nsx)[:-1]]).astype(np.int64)
+    sx = np.concatenate([np.linspace(-1 * a, 1 * a, n) for a, n in zip(apt_m, nsx)])
+    row_of = np.repeat(np.arange(len(i)), nsx)
+    sz = params["altitude"]
+    tz = params["altitude"] - rng[row_of]
+    sltrng = np.sqrt((sx - 0) ** 2 + (sz - tz) ** 2)
+    rows = np.mod(slantrange_to_rangebin(sltrng, params), Nr).astype(np.int32)
+    lam = params["lam"] if "lam" in params else C_PY / params["frequency"]
+    th_r = refraction_angle(sx - 0, sz, tz, params["eps_2"])
+    th_i = np.arcsin(np.sqrt(params["eps_2"]) * np.sin(th_r))
+    corrected_rng = sz / np.cos(th_i) + tz / np.cos(th_r) * np.sqrt(params["eps_2"])
+    filt = np.conj(np.exp(((-4j * np.pi) / lam) * corrected_rng)).astype(np.complex64)
+    return hs.astype(np.int32), nsx.astype(np.int32), offs, rows, filt
+
+
 def _rows(Nr, st, en):
     # the reference skips rows outside [st, en] only when both are truthy
     if st and en:
@@ -108,18 +145,7 @@ def focus_rdrgrm(rdrgrm, par, st=None, en=None, return_shifts=False,
     rdr_f = np.zeros_like(rdrgrm)
     if hi < lo:
         return rdr_f
-    hs, nsx, offs, rows, filts = [], [], [], [], []
-    off = 0
-    for i in range(lo, hi + 1):
-        n, h, rb, mf = row_tables(par, i, Nr)
-        hs.append(h)
-        nsx.append(n)
-        offs.append(off)
-        rows.append(np.mod(rb, Nr))
-        filts.append(mf)
-        off += n
-    rows = np.ascontiguousarray(np.concatenate(rows), dtype=np.int32)
-    filt = np.ascontiguousarray(np.concatenate(filts), dtype=np.complex64)
+    hs, nsx, offs, rows, filt = tables(par, lo, hi, Nr)
     if ctx is None:
         from .simulate import default_context
         ctx = default_context()

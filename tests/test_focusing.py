@@ -67,3 +67,29 @@ def test_row_selection_follows_the_reference():
     assert focusing._rows(100, 0, 50) == (0, 99)     # st falsy: every row
     assert focusing._rows(100, 10, 50) == (10, 50)
     assert focusing._rows(100, 90, 500) == (90, 99)
+
+
+def test_straight_ray_limit():
+    """eps_2 = 1: no refraction, so tan th = |x| / (h + d) and the corrected
+    range is the straight slant range sqrt((h + d)^2 + x^2)."""
+    par = dict(PAR, eps_2=1.0)
+    h, d = 5000.0, 1200.0
+    x = np.linspace(-900.0, 900.0, 31)
+    for mod in (ofo, focusing):
+        c = mod.get_target_phase(par, h, d, x, **(
+            {"cmplx_out": True} if mod is focusing else {}))
+        want = np.exp(-4j * np.pi / par["lam"] * np.hypot(h + d, x))
+        np.testing.assert_allclose(c, want, rtol=0, atol=1e-9)
+
+
+def test_packed_tables_match_per_row_tables():
+    """The tables handed to erp_focus (all rows at once, shifts modulo Nr,
+    complex64 filters) are the oracle's per-row tables."""
+    Nr, lo, hi = 64, 5, 40
+    hs, nsx, offs, rows, filt = focusing.tables(PAR, lo, hi, Nr)
+    assert offs[0] == 0 and rows.size == filt.size == int(nsx.sum())
+    for q, i in enumerate(range(lo, hi + 1)):
+        n, h, rb, mf = ofo.row_tables(PAR, i, Nr)
+        assert (n, h) == (nsx[q], hs[q])
+        np.testing.assert_array_equal(rows[offs[q]:offs[q] + n], np.mod(rb, Nr))
+        np.testing.assert_allclose(filt[offs[q]:offs[q] + n], mf, rtol=0, atol=2e-7)
