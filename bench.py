@@ -219,6 +219,48 @@ FOCUS_PAR = {"rx_window_offset_m": 24.75e3, "rx_sample_rate": 60e6, "aperture": 
              "spacing": 25.0, "altitude": 25e3, "eps_2": 3.15, "frequency": 60e6}
 
 
+def counters_summary(dirs, bench_json, out_path):
+    """Per-dispatch averages of every counter in rocprofv3 --pmc runs of this
+    script (csv output) for the hot kernels, with the ratios the design
+    choices rest on: wave-level VALU and LDS instructions per unit of work,
+    and LDS bank-conflict cycles per active LDS cycle."""
+    with open(bench_json) as f:
+        b = json.loads([l for l in f if l.startswith("{")][-1])
+    agg, disp = {}, {}
+    for d in dirs:
+        for r in _counter_rows(d):
+            for stage, kern in KERNELS.items():
+                if kern in r["Kernel_Name"]:
+                    c = agg.setdefault(kern, {})
+                    c[r["Counter_Name"]] = c.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+                    disp.setdefault((kern, r["Counter_Name"]), set()).add(r["Dispatch_Id"])
+    lines = [f"# Hardware counters of the hot kernels, workload {b['config']['workload']} "
+             f"(one MI355X)", ""]
+    for kern, c in agg.items():
+        stage = [s for s, k in KERNELS.items() if k == kern][0]
+        units = b["stage_units"][stage] / max(b["stage_launches"][stage], 1)
+        lines.append(f"{kern}: {units:.4g} units per dispatch")
+        avg = {}
+        for name, v in sorted(c.items()):
+            n = len(disp[(kern, name)])
+            avg[name] = v / n
+            lines.append(f"  {name:26s} {avg[name]:14.5g} per dispatch ({n} dispatches)")
+        if "SQ_INSTS_VALU" in avg and units:
+            lines.append(f"  VALU wave-instructions per unit      {avg['SQ_INSTS_VALU'] / units:10.4g}")
+        if "SQ_INSTS_LDS" in avg and units:
+            lines.append(f"  LDS wave-instructions per unit       {avg['SQ_INSTS_LDS'] / units:10.4g}")
+        if "SQ_LDS_BANK_CONFLICT" in avg and avg.get("SQ_ACTIVE_INST_LDS"):
+            lines.append(f"  LDS bank-conflict cycles / active LDS cycles "
+                         f"{avg['SQ_LDS_BANK_CONFLICT'] / avg['SQ_ACTIVE_INST_LDS']:.3f}")
+        if avg.get("SQ_BUSY_CYCLES") and "SQ_ACTIVE_INST_VALU" in avg:
+            lines.append(f"  VALU-active / SQ-busy cycles         "
+                         f"{avg['SQ_ACTIVE_INST_VALU'] / avg['SQ_BUSY_CYCLES']:10.4g}")
+        lines.append("")
+    with open(out_path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def focus_measure(ctx):
     """focus_rdrgrm on the device for two radargram shapes with the CHAOS
     example's focusing parameters (examples/CHAOS/code/params.py: 60 MHz,
@@ -353,6 +395,9 @@ def main():
     ap.add_argument("--flags", type=int, default=0)
     ap.add_argument("--ingest-pmc", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "BENCH_JSON"),
                     help="write profiles/traffic.json from two rocprofv3 --pmc runs")
+    ap.add_argument("--counters-summary", nargs="+", metavar="ARG",
+                    help="PMC_DIR [PMC_DIR ...] BENCH_JSON OUT_TXT: per-dispatch counter "
+                         "averages of the hot kernels from rocprofv3 --pmc runs")
     ap.add_argument("--serve", action="store_true",
                     help="served vs plain on the mixed-size job stream")
     ap.add_argument("--serve-jobs", type=int, default=48)
@@ -365,6 +410,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true",
                     help="leave the served-vs-plain record out of the default line")
     a = ap.parse_args()
+    if a.counters_summary:
+        counters_summary(a.counters_summary[:-2], a.counters_summary[-2], a.counters_summary[-1])
+        return
     if a.ingest_pmc:
         ingest_pmc(*a.ingest_pmc, os.path.join(ROOT, "profiles", "traffic.json"))
         return

@@ -151,6 +151,17 @@ __device__ __forceinline__ float2 cos_sin_ref(double ix, double iy, double iz, f
     return make_float2((float)c, sqrtf((float)((1.0 - c) * (1.0 + c))));
 }
 
+// cos_sin_ref with the hardware square root (about 1 ulp, no correctly
+// rounded expansion): sin = sqrt((1 - c)(1 + c)) with c in FP64, so the
+// cancellation near normal incidence stays exact and only the final FP32
+// root is approximate
+__device__ __forceinline__ float2 cos_sin_hw(double ix, double iy, double iz, float mx, float my,
+                                             float mz) {
+    double c = -(ix * mx + iy * my + iz * mz);
+    c = fmin(1.0, fmax(-1.0, c));
+    return make_float2((float)c, __builtin_amdgcn_sqrtf((float)((1.0 - c) * (1.0 + c))));
+}
+
 // (I.u, I.v) in FP64.  Needed for a general (non-orthonormal) facet frame,
 // where the azimuth atan2(I.u, I.v) is ill-conditioned near u x v and FP32
 // dot products lose it.
@@ -1380,7 +1391,7 @@ __device__ __forceinline__ Tri tri_eval(const RecV& rv, bool frame_ok, double sx
     const double invd = rd * fma(-0.5 * t2 * rd, rd, 1.5);
     const double ix = dx * invd, iy = dy * invd, iz = dz * invd;
     const float Ix = (float)ix, Iy = (float)iy, Iz = (float)iz;
-    const float2 csT = cos_sin_ref(ix, iy, iz, nx, ny, nz);
+    const float2 csT = cos_sin_hw(ix, iy, iz, nx, ny, nz);
     const float cT = csT.x, sT = csT.y;
     float Iu = Ix * ux + Iy * uy + Iz * uz;
     float Iv = Ix * vx + Iy * vy + Iz * vz;
@@ -1390,11 +1401,16 @@ __device__ __forceinline__ Tri tri_eval(const RecV& rv, bool frame_ok, double sx
         Iv = e.y;
     }
     // sin(TargetTh); tn = (0, 0, 1) is the common case
-    const float sTT = tn_up ? sqrtf(fmaxf(0.f, (float)((1.0 + iz) * (1.0 - iz))))
-                            : cos_sin_ref(ix, iy, iz, tnx, tny, tnz).y;
+    const float sTT = tn_up ? __builtin_amdgcn_sqrtf(fmaxf(0.f, (float)((1.0 + iz) * (1.0 - iz))))
+                            : cos_sin_hw(ix, iy, iz, tnx, tny, tnz).y;
+    // azimuth (cos Tph, sin Tph) = (Iv, Iu) / |(Iu, Iv)|; below 1e-30 the
+    // sinc arguments s sin(Tph), s cos(Tph) are < 1e-6 whatever the azimuth
+    // (s <= |(Iu, Iv)| for a unit I), so (1, 0) changes nothing and keeps
+    // the hardware rsq away from denormal inputs
     const float q2 = Iu * Iu + Iv * Iv;
-    const float rinv = (q2 > 0.f) ? rsqrtf(q2) : 0.f;
-    const float cph = (q2 > 0.f) ? Iv * rinv : 1.f;
+    const bool az = q2 > 1e-30f;
+    const float rinv = az ? __builtin_amdgcn_rsqf(q2) : 0.f;
+    const float cph = az ? Iv * rinv : 1.f;
     const float sph = Iu * rinv;
     const float a = kp.piLlam;
     float n1, d1, n2, d2, n3, d3, n4, d4;
@@ -1626,28 +1642,36 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
         __threadfence();
         __syncthreads();
 
-        // two facets per thread and iteration (two independent dependency
-        // chains), with the next iteration's two records loaded before this
-        // iteration's arithmetic so that their latency overlaps it
-        RecV na{}, nb{};
-        if (threadIdx.x < V) na = load_rec(rec, r0 + threadIdx.x);
-        if (threadIdx.x + NTF < V) nb = load_rec(rec, r0 + threadIdx.x + NTF);
-        for (int k = threadIdx.x; k < V; k += 2 * NTF) {
-            const RecV ca = na, cb = nb;
-            const bool two = k + NTF < V;
-            const int kn = k + 2 * NTF;
-            if (kn < V) na = load_rec(rec, r0 + kn);
-            if (kn + NTF < V) nb = load_rec(rec, r0 + kn + NTF);
-            const Tri ta = tri_eval<PH>(ca, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx, tny,
-                                        tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr, lossless,
-                                        prisms, np, kp);
-            Tri tb;
-            tb.valid = false;
-            if (two)
-                tb = tri_eval<PH>(cb, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx, tny, tnz,
-                                  tn_up, rst, inv_dr, n_d, two_cyc, nr, lossless, prisms, np, kp);
-            if (ta.valid) tri_add<ACC>(ta, base, len, nr, winA, rowA);
-            if (tb.valid) tri_add<ACC>(tb, base, len, nr, winA, rowA);
+        // two facets per thread and step (two independent dependency
+        // chains); the records of the next step are loaded before this
+        // step's arithmetic so that their latency overlaps it.  Two register
+        // sets (a0 b0, a1 b1) alternate, and every load reads a clamped,
+        // in-bounds index, so no set is copied or kept conditionally.
+        if (V > 0) {
+            const int vl = V - 1;
+            auto rec_at = [&](int k) { return load_rec(rec, r0 + min(k, vl)); };
+            auto step = [&](const RecV& ca, const RecV& cb, int k) {
+                const Tri ta = tri_eval<PH>(ca, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx,
+                                            tny, tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr,
+                                            lossless, prisms, np, kp);
+                Tri tb;
+                tb.valid = false;
+                if (k + NTF < V)
+                    tb = tri_eval<PH>(cb, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx, tny,
+                                      tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr, lossless,
+                                      prisms, np, kp);
+                if (ta.valid) tri_add<ACC>(ta, base, len, nr, winA, rowA);
+                if (tb.valid) tri_add<ACC>(tb, base, len, nr, winA, rowA);
+            };
+            RecV a0 = rec_at(threadIdx.x), b0 = rec_at(threadIdx.x + NTF);
+            for (int k = threadIdx.x; k < V; k += 4 * NTF) {
+                const RecV a1 = rec_at(k + 2 * NTF), b1 = rec_at(k + 3 * NTF);
+                step(a0, b0, k);
+                if (k + 2 * NTF >= V) break;
+                a0 = rec_at(k + 4 * NTF);
+                b0 = rec_at(k + 5 * NTF);
+                step(a1, b1, k + 2 * NTF);
+            }
         }
         __syncthreads();
         for (int k = threadIdx.x; k < len; k += NTF) {
