@@ -467,6 +467,19 @@ __device__ __forceinline__ double det_unfix(unsigned long long u, double inv_sca
     return (double)(long long)u * inv_scale;
 }
 
+// Record slot of the illuminated facet of rank i (facet order) among the V
+// of its shot.  The ranks are laid out as 64 rows of equal i mod 64, so
+// consecutive slots hold facets 64 ranks apart.  Neighbouring facets have
+// nearly equal ranges, and the target kernels, whose lanes read consecutive
+// slots, would otherwise add into the same few LDS bins in one instruction.
+// Every consumer sums over all of a shot's records, so the order is free.
+__device__ __forceinline__ int rec_slot(int i, int V) {
+    constexpr int G = 64;
+    const int q = V / G, rem = V - q * G;
+    const int a = i % G, c = i / G;
+    return (a < rem ? a * (q + 1) : rem * (q + 1) + (a - rem) * q) + c;
+}
+
 __global__ void __launch_bounds__(NT) k_surface(
     const float* __restrict__ mesh, int64_t Fpad, int64_t F, const erp_shot* __restrict__ shots,
     const unsigned long long* __restrict__ mask, const int* __restrict__ off,
@@ -500,7 +513,8 @@ __global__ void __launch_bounds__(NT) k_surface(
 
     const ShotK sh = load_shot(shots, s);
     const int V = Vs[s];
-    const int64_t rbase = rec_base[s] + off[(size_t)s * nblk + blk];
+    const int64_t rbase = rec_base[s];
+    const int rank0 = off[(size_t)s * nblk + blk];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const bool spec = kp.mode & ERP_MODE_SPECULAR;
     const bool lossless = kp.mode & ERP_MODE_LOSSLESS;
@@ -591,9 +605,7 @@ __global__ void __launch_bounds__(NT) k_surface(
         litmin = fmin(litmin, Itd); litmax = fmax(litmax, Itd);
 
         if (write_rec) {
-            const int64_t r = rbase + rank;
-#pragma unroll
-            for (int a = 0; a < 12; ++a) rec.geo[(size_t)a * rec.cap + r] = g[a];
+            const int64_t r = rbase + rec_slot(rank0 + rank, V);
             rec.itd[r] = Itd;
             // target amplitude factor: sqrt(P n Grefr G1 fRfrC) lam^2/(K Itd)
             const float fac = lossless ? 1.f : fRfrC;
@@ -602,17 +614,21 @@ __global__ void __launch_bounds__(NT) k_surface(
                                  ? fRe
                                  : sqrtf(kp.P * kp.nf * Grefr * G1 * fac) * kp.lam * kp.lam *
                                        kp.invK / Itdf;
-            rec.F[r] = Fr;
+            if (rec.geo) {
+#pragma unroll
+                for (int a = 0; a < 12; ++a) rec.geo[(size_t)a * rec.cap + r] = g[a];
+                rec.F[r] = Fr;
+                rec.fRfrC[r] = fRfrC;
+                rec.ang[r] = Ith;
+                rec.ang[rec.cap + r] = Iph;
+                rec.ang[2 * rec.cap + r] = Rth;
+            }
             if (rec.qa) {
                 rec.qa[r] = make_float4(g[0], g[1], g[2], Fr);
                 rec.qb[r] = make_float4(g[3], g[4], g[5], g[6]);
                 rec.qc[r] = make_float4(g[7], g[8], g[9], g[10]);
                 rec.qd[r] = g[11];
             }
-            rec.fRfrC[r] = fRfrC;
-            rec.ang[r] = Ith;
-            rec.ang[rec.cap + r] = Iph;
-            rec.ang[2 * rec.cap + r] = Rth;
         }
         if (!do_trace) continue;
         int b;
@@ -2309,14 +2325,20 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
             char* b = W.rec.as<char>();
             auto take = [&](size_t n) { char* r = b; b += (n + 255) & ~(size_t)255; return r; };
             rec.itd = (double*)take(cap * 8);
-            rec.geo = (float*)take(cap * 12 * 4);
-            rec.F = (float*)take(cap * 4);
-            rec.fRfrC = (float*)take(cap * 4);
-            rec.ang = (float*)take(cap * 3 * 4);
-            rec.qa = (float4*)take(cap * 16);
-            rec.qb = (float4*)take(cap * 16);
-            rec.qc = (float4*)take(cap * 16);
-            rec.qd = (float*)take(cap * 4);
+            // the per-field arrays feed the specular and type-0 kernels, the
+            // packed records the plain target kernel: write only what runs
+            if (spec || any_t0) {
+                rec.geo = (float*)take(cap * 12 * 4);
+                rec.F = (float*)take(cap * 4);
+                rec.fRfrC = (float*)take(cap * 4);
+                rec.ang = (float*)take(cap * 3 * 4);
+            }
+            if (!spec && any_typed) {
+                rec.qa = (float4*)take(cap * 16);
+                rec.qb = (float4*)take(cap * 16);
+                rec.qc = (float4*)take(cap * 16);
+                rec.qd = (float*)take(cap * 4);
+            }
             rec.cap = (int64_t)cap;
         }
 
