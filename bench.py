@@ -226,16 +226,27 @@ def counters_summary(dirs, bench_json, out_path):
     and LDS bank-conflict cycles per active LDS cycle."""
     with open(bench_json) as f:
         b = json.loads([l for l in f if l.startswith("{")][-1])
-    agg, disp = {}, {}
+    agg, disp, names = {}, {}, {}
     for d in dirs:
         for r in _counter_rows(d):
+            names.setdefault(d, set()).add(r["Counter_Name"])
             for stage, kern in KERNELS.items():
                 if kern in r["Kernel_Name"]:
                     c = agg.setdefault(kern, {})
                     c[r["Counter_Name"]] = c.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
                     disp.setdefault((kern, r["Counter_Name"]), set()).add(r["Dispatch_Id"])
     lines = [f"# Hardware counters of the hot kernels, workload {b['config']['workload']} "
-             f"(one MI355X)", ""]
+             f"(one MI355X)", "", "Commands (one counter pass each):"]
+    regex = "|".join(KERNELS.values())
+    for d in dirs:
+        lines.append(f"  cd /tmp && export TMPDIR=/tmp && timeout -s KILL 120 rocprofv3 --pmc "
+                     f"{' '.join(sorted(names.get(d, ())))} --kernel-include-regex '{regex}' "
+                     f"-f csv -d <out>/{os.path.basename(os.path.normpath(d))} -o run -- python3 "
+                     f"$GRAFT_REPO_ROOT/bench.py --steps {b['steps']} --warmup {b['warmup']} "
+                     f"--shots-per-step {b['config']['shots_per_step']} --no-cpu --no-serve")
+    lines.append(f"  python3 bench.py --counters-summary {' '.join('<out>/' + os.path.basename(os.path.normpath(d)) for d in dirs)} "
+                 f"<out>/{os.path.basename(bench_json)} profiles/{os.path.basename(out_path)}")
+    lines.append("")
     for kern, c in agg.items():
         stage = [s for s, k in KERNELS.items() if k == kern][0]
         units = b["stage_units"][stage] / max(b["stage_launches"][stage], 1)
@@ -252,9 +263,13 @@ def counters_summary(dirs, bench_json, out_path):
         if "SQ_LDS_BANK_CONFLICT" in avg and avg.get("SQ_ACTIVE_INST_LDS"):
             lines.append(f"  LDS bank-conflict cycles / active LDS cycles "
                          f"{avg['SQ_LDS_BANK_CONFLICT'] / avg['SQ_ACTIVE_INST_LDS']:.3f}")
-        if avg.get("SQ_BUSY_CYCLES") and "SQ_ACTIVE_INST_VALU" in avg:
-            lines.append(f"  VALU-active / SQ-busy cycles         "
-                         f"{avg['SQ_ACTIVE_INST_VALU'] / avg['SQ_BUSY_CYCLES']:10.4g}")
+        if avg.get("GRBM_GUI_ACTIVE") and "SQ_INSTS_VALU" in avg:
+            # GRBM_GUI_ACTIVE sums the busy cycles of the 8 XCDs; a wave64
+            # VALU instruction holds one 16-lane SIMD for 4 cycles; 1024 SIMDs
+            cyc = avg["GRBM_GUI_ACTIVE"] / 8.0
+            lines.append(f"  kernel cycles (GRBM_GUI_ACTIVE / 8)  {cyc:10.4g}")
+            lines.append(f"  VALU issue fraction, 4 x SQ_INSTS_VALU / (1024 SIMDs x cycles) "
+                         f"{4.0 * avg['SQ_INSTS_VALU'] / (1024.0 * cyc):.3f}")
         lines.append("")
     with open(out_path, "w") as f:
         f.write("\n".join(lines) + "\n")
