@@ -2223,6 +2223,18 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         throw ErpError{ERP_E_UNSUPPORTED,
                        "targets of type != 0 need convolution and convolution_linear "
                        "(sim.cu:410,872,912 otherwise use unallocated buffers)"};
+    if (F == 0 || ns == 0) {
+        // no facets: no surface echo and no refracted path, so every trace is
+        // zero and no facet is illuminated
+        zero(out_dev, (size_t)ns * nr, st);
+        if (nvalid_dev && ns > 0) {
+            const std::vector<int32_t> z((size_t)ns, 0);
+            ERP_HIP_TRY(hipMemcpyAsync(nvalid_dev, z.data(), sizeof(int32_t) * ns,
+                                       hipMemcpyHostToDevice, st));
+            ERP_HIP_TRY(hipStreamSynchronize(st));  // z is a host temporary
+        }
+        return;
+    }
 
     const int P = next_pow2(2 * (int64_t)nr - 1);
     const FftPlan planP = fft_plan(P);
