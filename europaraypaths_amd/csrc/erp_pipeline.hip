@@ -248,6 +248,23 @@ __device__ __forceinline__ void sinc_nd_hw(float x, float* num, float* den) {
     *den = tiny ? 1.0f : 3.14159265358979f * x;
 }
 
+// pi sinc(x) as numerator / denominator: sin(pi x) / x, and 1 / (1/pi)
+// where sinc rounds to 1.  A product of four needs one reciprocal and one
+// factor 1/pi^4, instead of four multiplications by pi.
+__device__ __forceinline__ void pisinc_nd_hw(float x, float* num, float* den) {
+    const bool tiny = fabsf(x) < 1e-6f;
+    *num = tiny ? 1.0f : sinpi_hw(x);
+    *den = tiny ? 0.318309886183791f : x;
+}
+
+// exp(-i pi t) for t = 2 r c given r and c (FP64): the half-cycle count
+// h = t / 2 = r c is formed directly, which is the same double as 0.5 (r 2c)
+__device__ __forceinline__ float2 phasor_rc_hw(double r, double c) {
+    const double h = r * c;
+    const float f = (float)(h - rint(h));
+    return make_float2(__builtin_amdgcn_cosf(f), -__builtin_amdgcn_sinf(f));
+}
+
 // exp(-i pi t) for FP64 t: exact reduction to revolutions in double, then
 // the hardware sine and cosine
 __device__ __forceinline__ float2 phasor_half_cycles_hw(double t) {
@@ -1432,12 +1449,14 @@ __device__ __forceinline__ Tri tri_eval(const RecV& rv, bool frame_ok, double sx
     float n1, d1, n2, d2, n3, d3, n4, d4;
     // sin(Tth) (cos Tph, sin Tph)
     const float scx = sT * cph, scy = sT * sph;
-    sinc_nd_hw(a * scx, &n1, &d1);
-    sinc_nd_hw(a * scy, &n2, &d2);
-    sinc_nd_hw(a * sTT * cph, &n3, &d3);
-    sinc_nd_hw(a * sTT * sph, &n4, &d4);
+    pisinc_nd_hw(a * scx, &n1, &d1);
+    pisinc_nd_hw(a * scy, &n2, &d2);
+    pisinc_nd_hw(a * sTT * cph, &n3, &d3);
+    pisinc_nd_hw(a * sTT * sph, &n4, &d4);
     const float den = fabsf((d1 * d2) * (d3 * d4));
-    float amp = qa.w * kp.Dmax * fabsf((n1 * n2) * (n3 * n4)) * __builtin_amdgcn_rcpf(den) * rs;
+    // Dmax / pi^4 takes the four factors pi of the pi-sincs back out
+    const float dmax_pi4 = kp.Dmax * 0.0102659823f;
+    float amp = qa.w * dmax_pi4 * fabsf((n1 * n2) * (n3 * n4)) * __builtin_amdgcn_rcpf(den) * rs;
     if (!lossless) {
         float expo = 0.f, tot = 0.f;
         for (int k = 0; k < np; ++k) {
@@ -1449,7 +1468,7 @@ __device__ __forceinline__ Tri tri_eval(const RecV& rv, bool frame_ok, double sx
         const float kc = kp.ks * cT;
         amp *= __expf(-2.f * (expo + kc * kc));
     }
-    const float2 ph = (PH == PH_F64) ? phasor_half_cycles_hw(rngt * two_cyc) : ph_st;
+    const float2 ph = (PH == PH_F64) ? phasor_rc_hw(rngt, kp.inv_cyc) : ph_st;
     o.c = make_float2(ph.x * amp, ph.y * amp);
     return o;
 }
@@ -1515,9 +1534,30 @@ __device__ __forceinline__ float2 win_get(WinS w, int idx) {
 template <int ACC = ACC_F64>
 __device__ __forceinline__ void tri_add(const Tri& t, int base, int len, int nr,
                                         WinS winA, float2* rowA) {
+    const int k0 = t.b - base;
+    if constexpr (ACC == ACC_F64) {
+        // split in FP64, the window's precision: c1 = c w, c0 = c - c1
+        const double cx = t.c.x, cy = t.c.y, w = t.w;
+        const double c1x = cx * w, c1y = cy * w;
+        const double c0x = cx - c1x, c0y = cy - c1y;
+        if (k0 >= 0 && k0 < len) {
+            atomicAdd(&winA.x[k0], c0x);
+            atomicAdd(&winA.y[k0], c0y);
+        } else {
+            glb_add2(rowA, t.b, make_float2((float)c0x, (float)c0y));
+        }
+        if (t.b + 1 < nr) {
+            if (k0 + 1 >= 0 && k0 + 1 < len) {
+                atomicAdd(&winA.x[k0 + 1], c1x);
+                atomicAdd(&winA.y[k0 + 1], c1y);
+            } else {
+                glb_add2(rowA, t.b + 1, make_float2((float)c1x, (float)c1y));
+            }
+        }
+        return;
+    }
     const float2 c0 = make_float2(t.c.x * (1.f - t.w), t.c.y * (1.f - t.w));
     const float2 c1 = make_float2(t.c.x * t.w, t.c.y * t.w);
-    const int k0 = t.b - base;
     if (k0 >= 0 && k0 < len) win_add<ACC>(winA, k0, c0); else glb_add2(rowA, t.b, c0);
     if (t.b + 1 < nr) {
         if (k0 + 1 >= 0 && k0 + 1 < len) win_add<ACC>(winA, k0 + 1, c1);
