@@ -49,6 +49,11 @@ constexpr int MASKW = FB / 64; // mask words per block (32)
 constexpr int SWIN = 2048;     // surface LDS window (bins)
 constexpr int TWIN = 4096;     // target LDS window (bins)
 constexpr int NTT = 512;       // threads per workgroup of the target kernels
+// (shot, target) pairs per target launch: their P-point rows take at most
+// kPairRowBytes per buffer set.  Large launches keep the last round of
+// workgroups a small part of each launch and need fewer host round trips.
+constexpr int64_t kPairRowBytes = (int64_t)6144 << 20;
+constexpr int kMaxPairsPerLaunch = 16384;
 constexpr int TWF = 4096;      // bin window of k_target_accum_fast (FP64, 64 KB of LDS)
 constexpr int NTF = 512;       // threads of k_target_accum_fast (two workgroups per CU)
 // Bin windows accumulate in FP64: LDS atomic adds of doubles (ds_add_f64) run
@@ -2203,8 +2208,8 @@ int64_t erp_footprint_impl(const erp_params* par, int64_t F, int64_t T, int64_t 
     b += S * nr * (16 + 8 + 16) + S * 4;      // FP64 surface bins, reflected, refracted
     if (T > 0) {
         const int64_t nbuf = spec ? 4 : 3;
-        int64_t C = std::max<int64_t>(1, ((int64_t)1536 << 20) / (P * 8 * nbuf));
-        C = std::min<int64_t>(C, 8192);
+        int64_t C = std::max<int64_t>(1, kPairRowBytes / (P * 8 * nbuf));
+        C = std::min<int64_t>(C, kMaxPairsPerLaunch);
         const int64_t gcap = C / 16 + S + 2;
         b += S * T * 8 + S * 8;               // target lists and counts
         b += P * 8 * (C * nbuf + gcap);       // per-target chunk rows and partial sums
@@ -2483,8 +2488,8 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                 // pairs per multiply-accumulate group (fused) or per summed slice (half)
                 const int GS = half ? target_half_slice() : 16;
                 const int nbuf = half ? 2 : (fused ? 3 : (spec ? 4 : 3));
-                int C = (int)std::max<int64_t>(1, ((int64_t)1536 << 20) / ((int64_t)P * 8 * nbuf));
-                C = std::min<int>(C, 8192);
+                int C = (int)std::max<int64_t>(1, kPairRowBytes / ((int64_t)P * 8 * nbuf));
+                C = std::min<int>(C, kMaxPairsPerLaunch);
                 const size_t gcap = (size_t)C / GS + S + 2;
                 // half: A rows, then one partial row per slice
                 W.tA.ensure(sizeof(float2) * (size_t)P *
