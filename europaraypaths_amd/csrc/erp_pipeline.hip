@@ -2500,9 +2500,14 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                 float2* Bs = spec ? Bq + (size_t)C * P : nullptr;
                 float2* Z = W.tC.as<float2>();                  // Z or fused workspace
                 std::vector<int> pshot, ptgt;
+                // each shot's list holds Tcap slots of which htc[2s] are used:
+                // copy only the leading columns that some shot uses
                 std::vector<int> hl1((size_t)S * Tcap);
-                ERP_HIP_TRY(hipMemcpyAsync(hl1.data(), list1, sizeof(int) * (size_t)S * Tcap,
-                                           hipMemcpyDeviceToHost, st));
+                int maxc = 0;
+                for (int s = 0; s < S; ++s) maxc = std::max(maxc, htc[2 * s]);
+                ERP_HIP_TRY(hipMemcpy2DAsync(hl1.data(), sizeof(int) * (size_t)Tcap, list1,
+                                             sizeof(int) * (size_t)Tcap, sizeof(int) * (size_t)maxc,
+                                             (size_t)S, hipMemcpyDeviceToHost, st));
                 ERP_HIP_TRY(hipStreamSynchronize(st));
                 W.pairs.ensure(sizeof(int) * ((size_t)C * 4 + 2 * gcap + 3 * (size_t)S + 16));
                 int* dps = W.pairs.as<int>();
