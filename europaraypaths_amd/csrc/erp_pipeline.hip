@@ -327,6 +327,15 @@ __global__ void k_zero(float2* __restrict__ p, size_t n) {
     for (; i < n; i += st) p[i] = make_float2(0.f, 0.f);
 }
 
+// pair i of a chunk: the host uploads its shot and its index in that
+// shot's target list; the target id is read here from the list the select
+// kernel wrote, so the lists never travel to the host
+__global__ void k_pair_targets(int* __restrict__ ptgt, const int* __restrict__ pshot,
+                               const int* __restrict__ list1, int Tcap, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ptgt[i] = list1[(size_t)pshot[i] * Tcap + ptgt[i]];
+}
+
 __global__ void k_init_bounds(Bounds* b, int S) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < S) {
@@ -2500,15 +2509,10 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                 float2* Bs = spec ? Bq + (size_t)C * P : nullptr;
                 float2* Z = W.tC.as<float2>();                  // Z or fused workspace
                 std::vector<int> pshot, ptgt;
+                pshot.reserve((size_t)C);
+                ptgt.reserve((size_t)C);
                 // each shot's list holds Tcap slots of which htc[2s] are used:
                 // copy only the leading columns that some shot uses
-                std::vector<int> hl1((size_t)S * Tcap);
-                int maxc = 0;
-                for (int s = 0; s < S; ++s) maxc = std::max(maxc, htc[2 * s]);
-                ERP_HIP_TRY(hipMemcpy2DAsync(hl1.data(), sizeof(int) * (size_t)Tcap, list1,
-                                             sizeof(int) * (size_t)Tcap, sizeof(int) * (size_t)maxc,
-                                             (size_t)S, hipMemcpyDeviceToHost, st));
-                ERP_HIP_TRY(hipStreamSynchronize(st));
                 W.pairs.ensure(sizeof(int) * ((size_t)C * 4 + 2 * gcap + 3 * (size_t)S + 16));
                 int* dps = W.pairs.as<int>();
                 int* dpt = dps + C;
@@ -2525,7 +2529,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                     while (cs < S && (int)pshot.size() < C) {
                         if (cj >= htc[2 * cs]) { ++cs; cj = 0; continue; }
                         pshot.push_back(cs);
-                        ptgt.push_back(hl1[(size_t)cs * Tcap + cj]);
+                        ptgt.push_back(cj);  // list index; k_pair_targets resolves it
                         ++cj;
                     }
                     const int n = (int)pshot.size();
@@ -2566,6 +2570,9 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                     std::copy(sege.begin(), sege.end(), up.begin() + so + 2 * S);
                     ERP_HIP_TRY(hipMemcpyAsync(dps, up.data(), sizeof(int) * up.size(),
                                                hipMemcpyHostToDevice, st));
+                    hipLaunchKernelGGL(k_pair_targets, dim3((n + 255) / 256), dim3(256), 0, st,
+                                       dpt, dps, list1, Tcap, n);
+                    ERP_CHECK_LAUNCH("k_pair_targets");
                     double triples = 0;
                     for (int i = 0; i < n; ++i) triples += hV[pshot[i]];
                     PT_.begin(ERP_STAGE_TARGET_ACCUM, st, triples);
