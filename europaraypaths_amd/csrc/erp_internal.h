@@ -69,6 +69,12 @@ struct DevBuf {
         }
         bytes = n;
     }
+    // for sizes that vary from call to call: grow by a quarter more than
+    // asked, so that a slowly rising size does not free and reallocate
+    // (hipFree waits for the device) on every call
+    void ensure_grow(size_t n) {
+        if (n > bytes) ensure(n + n / 4);
+    }
     void release() {
         if (p) hipFree(p);
         p = nullptr;

@@ -2387,7 +2387,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         if (need_rec && Vtot > 0) {
             const size_t cap = (size_t)Vtot;
             const size_t bytes = cap * (12 * 4 + 8 + 4 + 4 + 3 * 4 + 52) + 256 * 10;
-            W.rec.ensure(bytes);
+            W.rec.ensure_grow(bytes);
             char* b = W.rec.as<char>();
             auto take = [&](size_t n) { char* r = b; b += (n + 255) & ~(size_t)255; return r; };
             rec.itd = (double*)take(cap * 8);
@@ -2688,7 +2688,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                             pt.push_back(hl0[(size_t)s * Tcap + j]);
                         }
                     const int n = (int)ps.size();
-                    W.pairs.ensure(sizeof(int) * 2 * (size_t)n + 64);
+                    W.pairs.ensure_grow(sizeof(int) * 2 * (size_t)n + 64);
                     int* dps = W.pairs.as<int>();
                     std::vector<int> up(ps);
                     up.insert(up.end(), pt.begin(), pt.end());
