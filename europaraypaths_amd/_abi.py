@@ -24,6 +24,9 @@ ERP_RUN_RAW_SURFACE = 0x08
 ERP_RUN_ACC_F32 = 0x10
 ERP_RUN_ACC_COMP = 0x20
 
+ERP_TUNE_SHOTS_PER_BATCH = 1
+ERP_TUNE_PAIRS_PER_LAUNCH = 2
+
 _PARAM_FLOATS = ("lam", "k", "rng_res", "P", "fs", "ks", "eps_1", "eps_2", "c",
                  "c_1", "c_2", "nu_1", "nu_2", "alpha1", "alpha2", "aperture",
                  "smpl", "B", "f0", "dr")

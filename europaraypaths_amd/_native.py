@@ -23,7 +23,8 @@ EXPORTS = ("erp_ctx_create", "erp_ctx_destroy", "erp_last_error", "erp_mesh_uplo
            "erp_mesh_release", "erp_targets_upload", "erp_targets_release", "erp_run",
            "erp_run_dev", "erp_submit", "erp_poll", "erp_wait", "erp_run_footprint",
            "erp_profile_enable", "erp_profile_read", "erp_serve_slots",
-           "erp_mesh_from_heights", "erp_mesh_download", "erp_focus")
+           "erp_mesh_from_heights", "erp_mesh_download", "erp_focus", "erp_set_tuning",
+           "erp_get_tuning")
 
 STAGES = ("mask", "surface", "surface_conv", "target_select", "target_accum", "target_fft",
           "shot_final", "other", "target_sum")
@@ -88,6 +89,10 @@ def lib() -> ctypes.CDLL:
         L.erp_profile_enable.restype = i32
         L.erp_profile_read.argtypes = [vp, i32, P(ctypes.c_double), P(i64), P(ctypes.c_double)]
         L.erp_profile_read.restype = i32
+        L.erp_set_tuning.argtypes = [vp, i32, i64]
+        L.erp_set_tuning.restype = i32
+        L.erp_get_tuning.argtypes = [vp, i32, P(i64)]
+        L.erp_get_tuning.restype = i32
         _lib = L
         return L
 
@@ -278,6 +283,15 @@ class Context:
                                     _ptr(nsx), _ptr(offs), _ptr(rows), _ptr(filt), rows.size,
                                     _ptr(out)))
         return out
+
+    def set_tuning(self, key: int, value: int):
+        """erp_set_tuning: cap a batch shape (ERP_TUNE_*) on this context; 0 = default."""
+        self._check(lib().erp_set_tuning(self.handle, key, value))
+
+    def get_tuning(self, key: int) -> int:
+        v = ctypes.c_int64()
+        self._check(lib().erp_get_tuning(self.handle, key, ctypes.byref(v)))
+        return v.value
 
     def serve_slots(self, n: int):
         """Start worker slots for submit() up to n (each: thread, stream, buffers)."""

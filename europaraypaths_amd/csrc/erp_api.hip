@@ -573,6 +573,24 @@ int erp_wait(erp_job* j) {
     return job_finish(j);
 }
 
+int erp_set_tuning(erp_ctx* ctx, int key, int64_t value) {
+    if (!ctx || value < 0) return ERP_E_INVALID;
+    switch (key) {
+        case ERP_TUNE_SHOTS_PER_BATCH: ctx->tune_shots = value; return ERP_OK;
+        case ERP_TUNE_PAIRS_PER_LAUNCH: ctx->tune_pairs = value; return ERP_OK;
+        default: return ERP_E_INVALID;
+    }
+}
+
+int erp_get_tuning(erp_ctx* ctx, int key, int64_t* value) {
+    if (!ctx || !value) return ERP_E_INVALID;
+    switch (key) {
+        case ERP_TUNE_SHOTS_PER_BATCH: *value = ctx->tune_shots; return ERP_OK;
+        case ERP_TUNE_PAIRS_PER_LAUNCH: *value = ctx->tune_pairs; return ERP_OK;
+        default: return ERP_E_INVALID;
+    }
+}
+
 int erp_profile_enable(erp_ctx* ctx, int on) {
     if (!ctx) return ERP_E_INVALID;
     std::lock_guard<std::mutex> lk(ctx->mu);

@@ -199,6 +199,8 @@ struct erp_ctx {
     std::condition_variable qcv;
     bool stopping = false;
     std::atomic<int> active{0};  // jobs running in slots
+    // caps on the batch shapes (erp_set_tuning; 0 = default)
+    std::atomic<int64_t> tune_shots{0}, tune_pairs{0};
 };
 
 // ---------------------------------------------------------------------------

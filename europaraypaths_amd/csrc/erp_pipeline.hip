@@ -2318,6 +2318,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
 
     // batch size: bounded by memory of per-shot buffers
     int Smax = 32;
+    if (ctx->tune_shots > 0) Smax = (int)std::min<int64_t>(Smax, ctx->tune_shots);
     if ((int64_t)P * 8 * 4 * Smax > (int64_t)1 << 30) Smax = std::max<int>(1, (int)(((int64_t)1 << 30) / ((int64_t)P * 32)));
     const int64_t Tpad = targets ? targets->Tpad : 0;
 
@@ -2499,6 +2500,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                 const int nbuf = half ? 2 : (fused ? 3 : (spec ? 4 : 3));
                 int C = (int)std::max<int64_t>(1, kPairRowBytes / ((int64_t)P * 8 * nbuf));
                 C = std::min<int>(C, kMaxPairsPerLaunch);
+                if (ctx->tune_pairs > 0) C = (int)std::min<int64_t>(C, ctx->tune_pairs);
                 const size_t gcap = (size_t)C / GS + S + 2;
                 // half: A rows, then one partial row per slice
                 W.tA.ensure(sizeof(float2) * (size_t)P *

@@ -230,6 +230,21 @@ int erp_profile_enable(erp_ctx* ctx, int on);
  * launches and units for one stage. */
 int erp_profile_read(erp_ctx* ctx, int stage, double* ms, int64_t* launches, double* units);
 
+/* ---- batch shapes ------------------------------------------------------------
+ * Caps on the batch shapes erp_run / erp_run_dev / erp_submit use on this
+ * context; 0 restores the default.  They can only lower the defaults, so
+ * erp_run_footprint stays an upper bound.  No counterpart in the reference
+ * (its sim runs every shot in its own loop iteration, sim.cu:615-1008).
+ *   ERP_TUNE_SHOTS_PER_BATCH : shots per batch (default and maximum 32)
+ *   ERP_TUNE_PAIRS_PER_LAUNCH: (shot, target) pairs per target launch
+ *                              (default: the pairs whose rows fit 6 GB,
+ *                              at most 16384) */
+#define ERP_TUNE_SHOTS_PER_BATCH  1
+#define ERP_TUNE_PAIRS_PER_LAUNCH 2
+int erp_set_tuning(erp_ctx* ctx, int key, int64_t value);
+/* The value set for key (0 = default). */
+int erp_get_tuning(erp_ctx* ctx, int key, int64_t* value);
+
 /* ---- SAR focusing (simple_focusing.py:310-404) ------------------------------
  * Focus range bins row0 .. row0 + nrows - 1 of an Nr x Na complex radargram
  * (rdr: Nr * Na * 2 floats, row-major, as compile_rdrgrm lays it out).  For
