@@ -255,3 +255,20 @@ def test_mesh_from_heights_matches_host_soa(gpu_ctx, shape):
     finally:
         m.release()
     np.testing.assert_array_equal(dev, host)
+
+
+def test_targets_nearly_aligned_with_facet_rows(gpu_ctx):
+    """Targets 0.2 mm off a facet row and column: the azimuth of the facets
+    in that row and column is within 1e-5 of an axis, so two of the four
+    sinc arguments are about 1e-5, where sin(pi x) has to be relative-
+    accurate (an absolute error of 1e-7 would be 1 % of it)."""
+    par, soa, shots = scene(nx=48, ny=40, fs=20.0, rough=0.0, nr=1024, h=3000.0, rst=2600.0)
+    xyzn = np.zeros((6, 3), np.float32)
+    xyzn[0] = np.float32(20.0) + np.float32(2e-4) * np.array([1, -1, 1], np.float32)
+    xyzn[1] = np.float32(-40.0) + np.float32(2e-4) * np.array([1, 1, -1], np.float32)
+    xyzn[2] = -300.0
+    xyzn[5] = 1.0
+    tg = (xyzn, np.ones(3, np.uint8))
+    g, gv, o, ov = run_both(gpu_ctx, par, soa, shots, targets=tg)
+    np.testing.assert_array_equal(gv, ov)
+    print(f"worst errors {compare_traces(g, o)}", flush=True)
