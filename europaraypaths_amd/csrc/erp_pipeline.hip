@@ -1679,21 +1679,38 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
                 winS[TWF + k] = 0.0;
             }
             __syncthreads();
-            for (int k = threadIdx.x; k < V; k += NTF) {
-                const Tri ta = tri_eval(load_rec(rec, r0 + k), frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz,
-                                        tnx, tny, tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr,
-                                        lossless, prisms, np, kp);
-                if (!ta.valid) continue;
+            const bool first = wb == base, last = wb + TWF >= top;
+            auto add_det = [&](const Tri& ta) {
                 const float2 c0 = make_float2(ta.c.x * (1.f - ta.w), ta.c.y * (1.f - ta.w));
                 const float2 c1 = make_float2(ta.c.x * ta.w, ta.c.y * ta.w);
                 const int k0 = ta.b - wb;
-                const bool first = wb == base, last = wb + TWF >= top;
                 if (k0 >= 0 && k0 < wl) lds_add2_fix(wux, wuy, k0, c0, scale);
                 else if ((ta.b < base && first) || (ta.b >= top && last)) glb_add2(rowA, ta.b, c0);
                 if (ta.b + 1 < nr) {
                     if (k0 + 1 >= 0 && k0 + 1 < wl) lds_add2_fix(wux, wuy, k0 + 1, c1, scale);
                     else if ((ta.b + 1 < base && first) || (ta.b + 1 >= top && last))
                         glb_add2(rowA, ta.b + 1, c1);
+                }
+            };
+            // one facet per thread and step, the next step's record loaded
+            // ahead into the other register set (two chains, as the plain
+            // path runs, would cost this variant a wave per SIMD)
+            if (V > 0) {
+                const int vl = V - 1;
+                auto rec_at = [&](int k) { return load_rec(rec, r0 + min(k, vl)); };
+                auto step = [&](const RecV& ca) {
+                    const Tri ta = tri_eval(ca, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx,
+                                            tny, tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr,
+                                            lossless, prisms, np, kp);
+                    if (ta.valid) add_det(ta);
+                };
+                RecV a0 = rec_at(threadIdx.x);
+                for (int k = threadIdx.x; k < V; k += 2 * NTF) {
+                    const RecV a1 = rec_at(k + NTF);
+                    step(a0);
+                    if (k + NTF >= V) break;
+                    a0 = rec_at(k + 2 * NTF);
+                    step(a1);
                 }
             }
             __syncthreads();
