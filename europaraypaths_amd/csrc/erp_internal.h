@@ -69,11 +69,12 @@ struct DevBuf {
         }
         bytes = n;
     }
-    // for sizes that vary from call to call: grow by a quarter more than
-    // asked, so that a slowly rising size does not free and reallocate
-    // (hipFree waits for the device) on every call
-    void ensure_grow(size_t n) {
-        if (n > bytes) ensure(n + n / 4);
+    // for sizes that vary from call to call: grow by up to a quarter more
+    // than asked (never beyond cap, the caller's bound on n), so that a
+    // slowly rising size does not free and reallocate (hipFree waits for the
+    // device) on every call
+    void ensure_grow(size_t n, size_t cap) {
+        if (n > bytes) ensure(std::max(n, std::min(n + n / 4, cap)));
     }
     void release() {
         if (p) hipFree(p);

@@ -2229,7 +2229,7 @@ int64_t erp_footprint_impl(const erp_params* par, int64_t F, int64_t T, int64_t 
     const bool spec = par->mode & ERP_MODE_SPECULAR;
     int64_t b = 0;
     b += S * nblk * (MASKW * 8 + 8);          // mask + counts + offsets
-    b += S * F * 128;                         // facet records (128 B per facet-shot)
+    b += S * F * 136 + 2560;                  // facet records (136 B per facet-shot)
     b += S * P * 8 * 3 + S * M * 8;           // surface, spectral sum, type-0 rows; Bluestein
     b += S * nr * (16 + 8 + 16) + S * 4;      // FP64 surface bins, reflected, refracted
     if (T > 0) {
@@ -2404,8 +2404,11 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         Rec rec{};
         if (need_rec && Vtot > 0) {
             const size_t cap = (size_t)Vtot;
-            const size_t bytes = cap * (12 * 4 + 8 + 4 + 4 + 3 * 4 + 52) + 256 * 10;
-            W.rec.ensure_grow(bytes);
+            // at most every facet of every shot of the batch (the bound that
+            // erp_run_footprint counts)
+            const size_t rec_unit = 12 * 4 + 8 + 4 + 4 + 3 * 4 + 52;
+            const size_t bytes = cap * rec_unit + 256 * 10;
+            W.rec.ensure_grow(bytes, (size_t)S * (size_t)F * rec_unit + 256 * 10);
             char* b = W.rec.as<char>();
             auto take = [&](size_t n) { char* r = b; b += (n + 255) & ~(size_t)255; return r; };
             rec.itd = (double*)take(cap * 8);
@@ -2707,7 +2710,8 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                             pt.push_back(hl0[(size_t)s * Tcap + j]);
                         }
                     const int n = (int)ps.size();
-                    W.pairs.ensure_grow(sizeof(int) * 2 * (size_t)n + 64);
+                    W.pairs.ensure_grow(sizeof(int) * 2 * (size_t)n + 64,
+                                        sizeof(int) * 2 * (size_t)S * Tcap + 64);
                     int* dps = W.pairs.as<int>();
                     std::vector<int> up(ps);
                     up.insert(up.end(), pt.begin(), pt.end());
