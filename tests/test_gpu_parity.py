@@ -1,6 +1,6 @@
 """GPU parity of the HIP path against the FP64 oracle (called through the C ABI).
 
-Tolerance (north star): per-bin phase within 1e-3 rad and amplitude within
+Tolerance: per-bin phase within 1e-3 rad and amplitude within
 1e-4 relative on bins within 40 dB of each trace's peak, plus an absolute
 bound of 1e-4 x peak on every bin; illuminated-facet counts and, through the
 raw surface trace, range-bin indices are compared exactly.
