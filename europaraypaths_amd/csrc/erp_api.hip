@@ -296,6 +296,17 @@ const char* erp_last_error(const erp_ctx* ctx) {
     return ctx->last_error.c_str();
 }
 
+// Row length of a mesh whose facets come in rows of equal y (the first
+// facet whose y differs from the first one's starts the second row), when
+// it suits the surface kernels' 64 x 32 tiles; else 0.  Only the order in
+// which the kernels visit facets depends on it, never a result.
+static int tile_width(const float* y, int64_t F) {
+    int64_t k = 1;
+    while (k < F && y[k] == y[0]) ++k;
+    if (k >= F || k % 64 != 0 || k > (int64_t)1 << 20 || F % (k * 32) != 0) return 0;
+    return (int)k;
+}
+
 int erp_mesh_upload(erp_ctx* ctx, const float* soa, int64_t F, erp_mesh** out) {
     if (!ctx || !out || (F > 0 && !soa) || F < 0) return ERP_E_INVALID;
     *out = nullptr;
@@ -330,6 +341,7 @@ int erp_mesh_upload(erp_ctx* ctx, const float* soa, int64_t F, erp_mesh** out) {
             ok = std::min(ep, em) <= 1e-5;
         }
         m->frame_ok = ok;
+        m->tile_nx = tile_width(soa + F, F);
     });
     if (rc != ERP_OK) {
         m->soa.release();
@@ -370,6 +382,7 @@ int erp_mesh_from_heights(erp_ctx* ctx, const double* heights, int64_t nx, int64
         z.release();
         // tangents and normal form an orthonormal frame by construction
         m->frame_ok = true;
+        m->tile_nx = (nx % 64 == 0 && ny % 32 == 0 && nx <= ((int64_t)1 << 20)) ? (int)nx : 0;
     });
     if (rc != ERP_OK) {
         m->soa.release();

@@ -93,6 +93,9 @@ struct erp_mesh {
     // every facet normal equals +-(u x v) to 1e-5: kernels may derive n from
     // the tangents instead of loading it
     bool frame_ok = false;
+    // facets in rows of tile_nx (a multiple of 64, row count a multiple of
+    // 32): the surface kernels then take them in 64 x 32 tiles; 0 = in order
+    int tile_nx = 0;
 };
 
 struct erp_targets {

@@ -348,10 +348,23 @@ __global__ void k_init_bounds(Bounds* b, int S) {
 // ---------------------------------------------------------------------------
 // 1. aperture mask + block counts
 // ---------------------------------------------------------------------------
+// Facet of slot j (j = q NT + threadIdx.x) of block blk.  Blocks are FB
+// consecutive facets, or, for a mesh laid out as rows of tile_nx facets
+// (tile_nx a multiple of 64 and the row count a multiple of 32), tiles of
+// 64 columns x 32 rows: a tile's facets lie close together, so its ranges
+// span few bins and the surface window catches them.  Either way a
+// bijection onto [0, F); results do not depend on it.
+__device__ __forceinline__ int64_t facet_of(int blk, int j, int tile_nx) {
+    if (tile_nx == 0) return (int64_t)blk * FB + j;
+    const int tpr = tile_nx / 64;
+    const int trow = blk / tpr, tcol = blk - trow * tpr;
+    return ((int64_t)trow * 32 + (j >> 6)) * tile_nx + (int64_t)tcol * 64 + (j & 63);
+}
+
 __global__ void __launch_bounds__(NT) k_illum_mask(
     const float* __restrict__ mesh, int64_t Fpad, int64_t F,
     const erp_shot* __restrict__ shots, double cos_ap,
-    unsigned long long* __restrict__ mask, int* __restrict__ cnt, int nblk) {
+    unsigned long long* __restrict__ mask, int* __restrict__ cnt, int nblk, int tile_nx) {
     const int blk = blockIdx.x, s = blockIdx.y;
     const ShotK sh = load_shot(shots, s);
     const float sxf = (float)sh.sx, syf = (float)sh.sy, szf = (float)sh.sz;
@@ -365,7 +378,7 @@ __global__ void __launch_bounds__(NT) k_illum_mask(
     const float srcmag = fabsf(sxf) + fabsf(syf) + fabsf(szf);
 #pragma unroll
     for (int q = 0; q < FPT; ++q) {
-        const int64_t i = (int64_t)blk * FB + q * NT + threadIdx.x;
+        const int64_t i = facet_of(blk, q * NT + threadIdx.x, tile_nx);
         bool p = false;
         if (i < F) {
             const float fx = X[i], fy = Y[i], fz = Z[i];
@@ -516,7 +529,7 @@ __global__ void __launch_bounds__(NT) k_surface(
     const unsigned long long* __restrict__ mask, const int* __restrict__ off,
     const int* __restrict__ Vs, const int64_t* __restrict__ rec_base, int nblk, KPar kp,
     Rec rec, int write_rec, int do_trace, double2* __restrict__ srow, int64_t shot0,
-    Bounds* __restrict__ bounds, int det, unsigned* __restrict__ smax) {
+    Bounds* __restrict__ bounds, int det, unsigned* __restrict__ smax, int tile_nx) {
     // det 0: FP64 bins (LDS window, FP64 atomics into srow)
     // det 1: first deterministic pass, largest |component| per shot into smax
     // det 2: second deterministic pass, fixed-point bins (srow holds int64)
@@ -568,7 +581,7 @@ __global__ void __launch_bounds__(NT) k_surface(
         const int w = q * WPB + wv;
         const unsigned long long m = m_s[w];
         if (!((m >> lane) & 1ull)) continue;
-        const int64_t i = (int64_t)blk * FB + q * NT + threadIdx.x;
+        const int64_t i = facet_of(blk, q * NT + threadIdx.x, tile_nx);
         const int rank = pre_s[w] + __popcll(m & ((1ull << lane) - 1ull));
         float g[12];
 #pragma unroll
@@ -2313,6 +2326,12 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     kp.ortho = mesh->frame_ok ? 1 : 0;
     const int L = 2 * nr - 1;
     const int nblk = (int)((F + FB - 1) / FB);
+    // facet order of the surface kernels: tiles where a wide aperture lights
+    // most of the mesh (each block's ranges then stay within its window);
+    // rows where a narrow one lights a small disk (rows give more, smaller
+    // lit blocks and a better balanced launch).  Measured on the 2048^2 fBm
+    // mesh: 60 deg, tiles 4.2x faster; 10 deg, rows 1.7x faster.
+    const int tile_nx = p.cos_ap < 0.94 ? mesh->tile_nx : 0;
     const bool need_rec = (T > 0) || (spec && !conv);
     const bool do_trace = !spec || (conv && lin);
     const float res_t = spec ? p.rng_res : 2.f * p.rng_res;  // sim.cu:853-858
@@ -2388,7 +2407,8 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         // 1-2: aperture mask, counts, offsets
         PT_.begin(ERP_STAGE_MASK, st, (double)F * S);
         hipLaunchKernelGGL(k_illum_mask, dim3(nblk, S), dim3(NT), 0, st, mesh->soa.as<float>(),
-                           mesh->Fpad, F, dshots, p.cos_ap, dmask, W.cnt.as<int>(), nblk);
+                           mesh->Fpad, F, dshots, p.cos_ap, dmask, W.cnt.as<int>(), nblk,
+                           tile_nx);
         ERP_CHECK_LAUNCH("k_illum_mask");
         hipLaunchKernelGGL(k_scan_blocks, dim3(S), dim3(1024), 0, st, W.cnt.as<int>(),
                            W.off.as<int>(), dV, nblk);
@@ -2440,14 +2460,15 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         const int write = (need_rec && Vtot > 0) ? 1 : 0;
         hipLaunchKernelGGL(k_surface, dim3(nblk, S), dim3(NT), 0, st, mesh->soa.as<float>(),
                            mesh->Fpad, F, dshots, dmask, W.off.as<int>(), dV, dbase, nblk, kp, rec,
-                           write, do_trace ? 1 : 0, srow, s0, dbounds, det ? 1 : 0, smax);
+                           write, do_trace ? 1 : 0, srow, s0, dbounds, det ? 1 : 0, smax,
+                           tile_nx);
         ERP_CHECK_LAUNCH("k_surface");
         if (det && do_trace) {
             // second pass: the same terms as fixed-point values scaled by the
             // per-shot maxima of the first
             hipLaunchKernelGGL(k_surface, dim3(nblk, S), dim3(NT), 0, st, mesh->soa.as<float>(),
                                mesh->Fpad, F, dshots, dmask, W.off.as<int>(), dV, dbase, nblk, kp,
-                               rec, 0, 1, srow, s0, (Bounds*)nullptr, 2, smax);
+                               rec, 0, 1, srow, s0, (Bounds*)nullptr, 2, smax, tile_nx);
             ERP_CHECK_LAUNCH("k_surface");
         }
         if (do_trace) {
