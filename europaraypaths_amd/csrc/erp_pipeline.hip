@@ -602,23 +602,17 @@ __global__ void __launch_bounds__(NT) k_surface(
             Iu = e.x;
             Iv = e.y;
         }
-        const float Ith = atan2f(sI, cI);
-        const float Iph = atan2f(Iu, Iv);
-        // compRefractedRays (snellsLaw)
+        // compRefractedRays (snellsLaw): past the critical angle the
+        // reference sets Rth = 1e4 and uses cos(1e4); Rth itself (and the
+        // angles Ith, Iph) are only stored for the kernels that read them
         const float sR = kp.sqrt_e1e2 * sI;
-        float Rth, cR;
-        if (sR > 1.0f) {
-            Rth = 1e4f;
-            cR = cosf(1e4f);
-        } else {
-            Rth = asinf(sR);
-            cR = sqrtf(fmaxf(0.f, 1.f - sR * sR));
-        }
+        const bool tir = sR > 1.0f;
+        const float cR = tir ? cosf(1e4f) : sqrtf(fmaxf(0.f, 1.f - sR * sR));
         // compReflectedEnergy (pol 0, H)
         const float rho = (kp.nu2 * cI - kp.nu1 * cR) / (kp.nu2 * cI + kp.nu1 * cR);
         const float Itdf = (float)Itd;
         const float atm = expf(-4.0f * kp.alpha1 * Itdf);
-        float fRfrC = (Rth > 1e3f) ? 0.f : (1.f - rho * rho);
+        float fRfrC = tir ? 0.f : (1.f - rho * rho);
         fRfrC *= atm;
         const float kc = kp.ks * cI;
         const float rough = expf(-4.f * (kc * kc));
@@ -663,9 +657,9 @@ __global__ void __launch_bounds__(NT) k_surface(
                 for (int a = 0; a < 12; ++a) rec.geo[(size_t)a * rec.cap + r] = g[a];
                 rec.F[r] = Fr;
                 rec.fRfrC[r] = fRfrC;
-                rec.ang[r] = Ith;
-                rec.ang[rec.cap + r] = Iph;
-                rec.ang[2 * rec.cap + r] = Rth;
+                rec.ang[r] = atan2f(sI, cI);                 // Ith
+                rec.ang[rec.cap + r] = atan2f(Iu, Iv);       // Iph
+                rec.ang[2 * rec.cap + r] = tir ? 1e4f : asinf(sR);  // Rth
             }
             if (rec.qa) {
                 rec.qa[r] = make_float4(g[0], g[1], g[2], Fr);
