@@ -593,7 +593,8 @@ template <int NTB>
 __device__ __forceinline__ void fft_regs(float2* s, float2 (&x)[PPT],
                                          const float2* __restrict__ lad) {
     constexpr int n = NTB * PPT;
-    constexpr int L = NTB == 256 ? 12 : (NTB == 512 ? 13 : 14);
+    constexpr int L = NTB == 64 ? 10 : NTB == 128 ? 11 : NTB == 256 ? 12 : NTB == 512 ? 13 : 14;
+    static_assert(NTB >= 64 && NTB <= 1024 && (NTB & (NTB - 1)) == 0, "NTB");
     constexpr int r0 = L & 3;
     constexpr int R0 = r0 == 0 ? 16 : (1 << r0);
     constexpr int Q0 = PPT / R0;
@@ -911,7 +912,8 @@ void target_spectra_fused(erp_ctx* ctx, const FftPlan& p, float2* A, float2* Y,
 int target_half_slice() { return SLICE; }
 
 bool target_half_supported(const FftPlan& p, int nr) {
-    return p.N == 2 * nr && (nr == 4096 || nr == 8192 || nr == 16384);
+    return p.N == 2 * nr && (nr == 1024 || nr == 2048 || nr == 4096 || nr == 8192 ||
+                             nr == 16384);
 }
 
 void fft_natural_order(const FftPlan& p, const float2* src, float2* dst, hipStream_t st) {
@@ -931,11 +933,20 @@ void target_pairs_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const in
                        const int* pwin, int nr, int n, const float2* Gnat, float2* work,
                        hipStream_t st) {
     if (!target_half_supported(p, nr))
-        throw ErpError{ERP_E_INVALID, "per-pair LDS chain needs P = 2 nr, nr in {4096, 8192, 16384}"};
+        throw ErpError{ERP_E_INVALID,
+                       "per-pair LDS chain needs P = 2 nr, nr a power of two in [1024, 16384]"};
     const float2* lad = ctx->twiddle.as<float2>();
     const float4* G4 = reinterpret_cast<const float4*>(Gnat);
     float2* o4 = work;
     switch (nr) {
+        case 1024:
+            hipLaunchKernelGGL(k_tpair_half<64>, dim3(n), dim3(64), 0, st, A, p.N, pbase, pwin,
+                               G4, o4, lad);
+            break;
+        case 2048:
+            hipLaunchKernelGGL(k_tpair_half<128>, dim3(n), dim3(128), 0, st, A, p.N, pbase, pwin,
+                               G4, o4, lad);
+            break;
         case 4096:
             hipLaunchKernelGGL(k_tpair_half<256>, dim3(n), dim3(256), 0, st, A, p.N, pbase, pwin,
                                G4, o4, lad);
