@@ -145,14 +145,16 @@ def ingest_pmc(fetch_dir, write_dir, bench_json, out_path):
     print(json.dumps(out, indent=1))
 
 
-def build_workload(name, ns):
+def build_workload(name, ns=None):
+    """The workload and its track: ns shots, or the workload's own track
+    (2000 shots over the fBm mesh, 100 over the 200 x 200 Gaussian one)."""
     from europaraypaths_amd import workloads
     if name == "fbm_two_layer":
-        return workloads.fbm_two_layer(ns=ns)
+        return workloads.fbm_two_layer(ns=ns or 2000)
     if name == "fbm_surface_only":
-        return workloads.fbm_two_layer(ns=ns, aperture=60.0, with_targets=False)
+        return workloads.fbm_two_layer(ns=ns or 2000, aperture=60.0, with_targets=False)
     if name == "gaussian_layer":
-        return workloads.gaussian_layer(ns=ns)
+        return workloads.gaussian_layer(ns=ns or 100)
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -404,7 +406,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--shots-per-step", type=int, default=50)
     ap.add_argument("--workload", default="fbm_two_layer")
-    ap.add_argument("--track-shots", type=int, default=2000)
+    ap.add_argument("--track-shots", type=int, default=None,
+                    help="shots of the track (default: the workload's own)")
     ap.add_argument("--cpu-shots", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--flags", type=int, default=0)
