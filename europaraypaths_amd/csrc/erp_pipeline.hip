@@ -77,6 +77,8 @@ struct KPar {
     double drd, inv_dr, cos_ap, n_d, inv_cyc, lam_d;
     double B, smpl;
     int nr, P_fft, rerad;
+    int row_live;      // entries of a pair row the spectral chain reads: nr for
+                       // the per-pair LDS chain, P_fft otherwise
     uint32_t mode;
     uint64_t seed;
 };
@@ -1650,7 +1652,7 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
         // bins outside the window(s) [base, base + len) or, deterministic,
         // [base, top): zero, then reached only by out-of-window atomics
         const int hi = fits ? base + len : (DET ? max(top, base) : base + TWF);
-        for (int k = threadIdx.x; k < P; k += NTF)
+        for (int k = threadIdx.x; k < kp.row_live; k += NTF)
             if (k < base || k >= hi || k >= nr) rowA[k] = make_float2(0.f, 0.f);
     }
 
@@ -2044,6 +2046,7 @@ KPar make_kpar(const erp_params& p, int P_fft) {
     k.smpl = p.smpl;
     k.nr = p.nr;
     k.P_fft = P_fft;
+    k.row_live = P_fft;
     k.rerad = p.rerad_funct;
     k.mode = p.mode;
     k.seed = p.seed;
@@ -2337,6 +2340,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     if (T > 0 && conv && lin) g_targ = get_chirp(ctx, nr, p.dr, res_t, 0, 0.f, P, planP, st);
     // plain mode: per-pair spectral chain in one workgroup when P = 2 nr fits LDS
     const bool half = !spec && T > 0 && conv && lin && target_half_supported(planP, nr);
+    kp.row_live = half ? nr : P;
     const float2* g_targ_nat =
         half ? get_chirp(ctx, nr, p.dr, res_t, 0, 0.f, P, planP, st, true) : nullptr;
     BlueTables* blue = nullptr;
