@@ -333,9 +333,11 @@ __global__ void k_zero(float2* __restrict__ p, size_t n) {
 // shot's target list; the target id is read here from the list the select
 // kernel wrote, so the lists never travel to the host
 __global__ void k_pair_targets(int* __restrict__ ptgt, const int* __restrict__ pshot,
-                               const int* __restrict__ list1, int Tcap, int n) {
+                               const int* __restrict__ list1, int Tcap, int n,
+                               int* __restrict__ next_pair) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) ptgt[i] = list1[(size_t)pshot[i] * Tcap + ptgt[i]];
+    if (i == 0) *next_pair = 0;  // the target kernel's pair counter
 }
 
 __global__ void k_init_bounds(Bounds* b, int S) {
@@ -1602,179 +1604,191 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
     const int* __restrict__ ptgt, const erp_shot* __restrict__ shots,
     const int* __restrict__ Vs, const int64_t* __restrict__ rec_base, Rec rec,
     const Bounds* __restrict__ bounds, const erp_prism* __restrict__ prisms, int np, KPar kp,
-    float2* __restrict__ outA, int* __restrict__ pbase, int* __restrict__ pwin, int frame_ok) {
+    float2* __restrict__ outA, int* __restrict__ pbase, int* __restrict__ pwin, int frame_ok,
+    int npairs, int* __restrict__ next_pair) {
     __shared__ double winS[2 * TWF];
+    __shared__ int s_pi;
     const WinS winA{winS, winS + TWF};
-    const int pi = blockIdx.x;
-    const int s = pshot[pi], t = ptgt[pi];
-    const ShotK sh = load_shot(shots, s);
-    const int V = Vs[s];
-    const int64_t r0 = rec_base[s];
-    const float txf = tg[t], tyf = tg[Tpad + t], tzf = tg[2 * Tpad + t];
-    const float tnx = tg[3 * Tpad + t], tny = tg[4 * Tpad + t], tnz = tg[5 * Tpad + t];
-    const double tx = txf, ty = tyf, tz = tzf;
-    const double rst = (double)sh.rst;
-    const int nr = kp.nr, P = kp.P_fft;
-    const bool lossless = kp.mode & ERP_MODE_LOSSLESS;
-
-    // window from the shot bounds (see k_target_accum)
-    const Bounds bd = bounds[s];
-    const float bx = fmaxf(fmaxf(o2f(bd.xmin) - txf, txf - o2f(bd.xmax)), 0.f);
-    const float by = fmaxf(fmaxf(o2f(bd.ymin) - tyf, tyf - o2f(bd.ymax)), 0.f);
-    const float bz = fmaxf(fmaxf(o2f(bd.zmin) - tzf, tzf - o2f(bd.zmax)), 0.f);
-    const double tmin = sqrt((double)bx * bx + (double)by * by + (double)bz * bz);
-    const double itmin = __longlong_as_double((long long)bd.itd_min);
-    int base = (int)floor((itmin + kp.n_d * tmin - rst) / kp.drd) - 2;
-    base = max(base, 0);
-    bool fits = false;
-    int len = TWF;
-    int top = nr;
-    if (pbase || DET) {
-        const float fx = fmaxf(fabsf(o2f(bd.xmin) - txf), fabsf(o2f(bd.xmax) - txf));
-        const float fy = fmaxf(fabsf(o2f(bd.ymin) - tyf), fabsf(o2f(bd.ymax) - tyf));
-        const float fz = fmaxf(fabsf(o2f(bd.zmin) - tzf), fabsf(o2f(bd.zmax) - tzf));
-        const double tmax = sqrt((double)fx * fx + (double)fy * fy + (double)fz * fz);
-        const double ithmax = __longlong_as_double((long long)bd.itd_max);
-        const double rhi = ithmax + kp.n_d * tmax * (1.0 + 1e-6) + 1.0;
-        // every bin written (b and b + 1) is below top
-        top = min((int)ceil((rhi - rst) / kp.drd) + 2, nr);
-        fits = top - base <= TWF;
-        // live window: [base, base + len), len a multiple of 64 (only it is
-        // zeroed, flushed and read by the spectral chain)
-        if (fits) len = max(0, min(TWF, (top - base + 63) & ~63));
-    }
-    if (pbase && threadIdx.x == 0) {
-        pbase[pi] = fits ? base : -1;
-        pwin[pi] = len;
-    }
-    float2* rowA = outA + (size_t)pi * P;
-    if (!fits || (DET && !pbase)) {
-        // bins outside the window(s) [base, base + len) or, deterministic,
-        // [base, top): zero, then reached only by out-of-window atomics
-        const int hi = fits ? base + len : (DET ? max(top, base) : base + TWF);
-        for (int k = threadIdx.x; k < kp.row_live; k += NTF)
-            if (k < base || k >= hi || k >= nr) rowA[k] = make_float2(0.f, 0.f);
-    }
-
-    const double inv_dr = kp.inv_dr, n_d = kp.n_d, two_cyc = 2.0 * kp.inv_cyc;
-    const bool tn_up = tnx == 0.f && tny == 0.f && tnz == 1.f;
-
-    if constexpr (DET) {
-        // bound of every term of this pair: |component| <= F Dmax / Ttd (the
-        // four sinc factors, the loss factor and the two-bin weights are <= 1)
-        __shared__ float red[NTF / 64];
-        float m = 0.f;
-        for (int k = threadIdx.x; k < V; k += NTF) {
-            const float4 qa = rec.qa[r0 + k];
-            const double dx = tx - (double)qa.x, dy = ty - (double)qa.y, dz = tz - (double)qa.z;
-            const double t2 = fma(dx, dx, fma(dy, dy, dz * dz));
-            m = fmaxf(m, qa.w * __builtin_amdgcn_rsqf((float)t2));
-        }
-        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    // persistent: each workgroup takes the next pair from a counter until
+    // none is left (the counter starts at 0: k_pair_targets clears it), so
+    // short pairs pay no workgroup launch and uneven pairs balance
+    for (;;) {
+        if (threadIdx.x == 0) s_pi = atomicAdd(next_pair, 1);
         __syncthreads();
-        m = red[0];
-        for (int w = 1; w < NTF / 64; ++w) m = fmaxf(m, red[w]);
-        // 2x margin for the rounding of the sinc quotients
-        const double scale = det_scale((double)V, 2.0 * (double)m * (double)kp.Dmax);
-        const double inv = scale > 0.0 ? 1.0 / scale : 0.0;
-        unsigned long long* wux = reinterpret_cast<unsigned long long*>(winS);
-        unsigned long long* wuy = wux + TWF;
-        // one LDS window of TWF bins at a time over [base, top)
-        for (int wb = base; wb < top || wb == base; wb += TWF) {
-            const int wl = fits ? len : min(TWF, max(0, top - wb));
-            for (int k = threadIdx.x; k < wl; k += NTF) {
-                winS[k] = 0.0;
-                winS[TWF + k] = 0.0;
+        const int pi = s_pi;
+        if (pi >= npairs) break;
+        const int s = pshot[pi], t = ptgt[pi];
+        const ShotK sh = load_shot(shots, s);
+        const int V = Vs[s];
+        const int64_t r0 = rec_base[s];
+        const float txf = tg[t], tyf = tg[Tpad + t], tzf = tg[2 * Tpad + t];
+        const float tnx = tg[3 * Tpad + t], tny = tg[4 * Tpad + t], tnz = tg[5 * Tpad + t];
+        const double tx = txf, ty = tyf, tz = tzf;
+        const double rst = (double)sh.rst;
+        const int nr = kp.nr, P = kp.P_fft;
+        const bool lossless = kp.mode & ERP_MODE_LOSSLESS;
+
+        // window from the shot bounds (see k_target_accum)
+        const Bounds bd = bounds[s];
+        const float bx = fmaxf(fmaxf(o2f(bd.xmin) - txf, txf - o2f(bd.xmax)), 0.f);
+        const float by = fmaxf(fmaxf(o2f(bd.ymin) - tyf, tyf - o2f(bd.ymax)), 0.f);
+        const float bz = fmaxf(fmaxf(o2f(bd.zmin) - tzf, tzf - o2f(bd.zmax)), 0.f);
+        const double tmin = sqrt((double)bx * bx + (double)by * by + (double)bz * bz);
+        const double itmin = __longlong_as_double((long long)bd.itd_min);
+        int base = (int)floor((itmin + kp.n_d * tmin - rst) / kp.drd) - 2;
+        base = max(base, 0);
+        bool fits = false;
+        int len = TWF;
+        int top = nr;
+        if (pbase || DET) {
+            const float fx = fmaxf(fabsf(o2f(bd.xmin) - txf), fabsf(o2f(bd.xmax) - txf));
+            const float fy = fmaxf(fabsf(o2f(bd.ymin) - tyf), fabsf(o2f(bd.ymax) - tyf));
+            const float fz = fmaxf(fabsf(o2f(bd.zmin) - tzf), fabsf(o2f(bd.zmax) - tzf));
+            const double tmax = sqrt((double)fx * fx + (double)fy * fy + (double)fz * fz);
+            const double ithmax = __longlong_as_double((long long)bd.itd_max);
+            const double rhi = ithmax + kp.n_d * tmax * (1.0 + 1e-6) + 1.0;
+            // every bin written (b and b + 1) is below top
+            top = min((int)ceil((rhi - rst) / kp.drd) + 2, nr);
+            fits = top - base <= TWF;
+            // live window: [base, base + len), len a multiple of 64 (only it is
+            // zeroed, flushed and read by the spectral chain)
+            if (fits) len = max(0, min(TWF, (top - base + 63) & ~63));
+        }
+        if (pbase && threadIdx.x == 0) {
+            pbase[pi] = fits ? base : -1;
+            pwin[pi] = len;
+        }
+        float2* rowA = outA + (size_t)pi * P;
+        if (!fits || (DET && !pbase)) {
+            // bins outside the window(s) [base, base + len) or, deterministic,
+            // [base, top): zero, then reached only by out-of-window atomics
+            const int hi = fits ? base + len : (DET ? max(top, base) : base + TWF);
+            for (int k = threadIdx.x; k < kp.row_live; k += NTF)
+                if (k < base || k >= hi || k >= nr) rowA[k] = make_float2(0.f, 0.f);
+        }
+
+        const double inv_dr = kp.inv_dr, n_d = kp.n_d, two_cyc = 2.0 * kp.inv_cyc;
+        const bool tn_up = tnx == 0.f && tny == 0.f && tnz == 1.f;
+
+        if constexpr (DET) {
+            // bound of every term of this pair: |component| <= F Dmax / Ttd (the
+            // four sinc factors, the loss factor and the two-bin weights are <= 1)
+            __shared__ float red[NTF / 64];
+            float m = 0.f;
+            for (int k = threadIdx.x; k < V; k += NTF) {
+                const float4 qa = rec.qa[r0 + k];
+                const double dx = tx - (double)qa.x, dy = ty - (double)qa.y, dz = tz - (double)qa.z;
+                const double t2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                m = fmaxf(m, qa.w * __builtin_amdgcn_rsqf((float)t2));
             }
+            for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
             __syncthreads();
-            const bool first = wb == base, last = wb + TWF >= top;
-            auto add_det = [&](const Tri& ta) {
-                const float2 c0 = make_float2(ta.c.x * (1.f - ta.w), ta.c.y * (1.f - ta.w));
-                const float2 c1 = make_float2(ta.c.x * ta.w, ta.c.y * ta.w);
-                const int k0 = ta.b - wb;
-                if (k0 >= 0 && k0 < wl) lds_add2_fix(wux, wuy, k0, c0, scale);
-                else if ((ta.b < base && first) || (ta.b >= top && last)) glb_add2(rowA, ta.b, c0);
-                if (ta.b + 1 < nr) {
-                    if (k0 + 1 >= 0 && k0 + 1 < wl) lds_add2_fix(wux, wuy, k0 + 1, c1, scale);
-                    else if ((ta.b + 1 < base && first) || (ta.b + 1 >= top && last))
-                        glb_add2(rowA, ta.b + 1, c1);
+            m = red[0];
+            for (int w = 1; w < NTF / 64; ++w) m = fmaxf(m, red[w]);
+            // 2x margin for the rounding of the sinc quotients
+            const double scale = det_scale((double)V, 2.0 * (double)m * (double)kp.Dmax);
+            const double inv = scale > 0.0 ? 1.0 / scale : 0.0;
+            unsigned long long* wux = reinterpret_cast<unsigned long long*>(winS);
+            unsigned long long* wuy = wux + TWF;
+            // one LDS window of TWF bins at a time over [base, top)
+            for (int wb = base; wb < top || wb == base; wb += TWF) {
+                const int wl = fits ? len : min(TWF, max(0, top - wb));
+                for (int k = threadIdx.x; k < wl; k += NTF) {
+                    winS[k] = 0.0;
+                    winS[TWF + k] = 0.0;
                 }
-            };
-            // one facet per thread and step, the next step's record loaded
-            // ahead into the other register set (two chains, as the plain
-            // path runs, would cost this variant a wave per SIMD)
+                __syncthreads();
+                const bool first = wb == base, last = wb + TWF >= top;
+                auto add_det = [&](const Tri& ta) {
+                    const float2 c0 = make_float2(ta.c.x * (1.f - ta.w), ta.c.y * (1.f - ta.w));
+                    const float2 c1 = make_float2(ta.c.x * ta.w, ta.c.y * ta.w);
+                    const int k0 = ta.b - wb;
+                    if (k0 >= 0 && k0 < wl) lds_add2_fix(wux, wuy, k0, c0, scale);
+                    else if ((ta.b < base && first) || (ta.b >= top && last)) glb_add2(rowA, ta.b, c0);
+                    if (ta.b + 1 < nr) {
+                        if (k0 + 1 >= 0 && k0 + 1 < wl) lds_add2_fix(wux, wuy, k0 + 1, c1, scale);
+                        else if ((ta.b + 1 < base && first) || (ta.b + 1 >= top && last))
+                            glb_add2(rowA, ta.b + 1, c1);
+                    }
+                };
+                // one facet per thread and step, the next step's record loaded
+                // ahead into the other register set (two chains, as the plain
+                // path runs, would cost this variant a wave per SIMD)
+                if (V > 0) {
+                    const int vl = V - 1;
+                    auto rec_at = [&](int k) { return load_rec(rec, r0 + min(k, vl)); };
+                    auto step = [&](const RecV& ca) {
+                        const Tri ta = tri_eval(ca, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx,
+                                                tny, tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr,
+                                                lossless, prisms, np, kp);
+                        if (ta.valid) add_det(ta);
+                    };
+                    RecV a0 = rec_at(threadIdx.x);
+                    for (int k = threadIdx.x; k < V; k += 2 * NTF) {
+                        const RecV a1 = rec_at(k + NTF);
+                        step(a0);
+                        if (k + NTF >= V) break;
+                        a0 = rec_at(k + 2 * NTF);
+                        step(a1);
+                    }
+                }
+                __syncthreads();
+                for (int k = threadIdx.x; k < wl; k += NTF) {
+                    const int b = wb + k;
+                    if (b >= nr) break;
+                    rowA[b] = make_float2((float)det_unfix(wux[k], inv),
+                                          (float)det_unfix(wuy[k], inv));
+                }
+                __syncthreads();
+                if (fits) break;
+            }
+            continue;  // the window loop ended on a barrier
+        } else {
+            for (int k = threadIdx.x; k < len; k += NTF) win_zero<ACC>(winA, k);
+            __threadfence();
+            __syncthreads();
+
+            // two facets per thread and step (two independent dependency
+            // chains); the records of the next step are loaded before this
+            // step's arithmetic so that their latency overlaps it.  Two register
+            // sets (a0 b0, a1 b1) alternate, and every load reads a clamped,
+            // in-bounds index, so no set is copied or kept conditionally.
             if (V > 0) {
                 const int vl = V - 1;
                 auto rec_at = [&](int k) { return load_rec(rec, r0 + min(k, vl)); };
-                auto step = [&](const RecV& ca) {
-                    const Tri ta = tri_eval(ca, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx,
-                                            tny, tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr,
-                                            lossless, prisms, np, kp);
-                    if (ta.valid) add_det(ta);
+                auto step = [&](const RecV& ca, const RecV& cb, int k) {
+                    const Tri ta = tri_eval<PH>(ca, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx,
+                                                tny, tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr,
+                                                lossless, prisms, np, kp);
+                    Tri tb;
+                    tb.valid = false;
+                    if (k + NTF < V)
+                        tb = tri_eval<PH>(cb, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx, tny,
+                                          tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr, lossless,
+                                          prisms, np, kp);
+                    if (ta.valid) tri_add<ACC>(ta, base, len, nr, winA, rowA);
+                    if (tb.valid) tri_add<ACC>(tb, base, len, nr, winA, rowA);
                 };
-                RecV a0 = rec_at(threadIdx.x);
-                for (int k = threadIdx.x; k < V; k += 2 * NTF) {
-                    const RecV a1 = rec_at(k + NTF);
-                    step(a0);
-                    if (k + NTF >= V) break;
-                    a0 = rec_at(k + 2 * NTF);
-                    step(a1);
+                RecV a0 = rec_at(threadIdx.x), b0 = rec_at(threadIdx.x + NTF);
+                for (int k = threadIdx.x; k < V; k += 4 * NTF) {
+                    const RecV a1 = rec_at(k + 2 * NTF), b1 = rec_at(k + 3 * NTF);
+                    step(a0, b0, k);
+                    if (k + 2 * NTF >= V) break;
+                    a0 = rec_at(k + 4 * NTF);
+                    b0 = rec_at(k + 5 * NTF);
+                    step(a1, b1, k + 2 * NTF);
                 }
             }
             __syncthreads();
-            for (int k = threadIdx.x; k < wl; k += NTF) {
-                const int b = wb + k;
+            for (int k = threadIdx.x; k < len; k += NTF) {
+                const int b = base + k;
                 if (b >= nr) break;
-                rowA[b] = make_float2((float)det_unfix(wux[k], inv),
-                                      (float)det_unfix(wuy[k], inv));
-            }
-            __syncthreads();
-            if (fits) break;
-        }
-        return;
-    } else {
-        for (int k = threadIdx.x; k < len; k += NTF) win_zero<ACC>(winA, k);
-        __threadfence();
-        __syncthreads();
-
-        // two facets per thread and step (two independent dependency
-        // chains); the records of the next step are loaded before this
-        // step's arithmetic so that their latency overlaps it.  Two register
-        // sets (a0 b0, a1 b1) alternate, and every load reads a clamped,
-        // in-bounds index, so no set is copied or kept conditionally.
-        if (V > 0) {
-            const int vl = V - 1;
-            auto rec_at = [&](int k) { return load_rec(rec, r0 + min(k, vl)); };
-            auto step = [&](const RecV& ca, const RecV& cb, int k) {
-                const Tri ta = tri_eval<PH>(ca, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx,
-                                            tny, tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr,
-                                            lossless, prisms, np, kp);
-                Tri tb;
-                tb.valid = false;
-                if (k + NTF < V)
-                    tb = tri_eval<PH>(cb, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx, tny,
-                                      tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr, lossless,
-                                      prisms, np, kp);
-                if (ta.valid) tri_add<ACC>(ta, base, len, nr, winA, rowA);
-                if (tb.valid) tri_add<ACC>(tb, base, len, nr, winA, rowA);
-            };
-            RecV a0 = rec_at(threadIdx.x), b0 = rec_at(threadIdx.x + NTF);
-            for (int k = threadIdx.x; k < V; k += 4 * NTF) {
-                const RecV a1 = rec_at(k + 2 * NTF), b1 = rec_at(k + 3 * NTF);
-                step(a0, b0, k);
-                if (k + 2 * NTF >= V) break;
-                a0 = rec_at(k + 4 * NTF);
-                b0 = rec_at(k + 5 * NTF);
-                step(a1, b1, k + 2 * NTF);
+                rowA[b] = win_get<ACC>(winA, k);
             }
         }
+        // the next pair zeroes the window this one has just read
         __syncthreads();
-        for (int k = threadIdx.x; k < len; k += NTF) {
-            const int b = base + k;
-            if (b >= nr) break;
-            rowA[b] = win_get<ACC>(winA, k);
-        }
     }
 }
 
@@ -2611,19 +2625,23 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                     std::copy(sege.begin(), sege.end(), up.begin() + so + 2 * S);
                     ERP_HIP_TRY(hipMemcpyAsync(dps, up.data(), sizeof(int) * up.size(),
                                                hipMemcpyHostToDevice, st));
+                    int* dnext = dpw + C;  // pair counter, in the spare tail of the list
                     hipLaunchKernelGGL(k_pair_targets, dim3((n + 255) / 256), dim3(256), 0, st,
-                                       dpt, dps, list1, Tcap, n);
+                                       dpt, dps, list1, Tcap, n, dnext);
                     ERP_CHECK_LAUNCH("k_pair_targets");
                     double triples = 0;
                     for (int i = 0; i < n; ++i) triples += hV[pshot[i]];
                     PT_.begin(ERP_STAGE_TARGET_ACCUM, st, triples);
                     if (!spec) {
                         auto kern = pick_target_accum(det, ph, acc);
-                        hipLaunchKernelGGL(kern, dim3(n), dim3(NTF), 0, st,
+                        // persistent grid: two workgroups per CU (the LDS window
+                        // allows two), each taking pairs until none is left
+                        const int grid = std::max(1, std::min(n, 2 * ctx->num_cu));
+                        hipLaunchKernelGGL(kern, dim3(grid), dim3(NTF), 0, st,
                                            targets->xyzn.as<float>(), Tpad, dps, dpt, dshots, dV,
                                            dbase, rec, dbounds, dpr, (int)nprisms, kp, A,
                                            (fused || half) ? dpb : nullptr, dpw,
-                                           mesh->frame_ok ? 1 : 0);
+                                           mesh->frame_ok ? 1 : 0, n, dnext);
                         ERP_CHECK_LAUNCH("k_target_accum_fast");
                     } else {
                         hipLaunchKernelGGL(k_target_accum<1>, dim3(n), dim3(NTT),
