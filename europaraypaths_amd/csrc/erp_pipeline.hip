@@ -1746,7 +1746,10 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
             continue;  // the window loop ended on a barrier
         } else {
             for (int k = threadIdx.x; k < len; k += NTF) win_zero<ACC>(winA, k);
-            __threadfence();
+            // the zeros written to the row outside the window must land
+            // before any thread's out-of-window atomics; a fitting pair
+            // wrote none (and the LDS zeros need only the barrier)
+            if (!fits) __threadfence();
             __syncthreads();
 
             // two facets per thread and step (two independent dependency
