@@ -1125,7 +1125,8 @@ __global__ void __launch_bounds__(NTT) k_target_accum(
         winA[k] = make_double2(0.0, 0.0);
         if (MODE == 1) winB[k] = make_double2(0.0, 0.0);
     }
-    __threadfence();
+    // only the row zeros above must land before other threads' atomics
+    if (MODE != 2 && !fits) __threadfence();
     __syncthreads();
 
     for (int k = threadIdx.x; k < V; k += NTT) {
