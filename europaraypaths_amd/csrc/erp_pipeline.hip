@@ -453,9 +453,9 @@ struct Rec {
     float* F;       // cap: sqrt(P n Grefr G1 [fRfrC]) lam^2 / (K Itd)
     float* fRfrC;   // cap
     float* ang;     // 3 x cap: Ith, Iph, Rth
-    // the facet record packed for k_target_accum_fast (52 bytes, four wide
-    // loads): qa = (x, y, z, F), qb = (nx, ny, nz, ux), qc = (uy, uz, vx, vy),
-    // qd = vz; Itd is recomputed from the source position
+    // the facet record packed for k_target_accum_fast (52 bytes in four
+    // loads, plus Itd from itd above): qa = (x, y, z, F),
+    // qb = (nx, ny, nz, ux), qc = (uy, uz, vx, vy), qd = vz
     float4* qa;
     float4* qb;
     float4* qc;
@@ -1374,6 +1374,35 @@ __device__ __forceinline__ RecV load_rec(const Rec& rec, int64_t r) {
     v.itd = rec.itd[r];
     return v;
 }
+// The packed records of one shot, based at the shot's first record (a value
+// uniform over the workgroup, so the five bases sit in scalar registers).
+// Record k of the shot is then one 32-bit byte offset per array off a scalar
+// base (k < 2^27: 16 k fits 32 bits), which the loads take as is, instead of
+// five 64-bit address sums per record.
+struct RecShot {
+    const char* qa;
+    const char* qb;
+    const char* qc;
+    const char* qd;
+    const char* itd;
+};
+__device__ __forceinline__ RecShot rec_shot(const Rec& rec, int64_t r0) {
+    return RecShot{reinterpret_cast<const char*>(rec.qa + r0),
+                   reinterpret_cast<const char*>(rec.qb + r0),
+                   reinterpret_cast<const char*>(rec.qc + r0),
+                   reinterpret_cast<const char*>(rec.qd + r0),
+                   reinterpret_cast<const char*>(rec.itd + r0)};
+}
+__device__ __forceinline__ RecV load_rec(const RecShot& rs, unsigned k) {
+    RecV v;
+    const unsigned o16 = k << 4;
+    v.qa = *reinterpret_cast<const float4*>(rs.qa + o16);
+    v.qb = *reinterpret_cast<const float4*>(rs.qb + o16);
+    v.qc = *reinterpret_cast<const float4*>(rs.qc + o16);
+    v.qd = *reinterpret_cast<const float*>(rs.qd + (k << 2));
+    v.itd = *reinterpret_cast<const double*>(rs.itd + (k << 3));
+    return v;
+}
 
 template <int PH = PH_F64>
 __device__ __forceinline__ Tri tri_eval(const RecV& rv, bool frame_ok, double sx,
@@ -1622,6 +1651,7 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
         const ShotK sh = load_shot(shots, s);
         const int V = Vs[s];
         const int64_t r0 = rec_base[s];
+        const RecShot rs = rec_shot(rec, r0);
         const float txf = tg[t], tyf = tg[Tpad + t], tzf = tg[2 * Tpad + t];
         const float tnx = tg[3 * Tpad + t], tny = tg[4 * Tpad + t], tnz = tg[5 * Tpad + t];
         const double tx = txf, ty = tyf, tz = tzf;
@@ -1718,7 +1748,7 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
                 // path runs, would cost this variant a wave per SIMD)
                 if (V > 0) {
                     const int vl = V - 1;
-                    auto rec_at = [&](int k) { return load_rec(rec, r0 + min(k, vl)); };
+                    auto rec_at = [&](int k) { return load_rec(rs, (unsigned)min(k, vl)); };
                     auto step = [&](const RecV& ca) {
                         const Tri ta = tri_eval(ca, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx,
                                                 tny, tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr,
@@ -1760,7 +1790,7 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
             // in-bounds index, so no set is copied or kept conditionally.
             if (V > 0) {
                 const int vl = V - 1;
-                auto rec_at = [&](int k) { return load_rec(rec, r0 + min(k, vl)); };
+                auto rec_at = [&](int k) { return load_rec(rs, (unsigned)min(k, vl)); };
                 auto step = [&](const RecV& ca, const RecV& cb, int k) {
                     const Tri ta = tri_eval<PH>(ca, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx,
                                                 tny, tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr,
@@ -2434,6 +2464,10 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         ERP_HIP_TRY(hipStreamSynchronize(st));
         hbase[0] = 0;
         for (int s = 0; s < S; ++s) hbase[s + 1] = hbase[s] + (need_rec ? hV[s] : 0);
+        // the target kernel reads a shot's records at 32-bit byte offsets
+        for (int s = 0; s < S; ++s)
+            if (need_rec && hV[s] >= (1 << 28))
+                throw ErpError{ERP_E_INVALID, "a shot illuminates 2^28 facets or more"};
         const int64_t Vtot = hbase[S];
         ERP_HIP_TRY(hipMemcpyAsync(dbase, hbase.data(), sizeof(int64_t) * S,
                                    hipMemcpyHostToDevice, st));
