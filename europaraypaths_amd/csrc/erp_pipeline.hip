@@ -255,13 +255,15 @@ __device__ __forceinline__ void sinc_nd_hw(float x, float* num, float* den) {
     *den = tiny ? 1.0f : 3.14159265358979f * x;
 }
 
-// pi sinc(x) as numerator / denominator: sin(pi x) / x, and 1 / (1/pi)
-// where sinc rounds to 1.  A product of four needs one reciprocal and one
-// factor 1/pi^4, instead of four multiplications by pi.
+// pi sinc(x) as numerator / denominator: sin(pi |x|) / |x| (an even
+// function), with |x| raised to at least 1e-6: below that sin(pi x) / x
+// equals pi to within 2e-12 relative, so x = 0 needs no select.  A product
+// of four needs one reciprocal and one factor 1/pi^4, instead of four
+// multiplications by pi.
 __device__ __forceinline__ void pisinc_nd_hw(float x, float* num, float* den) {
-    const bool tiny = fabsf(x) < 1e-6f;
-    *num = tiny ? 1.0f : sinpi_hw(x);
-    *den = tiny ? 0.318309886183791f : x;
+    const float xa = fmaxf(fabsf(x), 1e-6f);
+    *num = sinpi_hw(xa);
+    *den = xa;
 }
 
 // exp(-i pi t) for t = 2 r c given r and c (FP64): the half-cycle count
@@ -1595,14 +1597,15 @@ __device__ __forceinline__ void tri_add(const Tri& t, int base, int len, int nr,
         const double cx = t.c.x, cy = t.c.y, w = t.w;
         const double c1x = cx * w, c1y = cy * w;
         const double c0x = cx - c1x, c0y = cy - c1y;
-        if (k0 >= 0 && k0 < len) {
+        // 0 <= k < len as one unsigned compare (len >= 0)
+        if ((unsigned)k0 < (unsigned)len) {
             atomicAdd(&winA.x[k0], c0x);
             atomicAdd(&winA.y[k0], c0y);
         } else {
             glb_add2(rowA, t.b, make_float2((float)c0x, (float)c0y));
         }
-        if (t.b + 1 < nr) {
-            if (k0 + 1 >= 0 && k0 + 1 < len) {
+        if (t.b < nr - 1) {
+            if ((unsigned)(k0 + 1) < (unsigned)len) {
                 atomicAdd(&winA.x[k0 + 1], c1x);
                 atomicAdd(&winA.y[k0 + 1], c1y);
             } else {
