@@ -137,6 +137,13 @@ __device__ __forceinline__ bool bin_of(double rng, double rst, double dr, int nr
     *w = (float)(x - (double)bi);
     return true;
 }
+__device__ __forceinline__ bool bin_of(double rng, double rst, double dr, int nr, int* b,
+                                       double* w) {
+    float wf;
+    const bool ok = bin_of(rng, rst, dr, nr, b, &wf);
+    *w = wf;
+    return ok;
+}
 
 // exp(-i 2 pi cyc): phase from FP64 cycles, reduced before FP32 sincospi.
 __device__ __forceinline__ float2 phasor_cycles(double cyc) {
@@ -1298,7 +1305,7 @@ __global__ void __launch_bounds__(NTT) k_target_accum(
 // 767-830) for facet record r.
 struct Tri {
     int b;
-    float w;
+    double w;  // kept in FP64: the FP64 two-bin split takes it without conversions
     float2 c;
     bool valid;
 };
@@ -1424,16 +1431,19 @@ __device__ __forceinline__ Tri tri_eval(const RecV& rv, bool frame_ok, double sx
     // hardware reciprocal square root: t2 is a squared facet-target
     // distance, far from the denormal range rsqrtf guards against
     const float rs = __builtin_amdgcn_rsqf((float)t2);
-    const double y0 = t2 * (double)rs;
-    const double Ttd = fma(fma(-y0, y0, t2), 0.5 * (double)rs, y0);
+    const double rd = (double)rs, hr = 0.5 * rd;
+    const double y0 = t2 * rd;
+    const double Ttd = fma(fma(-y0, y0, t2), hr, y0);
     double rngt = 0.0;
     float2 ph_st = make_float2(1.f, 0.f);
     if constexpr (PH == PH_F64) {
         rngt = fma(Ttd, n_d, Itd);
-        const double xq = (rngt - rst) * inv_dr;
+        // (rngt - rst) / dr to within 1e-11 bin; the 1e-7 margin below
+        // covers it
+        const double xq = fma(rngt, inv_dr, -rst * inv_dr);
         o.b = (int)xq;
-        o.w = (float)(xq - (double)o.b);
-        if (o.w < 1e-7f || o.w > 1.f - 1e-7f) {
+        o.w = xq - (double)o.b;
+        if (o.w < 1e-7 || o.w > 1.0 - 1e-7) {
             // within about 1e-7 bin of an edge (every negative coordinate
             // included): the oracle's exact operation sequence
             rngt = madd_nc(Itd, dist3(dx, dy, dz), n_d);
@@ -1466,7 +1476,7 @@ __device__ __forceinline__ Tri tri_eval(const RecV& rv, bool frame_ok, double sx
         const double xq = (double)xd.hi + (double)xd.lo;
         o.valid = (xq > -1.0) && (xq < (double)nr);
         o.b = (int)xq;
-        o.w = (float)(xq - (double)o.b);
+        o.w = xq - (double)o.b;
         // exp(-i pi t), t = rngt * two_cyc half cycles, reduced in double-float
         const DF t = df_mul(rd2, df_of(0.5 * two_cyc));
         const float rr = rintf(t.hi);
@@ -1476,9 +1486,9 @@ __device__ __forceinline__ Tri tri_eval(const RecV& rv, bool frame_ok, double sx
     const float nx = qb.x, ny = qb.y, nz = qb.z;
     const float ux = qb.w, uy = qc.x, uz = qc.y;
     const float vx = qc.z, vy = qc.w, vz = qd;
-    // unit vector facet -> target in FP64: 1/Ttd by one Newton step from rs
-    const double rd = (double)rs;
-    const double invd = rd * fma(-0.5 * t2 * rd, rd, 1.5);
+    // unit vector facet -> target in FP64: 1/Ttd by one Newton step from rs,
+    // rd (1.5 - t2 rd^2 / 2), with t2 rd^2 / 2 = y0 hr exactly
+    const double invd = rd * fma(-y0, hr, 1.5);
     const double ix = dx * invd, iy = dy * invd, iz = dz * invd;
     const float Ix = (float)ix, Iy = (float)iy, Iz = (float)iz;
     const float2 csT = cos_sin_hw(ix, iy, iz, nx, ny, nz);
@@ -1614,8 +1624,9 @@ __device__ __forceinline__ void tri_add(const Tri& t, int base, int len, int nr,
         }
         return;
     }
-    const float2 c0 = make_float2(t.c.x * (1.f - t.w), t.c.y * (1.f - t.w));
-    const float2 c1 = make_float2(t.c.x * t.w, t.c.y * t.w);
+    const float wf = (float)t.w;
+    const float2 c0 = make_float2(t.c.x * (1.f - wf), t.c.y * (1.f - wf));
+    const float2 c1 = make_float2(t.c.x * wf, t.c.y * wf);
     if (k0 >= 0 && k0 < len) win_add<ACC>(winA, k0, c0); else glb_add2(rowA, t.b, c0);
     if (t.b + 1 < nr) {
         if (k0 + 1 >= 0 && k0 + 1 < len) win_add<ACC>(winA, k0 + 1, c1);
@@ -1735,8 +1746,9 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
                 __syncthreads();
                 const bool first = wb == base, last = wb + TWF >= top;
                 auto add_det = [&](const Tri& ta) {
-                    const float2 c0 = make_float2(ta.c.x * (1.f - ta.w), ta.c.y * (1.f - ta.w));
-                    const float2 c1 = make_float2(ta.c.x * ta.w, ta.c.y * ta.w);
+                    const float wf = (float)ta.w;
+                    const float2 c0 = make_float2(ta.c.x * (1.f - wf), ta.c.y * (1.f - wf));
+                    const float2 c1 = make_float2(ta.c.x * wf, ta.c.y * wf);
                     const int k0 = ta.b - wb;
                     if (k0 >= 0 && k0 < wl) lds_add2_fix(wux, wuy, k0, c0, scale);
                     else if ((ta.b < base && first) || (ta.b >= top && last)) glb_add2(rowA, ta.b, c0);
