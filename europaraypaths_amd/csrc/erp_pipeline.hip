@@ -171,9 +171,11 @@ __device__ __forceinline__ float2 cos_sin_ref(double ix, double iy, double iz, f
 // root is approximate
 __device__ __forceinline__ float2 cos_sin_hw(double ix, double iy, double iz, float mx, float my,
                                              float mz) {
-    double c = -(ix * mx + iy * my + iz * mz);
-    c = fmin(1.0, fmax(-1.0, c));
-    return make_float2((float)c, __builtin_amdgcn_sqrtf((float)((1.0 - c) * (1.0 + c))));
+    // |c| exceeds 1 by at most a few FP64 ulps, which neither (float)c nor
+    // the root clamped at 0 can see: the same values as clamping c first
+    const double c = -(ix * mx + iy * my + iz * mz);
+    return make_float2((float)c,
+                       __builtin_amdgcn_sqrtf(fmaxf(0.f, (float)((1.0 - c) * (1.0 + c)))));
 }
 
 // (I.u, I.v) in FP64.  Needed for a general (non-orthonormal) facet frame,
@@ -675,7 +677,10 @@ __global__ void __launch_bounds__(NT) k_surface(
                 rec.ang[2 * rec.cap + r] = tir ? 1e4f : asinf(sR);  // Rth
             }
             if (rec.qa) {
-                rec.qa[r] = make_float4(g[0], g[1], g[2], Fr);
+                // F Dmax / pi^4: the gain factor of the target kernel's
+                // amplitude, with 1/pi^4 taking out the four factors pi of
+                // its pi-sincs
+                rec.qa[r] = make_float4(g[0], g[1], g[2], Fr * (kp.Dmax * 0.0102659823f));
                 rec.qb[r] = make_float4(g[3], g[4], g[5], g[6]);
                 rec.qc[r] = make_float4(g[7], g[8], g[9], g[10]);
                 rec.qd[r] = g[11];
@@ -1521,9 +1526,9 @@ __device__ __forceinline__ Tri tri_eval(const RecV& rv, bool frame_ok, double sx
     pisinc_nd_hw(a * sTT * cph, &n3, &d3);
     pisinc_nd_hw(a * sTT * sph, &n4, &d4);
     const float den = fabsf((d1 * d2) * (d3 * d4));
-    // Dmax / pi^4 takes the four factors pi of the pi-sincs back out
-    const float dmax_pi4 = kp.Dmax * 0.0102659823f;
-    float amp = qa.w * dmax_pi4 * fabsf((n1 * n2) * (n3 * n4)) * __builtin_amdgcn_rcpf(den) * rs;
+    // qa.w = F Dmax / pi^4 (the 1/pi^4 takes the four factors pi of the
+    // pi-sincs back out)
+    float amp = qa.w * fabsf((n1 * n2) * (n3 * n4)) * __builtin_amdgcn_rcpf(den) * rs;
     if (!lossless) {
         float expo = 0.f, tot = 0.f;
         for (int k = 0; k < np; ++k) {
@@ -1732,7 +1737,8 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
             m = red[0];
             for (int w = 1; w < NTF / 64; ++w) m = fmaxf(m, red[w]);
             // 2x margin for the rounding of the sinc quotients
-            const double scale = det_scale((double)V, 2.0 * (double)m * (double)kp.Dmax);
+            // qa.w = F Dmax / pi^4 and each pi-sinc factor is <= pi
+            const double scale = det_scale((double)V, 2.0 * (double)m * 97.40909103400244);
             const double inv = scale > 0.0 ? 1.0 / scale : 0.0;
             unsigned long long* wux = reinterpret_cast<unsigned long long*>(winS);
             unsigned long long* wuy = wux + TWF;
