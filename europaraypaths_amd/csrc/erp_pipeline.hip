@@ -171,11 +171,9 @@ __device__ __forceinline__ float2 cos_sin_ref(double ix, double iy, double iz, f
 // root is approximate
 __device__ __forceinline__ float2 cos_sin_hw(double ix, double iy, double iz, float mx, float my,
                                              float mz) {
-    // |c| exceeds 1 by at most a few FP64 ulps, which neither (float)c nor
-    // the root clamped at 0 can see: the same values as clamping c first
-    const double c = -(ix * mx + iy * my + iz * mz);
-    return make_float2((float)c,
-                       __builtin_amdgcn_sqrtf(fmaxf(0.f, (float)((1.0 - c) * (1.0 + c)))));
+    double c = -(ix * mx + iy * my + iz * mz);
+    c = fmin(1.0, fmax(-1.0, c));
+    return make_float2((float)c, __builtin_amdgcn_sqrtf((float)((1.0 - c) * (1.0 + c))));
 }
 
 // (I.u, I.v) in FP64.  Needed for a general (non-orthonormal) facet frame,
