@@ -59,6 +59,24 @@ template <class R> constexpr R PI_F() { return (R)3.14159265f; }
  * 411; file_io.cu:196-205). */
 constexpr double PI_5 = 3.14159;
 
+/* Planted mistakes for the pin tests (tests/test_oracle_closed_forms.py):
+ * 0 = none, the oracle as it is.  Each other value swaps one detail of the
+ * restatement for a plausible slip, so that a test can show that its pin
+ * catches that slip.  Set only through erp_oracle_set_mutation. */
+enum {
+    MUT_NONE = 0,
+    MUT_FACET_G_PI_ONCE = 1,    /* facet_G's sinc argument without the 2nd pi */
+    MUT_RADAREQ_NO_V = 2,       /* radarEq without the coherent-sum factor V */
+    MUT_RADAREQ_4PI_CUBED = 3,  /* radarEq over (4 pi)^3 as its comment reads */
+    MUT_TPH_SWAPPED = 4,        /* Tph = atan2(I.v, I.u): u and v exchanged */
+    MUT_TARGETTH_FACET_N = 5,   /* TargetTh against the facet's normal */
+    MUT_INCOHERENT_HALF = 6,    /* incoherent phases drawn from [0, pi) */
+    MUT_FACETRERAD_PI_TWICE = 7,/* facetReradiation's sinc argument times pi */
+    MUT_REFR_INDEX_DROPPED = 8, /* target range Itd + Ttd instead of Itd + n Ttd */
+    MUT_NO_TRANSMISSION = 9,    /* target power without the (1 - rho^2) factor */
+};
+static int g_mut = MUT_NONE;
+
 /* ------------------------------------------------------------------ math */
 
 /* elementary_kernels.cu:50-53: sinc with the double M_PI. */
@@ -76,6 +94,7 @@ template <class R> inline R sinc_ref(R x) {
 /* radar_kernels.cu:658-677 (pi appears twice: here and inside sinc). */
 template <class R> inline R facet_G(R th, R ph, R lam, R fs) {
     R piLlam = (PI_F<R>() * fs) / lam;
+    if (g_mut == MUT_FACET_G_PI_ONCE) piLlam = fs / lam;
     R s1 = sinc_ref<R>(piLlam * std::sin(th) * std::cos(ph));
     R s2 = sinc_ref<R>(piLlam * std::sin(th) * std::sin(ph));
     R d = s1 * s2;
@@ -97,6 +116,8 @@ template <class R> inline R radarEq(R P, R G, R fs, R lam, R dist, int64_t nf) {
     R num = P * G * G * fs * fs * fs * fs;
     double den = std::pow(4 * PI_5, 2) * (double)dist * (double)dist *
                  (double)dist * (double)dist;
+    if (g_mut == MUT_RADAREQ_NO_V) nf = 1;
+    if (g_mut == MUT_RADAREQ_4PI_CUBED) den *= 4 * PI_5;
     return (R)((double)nf * ((double)num / den));
 }
 
@@ -104,8 +125,9 @@ template <class R> inline R radarEq(R P, R G, R fs, R lam, R dist, int64_t nf) {
  * by the reference; only the squared sinc product is returned). */
 template <class R> inline R facetReradiation(R dist, R th, R ph, R lam, R fs) {
     (void)dist;
-    R s1 = sinc_ref<R>((fs / lam) * std::sin(th) * std::cos(ph));
-    R s2 = sinc_ref<R>((fs / lam) * std::sin(th) * std::sin(ph));
+    const R a = (g_mut == MUT_FACETRERAD_PI_TWICE) ? PI_F<R>() * fs / lam : fs / lam;
+    R s1 = sinc_ref<R>(a * std::sin(th) * std::cos(ph));
+    R s2 = sinc_ref<R>(a * std::sin(th) * std::sin(ph));
     R v = (fs * fs / lam) * s1 * s2;
     return v * v;
 }
@@ -587,10 +609,18 @@ inline PairRec<R> pair_record(const Par<R>& p, const MeshView& m,
     }
     R c1 = (-Ix) * (R)m.nx[i] + (-Iy) * (R)m.ny[i] + (-Iz) * (R)m.nz[i];
     q.Tth = std::acos(std::max((R)-1, std::min((R)1, c1)));
+    if (g_mut == MUT_TARGETTH_FACET_N) {
+        tnx = (R)m.nx[i];
+        tny = (R)m.ny[i];
+        tnz = (R)m.nz[i];
+    }
     R c2 = (-Ix) * tnx + (-Iy) * tny + (-Iz) * tnz;
     q.TargetTh = std::acos(std::max((R)-1, std::min((R)1, c2)));
     q.Tph = std::atan2(Ix * (R)m.ux[i] + Iy * (R)m.uy[i] + Iz * (R)m.uz[i],
                        Ix * (R)m.vx[i] + Iy * (R)m.vy[i] + Iz * (R)m.vz[i]);
+    if (g_mut == MUT_TPH_SWAPPED)
+        q.Tph = std::atan2(Ix * (R)m.vx[i] + Iy * (R)m.vy[i] + Iz * (R)m.vz[i],
+                           Ix * (R)m.ux[i] + Iy * (R)m.uy[i] + Iz * (R)m.uz[i]);
     R expo = 0, tot = 0;
     for (int64_t k = 0; k < np; ++k) {
         R d = prismIntersectionDistance<R>((R)m.x[i], (R)m.y[i], (R)m.z[i], Ix,
@@ -609,7 +639,7 @@ inline PairRec<R> pair_record(const Par<R>& p, const MeshView& m,
         R dth = f.Rth - q.Tth, dph = f.Iph - q.Tph;
         q.EI *= facetReradiation<R>(q.Ttd, dth, dph, p.lam, p.fs);
     }
-    q.EI = q.EI * f.fRfrC;
+    if (g_mut != MUT_NO_TRANSMISSION) q.EI = q.EI * f.fRfrC;
     R ct = p.ks * std::cos(q.Tth);
     R rough = std::exp((R)-4 * (ct * ct));
     q.EI = q.EI * rough;
@@ -618,7 +648,7 @@ inline PairRec<R> pair_record(const Par<R>& p, const MeshView& m,
         R RrTh = snellsLaw<R>(PI_F<R>() - q.Tth, p.eps_2, p.eps_1);
         q.EO *= facetReradiation<R>(f.Itd, RrTh, q.Tph - f.Iph, p.lam, p.fs);
     }
-    q.EO = q.EO * f.fRfrC;
+    if (g_mut != MUT_NO_TRANSMISSION) q.EO = q.EO * f.fRfrC;
     q.EO = q.EO * rough;
     return q;
 }
@@ -688,7 +718,7 @@ void target_one(const Par<R>& p, const MeshView& m, const std::vector<FacetRec<R
             R Gd = facet_G<R>(q[k].TargetTh, q[k].Tph, p.lam, p.fs);
             R G1 = facet_G<R>(f.Ith, f.Iph, p.lam, p.fs);
             R G2 = facet_G<R>(q[k].Tth, q[k].Tph, p.lam, p.fs);
-            R rngt = f.Itd + q[k].Ttd * n;
+            R rngt = f.Itd + q[k].Ttd * (g_mut == MUT_REFR_INDEX_DROPPED ? (R)1 : n);
             int b;
             R wgt;
             if (!bin_of<R>(rngt, rst, dr, nr, &b, &wgt)) continue;
@@ -810,6 +840,7 @@ int run_shot(const Par<R>& p, const MeshView& m, const float* tgt,
             cx<R> ph;
             if (p.incoherent) {
                 double u = incoherent_u(p.seed, is, f.idx);
+                if (g_mut == MUT_INCOHERENT_HALF) u *= 0.5;
                 R a = (R)(2.0 * M_PI * u);
                 ph = cx<R>(std::cos(a), std::sin(a));
             } else {
@@ -1040,6 +1071,12 @@ extern "C" int erp_oracle_target_pairs(const erp_params* par, const float* soa, 
                                  nprisms, out);
     return pairs_impl<double>(par, soa, F, tgt, ttype, T, it, shot, prisms,
                               nprisms, out);
+}
+
+extern "C" int erp_oracle_set_mutation(int m) {
+    const int prev = g_mut;
+    g_mut = m;
+    return prev;
 }
 
 extern "C" int erp_oracle_dft(const double* in, int64_t n, int inverse, double* out) {

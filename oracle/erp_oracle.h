@@ -58,6 +58,12 @@ int erp_oracle_conv(int kind, const double* x, const double* y, int nr, double d
                     double res, double rst, double smpl, double Bw, int precision,
                     double* out);
 
+/* Planted mistakes for the pin tests: 0 restores the oracle; any other
+ * value swaps one detail for a plausible slip (see the MUT_* list in
+ * erp_oracle.cpp).  Returns the previous value.  Not thread-safe: set it
+ * only between runs. */
+int erp_oracle_set_mutation(int m);
+
 /* Version string of the oracle build. */
 const char* erp_oracle_version(void);
 

@@ -45,6 +45,8 @@ def lib():
         L.erp_oracle_conv.argtypes = [i32, vp, vp, i32, d, d, d, d, d, i32, vp]
         L.erp_oracle_conv.restype = i32
         L.erp_oracle_version.restype = ctypes.c_char_p
+        L.erp_oracle_set_mutation.argtypes = [i32]
+        L.erp_oracle_set_mutation.restype = i32
         _lib = L
     return _lib
 
@@ -139,3 +141,30 @@ def conv(kind, x, y=None, nr=None, dr=1.0, res=1.0, rst=0.0, smpl=1.0, B=1.0, pr
     if rc:
         raise RuntimeError(rc)
     return o[:, 0] + 1j * o[:, 1]
+
+
+# planted mistakes (erp_oracle.cpp MUT_*): for tests that show a pin catches them
+MUT_FACET_G_PI_ONCE = 1
+MUT_RADAREQ_NO_V = 2
+MUT_RADAREQ_4PI_CUBED = 3
+MUT_TPH_SWAPPED = 4
+MUT_TARGETTH_FACET_N = 5
+MUT_INCOHERENT_HALF = 6
+MUT_FACETRERAD_PI_TWICE = 7
+MUT_REFR_INDEX_DROPPED = 8
+MUT_NO_TRANSMISSION = 9
+
+
+class mutation:
+    """Context manager: run the oracle with one planted mistake."""
+
+    def __init__(self, m: int):
+        self.m = m
+
+    def __enter__(self):
+        self.prev = lib().erp_oracle_set_mutation(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        lib().erp_oracle_set_mutation(self.prev)
+        return False
