@@ -24,7 +24,7 @@ EXPORTS = ("erp_ctx_create", "erp_ctx_destroy", "erp_last_error", "erp_mesh_uplo
            "erp_run_dev", "erp_submit", "erp_poll", "erp_wait", "erp_run_footprint",
            "erp_profile_enable", "erp_profile_read", "erp_serve_slots",
            "erp_mesh_from_heights", "erp_mesh_download", "erp_focus", "erp_set_tuning",
-           "erp_get_tuning")
+           "erp_get_tuning", "erp_run_dev_noalloc")
 
 STAGES = ("mask", "surface", "surface_conv", "target_select", "target_accum", "target_fft",
           "shot_final", "other", "target_sum")
@@ -75,6 +75,8 @@ def lib() -> ctypes.CDLL:
         L.erp_run.restype = i32
         L.erp_run_dev.argtypes = run_args
         L.erp_run_dev.restype = i32
+        L.erp_run_dev_noalloc.argtypes = run_args
+        L.erp_run_dev_noalloc.restype = i32
         L.erp_submit.argtypes = run_args + [P(vp)]
         L.erp_submit.restype = i32
         L.erp_poll.argtypes = [vp]
@@ -254,6 +256,18 @@ class Context:
                                       ctypes.c_void_p(out_ptr),
                                       ctypes.c_void_p(nvalid_ptr) if nvalid_ptr else None,
                                       flags))
+
+    def run_dev_noalloc(self, params: ErpParams, mesh: Mesh, targets: Targets | None, shots,
+                        out_ptr: int, nvalid_ptr: int | None = None, prisms=None,
+                        flags: int = 0):
+        """erp_run_dev_noalloc: erp_run_dev with every work buffer carved from
+        the library's per-device scratch array (no device allocation)."""
+        shots, prisms, npr = self._args(params, shots, prisms)
+        self._check(lib().erp_run_dev_noalloc(
+            self.handle, ctypes.byref(params), mesh.handle,
+            targets.handle if targets is not None else None, _ptr(shots), shots.shape[0],
+            _ptr(prisms), npr, ctypes.c_void_p(out_ptr),
+            ctypes.c_void_p(nvalid_ptr) if nvalid_ptr else None, flags))
 
     def submit(self, params: ErpParams, mesh: Mesh, targets: Targets | None, shots,
                prisms=None, flags: int = 0) -> Job:

@@ -13,6 +13,9 @@
 #include <exception>
 #include <new>
 
+// The scratch of erp_run_dev_noalloc (see scratch_init below).
+static __device__ __align__(256) unsigned char scratch[ERP_SCRATCH_BYTES];
+
 namespace {
 
 thread_local std::string g_null_ctx_error;
@@ -236,6 +239,34 @@ void stop_slots(erp_ctx* ctx) {
 constexpr int kDefaultSlots = 4;
 constexpr int kMaxSlots = 16;
 
+// ---------------------------------------------------------------------------
+// scratch of erp_run_dev_noalloc
+// ---------------------------------------------------------------------------
+// Each device holds its own copy of `scratch` from the moment the library's
+// code object is loaded on it until the process ends.  In host code its name
+// is not a device address: the address of the current device's copy is
+// looked up once per device with hipGetSymbolAddress (at the first context
+// creation on the device, that device current), and only that pointer is
+// handed to kernels and copies.
+constexpr int kScratchDevices = 64;
+struct ScratchDev {
+    void* ptr = nullptr;  // this device's copy of `scratch`
+    std::mutex run_mu;    // one allocation-free run at a time per device
+};
+ScratchDev g_scratch[kScratchDevices];
+std::mutex g_scratch_init_mu;
+
+// With `device` current: look up its copy of the array (once).
+void scratch_init(int device) {
+    if (device < 0 || device >= kScratchDevices) return;  // the entry point reports it
+    std::lock_guard<std::mutex> lk(g_scratch_init_mu);
+    if (g_scratch[device].ptr) return;
+    void* p = nullptr;
+    ERP_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw ErpError{ERP_E_HIP, "hipGetSymbolAddress(scratch) returned NULL"};
+    g_scratch[device].ptr = p;
+}
+
 }  // namespace
 
 
@@ -258,6 +289,7 @@ int erp_ctx_create(int device, erp_ctx** out) {
         ERP_HIP_TRY(hipGetDeviceProperties(&prop, device));
         ctx->num_cu = prop.multiProcessorCount;
         fft_init_twiddles(ctx);
+        scratch_init(device);
         ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
     });
     if (rc != ERP_OK) {
@@ -477,6 +509,39 @@ int erp_run_dev(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                      (int32_t*)nvalid_dev, flags, ctx->stream, ctx->work, &ctx->prof);
         ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
         ctx->prof.resolve();
+    });
+}
+
+int erp_run_dev_noalloc(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
+                        const erp_targets* targets, const erp_shot* shots, int64_t ns,
+                        const erp_prism* prisms, int64_t nprisms, void* out_dev,
+                        void* nvalid_dev, uint32_t flags) {
+    if (!ctx) return ERP_E_INVALID;
+    return guarded(ctx, [&] {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        validate_params(par);
+        if (!mesh || !shots || ns < 0 || !out_dev || nprisms < 0 || (nprisms > 0 && !prisms))
+            throw ErpError{ERP_E_INVALID, "invalid argument"};
+        if (mesh->ctx != ctx || (targets && targets->ctx != ctx))
+            throw ErpError{ERP_E_INVALID, "mesh/targets belong to another context"};
+        if (ctx->device >= kScratchDevices || !g_scratch[ctx->device].ptr)
+            throw ErpError{ERP_E_INVALID, "no allocation-free scratch on device " +
+                                              std::to_string(ctx->device)};
+        set_device(ctx);
+        std::lock_guard<std::mutex> sl(g_scratch[ctx->device].run_mu);
+        Arena a;
+        a.base = static_cast<char*>(g_scratch[ctx->device].ptr);
+        a.cap = (size_t)ERP_SCRATCH_BYTES;
+        WorkBufs w;  // every buffer is lent from the arena; nothing to release
+        try {
+            erp_run_impl(ctx, par, mesh, targets, shots, ns, prisms, nprisms, (float2*)out_dev,
+                         (int32_t*)nvalid_dev, flags, ctx->stream, w, nullptr, &a);
+            ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            // the scratch is free for the next call only once queued work ended
+            hipStreamSynchronize(ctx->stream);
+            throw;
+        }
     });
 }
 

@@ -807,11 +807,17 @@ FftPlan fft_plan(int N) {
     return p;
 }
 
+static const float2* inter_table(erp_ctx* ctx, const FftPlan& p, hipStream_t s);
+
+// The ladder table and the inter-pass tables of every two-pass size a run
+// can use (up to the 2^17-point Bluestein transforms of nr = 32767): built
+// once per context, so that no run allocates a table (1.9 MB in all).
 void fft_init_twiddles(erp_ctx* ctx) {
     ctx->twiddle.ensure(sizeof(float2) * (size_t)LADN);
     hipLaunchKernelGGL(k_twiddle_init, dim3(LADN / 256), dim3(256), 0, ctx->stream,
                        ctx->twiddle.as<float2>());
     ERP_CHECK_LAUNCH("k_twiddle_init");
+    for (int n = 2 * NPTS; n <= (1 << 17); n *= 2) inter_table(ctx, fft_plan(n), ctx->stream);
 }
 
 // inter-pass twiddle table of a two-pass plan (cached per size in the context)

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <algorithm>
 #include <string>
 #include <vector>
 #include <atomic>
@@ -55,8 +56,20 @@ struct ErpError {
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    // a region of memory the buffer does not own (see Arena): never freed,
+    // never grown; asking for more than it holds is an error
+    bool borrowed = false;
+    void borrow(void* q, size_t n) {
+        p = q;
+        bytes = n;
+        borrowed = true;
+    }
     void ensure(size_t n) {
         if (n <= bytes) return;
+        if (borrowed)
+            throw ErpError{ERP_E_NOMEM, "a work buffer needs " + std::to_string(n) +
+                                            " bytes, its scratch region holds " +
+                                            std::to_string(bytes)};
         if (p) hipFree(p);
         p = nullptr;
         bytes = 0;
@@ -77,11 +90,38 @@ struct DevBuf {
         if (n > bytes) ensure(std::max(n, std::min(n + n / 4, cap)));
     }
     void release() {
-        if (p) hipFree(p);
+        if (p && !borrowed) hipFree(p);
         p = nullptr;
         bytes = 0;
+        borrowed = false;
     }
     template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Device memory of one run carved from a region the caller provides (the
+// allocation-free entry point): 256-byte aligned bump allocation, nothing is
+// freed before the run ends.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+    static size_t round(size_t n) { return (n + 255) & ~(size_t)255; }
+    size_t left() const { return cap - used; }
+    void* take(size_t n, const char* what) {
+        const size_t r = round(std::max<size_t>(n, 1));
+        if (r > left())
+            throw ErpError{ERP_E_NOMEM, std::string("scratch exhausted: ") + what + " needs " +
+                                            std::to_string(r) + " bytes, " +
+                                            std::to_string(left()) + " of " +
+                                            std::to_string(cap) + " left"};
+        void* p = base + used;
+        used += r;
+        return p;
+    }
+    template <class T> T* take_n(size_t n, const char* what) {
+        return static_cast<T*>(take(sizeof(T) * n, what));
+    }
+    void lend(DevBuf& b, size_t n, const char* what) { b.borrow(take(n, what), round(std::max<size_t>(n, 1))); }
 };
 
 struct erp_mesh {
@@ -241,12 +281,15 @@ void target_spectra_fused(erp_ctx* ctx, const FftPlan& p, float2* A, float2* Y,
 // ---------------------------------------------------------------------------
 // pipeline (erp_pipeline.hip)
 // ---------------------------------------------------------------------------
-// timer: stage timers to record into, or nullptr (asynchronous jobs)
+// timer: stage timers to record into, or nullptr (asynchronous jobs).
+// arena: nullptr, or the region every work buffer and table of the run is
+// carved from (then `work` must be empty and nothing is allocated; batch
+// shapes shrink to fit and a job that cannot fit fails with ERP_E_NOMEM).
 void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                   const erp_targets* targets, const erp_shot* shots, int64_t ns,
                   const erp_prism* prisms, int64_t nprisms, float2* out_dev,
                   int32_t* nvalid_dev, uint32_t flags, hipStream_t stream,
-                  WorkBufs& work, StageTimer* timer);
+                  WorkBufs& work, StageTimer* timer, Arena* arena = nullptr);
 int64_t erp_footprint_impl(const erp_params* par, int64_t F, int64_t T, int64_t ns);
 // focused rows row0 .. row0 + nrows - 1 of an Nr x Na radargram (erp_focus.hip)
 void focus_rows(const float2* rdr, int Nr, int Na, int row0, int nrows, const int* row_hs,

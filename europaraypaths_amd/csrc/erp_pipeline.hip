@@ -2120,6 +2120,18 @@ KPar make_kpar(const erp_params& p, int P_fft) {
     return k;
 }
 
+// Range-compression kernel spectrum of (nr, dr, res, kind, rst) into dst:
+// P complex64 in the order fft_forward produces, scaled by 1/P.
+void build_chirp(erp_ctx* ctx, float2* dst, int nr, float dr, float res, int kind, float rst,
+                 int P, const FftPlan& plan, hipStream_t st) {
+    hipLaunchKernelGGL(k_gen_kernel, dim3((P + 255) / 256), dim3(256), 0, st, dst, P, nr,
+                       (double)dr, (double)res, (double)rst, kind);
+    ERP_CHECK_LAUNCH("k_gen_kernel");
+    fft_forward(ctx, plan, dst, 1, st);
+    hipLaunchKernelGGL(k_scale, dim3((P + 255) / 256), dim3(256), 0, st, dst, P, 1.0f / (float)P);
+    ERP_CHECK_LAUNCH("k_scale");
+}
+
 // Range-compression kernel spectrum, cached per context.  natural = true
 // returns it in natural frequency order (the per-pair LDS chain), otherwise
 // in the order fft_forward produces.  Only kind-1 entries (one per receive
@@ -2135,14 +2147,7 @@ const float2* get_chirp(erp_ctx* ctx, int nr, float dr, float res, int kind, flo
         c = new ChirpSpec();
         c->key = key;
         c->spec.ensure(sizeof(float2) * (size_t)P);
-        hipLaunchKernelGGL(k_gen_kernel, dim3((P + 255) / 256), dim3(256), 0, st,
-                           c->spec.as<float2>(), P, nr, (double)dr, (double)res, (double)rst,
-                           kind);
-        ERP_CHECK_LAUNCH("k_gen_kernel");
-        fft_forward(ctx, plan, c->spec.as<float2>(), 1, st);
-        hipLaunchKernelGGL(k_scale, dim3((P + 255) / 256), dim3(256), 0, st,
-                           c->spec.as<float2>(), P, 1.0f / (float)P);
-        ERP_CHECK_LAUNCH("k_scale");
+        build_chirp(ctx, c->spec.as<float2>(), nr, dr, res, kind, rst, P, plan, st);
         // other streams may use the new spectrum as soon as the lock is released
         ERP_HIP_TRY(hipStreamSynchronize(st));
         int n1 = 0;
@@ -2168,6 +2173,19 @@ const float2* get_chirp(erp_ctx* ctx, int nr, float dr, float res, int kind, flo
     return c->nat.as<float2>();
 }
 
+// Bluestein tables of b (L, M, smpl, B set; w, Bw and mask sized) built on st
+void build_blue(erp_ctx* ctx, BlueTables* b, hipStream_t st) {
+    hipLaunchKernelGGL(k_blue_tables, dim3((b->M + 255) / 256), dim3(256), 0, st,
+                       b->w.as<float2>(), b->Bw.as<float2>(), b->mask.as<unsigned char>(), b->L,
+                       b->M, (double)b->smpl, (double)b->B);
+    ERP_CHECK_LAUNCH("k_blue_tables");
+    FftPlan pm = fft_plan(b->M);
+    fft_forward(ctx, pm, b->Bw.as<float2>(), 1, st);
+    hipLaunchKernelGGL(k_scale, dim3((b->M + 255) / 256), dim3(256), 0, st, b->Bw.as<float2>(),
+                       b->M, 1.0f / (float)b->M);
+    ERP_CHECK_LAUNCH("k_scale");
+}
+
 BlueTables* get_blue(erp_ctx* ctx, int L, float smpl, float B, hipStream_t st) {
     std::lock_guard<std::recursive_mutex> lk(ctx->cache_mu);
     for (BlueTables* b : ctx->blues)
@@ -2180,15 +2198,7 @@ BlueTables* get_blue(erp_ctx* ctx, int L, float smpl, float B, hipStream_t st) {
     b->w.ensure(sizeof(float2) * (size_t)L);
     b->Bw.ensure(sizeof(float2) * (size_t)b->M);
     b->mask.ensure((size_t)L);
-    hipLaunchKernelGGL(k_blue_tables, dim3((b->M + 255) / 256), dim3(256), 0, st,
-                       b->w.as<float2>(), b->Bw.as<float2>(), b->mask.as<unsigned char>(), L,
-                       b->M, (double)smpl, (double)B);
-    ERP_CHECK_LAUNCH("k_blue_tables");
-    FftPlan pm = fft_plan(b->M);
-    fft_forward(ctx, pm, b->Bw.as<float2>(), 1, st);
-    hipLaunchKernelGGL(k_scale, dim3((b->M + 255) / 256), dim3(256), 0, st, b->Bw.as<float2>(),
-                       b->M, 1.0f / (float)b->M);
-    ERP_CHECK_LAUNCH("k_scale");
+    build_blue(ctx, b, st);
     if (ctx->blues.size() > 8 && ctx->active.load() == 0) {
         ERP_HIP_TRY(hipStreamSynchronize(st));
         delete ctx->blues.front();
@@ -2326,7 +2336,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                   const erp_targets* targets, const erp_shot* shots, int64_t ns,
                   const erp_prism* prisms, int64_t nprisms, float2* out_dev,
                   int32_t* nvalid_dev, uint32_t flags, hipStream_t st, WorkBufs& W,
-                  StageTimer* timer) {
+                  StageTimer* timer, Arena* arena) {
     const erp_params& p = *par;
     const uint32_t mode = p.mode;
     const bool conv = mode & ERP_MODE_CONVOLUTION;
@@ -2400,20 +2410,46 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     const bool do_trace = !spec || (conv && lin);
     const float res_t = spec ? p.rng_res : 2.f * p.rng_res;  // sim.cu:853-858
 
-    // chirp spectra (cached per context)
+    // chirp spectra: cached per context, or (arena) built into the region
+    auto chirp = [&](float res, bool natural) -> const float2* {
+        if (!arena) return get_chirp(ctx, nr, p.dr, res, 0, 0.f, P, planP, st, natural);
+        float2* spec_ = arena->take_n<float2>((size_t)P, "chirp spectrum");
+        build_chirp(ctx, spec_, nr, p.dr, res, 0, 0.f, P, planP, st);
+        if (!natural) return spec_;
+        float2* nat = arena->take_n<float2>((size_t)P, "chirp spectrum");
+        fft_natural_order(planP, spec_, nat, st);
+        return nat;
+    };
     const float2* g_surf = nullptr;
-    if (!spec || (conv && lin)) g_surf = get_chirp(ctx, nr, p.dr, p.rng_res, 0, 0.f, P, planP, st);
+    if (!spec || (conv && lin)) g_surf = chirp(p.rng_res, false);
     const float2* g_targ = nullptr;
-    if (T > 0 && conv && lin) g_targ = get_chirp(ctx, nr, p.dr, res_t, 0, 0.f, P, planP, st);
+    if (T > 0 && conv && lin) g_targ = chirp(res_t, false);
     // plain mode: per-pair spectral chain in one workgroup when P = 2 nr fits LDS
     const bool half = !spec && T > 0 && conv && lin && target_half_supported(planP, nr);
     kp.row_live = half ? nr : P;
-    const float2* g_targ_nat =
-        half ? get_chirp(ctx, nr, p.dr, res_t, 0, 0.f, P, planP, st, true) : nullptr;
+    const float2* g_targ_nat = half ? chirp(res_t, true) : nullptr;
+    // circular convolution of type-0 targets: one genChirp table per rx window
+    // start; with an arena one table is rebuilt for each shot
+    float2* g_circ = (arena && any_t0 && conv && !lin)
+                         ? arena->take_n<float2>((size_t)P, "circular chirp spectrum")
+                         : nullptr;
     BlueTables* blue = nullptr;
+    BlueTables blue_local;
     FftPlan planM;
     if (any_typed) {
-        blue = get_blue(ctx, L, p.smpl, p.B, st);
+        if (arena) {
+            blue = &blue_local;
+            blue->L = L;
+            blue->M = next_pow2(2 * (int64_t)L - 1);
+            blue->smpl = p.smpl;
+            blue->B = p.B;
+            arena->lend(blue->w, sizeof(float2) * (size_t)L, "Bluestein chirp");
+            arena->lend(blue->Bw, sizeof(float2) * (size_t)blue->M, "Bluestein spectrum");
+            arena->lend(blue->mask, (size_t)L, "band mask");
+            build_blue(ctx, blue, st);
+        } else {
+            blue = get_blue(ctx, L, p.smpl, p.B, st);
+        }
         planM = fft_plan(blue->M);
     }
 
@@ -2422,6 +2458,85 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     if (ctx->tune_shots > 0) Smax = (int)std::min<int64_t>(Smax, ctx->tune_shots);
     if ((int64_t)P * 8 * 4 * Smax > (int64_t)1 << 30) Smax = std::max<int>(1, (int)(((int64_t)1 << 30) / ((int64_t)P * 32)));
     const int64_t Tpad = targets ? targets->Tpad : 0;
+
+    // (shot, target) pairs per target launch and the per-pair buffer layout
+    const bool fused = !spec && !half && (!planP.two_pass || nr % planP.N1 == 0);
+    // pairs per multiply-accumulate group (fused) or per summed slice (half)
+    const int GS = half ? target_half_slice() : 16;
+    const int nbuf = half ? 2 : (fused ? 3 : (spec ? 4 : 3));
+    int C = (int)std::max<int64_t>(1, kPairRowBytes / ((int64_t)P * 8 * nbuf));
+    C = std::min<int>(C, kMaxPairsPerLaunch);
+    if (ctx->tune_pairs > 0) C = (int)std::min<int64_t>(C, ctx->tune_pairs);
+
+    // bytes of the facet records of cap illuminated facets (only the layouts
+    // the run reads; each array 256-byte aligned, as the batch loop carves them)
+    auto rec_need = [&](size_t cap) -> size_t {
+        size_t b = Arena::round(cap * 8);
+        if (spec || any_t0)
+            b += Arena::round(cap * 48) + 2 * Arena::round(cap * 4) + Arena::round(cap * 12);
+        if (!spec && any_typed) b += 3 * Arena::round(cap * 16) + Arena::round(cap * 4);
+        return b;
+    };
+    // Allocation-free run: every batch buffer is lent from the arena before
+    // the loop (the loop's ensure calls then find it large enough).  Shots per
+    // batch and pairs per launch shrink until the fixed buffers take at most
+    // three quarters of what is left; the facet records get the rest.
+    if (arena) {
+        auto layout = [&](int S_, int C_, bool lend) -> size_t {
+            size_t tot = 0;
+            auto buf = [&](DevBuf& b, size_t n, const char* what) {
+                tot += Arena::round(std::max<size_t>(n, 1));
+                if (lend) arena->lend(b, n, what);
+            };
+            const size_t s = (size_t)S_;
+            buf(W.shots, sizeof(erp_shot) * s, "shot table");
+            if (nprisms > 0) buf(W.prisms, sizeof(erp_prism) * (size_t)nprisms, "prism table");
+            buf(W.cnt, sizeof(int) * s * nblk, "block counts");
+            buf(W.off, sizeof(int) * s * nblk, "block offsets");
+            buf(W.V, sizeof(int) * s, "illuminated counts");
+            buf(W.Vd, sizeof(unsigned long long) * s * nblk * MASKW, "aperture mask");
+            buf(W.bounds, sizeof(Bounds) * s, "shot bounds");
+            buf(W.tcount, sizeof(int64_t) * s, "record bases");
+            buf(W.psurf, sizeof(float2) * s * P, "surface traces");
+            buf(W.srow, sizeof(float2) * (s * nr * 2 + (s + 1) / 2), "surface bins");
+            buf(W.refl, sizeof(float2) * s * nr, "reflected traces");
+            buf(W.out, sizeof(float2) * s * nr * 2, "refracted traces");
+            if (any_typed) {
+                buf(W.shotS, sizeof(float2) * s * P, "spectral sums");
+                buf(W.blu, sizeof(float2) * s * planM.N, "Bluestein rows");
+            }
+            if (any_t0) buf(W.t0trace, sizeof(float2) * s * P, "type-0 traces");
+            if (T > 0) {
+                buf(W.tlist, sizeof(int) * s * (size_t)T * 2, "target lists");
+                buf(W.tcnt, sizeof(int) * s * 2, "target counts");
+            }
+            size_t pb = 0;
+            if (any_typed) {
+                const size_t gc = (size_t)C_ / GS + s + 2;
+                buf(W.tA, sizeof(float2) * (size_t)P * ((size_t)C_ * (nbuf - 1) + (half ? gc : 0)),
+                    "pair rows");
+                buf(W.tC, sizeof(float2) * (size_t)C_ * P, "pair products");
+                pb = sizeof(int) * ((size_t)C_ * 4 + 2 * gc + 3 * s + 16);
+            }
+            if (any_t0) pb = std::max(pb, sizeof(int) * 2 * s * (size_t)T + 64);
+            if (pb) buf(W.pairs, pb, "pair lists");
+            return tot;
+        };
+        const size_t avail = arena->left();
+        while (layout(Smax, C, false) > avail / 4 * 3) {
+            if (any_typed && C > 256) C /= 2;
+            else if (Smax > 1) Smax = (Smax + 1) / 2;
+            else if (any_typed && C > 1) C /= 2;
+            else
+                throw ErpError{ERP_E_NOMEM,
+                               "job too large for the allocation-free entry point: one shot's "
+                               "work buffers need " + std::to_string(layout(1, 1, false)) +
+                                   " bytes, the scratch has " + std::to_string(avail) +
+                                   " left after the tables"};
+        }
+        layout(Smax, C, true);
+        if (need_rec) arena->lend(W.rec, arena->left(), "facet records");
+    }
 
     W.shots.ensure(sizeof(erp_shot) * (size_t)Smax);
     if (nprisms > 0) {
@@ -2461,8 +2576,11 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     float2* refr = W.out.as<float2>();
     float2* refr0 = refr + (size_t)Smax * nr;
 
-    for (int64_t s0 = 0; s0 < ns; s0 += Smax) {
-        const int S = (int)std::min<int64_t>(Smax, ns - s0);
+    // S: shots of the current batch (an allocation-free run may take fewer
+    // than Smax when their facet records do not fit the region)
+    int S = 0;
+    for (int64_t s0 = 0; s0 < ns; s0 += S) {
+        S = (int)std::min<int64_t>(Smax, ns - s0);
         ERP_HIP_TRY(hipMemcpyAsync(W.shots.p, shots + s0, sizeof(erp_shot) * S,
                                    hipMemcpyHostToDevice, st));
         const erp_shot* dshots = W.shots.as<erp_shot>();
@@ -2481,6 +2599,21 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         PT_.end(st);
         ERP_HIP_TRY(hipMemcpyAsync(hV.data(), dV, sizeof(int) * S, hipMemcpyDeviceToHost, st));
         ERP_HIP_TRY(hipStreamSynchronize(st));
+        if (arena && need_rec) {
+            // keep the leading shots whose records fit the region; the others
+            // are masked again as the next batch
+            int fit = 0;
+            size_t acc = 0;
+            while (fit < S && rec_need(acc + (size_t)hV[fit]) <= W.rec.bytes) acc += hV[fit++];
+            if (fit == 0)
+                throw ErpError{ERP_E_NOMEM,
+                               "shot " + std::to_string(s0) + " illuminates " +
+                                   std::to_string(hV[0]) + " facets: their records need " +
+                                   std::to_string(rec_need((size_t)hV[0])) +
+                                   " bytes, the allocation-free scratch has " +
+                                   std::to_string(W.rec.bytes) + " for records"};
+            S = fit;
+        }
         hbase[0] = 0;
         for (int s = 0; s < S; ++s) hbase[s + 1] = hbase[s] + (need_rec ? hV[s] : 0);
         // the target kernel reads a shot's records at 32-bit byte offsets
@@ -2496,8 +2629,9 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
             // at most every facet of every shot of the batch (the bound that
             // erp_run_footprint counts)
             const size_t rec_unit = 12 * 4 + 8 + 4 + 4 + 3 * 4 + 52;
-            const size_t bytes = cap * rec_unit + 256 * 10;
-            W.rec.ensure_grow(bytes, (size_t)S * (size_t)F * rec_unit + 256 * 10);
+            const size_t bytes = rec_need(cap);
+            if (arena) W.rec.ensure(bytes);  // fits: the batch was cut to fit
+            else W.rec.ensure_grow(bytes, (size_t)S * (size_t)F * rec_unit + 256 * 10);
             char* b = W.rec.as<char>();
             auto take = [&](size_t n) { char* r = b; b += (n + 255) & ~(size_t)255; return r; };
             rec.itd = (double*)take(cap * 8);
@@ -2603,14 +2737,6 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
             if (npairs > 0) {
                 float2* Sacc = W.shotS.as<float2>();
                 zero(Sacc, (size_t)S * P, st);
-                // plain mode with a suitable size: fused LDS spectral chain
-                const bool fused = !spec && !half && (!planP.two_pass || nr % planP.N1 == 0);
-                // pairs per multiply-accumulate group (fused) or per summed slice (half)
-                const int GS = half ? target_half_slice() : 16;
-                const int nbuf = half ? 2 : (fused ? 3 : (spec ? 4 : 3));
-                int C = (int)std::max<int64_t>(1, kPairRowBytes / ((int64_t)P * 8 * nbuf));
-                C = std::min<int>(C, kMaxPairsPerLaunch);
-                if (ctx->tune_pairs > 0) C = (int)std::min<int64_t>(C, ctx->tune_pairs);
                 const size_t gcap = (size_t)C / GS + S + 2;
                 // half: A rows, then one partial row per slice
                 W.tA.ensure(sizeof(float2) * (size_t)P *
@@ -2822,8 +2948,13 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                         // circular: genChirp table depends on rst (per shot)
                         std::vector<erp_shot> hs(shots + s0, shots + s0 + S);
                         for (int s = 0; s < S; ++s) {
-                            const float2* gc = get_chirp(ctx, nr, p.dr, p.rng_res, 1, hs[s].rst, P,
-                                                         planP, st);
+                            const float2* gc = g_circ;
+                            if (g_circ)  // rebuilt in place: stream order keeps the last use
+                                build_chirp(ctx, g_circ, nr, p.dr, p.rng_res, 1, hs[s].rst, P,
+                                            planP, st);
+                            else
+                                gc = get_chirp(ctx, nr, p.dr, p.rng_res, 1, hs[s].rst, P, planP,
+                                               st);
                             window_conv(ctx, planP, t0 + (size_t)s * P, 1, gc,
                                         refr0 + (size_t)s * nr, nr, st);
                         }

@@ -186,6 +186,25 @@ int erp_run_dev(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                 const erp_prism* prisms, int64_t nprisms,
                 void* out_dev, void* nvalid_dev, uint32_t flags);
 
+/* Allocation-free form of erp_run_dev.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory while they run a job
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as erp_run_dev, but every work
+ * buffer and table of the run is carved from a per-device scratch array of
+ * ERP_SCRATCH_BYTES that the library holds from the first context on that
+ * device until the process ends; the call allocates nothing.  Shots per
+ * batch and pairs per target launch shrink until the buffers fit (slower
+ * than erp_run_dev for large jobs); a job of which one shot cannot fit
+ * returns ERP_E_NOMEM with the sizes in erp_last_error.  All calls on one
+ * device share the array and would race if they overlapped, so the library
+ * serialises them (a call waits for the previous one to finish); nothing of
+ * one call's scratch contents is used by the next.  No stage timers. */
+#define ERP_SCRATCH_BYTES ((int64_t)256 << 20)
+int erp_run_dev_noalloc(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
+                        const erp_targets* targets, const erp_shot* shots, int64_t ns,
+                        const erp_prism* prisms, int64_t nprisms,
+                        void* out_dev, void* nvalid_dev, uint32_t flags);
+
 /* Asynchronous form of erp_run: queue a job and return at once.  Jobs run
  * in worker slots (erp_serve_slots; 4 are started at the first submit), each
  * a host thread with its own stream and work buffers, so jobs submitted
