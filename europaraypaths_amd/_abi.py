@@ -36,7 +36,8 @@ class ErpParams(ctypes.Structure):
     _fields_ = ([(n, ctypes.c_float) for n in _PARAM_FLOATS] +
                 [("nr", ctypes.c_int32), ("rerad_funct", ctypes.c_int32),
                  ("pol", ctypes.c_int32), ("mode", ctypes.c_uint32),
-                 ("cos_ap", ctypes.c_double), ("seed", ctypes.c_uint64)])
+                 ("cos_ap", ctypes.c_double), ("seed", ctypes.c_uint64),
+                 ("eps_3", ctypes.c_float), ("z_int", ctypes.c_float)])
 
 
 class ErpShot(ctypes.Structure):
@@ -60,4 +61,4 @@ PRISM_DTYPE = np.dtype([(n, "<f4") for n in
 
 assert SHOT_DTYPE.itemsize == ctypes.sizeof(ErpShot) == 48
 assert PRISM_DTYPE.itemsize == ctypes.sizeof(ErpPrism) == 28
-assert ctypes.sizeof(ErpParams) == 112
+assert ctypes.sizeof(ErpParams) == 120

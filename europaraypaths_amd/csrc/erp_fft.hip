@@ -663,16 +663,24 @@ __device__ __forceinline__ void load_window(float2 (&x)[PPT], const float2* __re
     }
 }
 
-template <int NTB>
+template <int NTB, bool MT = false>
 __global__ void __launch_bounds__(NTB) k_tpair_half(const float2* __restrict__ A, int P,
                                                     const int* __restrict__ pbase,
                                                     const int* __restrict__ pwin,
                                                     const float4* __restrict__ Gn,
                                                     float2* __restrict__ out,
-                                                    const float2* __restrict__ lad) {
+                                                    const float2* __restrict__ lad,
+                                                    const float4* const* __restrict__ Gpair,
+                                                    const int* __restrict__ npairs) {
     constexpr int n = NTB * PPT;
     __shared__ float2 s[n + n / 16];
     const int p = blockIdx.x;
+    // a multi-track launch: a grid of the pair bound, the count on the
+    // device, and each pair's own range kernel spectrum
+    if (MT) {
+        if (p >= *npairs) return;
+        Gn = Gpair[p];
+    }
     const int base = pbase[p], win = pwin[p];
     const float2* a = A + (size_t)p * P;
     const float2* mod = lad + 2 * n;  // m(j) = lad[2n + j]
@@ -736,6 +744,7 @@ __global__ void __launch_bounds__(256) k_sum_slices(float4* __restrict__ partial
     const int k = blockIdx.x * 256 + threadIdx.x;  // frequency pair (2k, 2k+1), P2 = P / 2
     if (k >= P2) return;
     const int b = sl_beg[blockIdx.y], e = sl_end[blockIdx.y];
+    if (b == e) return;  // padding slice of a replayed launch: never read
     const size_t P = 2 * (size_t)P2;
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     int r = b;
@@ -765,10 +774,13 @@ __global__ void __launch_bounds__(256) k_sum_final(float2* __restrict__ S,
                                                    const int* __restrict__ seg_shot,
                                                    const int* __restrict__ seg_beg,
                                                    const int* __restrict__ seg_end, int P,
-                                                   float scale, int pN1, int pN2) {
+                                                   float scale, int pN1, int pN2,
+                                                   const float* __restrict__ seg_scale) {
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= P) return;
     const int sg = blockIdx.y;
+    if (seg_beg[sg] == seg_end[sg]) return;  // padding segment
+    if (seg_scale) scale = seg_scale[sg];   // segments of several tracks
     float2 acc = make_float2(0.f, 0.f);
     for (int r = seg_beg[sg]; r < seg_end[sg]; ++r) {
         const float2 v = partial[(size_t)r * P + k];
@@ -937,34 +949,26 @@ void fft_natural_order(const FftPlan& p, const float2* src, float2* dst, hipStre
 // pairs of A (rows of P) -> per-pair products in work (even/odd planes).
 void target_pairs_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const int* pbase,
                        const int* pwin, int nr, int n, const float2* Gnat, float2* work,
-                       hipStream_t st) {
+                       hipStream_t st, const float4* const* Gpair, const int* npairs_dev) {
     if (!target_half_supported(p, nr))
         throw ErpError{ERP_E_INVALID,
                        "per-pair LDS chain needs P = 2 nr, nr a power of two in [1024, 16384]"};
     const float2* lad = ctx->twiddle.as<float2>();
     const float4* G4 = reinterpret_cast<const float4*>(Gnat);
     float2* o4 = work;
+    // n: the pairs, or (multi-track, Gpair set) the bound of a grid whose
+    // pairs past *npairs_dev exit at once
+    auto go = [&](auto kern, int nt) {
+        hipLaunchKernelGGL(kern, dim3(n), dim3(nt), 0, st, A, p.N, pbase, pwin, G4, o4, lad,
+                           Gpair, npairs_dev);
+    };
+    const bool mt = Gpair != nullptr;
     switch (nr) {
-        case 1024:
-            hipLaunchKernelGGL(k_tpair_half<64>, dim3(n), dim3(64), 0, st, A, p.N, pbase, pwin,
-                               G4, o4, lad);
-            break;
-        case 2048:
-            hipLaunchKernelGGL(k_tpair_half<128>, dim3(n), dim3(128), 0, st, A, p.N, pbase, pwin,
-                               G4, o4, lad);
-            break;
-        case 4096:
-            hipLaunchKernelGGL(k_tpair_half<256>, dim3(n), dim3(256), 0, st, A, p.N, pbase, pwin,
-                               G4, o4, lad);
-            break;
-        case 8192:
-            hipLaunchKernelGGL(k_tpair_half<512>, dim3(n), dim3(512), 0, st, A, p.N, pbase, pwin,
-                               G4, o4, lad);
-            break;
-        default:
-            hipLaunchKernelGGL(k_tpair_half<1024>, dim3(n), dim3(1024), 0, st, A, p.N, pbase,
-                               pwin, G4, o4, lad);
-            break;
+        case 1024: mt ? go(k_tpair_half<64, true>, 64) : go(k_tpair_half<64>, 64); break;
+        case 2048: mt ? go(k_tpair_half<128, true>, 128) : go(k_tpair_half<128>, 128); break;
+        case 4096: mt ? go(k_tpair_half<256, true>, 256) : go(k_tpair_half<256>, 256); break;
+        case 8192: mt ? go(k_tpair_half<512, true>, 512) : go(k_tpair_half<512>, 512); break;
+        default: mt ? go(k_tpair_half<1024, true>, 1024) : go(k_tpair_half<1024>, 1024); break;
     }
     ERP_CHECK_LAUNCH("k_tpair_half");
 }
@@ -975,13 +979,14 @@ void target_pairs_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const in
 // shot into S, scaled and in the order fft_inverse expects.
 void target_sum_half(const FftPlan& p, const float2* work, float2* partial, const int* sl_beg,
                      const int* sl_end, int nslices, const int* seg_shot, const int* seg_beg,
-                     const int* seg_end, int nseg, float2* S, float scale, hipStream_t st) {
+                     const int* seg_end, int nseg, float2* S, float scale, hipStream_t st,
+                     const float* seg_scale) {
     const int P2 = p.N / 2;
     hipLaunchKernelGGL(k_sum_slices, dim3((P2 + 255) / 256, nslices), dim3(256), 0, st,
                        reinterpret_cast<float4*>(partial), work, sl_beg, sl_end, P2);
     ERP_CHECK_LAUNCH("k_sum_slices");
     hipLaunchKernelGGL(k_sum_final, dim3((p.N + 255) / 256, nseg), dim3(256), 0, st, S, partial,
                        seg_shot, seg_beg, seg_end, p.N, scale, p.two_pass ? p.N1 : 0,
-                       p.two_pass ? p.N2 : 0);
+                       p.two_pass ? p.N2 : 0, seg_scale);
     ERP_CHECK_LAUNCH("k_sum_final");
 }

@@ -265,12 +265,17 @@ void fft_init_twiddles(erp_ctx* ctx);
 // one workgroup per (shot, target) pair, whole chain in LDS (P = 2 nr)
 bool target_half_supported(const FftPlan& p, int nr);
 int target_half_slice();
+// Gpair / npairs_dev (multi-track batches): per-pair range kernel spectra
+// and the pair count on the device, n then being the grid's bound
 void target_pairs_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const int* pbase,
                        const int* pwin, int nr, int n, const float2* Gnat, float2* work,
-                       hipStream_t st);
+                       hipStream_t st, const float4* const* Gpair = nullptr,
+                       const int* npairs_dev = nullptr);
+// seg_scale (multi-track batches): per-segment scale instead of `scale`
 void target_sum_half(const FftPlan& p, const float2* work, float2* partial, const int* sl_beg,
                      const int* sl_end, int nslices, const int* seg_shot, const int* seg_beg,
-                     const int* seg_end, int nseg, float2* S, float scale, hipStream_t st);// natural-order copy of a forward spectrum produced by fft_forward
+                     const int* seg_end, int nseg, float2* S, float scale, hipStream_t st,
+                     const float* seg_scale = nullptr);// natural-order copy of a forward spectrum produced by fft_forward
 void fft_natural_order(const FftPlan& p, const float2* src, float2* dst, hipStream_t st);
 void target_spectra_fused(erp_ctx* ctx, const FftPlan& p, float2* A, float2* Y,
                           const int* pbase, const int* pwin, int nr, int n, const float2* G,

@@ -75,6 +75,16 @@ struct KPar {
     float sqrt_e1e2;   // sqrt(eps_1/eps_2)
     float sqrt_e2e1;   // sqrt(eps_2/eps_1)
     double drd, inv_dr, cos_ap, n_d, inv_cyc, lam_d;
+    // c1/c2 and c/c2 (genRefrPhasor, refrRadarSignal) divided in FP64: a
+    // float quotient is off by up to 6e-8, i.e. 5e-5 rad of phase over a
+    // 300 m refracted leg at 60 MHz
+    double c1c2, cc2;
+    // second subsurface medium below z = zi (has3 = 0: none): dn = n3 - n2,
+    // tsq = sqrt(1 - rho23^2) (amplitude factor of a crossing path),
+    // n_lo / n_hi = min / max(n2, n3) (window bounds)
+    int has3;
+    double zi, dn, n_lo, n_hi;
+    float tsq;
     double B, smpl;
     int nr, P_fft, rerad;
     int row_live;      // entries of a pair row the spectral chain reads: nr for
@@ -86,6 +96,32 @@ struct KPar {
 struct ShotK {
     double sx, sy, sz;
     float snx, sny, snz, rst, grefl, grefr;
+};
+
+// Multi-track batches (erp_run_tracks): the rows of a batch are shots of
+// several tracks of one range length; every kernel that reads a track's
+// data takes it from that track's descriptor instead of its arguments.
+struct TrackK {
+    KPar kp;
+    const float* mesh;             // 12 x Fpad SoA
+    int64_t Fpad, F;
+    int nblk, tile_nx, frame_ok;
+    const float* tg;               // 6 x Tpad targets
+    int64_t Tpad;
+    int T;
+    const unsigned char* ttype;
+    const float2* g_surf;          // surface range kernel spectrum (fft order)
+    const float2* g_targ_nat;      // target range kernel spectrum (natural order)
+    const float2* blue_w;          // Bluestein tables of the track's band
+    const float2* blue_Bw;
+    const unsigned char* blue_mask;
+    float2* out;                   // the track's radargram, ns x nr (device)
+    int32_t* nvalid;               // its illuminated counts (device) or null
+    float tscale;                  // -1 / sqrt(P): outward trace of a target
+};
+struct RowK {
+    int trk;   // track of the row
+    int shot;  // the row's shot index within its track
 };
 
 __device__ __forceinline__ ShotK load_shot(const erp_shot* __restrict__ shots, int s) {
@@ -329,6 +365,64 @@ struct Bounds {
     unsigned xmin, ymin, zmin, xmax, ymax, zmax;  // f2o-encoded
 };
 
+// Arithmetic of the range, bin and phase of a triple (precision study):
+// FP64 (the product), FP32 (the reference's own arithmetic,
+// radar_kernels.cu:681-686,811-813) or double-float (pairs of floats).
+constexpr int PH_F64 = 0, PH_F32 = 1, PH_DF = 2;
+// Accumulation of the bin window: FP64 (the product), plain FP32, or FP32
+// with the rounding error of every addition recovered (TwoSum on the value
+// an atomic returns) and accumulated beside it.
+constexpr int ACC_F64 = 0, ACC_F32 = 1, ACC_COMP = 2;
+
+// double-float: an unevaluated sum hi + lo of two floats (Dekker, Knuth).
+// Contraction is off inside: an FMA formed from a product and a later sum
+// would use the exact product where the error term assumes the rounded one.
+struct DF {
+    float hi, lo;
+};
+__device__ __forceinline__ DF df_two_sum(float a, float b) {
+#pragma clang fp contract(off)
+    const float s = a + b;
+    const float bb = s - a;
+    return DF{s, (a - (s - bb)) + (b - bb)};
+}
+__device__ __forceinline__ DF df_fast(float a, float b) {
+#pragma clang fp contract(off)
+    const float s = a + b;
+    return DF{s, b - (s - a)};
+}
+__device__ __forceinline__ DF df_of(double x) {
+    const float h = (float)x;
+    return DF{h, (float)(x - (double)h)};
+}
+__device__ __forceinline__ DF df_add(DF a, DF b) {
+#pragma clang fp contract(off)
+    const DF s = df_two_sum(a.hi, b.hi);
+    return df_fast(s.hi, s.lo + (a.lo + b.lo));
+}
+__device__ __forceinline__ DF df_neg(DF a) { return DF{-a.hi, -a.lo}; }
+__device__ __forceinline__ DF df_mul(DF a, DF b) {
+#pragma clang fp contract(off)
+    const float p = a.hi * b.hi;
+    const float e = fmaf(a.hi, b.hi, -p);
+    return df_fast(p, e + (a.hi * b.lo + a.lo * b.hi));
+}
+__device__ __forceinline__ DF df_sqrt(DF a) {
+#pragma clang fp contract(off)
+    if (!(a.hi > 0.f)) return DF{0.f, 0.f};
+    const float x = sqrtf(a.hi);
+    const float p = x * x;
+    const float e = fmaf(x, x, -p);
+    return df_fast(x, ((a.hi - p) - e + a.lo) / (2.f * x));
+}
+// |a - b| for float a and double-float b components
+__device__ __forceinline__ DF df_dist(float ax, float ay, float az, DF bx, DF by, DF bz) {
+    const DF dx = df_add(DF{ax, 0.f}, df_neg(bx));
+    const DF dy = df_add(DF{ay, 0.f}, df_neg(by));
+    const DF dz = df_add(DF{az, 0.f}, df_neg(bz));
+    return df_sqrt(df_add(df_add(df_mul(dx, dx), df_mul(dy, dy)), df_mul(dz, dz)));
+}
+
 // ---------------------------------------------------------------------------
 // generic small kernels
 // ---------------------------------------------------------------------------
@@ -377,8 +471,19 @@ __device__ __forceinline__ int64_t facet_of(int blk, int j, int tile_nx) {
 __global__ void __launch_bounds__(NT) k_illum_mask(
     const float* __restrict__ mesh, int64_t Fpad, int64_t F,
     const erp_shot* __restrict__ shots, double cos_ap,
-    unsigned long long* __restrict__ mask, int* __restrict__ cnt, int nblk, int tile_nx) {
+    unsigned long long* __restrict__ mask, int* __restrict__ cnt, int nblk, int tile_nx,
+    const TrackK* __restrict__ trk, const RowK* __restrict__ rows) {
     const int blk = blockIdx.x, s = blockIdx.y;
+    if (trk) {
+        // multi-track: the row's mesh; blocks past it hold nothing
+        const TrackK& t = trk[rows[s].trk];
+        if (blk >= t.nblk) return;
+        mesh = t.mesh;
+        Fpad = t.Fpad;
+        F = t.F;
+        cos_ap = t.kp.cos_ap;
+        tile_nx = t.tile_nx;
+    }
     const ShotK sh = load_shot(shots, s);
     const float sxf = (float)sh.sx, syf = (float)sh.sy, szf = (float)sh.sz;
     const float cosf_ap = (float)cos_ap;
@@ -427,15 +532,19 @@ __global__ void __launch_bounds__(NT) k_illum_mask(
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(1024) k_scan_blocks(const int* __restrict__ cnt,
                                                       int* __restrict__ off,
-                                                      int* __restrict__ V, int nblk) {
+                                                      int* __restrict__ V, int nblk,
+                                                      const TrackK* __restrict__ trk,
+                                                      const RowK* __restrict__ rows) {
     const int s = blockIdx.x;
+    // rows of a multi-track batch: stride nblk, the row's mesh's own count
+    const int n = trk ? trk[rows[s].trk].nblk : nblk;
     __shared__ int buf[1024];
     __shared__ int carry;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
-    for (int base = 0; base < nblk; base += 1024) {
+    for (int base = 0; base < n; base += 1024) {
         const int i = base + threadIdx.x;
-        const int v = (i < nblk) ? cnt[(size_t)s * nblk + i] : 0;
+        const int v = (i < n) ? cnt[(size_t)s * nblk + i] : 0;
         buf[threadIdx.x] = v;
         __syncthreads();
         for (int o = 1; o < 1024; o <<= 1) {
@@ -444,7 +553,7 @@ __global__ void __launch_bounds__(1024) k_scan_blocks(const int* __restrict__ cn
             buf[threadIdx.x] += t;
             __syncthreads();
         }
-        if (i < nblk) off[(size_t)s * nblk + i] = carry + buf[threadIdx.x] - v;
+        if (i < n) off[(size_t)s * nblk + i] = carry + buf[threadIdx.x] - v;
         __syncthreads();
         if (threadIdx.x == 1023) carry += buf[1023];
         __syncthreads();
@@ -515,6 +624,62 @@ __device__ __forceinline__ void lds_add2_fix(unsigned long long* wx, unsigned lo
     atomicAdd(&wx[idx], det_fix(v.x, scale));
     atomicAdd(&wy[idx], det_fix(v.y, scale));
 }
+// FP32 window add; ACC_COMP also accumulates the rounding error of the
+// addition the atomic performed (TwoSum of the returned old value and v)
+template <int ACC>
+__device__ __forceinline__ void lds_add_f32(float* w, int idx, float v, int stride2) {
+    if constexpr (ACC == ACC_COMP) {
+        const float old = atomicAdd(&w[idx], v);
+        const float s = old + v;
+        const float bb = s - old;
+        const float e = (old - (s - bb)) + (v - bb);
+        atomicAdd(&w[stride2 + idx], e);
+    } else {
+        atomicAdd(&w[idx], v);
+    }
+}
+template <int ACC>
+__device__ __forceinline__ void win_add(WinS w, int idx, float2 v) {
+    if constexpr (ACC == ACC_F64) {
+        lds_add2s(w, idx, v);
+    } else {
+        // FP32 layout inside the same LDS: x at [0, n), y at [n, 2n),
+        // compensations at [2n, 4n) (n = the window's FP64 capacity)
+        float* f = reinterpret_cast<float*>(w.x);
+        const int n = (int)(w.y - w.x);
+        lds_add_f32<ACC>(f, idx, v.x, 2 * n);
+        lds_add_f32<ACC>(f + n, idx, v.y, 2 * n);
+    }
+}
+template <int ACC>
+__device__ __forceinline__ void win_zero(WinS w, int idx) {
+    if constexpr (ACC == ACC_F64) {
+        w.x[idx] = 0.0;
+        w.y[idx] = 0.0;
+    } else {
+        float* f = reinterpret_cast<float*>(w.x);
+        const int n = (int)(w.y - w.x);
+        f[idx] = 0.f;
+        f[n + idx] = 0.f;
+        if constexpr (ACC == ACC_COMP) {
+            f[2 * n + idx] = 0.f;
+            f[3 * n + idx] = 0.f;
+        }
+    }
+}
+template <int ACC>
+__device__ __forceinline__ float2 win_get(WinS w, int idx) {
+    if constexpr (ACC == ACC_F64) {
+        return make_float2((float)w.x[idx], (float)w.y[idx]);
+    } else {
+        const float* f = reinterpret_cast<const float*>(w.x);
+        const int n = (int)(w.y - w.x);
+        if constexpr (ACC == ACC_COMP)
+            return make_float2(f[idx] + f[2 * n + idx], f[n + idx] + f[3 * n + idx]);
+        return make_float2(f[idx], f[n + idx]);
+    }
+}
+
 __device__ __forceinline__ void glb_add2_fix(unsigned long long* row, int idx, float2 v,
                                              double scale) {
     atomicAdd(&row[2 * idx], det_fix(v.x, scale));
@@ -542,11 +707,31 @@ __global__ void __launch_bounds__(NT) k_surface(
     const unsigned long long* __restrict__ mask, const int* __restrict__ off,
     const int* __restrict__ Vs, const int64_t* __restrict__ rec_base, int nblk, KPar kp,
     Rec rec, int write_rec, int do_trace, double2* __restrict__ srow, int64_t shot0,
-    Bounds* __restrict__ bounds, int det, unsigned* __restrict__ smax, int tile_nx) {
+    Bounds* __restrict__ bounds, int det, unsigned* __restrict__ smax, int tile_nx,
+    int prec_ph, int prec_acc, const TrackK* __restrict__ trk, const RowK* __restrict__ rows) {
     // det 0: FP64 bins (LDS window, FP64 atomics into srow)
     // det 1: first deterministic pass, largest |component| per shot into smax
     // det 2: second deterministic pass, fixed-point bins (srow holds int64)
+    // prec_ph, prec_acc (precision study, det 0, non-specular): the trace's range, bin
+    // and phase in FP64 / FP32 (the reference's surfacePT arithmetic) /
+    // double-float, and its LDS window sums in FP64 / FP32 / compensated
+    // FP32 (the window partial sums enter the FP64 row as they are); the
+    // facet records for the target kernels always keep the FP64 range
     const int blk = blockIdx.x, s = blockIdx.y;
+    // the shot's index within its track (the incoherent phase hash)
+    int64_t shot_id = shot0 + s;
+    if (trk) {
+        // multi-track: the row's mesh and parameters; blocks past its mesh
+        // hold nothing
+        const TrackK& t = trk[rows[s].trk];
+        if (blk >= t.nblk) return;
+        mesh = t.mesh;
+        Fpad = t.Fpad;
+        F = t.F;
+        kp = t.kp;
+        tile_nx = t.tile_nx;
+        shot_id = rows[s].shot;
+    }
     const unsigned long long* mw = mask + ((size_t)s * nblk + blk) * MASKW;
     __shared__ unsigned long long m_s[MASKW];
     __shared__ int pre_s[MASKW + 1];
@@ -689,14 +874,44 @@ __global__ void __launch_bounds__(NT) k_surface(
         float wgt;
         if (!spec) {
             // surfacePT: sqrt(friis(friis(P,Gr,G,lam,R),G,Gr,lam,R) [* fRe])
-            if (!bin_of(Itd, rst, kp.drd, kp.nr, &b, &wgt)) continue;
+            float2 pst = make_float2(1.f, 0.f);
+            if (prec_ph == PH_F32) {
+                // Itd, the bin coordinate and phasor(2 Itd) in FP32
+                // (compIncidentRays, radar_kernels.cu:681-686, 733-737)
+                const float ax = g[0] - (float)sh.sx, ay = g[1] - (float)sh.sy,
+                            az = g[2] - (float)sh.sz;
+                const float I32 = sqrtf(ax * ax + ay * ay + az * az);
+                const float x32 = (I32 - sh.rst) / kp.dr;
+                if (!(x32 > -1.f) || !(x32 < (float)kp.nr)) continue;
+                b = (int)x32;
+                wgt = x32 - (float)b;
+                float sn, cs;
+                sincosf(-(2.0f * ERP_PI_F / kp.lam) * (2.f * I32), &sn, &cs);
+                pst = make_float2(cs, sn);
+            } else if (prec_ph == PH_DF) {
+                const DF Id = df_dist(g[0], g[1], g[2], df_of(sh.sx), df_of(sh.sy), df_of(sh.sz));
+                const DF xd = df_mul(df_add(Id, df_neg(df_of(rst))), df_of(kp.inv_dr));
+                const double xq = (double)xd.hi + (double)xd.lo;
+                if (!(xq > -1.0) || !(xq < (double)kp.nr)) continue;
+                b = (int)xq;
+                wgt = (float)(xq - (double)b);
+                // exp(-i pi t), t = 2 Itd 2 inv_cyc half cycles, reduced in double-float
+                const DF t = df_mul(Id, df_of(2.0 * kp.inv_cyc));
+                const float rr = rintf(t.hi);
+                const float fr = (t.hi - rr) + t.lo;
+                pst = make_float2(__builtin_amdgcn_cosf(fr), -__builtin_amdgcn_sinf(fr));
+            } else if (!bin_of(Itd, rst, kp.drd, kp.nr, &b, &wgt)) {
+                continue;
+            }
             float amp = kp.sqrtP * kp.lam * kp.lam * Grefr * G1 * kp.invK / (Itdf * Itdf);
             if (!lossless) amp *= sqrtf(fRe);
             float2 ph;
             if (incoh) {
                 float sn, cs;
-                sincospif(2.0f * incoherent_u(kp.seed, shot0 + s, i), &sn, &cs);
+                sincospif(2.0f * incoherent_u(kp.seed, shot_id, i), &sn, &cs);
                 ph = make_float2(cs, sn);
+            } else if (prec_ph != PH_F64) {
+                ph = pst;
             } else {
                 ph = phasor_cycles(2.0 * Itd * kp.inv_cyc);
             }
@@ -736,6 +951,7 @@ __global__ void __launch_bounds__(NT) k_surface(
         red_f[3][wv] = lxmax; red_f[4][wv] = lymax; red_f[5][wv] = lzmax;
         red_d[0][wv] = litmin; red_d[1][wv] = litmax;
     }
+    // (FP64 window, or the FP32 layout of the study's sums in the same LDS)
     for (int k = threadIdx.x; k < 2 * SWIN; k += NT) swin[k] = 0.0;
     __syncthreads();
     if (threadIdx.x == 0 && bounds) {
@@ -793,6 +1009,13 @@ __global__ void __launch_bounds__(NT) k_surface(
         }
         return;
     }
+    const WinS W{swin, swin + SWIN};
+    // one window add of the study's arithmetic (a uniform choice)
+    auto wadd = [&](int k, float2 v) {
+        if (prec_acc == ACC_F32) win_add<ACC_F32>(W, k, v);
+        else if (prec_acc == ACC_COMP) win_add<ACC_COMP>(W, k, v);
+        else lds_add2s(W, k, v);
+    };
 #pragma unroll
     for (int q = 0; q < FPT; ++q) {
         const int b = bq[q];
@@ -801,16 +1024,23 @@ __global__ void __launch_bounds__(NT) k_surface(
         const float2 c = cq[q];
         const float2 c0 = make_float2(c.x * (1.f - w), c.y * (1.f - w));
         const int k0 = b - base;
-        const WinS W{swin, swin + SWIN};
-        if (k0 < SWIN) lds_add2s(W, k0, c0); else glb_add2d(row, b, c0.x, c0.y);
+        if (k0 < SWIN) wadd(k0, c0); else glb_add2d(row, b, c0.x, c0.y);
         if (!spec && b + 1 < nr) {
             const float2 c1 = make_float2(c.x * w, c.y * w);
-            if (k0 + 1 < SWIN) lds_add2s(W, k0 + 1, c1); else glb_add2d(row, b + 1, c1.x, c1.y);
+            if (k0 + 1 < SWIN) wadd(k0 + 1, c1); else glb_add2d(row, b + 1, c1.x, c1.y);
         }
     }
     __syncthreads();
     for (int k = threadIdx.x; k < SWIN; k += NT) {
-        const double vx = swin[k], vy = swin[SWIN + k];
+        double vx, vy;
+        if (prec_acc == ACC_F64) {
+            vx = swin[k];
+            vy = swin[SWIN + k];
+        } else {
+            const float2 v = prec_acc == ACC_F32 ? win_get<ACC_F32>(W, k) : win_get<ACC_COMP>(W, k);
+            vx = v.x;
+            vy = v.y;
+        }
         const int b = base + k;
         if (b < nr && (vx != 0.0 || vy != 0.0)) glb_add2d(row, b, vx, vy);
     }
@@ -965,9 +1195,19 @@ __global__ void k_blue_tables(float2* __restrict__ w, float2* __restrict__ b,
 __global__ void __launch_bounds__(1024) k_target_select(
     const float* __restrict__ tg, int64_t Tpad, int T, const unsigned char* __restrict__ ttype,
     const erp_shot* __restrict__ shots, const int* __restrict__ Vs, double cos_ap,
-    int* __restrict__ list1, int* __restrict__ list0, int* __restrict__ tcnt, int Tcap) {
+    int* __restrict__ list1, int* __restrict__ list0, int* __restrict__ tcnt, int Tcap,
+    const TrackK* __restrict__ trk, const RowK* __restrict__ rows) {
 #pragma clang fp contract(off)
     const int s = blockIdx.x;
+    if (trk) {
+        // multi-track: the row's track's targets (Tcap >= every track's T)
+        const TrackK& tk = trk[rows[s].trk];
+        tg = tk.tg;
+        Tpad = tk.Tpad;
+        T = tk.T;
+        ttype = tk.ttype;
+        cos_ap = tk.kp.cos_ap;
+    }
     const ShotK sh = load_shot(shots, s);
     __shared__ int buf1[1024], buf0[1024];
     __shared__ int c1, c0;
@@ -1103,7 +1343,7 @@ __global__ void __launch_bounds__(NTT) k_target_accum(
     const double tmin = sqrt((double)bx * bx + (double)by * by + (double)bz * bz);
     const double itmin = __longlong_as_double((long long)bd.itd_min);
     double rlo;
-    if (MODE == 2) rlo = (itmin + tmin) - tmin + tmin * (double)(kp.c1 / kp.c2);
+    if (MODE == 2) rlo = (itmin + tmin) - tmin + tmin * kp.c1c2;
     else rlo = itmin + kp.n_d * tmin;
     int base = (int)floor((rlo - rst) / kp.drd) - 2;
     base = max(base, 0);
@@ -1158,7 +1398,7 @@ __global__ void __launch_bounds__(NTT) k_target_accum(
         if (MODE == 2) {
             // genRefrPhasor: sltrng = Itd + Ttd; rngt = (sltrng - Ttd) + Ttd*(c1/c2)
             const double sl = Itd + Ttd;
-            rngt = madd_nc(sl - Ttd, Ttd, (double)(kp.c1 / kp.c2));
+            rngt = madd_nc(sl - Ttd, Ttd, kp.c1c2);
         } else {
             rngt = madd_nc(Itd, Ttd, kp.n_d);
         }
@@ -1313,64 +1553,6 @@ struct Tri {
     bool valid;
 };
 
-// Arithmetic of the range, bin and phase of a triple (precision study):
-// FP64 (the product), FP32 (the reference's own arithmetic,
-// radar_kernels.cu:681-686,811-813) or double-float (pairs of floats).
-constexpr int PH_F64 = 0, PH_F32 = 1, PH_DF = 2;
-// Accumulation of the bin window: FP64 (the product), plain FP32, or FP32
-// with the rounding error of every addition recovered (TwoSum on the value
-// an atomic returns) and accumulated beside it.
-constexpr int ACC_F64 = 0, ACC_F32 = 1, ACC_COMP = 2;
-
-// double-float: an unevaluated sum hi + lo of two floats (Dekker, Knuth).
-// Contraction is off inside: an FMA formed from a product and a later sum
-// would use the exact product where the error term assumes the rounded one.
-struct DF {
-    float hi, lo;
-};
-__device__ __forceinline__ DF df_two_sum(float a, float b) {
-#pragma clang fp contract(off)
-    const float s = a + b;
-    const float bb = s - a;
-    return DF{s, (a - (s - bb)) + (b - bb)};
-}
-__device__ __forceinline__ DF df_fast(float a, float b) {
-#pragma clang fp contract(off)
-    const float s = a + b;
-    return DF{s, b - (s - a)};
-}
-__device__ __forceinline__ DF df_of(double x) {
-    const float h = (float)x;
-    return DF{h, (float)(x - (double)h)};
-}
-__device__ __forceinline__ DF df_add(DF a, DF b) {
-#pragma clang fp contract(off)
-    const DF s = df_two_sum(a.hi, b.hi);
-    return df_fast(s.hi, s.lo + (a.lo + b.lo));
-}
-__device__ __forceinline__ DF df_neg(DF a) { return DF{-a.hi, -a.lo}; }
-__device__ __forceinline__ DF df_mul(DF a, DF b) {
-#pragma clang fp contract(off)
-    const float p = a.hi * b.hi;
-    const float e = fmaf(a.hi, b.hi, -p);
-    return df_fast(p, e + (a.hi * b.lo + a.lo * b.hi));
-}
-__device__ __forceinline__ DF df_sqrt(DF a) {
-#pragma clang fp contract(off)
-    if (!(a.hi > 0.f)) return DF{0.f, 0.f};
-    const float x = sqrtf(a.hi);
-    const float p = x * x;
-    const float e = fmaf(x, x, -p);
-    return df_fast(x, ((a.hi - p) - e + a.lo) / (2.f * x));
-}
-// |a - b| for float a and double-float b components
-__device__ __forceinline__ DF df_dist(float ax, float ay, float az, DF bx, DF by, DF bz) {
-    const DF dx = df_add(DF{ax, 0.f}, df_neg(bx));
-    const DF dy = df_add(DF{ay, 0.f}, df_neg(by));
-    const DF dz = df_add(DF{az, 0.f}, df_neg(bz));
-    return df_sqrt(df_add(df_add(df_mul(dx, dx), df_mul(dy, dy)), df_mul(dz, dz)));
-}
-
 // one facet record in registers
 struct RecV {
     float4 qa, qb, qc;
@@ -1416,13 +1598,40 @@ __device__ __forceinline__ RecV load_rec(const RecShot& rs, unsigned k) {
     return v;
 }
 
-template <int PH = PH_F64>
+// The second subsurface medium for one pair: deep = the target lies below
+// the interface z = zi, so part of each facet's forced ray runs in eps_3
+// (a uniform flag per pair).
+struct Strat {
+    bool deep;
+    double zi, tz, dn;
+    float tsq;
+};
+
+// Fraction of the straight facet (fz) -> target (tz) segment below z = zi
+// for a target below it: (zi - tz) / (fz - tz), or 1 when the facet is not
+// above the interface.  Exact: the oracle's FP64 division.
+__device__ __forceinline__ double below_frac_exact(float fz, const Strat& st) {
+#pragma clang fp contract(off)
+    if (!((double)fz > st.zi)) return 1.0;
+    return (st.zi - st.tz) / ((double)fz - st.tz);
+}
+// The same from an FP32 reciprocal and one Newton correction in FP64
+// (relative error about 1e-14)
+__device__ __forceinline__ double below_frac(float fz, const Strat& st) {
+    if (!((double)fz > st.zi)) return 1.0;
+    const double num = st.zi - st.tz, den = (double)fz - st.tz;
+    const double r0 = (double)__builtin_amdgcn_rcpf((float)den);
+    const double q = num * r0;
+    return fma(fma(-q, den, num), r0, q);
+}
+
+template <int PH = PH_F64, bool S3 = false>
 __device__ __forceinline__ Tri tri_eval(const RecV& rv, bool frame_ok, double sx,
                                         double sy, double sz, double tx, double ty, double tz,
                                         float tnx, float tny, float tnz, bool tn_up, double rst,
                                         double inv_dr, double n_d, double two_cyc, int nr,
                                         bool lossless, const erp_prism* __restrict__ prisms,
-                                        int np, const KPar& kp) {
+                                        int np, const KPar& kp, const Strat& st) {
     Tri o;
     const float4 qa = rv.qa, qb = rv.qb, qc = rv.qc;
     const float qd = rv.qd;
@@ -1441,6 +1650,8 @@ __device__ __forceinline__ Tri tri_eval(const RecV& rv, bool frame_ok, double sx
     float2 ph_st = make_float2(1.f, 0.f);
     if constexpr (PH == PH_F64) {
         rngt = fma(Ttd, n_d, Itd);
+        // second medium: + (n3 - n2) times the ray's length below the interface
+        if (S3 && st.deep) rngt = fma(Ttd * below_frac(fz, st), st.dn, rngt);
         // (rngt - rst) / dr to within 1e-11 bin; the 1e-7 margin below
         // covers it
         const double xq = fma(rngt, inv_dr, -rst * inv_dr);
@@ -1449,7 +1660,13 @@ __device__ __forceinline__ Tri tri_eval(const RecV& rv, bool frame_ok, double sx
         if (o.w < 1e-7 || o.w > 1.0 - 1e-7) {
             // within about 1e-7 bin of an edge (every negative coordinate
             // included): the oracle's exact operation sequence
-            rngt = madd_nc(Itd, dist3(dx, dy, dz), n_d);
+            const double Te = dist3(dx, dy, dz);
+            rngt = madd_nc(Itd, Te, n_d);
+            if (S3 && st.deep) {
+#pragma clang fp contract(off)
+                const double L3 = Te * below_frac_exact(fz, st);
+                rngt = madd_nc(rngt, L3, st.dn);
+            }
             o.valid = bin_of(rngt, rst, kp.drd, nr, &o.b, &o.w);
         } else {
             o.valid = (xq > -1.0) && (xq < (double)nr);
@@ -1461,7 +1678,12 @@ __device__ __forceinline__ Tri tri_eval(const RecV& rv, bool frame_ok, double sx
         const float bx = (float)tx - fx, by = (float)ty - fy, bz = (float)tz - fz;
         const float I32 = sqrtf(ax * ax + ay * ay + az * az);
         const float T32 = sqrtf(bx * bx + by * by + bz * bz);
-        const float r32 = I32 + T32 * kp.nf;
+        float r32 = I32 + T32 * kp.nf;
+        if (S3 && st.deep) {
+            const float fr = fz > (float)st.zi ? ((float)st.zi - (float)st.tz) / (fz - (float)st.tz)
+                                                : 1.f;
+            r32 = r32 + (T32 * fr) * (float)st.dn;
+        }
         const float x32 = (r32 - (float)rst) / kp.dr;
         o.valid = (x32 > -1.f) && (x32 < (float)nr);
         o.b = (int)x32;
@@ -1474,7 +1696,9 @@ __device__ __forceinline__ Tri tri_eval(const RecV& rv, bool frame_ok, double sx
         // coordinates and the shot position
         const DF Id = df_dist(fx, fy, fz, df_of(sx), df_of(sy), df_of(sz));
         const DF Td = df_dist(fx, fy, fz, df_of(tx), df_of(ty), df_of(tz));
-        const DF rd2 = df_add(Id, df_mul(Td, df_of(n_d)));
+        DF rd2 = df_add(Id, df_mul(Td, df_of(n_d)));
+        if (S3 && st.deep)
+            rd2 = df_add(rd2, df_mul(df_mul(Td, df_of(below_frac_exact(fz, st))), df_of(st.dn)));
         const DF xd = df_mul(df_add(rd2, df_neg(df_of(rst))), df_of(inv_dr));
         const double xq = (double)xd.hi + (double)xd.lo;
         o.valid = (xq > -1.0) && (xq < (double)nr);
@@ -1527,6 +1751,9 @@ __device__ __forceinline__ Tri tri_eval(const RecV& rv, bool frame_ok, double sx
     // qa.w = F Dmax / pi^4 (the 1/pi^4 takes the four factors pi of the
     // pi-sincs back out)
     float amp = qa.w * fabsf((n1 * n2) * (n3 * n4)) * __builtin_amdgcn_rcpf(den) * rs;
+    // interface transmission of a ray that crosses into the second medium
+    // (a loss factor like fRfrC: not applied to lossless runs)
+    if (S3 && !lossless && st.deep && (double)fz > st.zi) amp *= st.tsq;
     if (!lossless) {
         float expo = 0.f, tot = 0.f;
         for (int k = 0; k < np; ++k) {
@@ -1545,62 +1772,6 @@ __device__ __forceinline__ Tri tri_eval(const RecV& rv, bool frame_ok, double sx
 
 // two-bin split of one term into the LDS window [base, base + len) or, past
 // it, the global row
-// FP32 window add; ACC_COMP also accumulates the rounding error of the
-// addition the atomic performed (TwoSum of the returned old value and v)
-template <int ACC>
-__device__ __forceinline__ void lds_add_f32(float* w, int idx, float v, int stride2) {
-    if constexpr (ACC == ACC_COMP) {
-        const float old = atomicAdd(&w[idx], v);
-        const float s = old + v;
-        const float bb = s - old;
-        const float e = (old - (s - bb)) + (v - bb);
-        atomicAdd(&w[stride2 + idx], e);
-    } else {
-        atomicAdd(&w[idx], v);
-    }
-}
-template <int ACC>
-__device__ __forceinline__ void win_add(WinS w, int idx, float2 v) {
-    if constexpr (ACC == ACC_F64) {
-        lds_add2s(w, idx, v);
-    } else {
-        // FP32 layout inside the same LDS: x at [0, n), y at [n, 2n),
-        // compensations at [2n, 4n) (n = the window's FP64 capacity)
-        float* f = reinterpret_cast<float*>(w.x);
-        const int n = (int)(w.y - w.x);
-        lds_add_f32<ACC>(f, idx, v.x, 2 * n);
-        lds_add_f32<ACC>(f + n, idx, v.y, 2 * n);
-    }
-}
-template <int ACC>
-__device__ __forceinline__ void win_zero(WinS w, int idx) {
-    if constexpr (ACC == ACC_F64) {
-        w.x[idx] = 0.0;
-        w.y[idx] = 0.0;
-    } else {
-        float* f = reinterpret_cast<float*>(w.x);
-        const int n = (int)(w.y - w.x);
-        f[idx] = 0.f;
-        f[n + idx] = 0.f;
-        if constexpr (ACC == ACC_COMP) {
-            f[2 * n + idx] = 0.f;
-            f[3 * n + idx] = 0.f;
-        }
-    }
-}
-template <int ACC>
-__device__ __forceinline__ float2 win_get(WinS w, int idx) {
-    if constexpr (ACC == ACC_F64) {
-        return make_float2((float)w.x[idx], (float)w.y[idx]);
-    } else {
-        const float* f = reinterpret_cast<const float*>(w.x);
-        const int n = (int)(w.y - w.x);
-        if constexpr (ACC == ACC_COMP)
-            return make_float2(f[idx] + f[2 * n + idx], f[n + idx] + f[3 * n + idx]);
-        return make_float2(f[idx], f[n + idx]);
-    }
-}
-
 template <int ACC = ACC_F64>
 __device__ __forceinline__ void tri_add(const Tri& t, int base, int len, int nr,
                                         WinS winA, float2* rowA) {
@@ -1645,17 +1816,28 @@ __device__ __forceinline__ void tri_add(const Tri& t, int base, int len, int nr,
 // decided on the approximate range unless it lies within 1e-7 bin of a bin
 // edge, where the exact FP64 sqrt and division of the oracle are repeated, so
 // bin indices stay identical to the FP64 reference.
-template <bool DET, int PH = PH_F64, int ACC = ACC_F64>
-__global__ void __launch_bounds__(NTF) k_target_accum_fast(
-    const float* __restrict__ tg, int64_t Tpad, const int* __restrict__ pshot,
-    const int* __restrict__ ptgt, const erp_shot* __restrict__ shots,
-    const int* __restrict__ Vs, const int64_t* __restrict__ rec_base, Rec rec,
-    const Bounds* __restrict__ bounds, const erp_prism* __restrict__ prisms, int np, KPar kp,
-    float2* __restrict__ outA, int* __restrict__ pbase, int* __restrict__ pwin, int frame_ok,
-    int npairs, int* __restrict__ next_pair) {
+// S3: the second subsurface medium.  Its instantiations are held to 128
+// VGPRs (4 waves per SIMD, the two 512-thread workgroups per CU that the LDS
+// window allows; unbounded they take 132 and one workgroup per CU); the
+// others keep the default bound.
+// MT: a multi-track batch: each pair's row names its track, whose targets,
+// parameters and facet frames the pair uses; the pair count is read from
+// device memory (the launch is replayed from a graph).
+template <bool DET, int PH = PH_F64, int ACC = ACC_F64, bool S3 = false, bool MT = false>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, NTF),
+                               amdgpu_waves_per_eu(S3 ? 4 : 1)))
+k_target_accum_fast(const float* __restrict__ tg0, int64_t Tpad0, const int* __restrict__ pshot,
+                    const int* __restrict__ ptgt, const erp_shot* __restrict__ shots,
+                    const int* __restrict__ Vs, const int64_t* __restrict__ rec_base, Rec rec,
+                    const Bounds* __restrict__ bounds, const erp_prism* __restrict__ prisms,
+                    int np, KPar kp0, float2* __restrict__ outA, int* __restrict__ pbase,
+                    int* __restrict__ pwin, int frame_ok0, int npairs0,
+                    int* __restrict__ next_pair, const TrackK* __restrict__ trk,
+                    const RowK* __restrict__ rows, const int* __restrict__ npairs_dev) {
     __shared__ double winS[2 * TWF];
     __shared__ int s_pi;
     const WinS winA{winS, winS + TWF};
+    const int npairs = MT ? *npairs_dev : npairs0;
     // persistent: each workgroup takes the next pair from a counter until
     // none is left (the counter starts at 0: k_pair_targets clears it), so
     // short pairs pay no workgroup launch and uneven pairs balance
@@ -1665,6 +1847,11 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
         const int pi = s_pi;
         if (pi >= npairs) break;
         const int s = pshot[pi], t = ptgt[pi];
+        const TrackK* tk = MT ? &trk[rows[s].trk] : nullptr;
+        const KPar& kp = MT ? tk->kp : kp0;
+        const float* __restrict__ tg = MT ? tk->tg : tg0;
+        const int64_t Tpad = MT ? tk->Tpad : Tpad0;
+        const int frame_ok = MT ? tk->frame_ok : frame_ok0;
         const ShotK sh = load_shot(shots, s);
         const int V = Vs[s];
         const int64_t r0 = rec_base[s];
@@ -1676,14 +1863,18 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
         const int nr = kp.nr, P = kp.P_fft;
         const bool lossless = kp.mode & ERP_MODE_LOSSLESS;
 
-        // window from the shot bounds (see k_target_accum)
+        // second medium: a uniform flag per pair
+        const Strat st{S3 && (double)tzf < kp.zi, kp.zi, (double)tzf, kp.dn, kp.tsq};
+        // window from the shot bounds (see k_target_accum); rngt lies between
+        // Itd + n_lo Ttd and Itd + n_hi Ttd
+        const double n_lo = st.deep ? kp.n_lo : kp.n_d, n_hi = st.deep ? kp.n_hi : kp.n_d;
         const Bounds bd = bounds[s];
         const float bx = fmaxf(fmaxf(o2f(bd.xmin) - txf, txf - o2f(bd.xmax)), 0.f);
         const float by = fmaxf(fmaxf(o2f(bd.ymin) - tyf, tyf - o2f(bd.ymax)), 0.f);
         const float bz = fmaxf(fmaxf(o2f(bd.zmin) - tzf, tzf - o2f(bd.zmax)), 0.f);
         const double tmin = sqrt((double)bx * bx + (double)by * by + (double)bz * bz);
         const double itmin = __longlong_as_double((long long)bd.itd_min);
-        int base = (int)floor((itmin + kp.n_d * tmin - rst) / kp.drd) - 2;
+        int base = (int)floor((itmin + n_lo * tmin - rst) / kp.drd) - 2;
         base = max(base, 0);
         bool fits = false;
         int len = TWF;
@@ -1694,7 +1885,7 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
             const float fz = fmaxf(fabsf(o2f(bd.zmin) - tzf), fabsf(o2f(bd.zmax) - tzf));
             const double tmax = sqrt((double)fx * fx + (double)fy * fy + (double)fz * fz);
             const double ithmax = __longlong_as_double((long long)bd.itd_max);
-            const double rhi = ithmax + kp.n_d * tmax * (1.0 + 1e-6) + 1.0;
+            const double rhi = ithmax + n_hi * tmax * (1.0 + 1e-6) + 1.0;
             // every bin written (b and b + 1) is below top
             top = min((int)ceil((rhi - rst) / kp.drd) + 2, nr);
             fits = top - base <= TWF;
@@ -1769,9 +1960,9 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
                     const int vl = V - 1;
                     auto rec_at = [&](int k) { return load_rec(rs, (unsigned)min(k, vl)); };
                     auto step = [&](const RecV& ca) {
-                        const Tri ta = tri_eval(ca, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx,
+                        const Tri ta = tri_eval<PH_F64, S3>(ca, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx,
                                                 tny, tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr,
-                                                lossless, prisms, np, kp);
+                                                lossless, prisms, np, kp, st);
                         if (ta.valid) add_det(ta);
                     };
                     RecV a0 = rec_at(threadIdx.x);
@@ -1811,15 +2002,15 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
                 const int vl = V - 1;
                 auto rec_at = [&](int k) { return load_rec(rs, (unsigned)min(k, vl)); };
                 auto step = [&](const RecV& ca, const RecV& cb, int k) {
-                    const Tri ta = tri_eval<PH>(ca, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx,
+                    const Tri ta = tri_eval<PH, S3>(ca, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx,
                                                 tny, tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr,
-                                                lossless, prisms, np, kp);
+                                                lossless, prisms, np, kp, st);
                     Tri tb;
                     tb.valid = false;
                     if (k + NTF < V)
-                        tb = tri_eval<PH>(cb, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx, tny,
+                        tb = tri_eval<PH, S3>(cb, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx, tny,
                                           tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr, lossless,
-                                          prisms, np, kp);
+                                          prisms, np, kp, st);
                     if (ta.valid) tri_add<ACC>(ta, base, len, nr, winA, rowA);
                     if (tb.valid) tri_add<ACC>(tb, base, len, nr, winA, rowA);
                 };
@@ -1844,6 +2035,7 @@ __global__ void __launch_bounds__(NTF) k_target_accum_fast(
         __syncthreads();
     }
 }
+
 
 // S[s][k] += scale * sum over the shot's pairs of X[p][k] * Z[p][k]
 __global__ void k_accum_spectra(float2* __restrict__ S, const float2* __restrict__ X,
@@ -1938,6 +2130,122 @@ __global__ void k_finalize(float2* __restrict__ out, const float2* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// multi-track batches: the row-wise forms of the per-shot kernels above,
+// each row taking its track's tables from the track descriptors
+// ---------------------------------------------------------------------------
+// record base of each row: exclusive sum of the rows' illuminated counts
+__global__ void k_rec_base(const int* __restrict__ V, int64_t* __restrict__ base, int rows) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        int64_t acc = 0;
+        for (int r = 0; r < rows; ++r) {
+            base[r] = acc;
+            acc += V[r];
+        }
+    }
+}
+
+// dst[r][k] = src[r][k] * table[k]: which = 0 the track's surface range
+// kernel spectrum (none: a track without a surface echo gets zeros),
+// which = 1 its Bluestein spectrum
+__global__ void k_mul_spec_rows(float2* __restrict__ dst, const float2* __restrict__ src, int P,
+                                int rows, const TrackK* __restrict__ trk,
+                                const RowK* __restrict__ rk, int which) {
+    const size_t n = (size_t)P * rows;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const TrackK& t = trk[rk[i / P].trk];
+        const float2* spec = which == 0 ? t.g_surf : t.blue_Bw;
+        dst[i] = spec ? cmul(src[i], spec[i & (size_t)(P - 1)]) : make_float2(0.f, 0.f);
+    }
+}
+
+__global__ void k_blue_in_rows(float2* __restrict__ a, int M, const float2* __restrict__ c, int P,
+                               int L, float cscale, int rows, const TrackK* __restrict__ trk,
+                               const RowK* __restrict__ rk) {
+    const size_t n = (size_t)M * rows;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / M), k = (int)(i - (size_t)r * M);
+        float2 v = make_float2(0.f, 0.f);
+        if (k < L) {
+            float2 x = c[(size_t)r * P + k];
+            x.x *= cscale;
+            x.y *= cscale;
+            v = cmul(x, trk[rk[r].trk].blue_w[k]);
+        }
+        a[i] = v;
+    }
+}
+
+__global__ void k_blue_mid_rows(float2* __restrict__ a, int M, int L, int rows,
+                                const TrackK* __restrict__ trk, const RowK* __restrict__ rk) {
+    const size_t n = (size_t)M * rows;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / M), k = (int)(i - (size_t)r * M);
+        float2 v = make_float2(0.f, 0.f);
+        if (k < L && trk[rk[r].trk].blue_mask[k]) {
+            v = a[i];
+            v.y = -v.y;
+        }
+        a[i] = v;
+    }
+}
+
+__global__ void k_blue_out_rows(float2* __restrict__ refr, int nr, const float2* __restrict__ a,
+                                int M, float invL, int rows, const int* __restrict__ live,
+                                const TrackK* __restrict__ trk, const RowK* __restrict__ rk) {
+    const size_t n = (size_t)nr * rows;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / nr), j = (int)(i - (size_t)r * nr);
+        float2 v = make_float2(0.f, 0.f);
+        if (live[r]) {
+            v = cmul(trk[rk[r].trk].blue_w[2 * j], a[(size_t)r * M + 2 * j]);
+            v = make_float2(v.x * invL, -v.y * invL);
+        }
+        refr[i] = v;
+    }
+}
+
+// out = refl + refr into each row's place in its track's radargram (rows of
+// shot -1 pad a batch and write nothing)
+__global__ void k_finalize_rows(const float2* __restrict__ refl, const float2* __restrict__ refr,
+                                const int* __restrict__ Vs, int nr, int rows,
+                                const TrackK* __restrict__ trk, const RowK* __restrict__ rk) {
+    const size_t n = (size_t)nr * rows;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / nr), j = (int)(i - (size_t)r * nr);
+        const RowK row = rk[r];
+        if (row.shot < 0) continue;
+        const TrackK& t = trk[row.trk];
+        float2 v = make_float2(0.f, 0.f);
+        if (Vs[r] > 0) v = make_float2(refl[i].x + refr[i].x, refl[i].y + refr[i].y);
+        t.out[(size_t)row.shot * nr + j] = v;
+        if (t.nvalid && j == 0) t.nvalid[row.shot] = Vs[r];
+    }
+}
+
+// pairs of one chunk of a multi-track batch: the target id from the row's
+// list, the track's range kernel spectrum per pair, and the accumulation
+// kernel's pair counter cleared; the chunk's pair count is on the device
+__global__ void k_pair_targets_rows(int* __restrict__ ptgt, const int* __restrict__ pshot,
+                                    const int* __restrict__ list1, int Tcap,
+                                    const int* __restrict__ n_dev, int* __restrict__ next_pair,
+                                    const float4** __restrict__ Gpair,
+                                    const TrackK* __restrict__ trk,
+                                    const RowK* __restrict__ rk) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < *n_dev) {
+        const int r = pshot[i];
+        ptgt[i] = list1[(size_t)r * Tcap + ptgt[i]];
+        Gpair[i] = reinterpret_cast<const float4*>(trk[rk[r].trk].g_targ_nat);
+    }
+    if (i == 0) *next_pair = 0;
+}
+
+// ---------------------------------------------------------------------------
 // NoConv direct sums (reflRadarSignal / refrRadarSignal, radar_kernels.cu:
 // 58-159, 201-299): one thread per (shot, bin), facets streamed through LDS.
 // ---------------------------------------------------------------------------
@@ -1954,7 +2262,9 @@ __global__ void __launch_bounds__(NT) k_refl_direct(
     const double r = (double)sh.rst + (double)ir * kp.drd;
     const bool lossless = kp.mode & ERP_MODE_LOSSLESS;
     (void)lossless;
-    float2 acc = make_float2(0.f, 0.f);
+    // FP64 sums: V terms of either sign per bin (sinc sidelobes of facets at
+    // other ranges) cancel to a bin far smaller than its largest terms
+    double ax = 0.0, ay = 0.0;
     for (int k0 = 0; k0 < V; k0 += NT) {
         const int k = k0 + threadIdx.x;
         if (k < V) {
@@ -1968,12 +2278,12 @@ __global__ void __launch_bounds__(NT) k_refl_direct(
             const float dl = (float)((r - itd) / (double)kp.rng_res);
             const float2 ph = phasor_cycles(-2.0 * itd * kp.inv_cyc);
             const float sc = sinc_f(dl) * fre_s[j];
-            acc.x += ph.x * sc;
-            acc.y += ph.y * sc;
+            ax += (double)(ph.x * sc);
+            ay += (double)(ph.y * sc);
         }
         __syncthreads();
     }
-    if (ir < kp.nr) refl[(size_t)s * kp.nr + ir] = acc;
+    if (ir < kp.nr) refl[(size_t)s * kp.nr + ir] = make_float2((float)ax, (float)ay);
 }
 
 __global__ void __launch_bounds__(NT) k_refr0_direct(
@@ -1991,7 +2301,7 @@ __global__ void __launch_bounds__(NT) k_refr0_direct(
     __shared__ double rng_s[NT];
     __shared__ float rc_s[NT];
     const double r = (double)sh.rst + (double)ir * kp.drd;
-    float2 acc = make_float2(0.f, 0.f);
+    double ax = 0.0, ay = 0.0;  // FP64 sums, as in k_refl_direct
     for (int ti = 0; ti < n0; ++ti) {
         const int t = list0[(size_t)s * Tcap + ti];
         const double tx = tg[t], ty = tg[Tpad + t], tz = tg[2 * Tpad + t];
@@ -2059,7 +2369,7 @@ __global__ void __launch_bounds__(NT) k_refr0_direct(
                 // refrRadarSignal passes d_Tth to rerad_funct (sim.cu:930-934)
                 rc_s[threadIdx.x] =
                     rerad_f(kp.rerad, atan2f(csT.y, cT)) * EI * EO * req;
-                rng_s[threadIdx.x] = madd_nc(sl - Ttd, Ttd, (double)(kp.c / kp.c2));
+                rng_s[threadIdx.x] = madd_nc(sl - Ttd, Ttd, kp.cc2);
             }
             __syncthreads();
             const int m = min(NT, V - k0);
@@ -2068,13 +2378,13 @@ __global__ void __launch_bounds__(NT) k_refr0_direct(
                 const float dl = (float)((r - rngt) / (double)kp.rng_res);
                 const float2 ph = phasor_cycles(-2.0 * rngt * kp.inv_cyc);
                 const float sc = sinc_f(dl) * rc_s[j];
-                acc.x += ph.x * sc;
-                acc.y += ph.y * sc;
+                ax += (double)(ph.x * sc);
+                ay += (double)(ph.y * sc);
             }
             __syncthreads();
         }
     }
-    if (ir < kp.nr) refr[(size_t)s * kp.nr + ir] = acc;
+    if (ir < kp.nr) refr[(size_t)s * kp.nr + ir] = make_float2((float)ax, (float)ay);
 }
 
 // ---------------------------------------------------------------------------
@@ -2106,6 +2416,21 @@ KPar make_kpar(const erp_params& p, int P_fft) {
     k.inv_dr = 1.0 / (double)p.dr;
     k.cos_ap = p.cos_ap;
     k.n_d = std::sqrt((double)p.eps_2);
+    k.c1c2 = (double)p.c_1 / (double)p.c_2;
+    k.cc2 = (double)p.c / (double)p.c_2;
+    k.has3 = p.eps_3 > 0.f ? 1 : 0;
+    k.zi = p.z_int;
+    k.dn = 0.0;
+    k.tsq = 1.f;
+    k.n_lo = k.n_hi = k.n_d;
+    if (k.has3) {
+        const double n3 = std::sqrt((double)p.eps_3);
+        const double rho = (k.n_d - n3) / (k.n_d + n3);
+        k.dn = n3 - k.n_d;
+        k.tsq = (float)std::sqrt(1.0 - rho * rho);
+        k.n_lo = std::min(k.n_d, n3);
+        k.n_hi = std::max(k.n_d, n3);
+    }
     // phasor(r) = exp(-i (2 PI_F / lam) r) = exp(-i 2 pi r * inv_cyc)
     k.inv_cyc = (double)ERP_PI_F / (M_PI * (double)p.lam);
     k.lam_d = p.lam;
@@ -2288,7 +2613,23 @@ void mesh_from_heights(const double* heights, int64_t nx, int64_t ny, double fs,
 
 using TargetAccumFn = decltype(&k_target_accum_fast<false>);
 
-TargetAccumFn pick_target_accum(bool det, int ph, int acc) {
+TargetAccumFn pick_target_accum(bool det, int ph, int acc, bool s3) {
+    if (s3) {
+        // second subsurface medium: its own instantiations, so that runs
+        // without it keep the kernel as it was tuned
+        if (det) return k_target_accum_fast<true, PH_F64, ACC_F64, true>;
+        switch (ph * 3 + acc) {
+        case PH_F64 * 3 + ACC_F32: return k_target_accum_fast<false, PH_F64, ACC_F32, true>;
+        case PH_F64 * 3 + ACC_COMP: return k_target_accum_fast<false, PH_F64, ACC_COMP, true>;
+        case PH_F32 * 3 + ACC_F64: return k_target_accum_fast<false, PH_F32, ACC_F64, true>;
+        case PH_F32 * 3 + ACC_F32: return k_target_accum_fast<false, PH_F32, ACC_F32, true>;
+        case PH_F32 * 3 + ACC_COMP: return k_target_accum_fast<false, PH_F32, ACC_COMP, true>;
+        case PH_DF * 3 + ACC_F64: return k_target_accum_fast<false, PH_DF, ACC_F64, true>;
+        case PH_DF * 3 + ACC_F32: return k_target_accum_fast<false, PH_DF, ACC_F32, true>;
+        case PH_DF * 3 + ACC_COMP: return k_target_accum_fast<false, PH_DF, ACC_COMP, true>;
+        default: return k_target_accum_fast<false, PH_F64, ACC_F64, true>;
+        }
+    }
     if (det) return k_target_accum_fast<true>;
     switch (ph * 3 + acc) {
     case PH_F64 * 3 + ACC_F32: return k_target_accum_fast<false, PH_F64, ACC_F32>;
@@ -2381,6 +2722,10 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         throw ErpError{ERP_E_UNSUPPORTED,
                        "targets of type != 0 need convolution and convolution_linear "
                        "(sim.cu:410,872,912 otherwise use unallocated buffers)"};
+    if (p.eps_3 > 0.f && (spec || any_t0))
+        throw ErpError{ERP_E_UNSUPPORTED,
+                       "the second subsurface medium (eps_3) is defined for the plain "
+                       "method's targets of type != 0"};
     if (F == 0 || ns == 0) {
         // no facets: no surface echo and no refracted path, so every trace is
         // zero and no facet is illuminated
@@ -2591,10 +2936,11 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         PT_.begin(ERP_STAGE_MASK, st, (double)F * S);
         hipLaunchKernelGGL(k_illum_mask, dim3(nblk, S), dim3(NT), 0, st, mesh->soa.as<float>(),
                            mesh->Fpad, F, dshots, p.cos_ap, dmask, W.cnt.as<int>(), nblk,
-                           tile_nx);
+                           tile_nx, (const TrackK*)nullptr, (const RowK*)nullptr);
         ERP_CHECK_LAUNCH("k_illum_mask");
         hipLaunchKernelGGL(k_scan_blocks, dim3(S), dim3(1024), 0, st, W.cnt.as<int>(),
-                           W.off.as<int>(), dV, nblk);
+                           W.off.as<int>(), dV, nblk, (const TrackK*)nullptr,
+                           (const RowK*)nullptr);
         ERP_CHECK_LAUNCH("k_scan_blocks");
         PT_.end(st);
         ERP_HIP_TRY(hipMemcpyAsync(hV.data(), dV, sizeof(int) * S, hipMemcpyDeviceToHost, st));
@@ -2664,14 +3010,15 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         hipLaunchKernelGGL(k_surface, dim3(nblk, S), dim3(NT), 0, st, mesh->soa.as<float>(),
                            mesh->Fpad, F, dshots, dmask, W.off.as<int>(), dV, dbase, nblk, kp, rec,
                            write, do_trace ? 1 : 0, srow, s0, dbounds, det ? 1 : 0, smax,
-                           tile_nx);
+                           tile_nx, ph, acc, (const TrackK*)nullptr, (const RowK*)nullptr);
         ERP_CHECK_LAUNCH("k_surface");
         if (det && do_trace) {
             // second pass: the same terms as fixed-point values scaled by the
             // per-shot maxima of the first
             hipLaunchKernelGGL(k_surface, dim3(nblk, S), dim3(NT), 0, st, mesh->soa.as<float>(),
                                mesh->Fpad, F, dshots, dmask, W.off.as<int>(), dV, dbase, nblk, kp,
-                               rec, 0, 1, srow, s0, (Bounds*)nullptr, 2, smax, tile_nx);
+                               rec, 0, 1, srow, s0, (Bounds*)nullptr, 2, smax, tile_nx, 0,
+                               0, (const TrackK*)nullptr, (const RowK*)nullptr);
             ERP_CHECK_LAUNCH("k_surface");
         }
         if (do_trace) {
@@ -2723,7 +3070,8 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
             hipLaunchKernelGGL(k_target_select, dim3(S), dim3(1024), 0, st,
                                targets->xyzn.as<float>(), Tpad, (int)T,
                                targets->type.as<unsigned char>(), dshots, dV, p.cos_ap, list1,
-                               list0, W.tcnt.as<int>(), Tcap);
+                               list0, W.tcnt.as<int>(), Tcap, (const TrackK*)nullptr,
+                               (const RowK*)nullptr);
             ERP_CHECK_LAUNCH("k_target_select");
             PT_.end(st);
             ERP_HIP_TRY(hipMemcpyAsync(htc.data(), W.tcnt.p, sizeof(int) * 2 * S,
@@ -2816,7 +3164,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                     for (int i = 0; i < n; ++i) triples += hV[pshot[i]];
                     PT_.begin(ERP_STAGE_TARGET_ACCUM, st, triples);
                     if (!spec) {
-                        auto kern = pick_target_accum(det, ph, acc);
+                        auto kern = pick_target_accum(det, ph, acc, kp.has3 != 0);
                         // persistent grid: two workgroups per CU (the LDS window
                         // allows two), each taking pairs until none is left
                         const int grid = std::max(1, std::min(n, 2 * ctx->num_cu));
@@ -2824,7 +3172,9 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                                            targets->xyzn.as<float>(), Tpad, dps, dpt, dshots, dV,
                                            dbase, rec, dbounds, dpr, (int)nprisms, kp, A,
                                            (fused || half) ? dpb : nullptr, dpw,
-                                           mesh->frame_ok ? 1 : 0, n, dnext);
+                                           mesh->frame_ok ? 1 : 0, n, dnext,
+                                           (const TrackK*)nullptr, (const RowK*)nullptr,
+                                           (const int*)nullptr);
                         ERP_CHECK_LAUNCH("k_target_accum_fast");
                     } else {
                         hipLaunchKernelGGL(k_target_accum<1>, dim3(n), dim3(NTT),

@@ -2,7 +2,7 @@
 
 Mirrors src/CUDA/file_io.cu:284-624 (facets, targets, sources, gains, rx
 windows, attenuation prisms, per-shot trace text) and the Python writers
-src/PYTHON/simple_surfaces.py:216-258.
+src/PYTHON/simple_surfaces.py:65-108.
 
 Record counting: the reference's count_lines returns newlines + 1
 (file_io.cu:314-335), so a file that ends with a newline gains one
@@ -167,7 +167,7 @@ def load_prisms(path: str, par) -> np.ndarray:
 
 def export_facets(fxs, fys, fzs, fnxs, fnys, fnzs, fuxs, fuys, fuzs, fvxs, fvys,
                   fvzs, filename, obj=None):
-    """Facet writer, `%.6f` with no trailing newline (simple_surfaces.py:216-236)."""
+    """Facet writer, `%.6f` with no trailing newline (simple_surfaces.py:65-86)."""
     cols = np.stack([fxs, fys, fzs, fnxs, fnys, fnzs, fuxs, fuys, fuzs, fvxs,
                      fvys, fvzs], axis=1).astype(np.float64)
     fmt = "%.6f,%.6f,%.6f:%.6f,%.6f,%.6f:%.6f,%.6f,%.6f:%.6f,%.6f,%.6f"
@@ -177,7 +177,7 @@ def export_facets(fxs, fys, fzs, fnxs, fnys, fnzs, fuxs, fuys, fuzs, fvxs, fvys,
 
 
 def export_targets(fxs, fys, fzs, fnxs, fnys, fnzs, filename, obj=None, ttype=1):
-    """Target writer, type 1, no trailing newline (simple_surfaces.py:239-258)."""
+    """Target writer, type 1, no trailing newline (simple_surfaces.py:88-108)."""
     cols = np.stack([fxs, fys, fzs, fnxs, fnys, fnzs], axis=1).astype(np.float64)
     fmt = "%.6f,%.6f,%.6f,%.6f,%.6f,%.6f," + str(int(ttype))
     with open(filename, "w") as f:

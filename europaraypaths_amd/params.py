@@ -225,6 +225,10 @@ class SimulationParameters:
     sdx: np.float32 = f32(0)
     ns: int = 0
     seed: int = 0
+    # second subsurface medium below z = z_int (an extension of the
+    # reference's single eps_2; eps_3 = 0 disables it)
+    eps_3: np.float32 = f32(0)
+    z_int: np.float32 = f32(0)
 
     @property
     def mode(self) -> int:
@@ -261,6 +265,8 @@ class SimulationParameters:
         e.mode = self.mode
         e.cos_ap = math.cos(float(self.aperture_threshold))
         e.seed = int(self.seed) & 0xFFFFFFFFFFFFFFFF
+        e.eps_3 = float(self.eps_3)
+        e.z_int = float(self.z_int)
         return e
 
 
@@ -269,6 +275,10 @@ def parse_parameters(j: dict | str) -> SimulationParameters:
 
     Raises KeyError for a missing required key (the reference throws), and
     ValueError where the reference would compute a non-positive bin count.
+    Two optional keys extend the reference's single subsurface medium:
+    "eps_3", the permittivity below the horizontal interface at
+    z = "z_interface" (m); absent or 0, there is one medium as in the
+    reference.
     """
     if isinstance(j, (str, os.PathLike)):
         with open(j) as f:
@@ -348,6 +358,7 @@ def parse_parameters(j: dict | str) -> SimulationParameters:
         atten_geom_path=j.get("attenuation_geometry_file", "NONE"),
         source_normal_multiplier=mult, altitude=altitude, sy=sy, sz=sz, sx0=sx0,
         sdx=sdx, ns=ns, seed=int(j.get("seed", 0)),
+        eps_3=f32(j.get("eps_3", 0.0)), z_int=f32(j.get("z_interface", 0.0)),
     )
 
 

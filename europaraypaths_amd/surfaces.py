@@ -58,12 +58,12 @@ def gen_convex(params, fxxs, fyys, fzzs):
 
 
 def calc_fresnel(params, debug=False):
-    """First Fresnel zone radius over a flat surface (simple_surfaces.py:275-278)."""
+    """First Fresnel zone radius over a flat surface (simple_surfaces.py:124-127)."""
     return np.sqrt((params["sz"] + params["lam"] / 4) ** 2 - params["sz"] ** 2)
 
 
 def calc_fresnel_spherical(params, debug=False):
-    """Fresnel radius over a sphere of radius = altitude (simple_surfaces.py:280-285)."""
+    """Fresnel radius over a sphere of radius = altitude (simple_surfaces.py:129-134)."""
     h = r = params["sz"]
     return np.sqrt((params["lam"] * h * r) / (2 * (h + r)))
 
@@ -96,7 +96,7 @@ def mesh_soa(fxxs, fyys, fzzs, params, mask=None) -> np.ndarray:
 
 
 def surface_grid(params, stype):
-    """Heights and mask of a reference surface type (simple_surfaces.py:298-317)."""
+    """Heights and mask of a reference surface type (simple_surfaces.py:136-170)."""
     fxxs, fyys = _grid(params)
     fzzs = np.zeros_like(fxxs)
     mask = None
@@ -114,7 +114,7 @@ def surface_grid(params, stype):
 
 
 def make_surface(params, stype, filename=None, obj=None):
-    """Build a surface, write its .fct file (simple_surfaces.py:298-352).
+    """Build a surface, write its .fct file (simple_surfaces.py:147-201).
 
     Returns the (12, F) float32 SoA (the reference returns nothing).  With
     filename=None nothing is written.
@@ -136,7 +136,7 @@ def make_surface(params, stype, filename=None, obj=None):
 
 
 def make_target_array(params, ttype, filename=None, obj=None, zoffset=0):
-    """Layer of point targets on the surface grid (simple_surfaces.py:355-405).
+    """Layer of point targets on the surface grid (simple_surfaces.py:204-253).
 
     Returns (xyzn (6, T) float32, type (T,) uint8 == 1).
     """

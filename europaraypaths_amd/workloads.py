@@ -9,9 +9,11 @@ fbm_two_layer    : 2048 x 2048 facets at 50 m, fBm (Hurst 0.8, RMS 10 m,
                    seed 2), target layers at 1.5 km and 6 km depth on a 500 m
                    grid (205 x 205 each), h = 50 km, 2000 shots, REASON_VHF
                    (60 MHz / 10 MHz), 16384 bins at 48 MHz, aperture 10 deg.
-The reference models one subsurface medium (a single eps_2, file_io.cu:81),
-so both layers of fbm_two_layer sit in eps_2 = 3.15; the 3.4 of the deeper
-interface has no counterpart in the reference's model.
+The reference models one subsurface medium (a single eps_2, file_io.cu:81).
+fbm_two_layer adds the second one of this build's extension (erp_params
+eps_3 / z_int): ice of eps 3.15 down to the 1.5 km interface and eps 3.4
+below it, so the 6 km layer's rays travel partly through each; with
+eps2_only=True both layers sit in eps 3.15, the reference's model.
 """
 from __future__ import annotations
 
@@ -63,15 +65,17 @@ def gaussian_layer(ns=100, scale=1.0, aperture=10.0, with_targets=True):
 
 
 def fbm_two_layer(ns=2000, grid=2048, layer_grid=205, aperture=10.0, with_targets=True,
-                  seed=2):
-    """The 2048 x 2048 fBm workload with two target layers."""
+                  seed=2, eps2_only=False):
+    """The 2048 x 2048 fBm workload with two target layers (eps 3.15 above
+    the 1.5 km interface, 3.4 below it)."""
     fs = 50.0
     ox = oy = -fs * grid / 2
     h = fbm_heights(grid, grid, fs, 10.0, 0.8, seed=seed)
     soa = terrain_soa(grid, grid, fs, ox, oy, h)
+    extra = {} if eps2_only else {"eps_3": 3.4, "z_interface": -1500.0}
     p = make_params("REASON_VHF", "planetary_ice", nr=16384, smpl=48e6, rst=49000.0,
                     aperture=aperture, fs=fs, ns=ns, sx0=-49975.0, sdx=50.0, sy=0.0,
-                    sz=50e3)
+                    sz=50e3, **extra)
     tg = None
     if with_targets:
         sp = 500.0

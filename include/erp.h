@@ -64,13 +64,14 @@ extern "C" {
                                        only.  Replaces the order-dependent float
                                        atomicAdd binning of radar_kernels.cu:
                                        754-758 and 821-825                      */
-/* Precision study of the subsurface-target accumulation (non-specular
- * method, not with ERP_RUN_DETERMINISTIC); the surface echo is unaffected.
+/* Precision study of the surface echo and the subsurface-target
+ * accumulation (non-specular method, not with ERP_RUN_DETERMINISTIC).
  * PHASE_*: arithmetic of Itd, Ttd, rngt = Itd + n Ttd, the bin coordinate
  * and the phase: FP32 as the reference computes them (radar_kernels.cu:
- * 681-686, 811-813) or double-float; the default is FP64.  ACC_*: the bin
- * window sums: plain FP32, or FP32 with the rounding error of every
- * addition recovered by TwoSum and summed beside it; the default is FP64. */
+ * 681-686, 733-737, 811-813) or double-float; the default is FP64.  ACC_*:
+ * the LDS bin window sums: plain FP32, or FP32 with the rounding error of
+ * every addition recovered by TwoSum and summed beside it; the default is
+ * FP64.  The facet records the target stage reads keep the FP64 Itd. */
 #define ERP_RUN_PHASE_F32     0x02u
 #define ERP_RUN_PHASE_DF      0x04u
 #define ERP_RUN_RAW_SURFACE   0x08u /* output the raw surface phasor trace before
@@ -111,6 +112,18 @@ typedef struct erp_params {
                            against it (geometry_kernels.cu:95-96,183) */
     uint64_t seed;      /* incoherent-mode RNG seed (replaces clock64(),
                            radar_kernels.cu:689-711) */
+    /* Second subsurface medium (an extension: the reference models one
+     * medium eps_2, file_io.cu:81).  Below the horizontal interface z = z_int
+     * the permittivity is eps_3; eps_3 <= 0 disables it.  The forced ray of
+     * a subsurface target (compTargetRays, geometry_kernels.cu:272-350) then
+     * has the optical length n_2 L_2 + n_3 L_3 over its parts above and
+     * below the interface, and a ray that crosses it carries the
+     * normal-incidence power transmission 1 - rho_23^2 each way,
+     * rho_23 = (n_2 - n_3) / (n_2 + n_3).  With eps_3 = eps_2 every result
+     * equals the single-medium one.  Plain method (non-specular, targets of
+     * type != 0) only. */
+    float eps_3;
+    float z_int;
 } erp_params;
 
 /* One shot (source position).  sx, sy, sz carry the reference's float source

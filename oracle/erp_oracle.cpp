@@ -315,6 +315,7 @@ template <class R> struct Par {
     double cos_ap;
     uint64_t seed;
     bool conv, linear, specular, lossless, incoherent, disable_surface;
+    R eps_3, z_int;  /* second subsurface medium (eps_3 <= 0: none) */
     explicit Par(const erp_params& p) {
         lam = p.lam; rng_res = p.rng_res; P = p.P; fs = p.fs; ks = p.ks;
         eps_1 = p.eps_1; eps_2 = p.eps_2; c = p.c; c_1 = p.c_1; c_2 = p.c_2;
@@ -328,8 +329,41 @@ template <class R> struct Par {
         lossless = mode & ERP_MODE_LOSSLESS;
         incoherent = mode & ERP_MODE_INCOHERENT;
         disable_surface = mode & ERP_MODE_DISABLE_SURFACE;
+        eps_3 = p.eps_3; z_int = p.z_int;
     }
 };
+
+/* Second subsurface medium (an extension of the reference, which has one
+ * medium eps_2, file_io.cu:81): the part of the straight facet -> target
+ * segment that lies below the horizontal interface z = z_int, as a fraction
+ * of its length.  0 when the target is not below the interface, 1 when the
+ * facet is not above it. */
+template <class R> inline R below_fraction(R fz, R tz, R zi) {
+    if (!(tz < zi)) return 0;
+    if (!(fz > zi)) return 1;
+    return (zi - tz) / (fz - tz);
+}
+/* Optical range of a target term: rngt = Itd + n Ttd (accumulateTarget,
+ * radar_kernels.cu:767-830) plus (n_3 - n) times the length below the
+ * interface; *tr receives the two-way-per-leg power transmission 1 - rho^2
+ * of the interface (1 when the ray does not cross it). */
+template <class R>
+inline R target_range(const Par<R>& p, R Itd, R Ttd, R n, R fz, R tz, R* tr) {
+    R rngt = Itd + Ttd * n;
+    *tr = 1;
+    if (p.eps_3 > 0) {
+        const R frac = below_fraction<R>(fz, tz, p.z_int);
+        if (frac > 0) {
+            const R n3 = std::sqrt(p.eps_3);
+            rngt = rngt + (Ttd * frac) * (n3 - n);
+            if (frac < 1) {
+                const R rho = (n - n3) / (n + n3);
+                *tr = 1 - rho * rho;
+            }
+        }
+    }
+    return rngt;
+}
 
 struct MeshView {
     const float *x, *y, *z, *nx, *ny, *nz, *ux, *uy, *uz, *vx, *vy, *vz;
@@ -686,6 +720,12 @@ inline int check_modes(const erp_params& p, const uint8_t* ttype, int64_t T) {
     for (int64_t t = 0; t < T; ++t)
         if (ttype && ttype[t] != 0) any_typed = true;
     if (any_typed && !(conv && lin)) return ERP_E_UNSUPPORTED;
+    /* the second medium is defined for the plain method's typed targets */
+    if (p.eps_3 > 0) {
+        if (spec) return ERP_E_UNSUPPORTED;
+        for (int64_t t = 0; t < T; ++t)
+            if (ttype && ttype[t] == 0) return ERP_E_UNSUPPORTED;
+    }
     return ERP_OK;
 }
 
@@ -718,7 +758,10 @@ void target_one(const Par<R>& p, const MeshView& m, const std::vector<FacetRec<R
             R Gd = facet_G<R>(q[k].TargetTh, q[k].Tph, p.lam, p.fs);
             R G1 = facet_G<R>(f.Ith, f.Iph, p.lam, p.fs);
             R G2 = facet_G<R>(q[k].Tth, q[k].Tph, p.lam, p.fs);
-            R rngt = f.Itd + q[k].Ttd * (g_mut == MUT_REFR_INDEX_DROPPED ? (R)1 : n);
+            R tr;
+            R rngt = target_range<R>(p, f.Itd, q[k].Ttd,
+                                     g_mut == MUT_REFR_INDEX_DROPPED ? (R)1 : n,
+                                     (R)m.z[f.idx], (R)tz, &tr);
             int b;
             R wgt;
             if (!bin_of<R>(rngt, rst, dr, nr, &b, &wgt)) continue;
@@ -728,8 +771,8 @@ void target_one(const Par<R>& p, const MeshView& m, const std::vector<FacetRec<R
             R Pout = friis<R>((R)1, Grefr, G1, p.lam, f.Itd);
             Pout = friis<R>(Pout, G2, Gd, p.lam, q[k].Ttd);
             if (!p.lossless) {
-                Pin = Pin * q[k].EI;
-                Pout = Pout * q[k].EO;
+                Pin = Pin * q[k].EI * tr;
+                Pout = Pout * q[k].EO * tr;
             }
             split_add<R>(PT.data(), nr, b, wgt,
                          phasor_neg<R>(rngt, p.lam) * std::sqrt(Pin * n));
@@ -1053,7 +1096,9 @@ static int pairs_impl(const erp_params* par, const float* soa, int64_t F,
             Pout = Pout * q.EO;
         }
         o[0] = ill[i]; o[1] = q.Ttd; o[2] = q.Tth; o[3] = q.TargetTh;
-        o[4] = q.Tph; o[5] = q.EI; o[6] = q.EO; o[7] = f.Itd + q.Ttd * n;
+        R tr;
+        o[4] = q.Tph; o[5] = q.EI; o[6] = q.EO;
+        o[7] = target_range<R>(p, f.Itd, q.Ttd, n, (R)m.z[i], (R)tgt[2 * T + it], &tr);
         o[8] = Pin; o[9] = Pout;
     }
     return ERP_OK;

@@ -10,8 +10,9 @@ summed in FP64 (the product), plain FP32 and compensated FP32.
 
 The scene is the fBm geometry scaled down so the FP64 oracle finishes in
 seconds: a 192 x 192 mesh at 50 m (Hurst 0.8, 10 m RMS), two target layers
-at 1.5 km and 6 km depth on a 500 m grid over the mesh, the surface echo
-off, 4096 bins at 48 MHz from 399 km, 10 deg aperture (every facet lit).
+at 1.5 km and 6 km depth on a 500 m grid over the mesh with eps 3.15 above
+the 1.5 km interface and 3.4 below it, the surface echo off, 4096 bins at
+48 MHz from 399 km, 10 deg aperture (every facet lit).
 Errors are taken over bins within 60 dB of each trace's peak.
 
 Run as a script on a GPU host:  python tests/precision_study.py [--json out]
@@ -40,14 +41,17 @@ PHASES = {"fp64": 0, "double-float": ERP_RUN_PHASE_DF, "fp32": ERP_RUN_PHASE_F32
 ACCUMS = {"fp64": 0, "fp32": ERP_RUN_ACC_F32, "fp32-compensated": ERP_RUN_ACC_COMP}
 
 
-def orbital_scene(platform, grid=192, ns=2, nr=4096, h=400e3):
+def orbital_scene(platform, grid=192, ns=2, nr=4096, h=400e3, surface=False):
+    """The orbital scene; surface=True keeps the surface echo on (the
+    study's surface leg runs it without targets)."""
     fs = 50.0
     ox = oy = -fs * grid / 2
     z = fbm_heights(grid, grid, fs, 10.0, 0.8, seed=3)
     soa = terrain_soa(grid, grid, fs, ox, oy, z)
     p = workloads.make_params(platform, "planetary_ice", nr=nr, smpl=48e6, rst=h - 1000.0,
                               aperture=10.0, fs=fs, ns=ns, sx0=-25.0 * (ns - 1), sdx=50.0,
-                              sy=0.0, sz=h, disable_surface=True)
+                              sy=0.0, sz=h, disable_surface=not surface, eps_3=3.4,
+                              z_interface=-1500.0)
     par = parse_parameters(p)
     m = max(2, int(grid * fs / 500.0))
     lo = -500.0 * (m - 1) / 2
@@ -75,29 +79,37 @@ def errors(g, o, floor_db=60.0):
             "abs_max": ab, "bins": int(ph.size)}
 
 
-def run_study(ctx, platforms=("REASON_HF", "REASON_VHF"), grid=192, ns=2, combos=None):
+def run_study(ctx, platforms=("REASON_HF", "REASON_VHF"), grid=192, ns=2, combos=None,
+              legs=("targets", "surface")):
+    """legs: "targets" = the two target layers with the surface echo off,
+    "surface" = the surface echo alone (no targets)."""
     from oracle import oracle
     rows = []
     combos = combos or [(p, a) for p in PHASES for a in ACCUMS]
-    for plat in platforms:
-        par, soa, tg, shots = orbital_scene(plat, grid=grid, ns=ns)
+    cases = [(leg, plat) for leg in legs for plat in platforms]
+    for leg, plat in cases:
+        par, soa, tg, shots = orbital_scene(plat, grid=grid, ns=ns, surface=leg == "surface")
+        if leg == "surface":
+            tg = None
         e = par.to_erp()
         ref, rv = oracle.run(e, soa, tg, shots)
         m = ctx.upload_mesh(soa)
-        t = ctx.upload_targets(*tg)
+        t = ctx.upload_targets(*tg) if tg is not None else None
         try:
             for pn, an in combos:
                 g, gv = ctx.run(e, m, t, shots, flags=PHASES[pn] | ACCUMS[an])
                 assert np.array_equal(gv, rv)
-                r = {"chirp_MHz": float(par.f0) / 1e6, "phase": pn, "accumulation": an}
+                r = {"leg": leg, "chirp_MHz": float(par.f0) / 1e6, "phase": pn,
+                     "accumulation": an}
                 r.update(errors(g, ref))
                 rows.append(r)
-                print(f"{r['chirp_MHz']:5.0f} MHz  phase {pn:13s} acc {an:17s} "
+                print(f"{leg:8s} {r['chirp_MHz']:5.0f} MHz  phase {pn:13s} acc {an:17s} "
                       f"phase max {r['phase_max']:.2e} rms {r['phase_rms']:.2e}  "
                       f"amp max {r['amp_max']:.2e} rms {r['amp_rms']:.2e}", flush=True)
         finally:
             m.release()
-            t.release()
+            if t is not None:
+                t.release()
     return rows
 
 
@@ -110,11 +122,11 @@ def main():
     from europaraypaths_amd._native import Context
     ctx = Context(0)
     rows = run_study(ctx, grid=a.grid, ns=a.shots)
-    print("\n| chirp | phase arithmetic | accumulation | phase max (rad) | phase rms | "
+    print("\n| leg | chirp | phase arithmetic | accumulation | phase max (rad) | phase rms | "
           "amp max (rel) | amp rms |")
-    print("|---|---|---|---|---|---|---|")
+    print("|---|---|---|---|---|---|---|---|")
     for r in rows:
-        print(f"| {r['chirp_MHz']:.0f} MHz | {r['phase']} | {r['accumulation']} | "
+        print(f"| {r['leg']} | {r['chirp_MHz']:.0f} MHz | {r['phase']} | {r['accumulation']} | "
               f"{r['phase_max']:.2e} | {r['phase_rms']:.2e} | {r['amp_max']:.2e} | "
               f"{r['amp_rms']:.2e} |")
     if a.json:

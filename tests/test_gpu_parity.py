@@ -294,3 +294,42 @@ def test_tiled_surface_order(gpu_ctx):
     np.testing.assert_array_equal(gv, ov)
     compare_traces(g, o)
     compare_traces(ga, o)
+
+
+@pytest.mark.parametrize("nr", [1024, 4096, 5000])
+def test_second_subsurface_medium(gpu_ctx, nr):
+    """Targets above and below an interface at z = -200 m with eps 3.4 below
+    it (the extension of erp_params eps_3 / z_int): against the FP64 oracle
+    in the default and the deterministic mode (per-pair chain at 1024 and
+    4096 bins, unfused chain at 5000)."""
+    print(f"second medium nr={nr}", flush=True)
+    par, soa, shots = scene(rough=0.3, nr=nr, h=3000.0, rst=2600.0, ns=2, eps_3=3.4,
+                            z_interface=-200.0)
+    assert float(par.eps_3) == pytest.approx(3.4) and float(par.z_int) == -200.0
+    xyzn = np.zeros((6, 4), np.float32)
+    xyzn[0] = [-100.0, -20.0, 60.0, 110.0]
+    xyzn[1] = 13.0
+    xyzn[2] = [-150.0, -300.0, -450.0, -200.0]
+    xyzn[5] = 1.0
+    tg = (xyzn, np.ones(4, np.uint8))
+    for flags in (0, ERP_RUN_DETERMINISTIC):
+        g, gv, o, ov = run_both(gpu_ctx, par, soa, shots, targets=tg, flags=flags)
+        np.testing.assert_array_equal(gv, ov)
+        print(f"  flags={flags} worst errors {compare_traces(g, o)}", flush=True)
+
+
+def test_second_medium_equal_permittivity_is_bitwise_single_medium(gpu_ctx):
+    """eps_3 = eps_2 runs the second-medium kernel variant, which must give
+    the single-medium result bit for bit in the deterministic mode."""
+    par, soa, shots = scene(rough=0.3, nr=1024, h=3000.0, rst=2600.0, ns=2)
+    tg = _targets(par, n=3, depth=350.0)
+    m = gpu_ctx.upload_mesh(soa)
+    t = gpu_ctx.upload_targets(*tg)
+    e = par.to_erp()
+    a, _ = gpu_ctx.run(e, m, t, shots, flags=ERP_RUN_DETERMINISTIC)
+    e.eps_3, e.z_int = e.eps_2, -200.0
+    b, _ = gpu_ctx.run(e, m, t, shots, flags=ERP_RUN_DETERMINISTIC)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    e.eps_3 = 3.4
+    c, _ = gpu_ctx.run(e, m, t, shots, flags=ERP_RUN_DETERMINISTIC)
+    assert not np.array_equal(a, c)

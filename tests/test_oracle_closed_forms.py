@@ -400,6 +400,99 @@ def test_interface_transmission_two_way():
     assert abs(err) < 2e-3, err
 
 
+# --------------------------------------------------- second subsurface medium
+
+def _two_medium_nadir(h, d, eps3, zi, lossless=True):
+    """A point target d deep under the nadir of a flat surface of 2.5 m
+    facets, surface echo off, with (eps3 > 0) the medium below z = zi of
+    permittivity eps3.  Returns the oracle trace, the vertical optical range
+    h + n2 d1 + n3 d2 and the parameters."""
+    n, fs = 400, 2.5
+    soa = workloads.flat_soa(n, n, fs, -fs * n / 2, -fs * n / 2)
+    n2 = math.sqrt(3.15)
+    extra = {"eps_3": eps3, "z_interface": zi} if eps3 else {}
+    rng = h + n2 * (-zi) + math.sqrt(eps3) * (d + zi) if eps3 else h + n2 * d
+    par = parse_parameters(workloads.make_params(
+        "REASON_VHF", "planetary_ice", nr=250, smpl=60e6, rst=rng - 500.0, aperture=70.0,
+        fs=fs, ns=1, sx0=0.0, sdx=0.0, sy=0.0, sz=h, lossless=lossless, disable_surface=True,
+        **extra))
+    xyzn = np.zeros((6, 1), np.float32)
+    xyzn[2], xyzn[5] = -d, 1.0
+    e = par.to_erp()
+    if not lossless:
+        e.alpha1 = e.alpha2 = 0.0
+        e.ks = 0.0
+    o, _ = oracle.run(e, soa, (xyzn, np.ones(1, np.uint8)), make_shots(par))
+    return o[0], rng, par
+
+
+def test_second_medium_with_equal_permittivity_is_the_single_medium():
+    """eps_3 = eps_2: every trace equals the reference's single-medium one,
+    bit for bit (the extra range term is (n3 - n2) L3 = 0, the transmission
+    1 - 0)."""
+    from tests.test_oracle_pins import _rough_scene
+    par, soa, shots = _rough_scene(nr=512, rst=4700.0)
+    xyzn = np.zeros((6, 3), np.float32)
+    xyzn[0] = [-30.0, 10.0, 40.0]
+    xyzn[2] = [-150.0, -250.0, -400.0]
+    xyzn[5] = 1.0
+    tg = (xyzn, np.ones(3, np.uint8))
+    e = par.to_erp()
+    a, _ = oracle.run(e, soa, tg, shots)
+    e.eps_3, e.z_int = e.eps_2, -200.0
+    b, _ = oracle.run(e, soa, tg, shots)
+    np.testing.assert_array_equal(a, b)
+    # and 3.4 below the interface changes the refracted part of the trace
+    e.eps_3 = 3.15
+    _, _, pa = oracle.run(e, soa, tg, shots, parts=True)
+    e.eps_3 = 3.4
+    _, _, pc = oracle.run(e, soa, tg, shots, parts=True)
+    assert np.abs(pc[:, 2] - pa[:, 2]).max() > 0.1 * np.abs(pa[:, 2]).max()
+
+
+@pytest.mark.parametrize("eps3,zi", [(3.4, -150.0), (3.4, -300.0), (4.5, -200.0)])
+def test_second_medium_nadir_target_closed_form(eps3, zi):
+    """Stationary phase over the surface facets with the forced (straight)
+    facet -> target ray: the ray's optical length is Ttd n_eff,
+    n_eff = (n2 d1 + n3 d2) / d for the depths d1, d2 above and below the
+    interface, so the nadir target's power is the single-medium closed form
+    (subsurface_target/code/output.py:13-36) with g_r = (h + d)/(h + d/n_eff)
+    and 1/n^2 replaced by n2^2 / n_eff^4 (the amplitude keeps the reference's
+    sqrt(P n2)); its phase is that of the vertical optical range
+    h + n2 d1 + n3 d2.  The single-medium case sits at +0.10 dB and +2.6 deg
+    on this scene (the test_oracle_pins bound: 0.5 dB, 7 deg)."""
+    h, d = 20e3, 400.0
+    o, rng, par = _two_medium_nadir(h, d, eps3, zi)
+    lam, fs = float(par.lam), float(par.fs)
+    n2, n3 = math.sqrt(3.15), math.sqrt(eps3)
+    d1 = -zi
+    neff = (n2 * d1 + n3 * (d - d1)) / d
+    gr = (h + d) / (h + d / neff)
+    GT = 4 * np.pi * fs ** 2 / lam ** 2
+    want = float(par.P) * _gain(par) ** 2 * lam ** 4 * GT ** 2 / ((4 * np.pi) ** 4 * (h + d) ** 4) \
+        * gr ** 4 * n2 ** 2 / neff ** 4
+    k = int(np.argmax(np.abs(o)))
+    db = 10 * np.log10(np.abs(o[k]) ** 2 / want)
+    ph = np.degrees(np.angle(o[k] * np.exp(4j * np.pi / lam * rng)))
+    assert abs(db) < 0.5, db
+    assert abs(ph) < 7.0, ph
+
+
+def test_second_medium_interface_transmission():
+    """With no conductivity and no roughness, the nadir target's power with
+    losses on is (1 - rho12^2)^2 (1 - rho23^2)^2 times its lossless value:
+    the surface's and the interface's normal-incidence power transmission,
+    each crossed twice."""
+    h, d, eps3, zi = 20e3, 400.0, 4.5, -200.0
+    lossy, _, _ = _two_medium_nadir(h, d, eps3, zi, lossless=False)
+    clean, _, _ = _two_medium_nadir(h, d, eps3, zi, lossless=True)
+    n2, n3 = math.sqrt(3.15), math.sqrt(eps3)
+    r12 = ((1 - n2) / (1 + n2)) ** 2
+    r23 = ((n2 - n3) / (n2 + n3)) ** 2
+    ratio = np.max(np.abs(lossy) ** 2) / np.max(np.abs(clean) ** 2)
+    assert abs(ratio / ((1 - r12) ** 2 * (1 - r23) ** 2) - 1) < 2e-3, ratio
+
+
 # ------------------------------------------------------- planted mistakes
 
 def _nadir_target_errors():
