@@ -26,6 +26,8 @@ ERP_RUN_ACC_COMP = 0x20
 
 ERP_TUNE_SHOTS_PER_BATCH = 1
 ERP_TUNE_PAIRS_PER_LAUNCH = 2
+ERP_TUNE_TRACK_ROWS = 3
+ERP_TUNE_DIRECT_LAUNCH = 4
 
 _PARAM_FLOATS = ("lam", "k", "rng_res", "P", "fs", "ks", "eps_1", "eps_2", "c",
                  "c_1", "c_2", "nu_1", "nu_2", "alpha1", "alpha2", "aperture",
@@ -46,6 +48,14 @@ class ErpShot(ctypes.Structure):
                 ("sny", ctypes.c_float), ("snz", ctypes.c_float),
                 ("rst", ctypes.c_float), ("grefl_lin", ctypes.c_float),
                 ("grefr_lin", ctypes.c_float)]
+
+
+class ErpTrack(ctypes.Structure):
+    """erp_track: one job of erp_run_tracks (pointers as erp_run_dev takes them)."""
+    _fields_ = [("par", ctypes.POINTER(ErpParams)), ("mesh", ctypes.c_void_p),
+                ("targets", ctypes.c_void_p), ("shots", ctypes.c_void_p),
+                ("ns", ctypes.c_int64), ("out_dev", ctypes.c_void_p),
+                ("nvalid_dev", ctypes.c_void_p)]
 
 
 class ErpPrism(ctypes.Structure):

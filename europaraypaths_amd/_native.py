@@ -11,7 +11,7 @@ import threading
 
 import numpy as np
 
-from ._abi import (ERP_E_NOTREADY, ERP_OK, PRISM_DTYPE, SHOT_DTYPE, ErpParams)
+from ._abi import (ERP_E_NOTREADY, ERP_OK, PRISM_DTYPE, SHOT_DTYPE, ErpParams, ErpTrack)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liberp.so")
@@ -24,7 +24,7 @@ EXPORTS = ("erp_ctx_create", "erp_ctx_destroy", "erp_last_error", "erp_mesh_uplo
            "erp_run_dev", "erp_submit", "erp_poll", "erp_wait", "erp_run_footprint",
            "erp_profile_enable", "erp_profile_read", "erp_serve_slots",
            "erp_mesh_from_heights", "erp_mesh_download", "erp_focus", "erp_set_tuning",
-           "erp_get_tuning", "erp_run_dev_noalloc")
+           "erp_get_tuning", "erp_run_dev_noalloc", "erp_run_tracks")
 
 STAGES = ("mask", "surface", "surface_conv", "target_select", "target_accum", "target_fft",
           "shot_final", "other", "target_sum")
@@ -77,6 +77,8 @@ def lib() -> ctypes.CDLL:
         L.erp_run_dev.restype = i32
         L.erp_run_dev_noalloc.argtypes = run_args
         L.erp_run_dev_noalloc.restype = i32
+        L.erp_run_tracks.argtypes = [vp, P(ErpTrack), i64, u32]
+        L.erp_run_tracks.restype = i32
         L.erp_submit.argtypes = run_args + [P(vp)]
         L.erp_submit.restype = i32
         L.erp_poll.argtypes = [vp]
@@ -101,6 +103,56 @@ def lib() -> ctypes.CDLL:
 
 def _ptr(a: np.ndarray | None):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+_hip = None
+
+
+def _hiprt() -> ctypes.CDLL:
+    """The HIP runtime, for the device buffers of DeviceTraces."""
+    global _hip
+    with _lock:
+        if _hip is None:
+            h = ctypes.CDLL("libamdhip64.so")
+            h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+            h.hipFree.argtypes = [ctypes.c_void_p]
+            h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                    ctypes.c_int]
+            _hip = h
+        return _hip
+
+
+class DeviceTraces:
+    """ns x nr complex64 traces and ns int32 illuminated counts in device
+    memory: the outputs of run_dev / run_tracks."""
+
+    def __init__(self, ns: int, nr: int):
+        self.ns, self.nr = ns, nr
+        self.out, self.nvalid = ctypes.c_void_p(), ctypes.c_void_p()
+        h = _hiprt()
+        for p, n in ((self.out, ns * nr * 8), (self.nvalid, ns * 4)):
+            rc = h.hipMalloc(ctypes.byref(p), max(1, n))
+            if rc != 0:
+                self.free()
+                raise ErpError(rc, f"hipMalloc of {n} bytes failed")
+
+    def read(self):
+        """(traces (ns, nr) complex64, nvalid (ns,) int32) copied to the host."""
+        o = np.zeros((self.ns, self.nr), np.complex64)
+        v = np.zeros(self.ns, np.int32)
+        h = _hiprt()
+        for a, p in ((o, self.out), (v, self.nvalid)):
+            if a.nbytes:
+                rc = h.hipMemcpy(a.ctypes.data, p, a.nbytes, 2)  # device to host
+                if rc != 0:
+                    raise ErpError(rc, "hipMemcpy device to host failed")
+        return o, v
+
+    def free(self):
+        for p in (self.out, self.nvalid):
+            if p.value:
+                _hiprt().hipFree(p)
+                p.value = None
 
 
 class Mesh:
@@ -268,6 +320,35 @@ class Context:
             targets.handle if targets is not None else None, _ptr(shots), shots.shape[0],
             _ptr(prisms), npr, ctypes.c_void_p(out_ptr),
             ctypes.c_void_p(nvalid_ptr) if nvalid_ptr else None, flags))
+
+    def run_tracks_dev(self, jobs, flags: int = 0):
+        """erp_run_tracks over jobs = [(params, mesh, targets or None, shots,
+        DeviceTraces)]: every job's radargram and counts into its buffers."""
+        keep = []
+        arr = (ErpTrack * max(1, len(jobs)))()
+        for i, (params, mesh, targets, shots, dev) in enumerate(jobs):
+            shots = np.ascontiguousarray(shots, dtype=SHOT_DTYPE)
+            if dev.ns != shots.shape[0] or dev.nr != params.nr:
+                raise ValueError(f"job {i}: output buffer shape does not match")
+            keep.append((params, shots))
+            arr[i] = ErpTrack(ctypes.pointer(params), mesh.handle,
+                              targets.handle if targets is not None else None,
+                              shots.ctypes.data, shots.shape[0], dev.out.value,
+                              dev.nvalid.value)
+        self._check(lib().erp_run_tracks(self.handle, arr, len(jobs), flags))
+
+    def run_tracks(self, jobs, flags: int = 0):
+        """erp_run_tracks over jobs = [(params, mesh, targets or None, shots)]
+        -> [(traces (ns, nr) complex64, nvalid (ns,) int32)] on the host."""
+        devs = []
+        try:
+            for params, mesh, targets, shots in jobs:
+                devs.append(DeviceTraces(len(shots), params.nr))
+            self.run_tracks_dev([j + (d,) for j, d in zip(jobs, devs)], flags)
+            return [d.read() for d in devs]
+        finally:
+            for d in devs:
+                d.free()
 
     def submit(self, params: ErpParams, mesh: Mesh, targets: Targets | None, shots,
                prisms=None, flags: int = 0) -> Job:

@@ -311,6 +311,8 @@ void erp_ctx_destroy(erp_ctx* ctx) {
     ctx->chirps.clear();
     ctx->blues.clear();
     release_work(ctx->work);
+    mt_state_free(ctx->mt);
+    ctx->mt = nullptr;
     ctx->twiddle.release();
     for (auto& t : ctx->inter) {
         t.second->release();
@@ -472,6 +474,7 @@ int erp_targets_upload(erp_ctx* ctx, const float* xyzn, const uint8_t* type, int
         if (T > 0)
             ERP_HIP_TRY(hipMemcpyAsync(t->type.p, type, (size_t)T, hipMemcpyHostToDevice,
                                        ctx->stream));
+        t->n0 = std::count(type, type + T, (uint8_t)0);
         ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
     });
     if (rc != ERP_OK) {
@@ -507,6 +510,35 @@ int erp_run_dev(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         set_device(ctx);
         erp_run_impl(ctx, par, mesh, targets, shots, ns, prisms, nprisms, (float2*)out_dev,
                      (int32_t*)nvalid_dev, flags, ctx->stream, ctx->work, &ctx->prof);
+        ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        ctx->prof.resolve();
+    });
+}
+
+int erp_run_tracks(erp_ctx* ctx, const erp_track* tracks, int64_t ntracks, uint32_t flags) {
+    if (!ctx) return ERP_E_INVALID;
+    return guarded(ctx, [&] {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (ntracks < 0 || (ntracks > 0 && !tracks))
+            throw ErpError{ERP_E_INVALID, "invalid track list"};
+        for (int64_t i = 0; i < ntracks; ++i) {
+            const erp_track& t = tracks[i];
+            validate_params(t.par);
+            if (!t.mesh || t.ns < 0 || (t.ns > 0 && (!t.shots || !t.out_dev)))
+                throw ErpError{ERP_E_INVALID, "invalid argument in track " + std::to_string(i)};
+            if (t.mesh->ctx != ctx || (t.targets && t.targets->ctx != ctx))
+                throw ErpError{ERP_E_INVALID, "mesh/targets of track " + std::to_string(i) +
+                                                  " belong to another context"};
+        }
+        set_device(ctx);
+        // the cached range kernel spectra and Bluestein tables the batches
+        // point to stay in the cache while the run holds them
+        ctx->active.fetch_add(1);
+        struct Hold {
+            erp_ctx* c;
+            ~Hold() { c->active.fetch_sub(1); }
+        } hold{ctx};
+        erp_run_tracks_impl(ctx, tracks, ntracks, flags, ctx->stream);
         ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
         ctx->prof.resolve();
     });
@@ -656,6 +688,8 @@ int erp_set_tuning(erp_ctx* ctx, int key, int64_t value) {
     switch (key) {
         case ERP_TUNE_SHOTS_PER_BATCH: ctx->tune_shots = value; return ERP_OK;
         case ERP_TUNE_PAIRS_PER_LAUNCH: ctx->tune_pairs = value; return ERP_OK;
+        case ERP_TUNE_TRACK_ROWS: ctx->tune_rows = value; return ERP_OK;
+        case ERP_TUNE_DIRECT_LAUNCH: ctx->tune_direct = value; return ERP_OK;
         default: return ERP_E_INVALID;
     }
 }
@@ -665,6 +699,8 @@ int erp_get_tuning(erp_ctx* ctx, int key, int64_t* value) {
     switch (key) {
         case ERP_TUNE_SHOTS_PER_BATCH: *value = ctx->tune_shots; return ERP_OK;
         case ERP_TUNE_PAIRS_PER_LAUNCH: *value = ctx->tune_pairs; return ERP_OK;
+        case ERP_TUNE_TRACK_ROWS: *value = ctx->tune_rows; return ERP_OK;
+        case ERP_TUNE_DIRECT_LAUNCH: *value = ctx->tune_direct; return ERP_OK;
         default: return ERP_E_INVALID;
     }
 }

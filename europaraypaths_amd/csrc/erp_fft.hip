@@ -769,26 +769,39 @@ __global__ void __launch_bounds__(256) k_sum_slices(float4* __restrict__ partial
 
 // Level 2: S[shot] += scale * sum of the slices [beg, end) of one shot,
 // written in the permuted order of a two-pass transform when pN1 > 0.
+// seq: each slice is added to S on its own, in order (S += scale * slice),
+// so that S depends only on the slices and not on how launches group them.
 __global__ void __launch_bounds__(256) k_sum_final(float2* __restrict__ S,
                                                    const float2* __restrict__ partial,
                                                    const int* __restrict__ seg_shot,
                                                    const int* __restrict__ seg_beg,
                                                    const int* __restrict__ seg_end, int P,
                                                    float scale, int pN1, int pN2,
-                                                   const float* __restrict__ seg_scale) {
+                                                   const float* __restrict__ seg_scale, int seq) {
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= P) return;
     const int sg = blockIdx.y;
     if (seg_beg[sg] == seg_end[sg]) return;  // padding segment
     if (seg_scale) scale = seg_scale[sg];   // segments of several tracks
+    const int kd = pN1 > 0 ? (k % pN2) * pN1 + k / pN2 : k;
+    float2* d = S + (size_t)seg_shot[sg] * P + kd;
+    if (seq) {
+        float2 a = *d;
+        for (int r = seg_beg[sg]; r < seg_end[sg]; ++r) {
+            const float2 v = partial[(size_t)r * P + k];
+            const float2 t = make_float2(scale * v.x, scale * v.y);
+            a.x += t.x;
+            a.y += t.y;
+        }
+        *d = a;
+        return;
+    }
     float2 acc = make_float2(0.f, 0.f);
     for (int r = seg_beg[sg]; r < seg_end[sg]; ++r) {
         const float2 v = partial[(size_t)r * P + k];
         acc.x += v.x;
         acc.y += v.y;
     }
-    const int kd = pN1 > 0 ? (k % pN2) * pN1 + k / pN2 : k;
-    float2* d = S + (size_t)seg_shot[sg] * P + kd;
     d->x += scale * acc.x;
     d->y += scale * acc.y;
 }
@@ -980,13 +993,13 @@ void target_pairs_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const in
 void target_sum_half(const FftPlan& p, const float2* work, float2* partial, const int* sl_beg,
                      const int* sl_end, int nslices, const int* seg_shot, const int* seg_beg,
                      const int* seg_end, int nseg, float2* S, float scale, hipStream_t st,
-                     const float* seg_scale) {
+                     const float* seg_scale, bool seq) {
     const int P2 = p.N / 2;
     hipLaunchKernelGGL(k_sum_slices, dim3((P2 + 255) / 256, nslices), dim3(256), 0, st,
                        reinterpret_cast<float4*>(partial), work, sl_beg, sl_end, P2);
     ERP_CHECK_LAUNCH("k_sum_slices");
     hipLaunchKernelGGL(k_sum_final, dim3((p.N + 255) / 256, nseg), dim3(256), 0, st, S, partial,
                        seg_shot, seg_beg, seg_end, p.N, scale, p.two_pass ? p.N1 : 0,
-                       p.two_pass ? p.N2 : 0, seg_scale);
+                       p.two_pass ? p.N2 : 0, seg_scale, seq ? 1 : 0);
     ERP_CHECK_LAUNCH("k_sum_final");
 }

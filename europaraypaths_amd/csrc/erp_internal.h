@@ -145,6 +145,7 @@ struct erp_targets {
     DevBuf xyzn;  // 6 x Tpad float32
     DevBuf type;  // T uint8
     int64_t Tpad;
+    int64_t n0 = 0;  // targets of type 0
 };
 
 // FFT plan for one power-of-two size N (see erp_fft.hip)
@@ -220,6 +221,9 @@ struct StageTimer {
 
 struct ErpSlot;  // asynchronous job worker (erp_api.hip)
 
+struct MtState;
+void mt_state_free(MtState* m);
+
 struct erp_ctx {
     int device = 0;
     StageTimer prof;
@@ -245,6 +249,10 @@ struct erp_ctx {
     std::atomic<int> active{0};  // jobs running in slots
     // caps on the batch shapes (erp_set_tuning; 0 = default)
     std::atomic<int64_t> tune_shots{0}, tune_pairs{0};
+    // multi-track batches: rows per batch (0 = default), launch directly
+    // instead of replaying captured graphs (> 0)
+    std::atomic<int64_t> tune_rows{0}, tune_direct{0};
+    MtState* mt = nullptr;  // buffers and graphs of erp_run_tracks
 };
 
 // ---------------------------------------------------------------------------
@@ -271,11 +279,14 @@ void target_pairs_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const in
                        const int* pwin, int nr, int n, const float2* Gnat, float2* work,
                        hipStream_t st, const float4* const* Gpair = nullptr,
                        const int* npairs_dev = nullptr);
-// seg_scale (multi-track batches): per-segment scale instead of `scale`
+// seg_scale (multi-track batches): per-segment scale instead of `scale`;
+// seq: each slice added to S on its own, in order, so that S does not
+// depend on how launches group the slices (deterministic mode)
 void target_sum_half(const FftPlan& p, const float2* work, float2* partial, const int* sl_beg,
                      const int* sl_end, int nslices, const int* seg_shot, const int* seg_beg,
                      const int* seg_end, int nseg, float2* S, float scale, hipStream_t st,
-                     const float* seg_scale = nullptr);// natural-order copy of a forward spectrum produced by fft_forward
+                     const float* seg_scale = nullptr, bool seq = false);
+// natural-order copy of a forward spectrum produced by fft_forward
 void fft_natural_order(const FftPlan& p, const float2* src, float2* dst, hipStream_t st);
 void target_spectra_fused(erp_ctx* ctx, const FftPlan& p, float2* A, float2* Y,
                           const int* pbase, const int* pwin, int nr, int n, const float2* G,
@@ -296,6 +307,9 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                   int32_t* nvalid_dev, uint32_t flags, hipStream_t stream,
                   WorkBufs& work, StageTimer* timer, Arena* arena = nullptr);
 int64_t erp_footprint_impl(const erp_params* par, int64_t F, int64_t T, int64_t ns);
+// multi-track batches of the eligible tracks, every other track alone
+void erp_run_tracks_impl(erp_ctx* ctx, const erp_track* tracks, int64_t ntracks, uint32_t flags,
+                         hipStream_t stream);
 // focused rows row0 .. row0 + nrows - 1 of an Nr x Na radargram (erp_focus.hip)
 void focus_rows(const float2* rdr, int Nr, int Na, int row0, int nrows, const int* row_hs,
                 const int* row_nsx, const int64_t* row_off, const int* rows, const float2* filt,

@@ -475,9 +475,10 @@ __global__ void __launch_bounds__(NT) k_illum_mask(
     const TrackK* __restrict__ trk, const RowK* __restrict__ rows) {
     const int blk = blockIdx.x, s = blockIdx.y;
     if (trk) {
-        // multi-track: the row's mesh; blocks past it hold nothing
+        // multi-track: the row's mesh; blocks past it and padding rows
+        // (shot -1) hold nothing
         const TrackK& t = trk[rows[s].trk];
-        if (blk >= t.nblk) return;
+        if (blk >= t.nblk || rows[s].shot < 0) return;
         mesh = t.mesh;
         Fpad = t.Fpad;
         F = t.F;
@@ -536,7 +537,12 @@ __global__ void __launch_bounds__(1024) k_scan_blocks(const int* __restrict__ cn
                                                       const TrackK* __restrict__ trk,
                                                       const RowK* __restrict__ rows) {
     const int s = blockIdx.x;
-    // rows of a multi-track batch: stride nblk, the row's mesh's own count
+    // rows of a multi-track batch: stride nblk, the row's mesh's own count;
+    // a padding row (shot -1) illuminates nothing
+    if (trk && rows[s].shot < 0) {
+        if (threadIdx.x == 0) V[s] = 0;
+        return;
+    }
     const int n = trk ? trk[rows[s].trk].nblk : nblk;
     __shared__ int buf[1024];
     __shared__ int carry;
@@ -722,15 +728,17 @@ __global__ void __launch_bounds__(NT) k_surface(
     int64_t shot_id = shot0 + s;
     if (trk) {
         // multi-track: the row's mesh and parameters; blocks past its mesh
-        // hold nothing
+        // and padding rows (shot -1) hold nothing; a track without targets
+        // keeps no facet records
         const TrackK& t = trk[rows[s].trk];
-        if (blk >= t.nblk) return;
+        if (blk >= t.nblk || rows[s].shot < 0) return;
         mesh = t.mesh;
         Fpad = t.Fpad;
         F = t.F;
         kp = t.kp;
         tile_nx = t.tile_nx;
         shot_id = rows[s].shot;
+        if (t.T == 0) write_rec = 0;
     }
     const unsigned long long* mw = mask + ((size_t)s * nblk + blk) * MASKW;
     __shared__ unsigned long long m_s[MASKW];
@@ -2133,17 +2141,6 @@ __global__ void k_finalize(float2* __restrict__ out, const float2* __restrict__ 
 // multi-track batches: the row-wise forms of the per-shot kernels above,
 // each row taking its track's tables from the track descriptors
 // ---------------------------------------------------------------------------
-// record base of each row: exclusive sum of the rows' illuminated counts
-__global__ void k_rec_base(const int* __restrict__ V, int64_t* __restrict__ base, int rows) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        int64_t acc = 0;
-        for (int r = 0; r < rows; ++r) {
-            base[r] = acc;
-            acc += V[r];
-        }
-    }
-}
-
 // dst[r][k] = src[r][k] * table[k]: which = 0 the track's surface range
 // kernel spectrum (none: a track without a surface echo gets zeros),
 // which = 1 its Bluestein spectrum
@@ -2208,8 +2205,9 @@ __global__ void k_blue_out_rows(float2* __restrict__ refr, int nr, const float2*
     }
 }
 
-// out = refl + refr into each row's place in its track's radargram (rows of
-// shot -1 pad a batch and write nothing)
+// out = refl + refr (refr null: no target in the batch) into each row's
+// place in its track's radargram (rows of shot -1 pad a batch and write
+// nothing)
 __global__ void k_finalize_rows(const float2* __restrict__ refl, const float2* __restrict__ refr,
                                 const int* __restrict__ Vs, int nr, int rows,
                                 const TrackK* __restrict__ trk, const RowK* __restrict__ rk) {
@@ -2221,7 +2219,10 @@ __global__ void k_finalize_rows(const float2* __restrict__ refl, const float2* _
         if (row.shot < 0) continue;
         const TrackK& t = trk[row.trk];
         float2 v = make_float2(0.f, 0.f);
-        if (Vs[r] > 0) v = make_float2(refl[i].x + refr[i].x, refl[i].y + refr[i].y);
+        if (Vs[r] > 0) {
+            const float2 b = refr ? refr[i] : make_float2(0.f, 0.f);
+            v = make_float2(refl[i].x + b.x, refl[i].y + b.y);
+        }
         t.out[(size_t)row.shot * nr + j] = v;
         if (t.nvalid && j == 0) t.nvalid[row.shot] = Vs[r];
     }
@@ -2556,10 +2557,49 @@ void window_conv(erp_ctx* ctx, const FftPlan& plan, float2* data, int rows, cons
     ERP_CHECK_LAUNCH("k_window");
 }
 
-struct BatchPlan {
-    int S;
-    int64_t shot0;
+// The (row, target) pairs of one target launch, taken in order from row
+// cs, list index cj: at most C pairs, row r holding tc[r].  Segments are the
+// runs of one row's pairs, slices runs of at most GS pairs of one segment.
+// canon: a row whose pairs go on into the next launch is cut at a multiple
+// of GS of its own pair index, so every row's slices are the same however
+// the launches fall (deterministic mode, multi-track batches).
+struct ChunkFill {
+    std::vector<int> pshot, ptgt;       // per pair: row, index in the row's list
+    std::vector<int> sg_s, sg_b, sg_e;  // per segment: row, first pair, end
+    std::vector<int> gb, ge;            // per slice: first pair, end
+    std::vector<int> gsb, gse;          // per segment: its slices
+    int n() const { return (int)pshot.size(); }
 };
+
+void fill_chunk(const int* tc, int rows, int& cs, int& cj, int C, int GS, bool canon,
+                ChunkFill& f) {
+    f.pshot.clear(); f.ptgt.clear();
+    f.sg_s.clear(); f.sg_b.clear(); f.sg_e.clear();
+    f.gb.clear(); f.ge.clear(); f.gsb.clear(); f.gse.clear();
+    canon = canon && C >= GS;
+    while (cs < rows && f.n() < C) {
+        const int rem = tc[cs] - cj;
+        if (rem <= 0) { ++cs; cj = 0; continue; }
+        int take = std::min(rem, C - f.n());
+        if (canon && take < rem) take -= take % GS;
+        if (take == 0) break;
+        const int b = f.n();
+        for (int j = 0; j < take; ++j) {
+            f.pshot.push_back(cs);
+            f.ptgt.push_back(cj + j);  // list index; resolved on the device
+        }
+        f.sg_s.push_back(cs);
+        f.sg_b.push_back(b);
+        f.sg_e.push_back(b + take);
+        f.gsb.push_back((int)f.gb.size());
+        for (int i = b; i < b + take; i += GS) {
+            f.gb.push_back(i);
+            f.ge.push_back(std::min(i + GS, b + take));
+        }
+        f.gse.push_back((int)f.gb.size());
+        cj += take;
+    }
+}
 
 }  // namespace
 
@@ -3094,9 +3134,6 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                 float2* Bq = A + (size_t)C * P;                 // y / Y' (and MODE 1: B)
                 float2* Bs = spec ? Bq + (size_t)C * P : nullptr;
                 float2* Z = W.tC.as<float2>();                  // Z or fused workspace
-                std::vector<int> pshot, ptgt;
-                pshot.reserve((size_t)C);
-                ptgt.reserve((size_t)C);
                 // each shot's list holds Tcap slots of which htc[2s] are used:
                 // copy only the leading columns that some shot uses
                 W.pairs.ensure(sizeof(int) * ((size_t)C * 4 + 2 * gcap + 3 * (size_t)S + 16));
@@ -3108,38 +3145,20 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                 int* dseg = dge + gcap;              // segment shot / begin / end
                 int* dpw = dseg + 3 * S;             // per-pair window length (fused)
                 // iterate pairs in (shot, target) order
+                std::vector<int> tc1((size_t)S);
+                for (int s = 0; s < S; ++s) tc1[s] = htc[2 * s];
+                ChunkFill cf;
                 int cs = 0, cj = 0;
                 while (cs < S) {
-                    pshot.clear();
-                    ptgt.clear();
-                    while (cs < S && (int)pshot.size() < C) {
-                        if (cj >= htc[2 * cs]) { ++cs; cj = 0; continue; }
-                        pshot.push_back(cs);
-                        ptgt.push_back(cj);  // list index; k_pair_targets resolves it
-                        ++cj;
-                    }
-                    const int n = (int)pshot.size();
+                    // deterministic: every shot's slices independent of the launches
+                    fill_chunk(tc1.data(), S, cs, cj, C, GS, det && half, cf);
+                    const int n = cf.n();
                     if (n == 0) break;
-                    // segments (contiguous pairs of one shot) and MAC groups
-                    std::vector<int> sg_s, sg_b, sg_e, gb, ge, gsb, gse;
-                    for (int i = 0; i < n; ++i) {
-                        if (i == 0 || pshot[i] != pshot[i - 1]) {
-                            sg_s.push_back(pshot[i]);
-                            sg_b.push_back(i);
-                            sg_e.push_back(i + 1);
-                        } else {
-                            sg_e.back() = i + 1;
-                        }
-                    }
+                    const std::vector<int>& pshot = cf.pshot;
+                    const std::vector<int>& ptgt = cf.ptgt;
+                    const std::vector<int>&sg_s = cf.sg_s, &sg_b = cf.sg_b, &sg_e = cf.sg_e;
+                    const std::vector<int>&gb = cf.gb, &ge = cf.ge, &gsb = cf.gsb, &gse = cf.gse;
                     const int nseg = (int)sg_s.size();
-                    for (int k = 0; k < nseg; ++k) {
-                        gsb.push_back((int)gb.size());
-                        for (int i = sg_b[k]; i < sg_e[k]; i += GS) {
-                            gb.push_back(i);
-                            ge.push_back(std::min(i + GS, sg_e[k]));
-                        }
-                        gse.push_back((int)gb.size());
-                    }
                     const int ngroups = (int)gb.size();
                     std::vector<int> up((size_t)C * 3 + 2 * gcap + 3 * (size_t)S, 0);
                     std::copy(pshot.begin(), pshot.end(), up.begin());
@@ -3192,7 +3211,8 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                         PT_.end(st);
                         PT_.begin(ERP_STAGE_TARGET_SUM, st, (double)n);
                         target_sum_half(planP, Z, A + (size_t)C * P, dgb, dge, ngroups, dseg,
-                                        dseg + S, dseg + 2 * S, nseg, Sacc, -1.0f / kp.sqrtP, st);
+                                        dseg + S, dseg + 2 * S, nseg, Sacc, -1.0f / kp.sqrtP, st,
+                                        nullptr, det);
                         PT_.end(st);
                         continue;
                     }
@@ -3322,4 +3342,523 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                            nvalid_dev ? nvalid_dev + s0 : nullptr);
         ERP_CHECK_LAUNCH("k_finalize");
     }
+}
+
+// ---------------------------------------------------------------------------
+// multi-track batches (erp_run_tracks)
+//
+// The (track, shot) rows of several tracks of one range length run in
+// batches of R rows: every launch of a batch covers all its rows, and each
+// row takes its mesh, targets, parameters, range kernel spectra and
+// Bluestein tables from its track's descriptor (TrackK).  Per batch:
+//   A1  aperture masks and counts of the R rows
+//       -> illuminated counts to the host: facet record bases
+//   A2  surface kernel, surface range compression, target selection
+//       -> pair counts to the host: pair lists, cut into launches of C
+//   chunk (per launch of C pairs) pair resolution, facet x target
+//       accumulation, per-pair LDS chain, slice and row sums
+//   tail  Bluestein crop and decimation of every row, then each row's trace
+//       into its track's radargram
+// Each of the four sequences has fixed launch shapes (a short last batch is
+// padded with rows of shot -1, a short launch exits past its pair count on
+// the device), so it is captured once as a HIP graph and replayed; the
+// tables the host writes between them live at fixed device addresses.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct MtKey {
+    int R = 0, nblk = 0, nr = 0, Tcap = 0, C = 0, ntrk = 0;
+    size_t reccap = 0;
+    bool det = false, s3 = false, targ = false;
+    bool operator==(const MtKey& o) const {
+        return R == o.R && nblk == o.nblk && nr == o.nr && Tcap == o.Tcap && C == o.C &&
+               ntrk == o.ntrk && reccap == o.reccap && det == o.det && s3 == o.s3 &&
+               targ == o.targ;
+    }
+};
+
+struct MtGraph {
+    hipGraphExec_t exec = nullptr;
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+    }
+};
+
+constexpr int kMtRowsDefault = 64;
+constexpr int kMtRowsMax = 256;
+
+}  // namespace
+
+struct MtState {
+    MtKey key;
+    DevBuf trk, rows, shots, cnt, off, V, mask, bounds, base, psurf, srow, refl, refr, Sacc, blu,
+        list, tcnt, A, Z, tabs, dtabs, rec;
+    MtGraph gA1, gA2, gChunk, gTail;
+    void reset_graphs() {
+        gA1.reset();
+        gA2.reset();
+        gChunk.reset();
+        gTail.reset();
+    }
+    void release() {
+        reset_graphs();
+        DevBuf* b[] = {&trk, &rows, &shots, &cnt, &off, &V, &mask, &bounds, &base, &psurf, &srow,
+                       &refl, &refr, &Sacc, &blu, &list, &tcnt, &A, &Z, &tabs, &dtabs, &rec};
+        for (DevBuf* d : b) d->release();
+        key = MtKey{};
+    }
+};
+
+void mt_state_free(MtState* m) {
+    if (!m) return;
+    m->release();
+    delete m;
+}
+
+namespace {
+
+// body() either launched directly or, the first time, captured into g and
+// then replayed from it
+template <class Fn>
+void mt_launch(MtGraph& g, bool direct, hipStream_t st, Fn&& body) {
+    if (direct) {
+        body();
+        return;
+    }
+    if (!g.exec) {
+        ERP_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            body();
+        } catch (...) {
+            hipGraph_t t = nullptr;
+            (void)hipStreamEndCapture(st, &t);
+            if (t) (void)hipGraphDestroy(t);
+            throw;
+        }
+        hipGraph_t graph = nullptr;
+        ERP_HIP_TRY(hipStreamEndCapture(st, &graph));
+        const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            g.exec = nullptr;
+            ERP_HIP_TRY(e);
+        }
+    }
+    ERP_HIP_TRY(hipGraphLaunch(g.exec, st));
+}
+
+// the tracks a multi-track batch can take: the plain method with linear
+// convolution, a range length whose per-pair LDS chain exists, no type-0
+// target (their direct histograms and circular chirps are per shot)
+bool mt_eligible(const erp_track& t, uint32_t flags) {
+    const erp_params& p = *t.par;
+    if (flags & ~ERP_RUN_DETERMINISTIC) return false;
+    const uint32_t m = p.mode;
+    if ((m & ERP_MODE_SPECULAR) || !(m & ERP_MODE_CONVOLUTION) || !(m & ERP_MODE_CONV_LINEAR))
+        return false;
+    if (p.nr < 1024 || p.nr > 16384 || (p.nr & (p.nr - 1)) != 0) return false;
+    if (t.targets && t.targets->n0 > 0) return false;
+    if (t.ns <= 0 || t.mesh->F == 0) return false;
+    return true;
+}
+
+// one group: tracks trk_idx of range length nr, all with (s3) or all
+// without the second subsurface medium
+void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64_t>& trk_idx,
+                  int nr, bool s3, bool det, hipStream_t st) {
+    const int P = 2 * nr;
+    const FftPlan planP = fft_plan(P);
+    if (!target_half_supported(planP, nr))
+        throw ErpError{ERP_E_INVALID, "multi-track batch: range length without a pair chain"};
+    const int L = 2 * nr - 1;
+    const FftPlan planM = fft_plan(next_pow2(2 * (int64_t)L - 1));
+    const int M = planM.N;
+    const int ntr = (int)trk_idx.size();
+
+    // track descriptors
+    std::vector<TrackK> tk((size_t)ntr);
+    int nblk_max = 1, Tcap = 0;
+    bool targ = false;
+    int64_t nrows = 0;
+    for (int k = 0; k < ntr; ++k) {
+        const erp_track& t = tracks[trk_idx[k]];
+        const erp_params& p = *t.par;
+        const erp_mesh* mesh = t.mesh;
+        TrackK& d = tk[k];
+        d = TrackK{};
+        d.kp = make_kpar(p, P);
+        d.kp.ortho = mesh->frame_ok ? 1 : 0;
+        d.kp.row_live = nr;
+        d.mesh = mesh->soa.as<float>();
+        d.Fpad = mesh->Fpad;
+        d.F = mesh->F;
+        d.nblk = (int)((mesh->F + FB - 1) / FB);
+        d.tile_nx = p.cos_ap < 0.94 ? mesh->tile_nx : 0;
+        d.frame_ok = mesh->frame_ok ? 1 : 0;
+        const int64_t T = t.targets ? t.targets->T : 0;
+        d.T = (int)T;
+        if (T > 0) {
+            d.tg = t.targets->xyzn.as<float>();
+            d.Tpad = t.targets->Tpad;
+            d.ttype = t.targets->type.as<unsigned char>();
+            // sim.cu:853-858: the target range kernel has twice the surface's
+            d.g_targ_nat = get_chirp(ctx, nr, p.dr, 2.f * p.rng_res, 0, 0.f, P, planP, st, true);
+            targ = true;
+        }
+        // a disabled surface echo: no spectrum, so the row's echo is zero
+        d.g_surf = (p.mode & ERP_MODE_DISABLE_SURFACE)
+                       ? nullptr
+                       : get_chirp(ctx, nr, p.dr, p.rng_res, 0, 0.f, P, planP, st);
+        d.out = static_cast<float2*>(t.out_dev);
+        d.nvalid = static_cast<int32_t*>(t.nvalid_dev);
+        d.tscale = -1.0f / d.kp.sqrtP;  // outward trace = -A / sqrt(P)
+        nblk_max = std::max(nblk_max, d.nblk);
+        Tcap = std::max<int>(Tcap, (int)T);
+        nrows += t.ns;
+    }
+
+    // Bluestein tables of every track once any has targets: the tail runs
+    // over all rows of a batch (a row without targets crops a zero sum)
+    if (targ)
+        for (int k = 0; k < ntr; ++k) {
+            const erp_params& p = *tracks[trk_idx[k]].par;
+            const BlueTables* b = get_blue(ctx, L, p.smpl, p.B, st);
+            tk[k].blue_w = b->w.as<float2>();
+            tk[k].blue_Bw = b->Bw.as<float2>();
+            tk[k].blue_mask = b->mask.as<unsigned char>();
+        }
+
+    // shapes
+    int R = kMtRowsDefault;
+    if (ctx->tune_rows > 0) R = (int)std::min<int64_t>(ctx->tune_rows, kMtRowsMax);
+    const int GS = target_half_slice();
+    int C = (int)std::max<int64_t>(1, kPairRowBytes / ((int64_t)P * 8 * 2));
+    C = std::min<int>(C, kMaxPairsPerLaunch);
+    if (ctx->tune_pairs > 0) C = (int)std::min<int64_t>(C, ctx->tune_pairs);
+    const int gcap = C / GS + R + 2;
+    const int segcap = R;
+    const bool direct = ctx->tune_direct > 0;
+
+    if (!ctx->mt) ctx->mt = new MtState();
+    MtState& m = *ctx->mt;
+    MtKey key;
+    key.R = R; key.nblk = nblk_max; key.nr = nr; key.Tcap = Tcap; key.C = C; key.ntrk = ntr;
+    key.reccap = m.key.reccap; key.det = det; key.s3 = s3; key.targ = targ;
+    if (!(key == m.key)) {
+        m.reset_graphs();
+        m.key = key;
+    }
+    const size_t r = (size_t)R;
+    m.trk.ensure(sizeof(TrackK) * (size_t)ntr);
+    m.rows.ensure(sizeof(RowK) * r);
+    m.shots.ensure(sizeof(erp_shot) * r);
+    m.cnt.ensure(sizeof(int) * r * nblk_max);
+    m.off.ensure(sizeof(int) * r * nblk_max);
+    m.V.ensure(sizeof(int) * r);
+    m.mask.ensure(sizeof(unsigned long long) * r * nblk_max * MASKW);
+    m.bounds.ensure(sizeof(Bounds) * r);
+    m.base.ensure(sizeof(int64_t) * r);
+    m.psurf.ensure(sizeof(float2) * r * P);
+    const size_t srow_f2 = r * nr * 2 + (r + 1) / 2;  // FP64 bins, then per-row maxima
+    m.srow.ensure(sizeof(float2) * srow_f2);
+    m.refl.ensure(sizeof(float2) * r * nr);
+    if (targ) {
+        m.refr.ensure(sizeof(float2) * r * nr);
+        m.Sacc.ensure(sizeof(float2) * r * P);
+        m.blu.ensure(sizeof(float2) * r * M);
+        m.list.ensure(sizeof(int) * r * (size_t)Tcap * 2);
+        m.tcnt.ensure(sizeof(int) * r * 2);
+        m.A.ensure(sizeof(float2) * (size_t)P * ((size_t)C + gcap));  // pair rows, slice sums
+        m.Z.ensure(sizeof(float2) * (size_t)P * C);                   // per-pair products
+    }
+    // host-written tables of a launch: pair rows and list indices, slices,
+    // segments, the pair count, segment scales
+    const size_t tab_ints = 2 * (size_t)C + 2 * (size_t)gcap + 3 * (size_t)segcap + 1 + segcap;
+    m.tabs.ensure(sizeof(int) * tab_ints);
+    // device-written: pair window bases and lengths, the pair counter, the
+    // per-pair range kernel spectra
+    m.dtabs.ensure(sizeof(int) * (2 * (size_t)C + 64) + sizeof(void*) * (size_t)C);
+    int* t_pshot = m.tabs.as<int>();
+    int* t_ptgt = t_pshot + C;
+    int* t_slb = t_ptgt + C;
+    int* t_sle = t_slb + gcap;
+    int* t_sgs = t_sle + gcap;
+    int* t_sgb = t_sgs + segcap;
+    int* t_sge = t_sgb + segcap;
+    int* t_n = t_sge + segcap;
+    float* t_scale = reinterpret_cast<float*>(t_n + 1);
+    int* d_pbase = m.dtabs.as<int>();
+    int* d_pwin = d_pbase + C;
+    int* d_next = d_pwin + C;  // 64 ints: the pair counter, then padding
+    const float4** d_G = reinterpret_cast<const float4**>(d_next + 64);
+
+    ERP_HIP_TRY(hipMemcpyAsync(m.trk.p, tk.data(), sizeof(TrackK) * (size_t)ntr,
+                               hipMemcpyHostToDevice, st));
+    const TrackK* dtrk = m.trk.as<TrackK>();
+    const RowK* drows = m.rows.as<RowK>();
+    const erp_shot* dshots = m.shots.as<erp_shot>();
+    int* dV = m.V.as<int>();
+    Bounds* dbounds = m.bounds.as<Bounds>();
+    int64_t* dbase = m.base.as<int64_t>();
+    double2* srow = m.srow.as<double2>();
+    unsigned* smax = reinterpret_cast<unsigned*>(srow + r * nr);
+    int* list1 = targ ? m.list.as<int>() : nullptr;
+    int* list0 = targ ? list1 + r * (size_t)Tcap : nullptr;
+    const KPar kp0 = tk[0].kp;  // launch argument only: every row reads its own
+
+    // all rows, track-major
+    std::vector<RowK> all;
+    all.reserve((size_t)nrows);
+    for (int k = 0; k < ntr; ++k)
+        for (int64_t s = 0; s < tracks[trk_idx[k]].ns; ++s) all.push_back(RowK{k, (int)s});
+
+    std::vector<RowK> hrows(r);
+    std::vector<erp_shot> hshots(r);
+    std::vector<int> hV(r), htc(2 * r), tc1(r);
+    std::vector<int64_t> hbase(r);
+    std::vector<int> htab(tab_ints);
+    StageTimer none_;
+    StageTimer& PT_ = ctx->prof.on ? ctx->prof : none_;
+    ChunkFill cf;
+
+    for (size_t b0 = 0; b0 < all.size(); b0 += r) {
+        const size_t nb = std::min(r, all.size() - b0);
+        for (size_t i = 0; i < r; ++i) {
+            const RowK row = i < nb ? all[b0 + i] : RowK{0, -1};
+            hrows[i] = row;
+            const erp_track& t = tracks[trk_idx[row.trk]];
+            hshots[i] = t.shots[row.shot < 0 ? 0 : row.shot];
+        }
+        ERP_HIP_TRY(hipMemcpyAsync(m.rows.p, hrows.data(), sizeof(RowK) * r,
+                                   hipMemcpyHostToDevice, st));
+        ERP_HIP_TRY(hipMemcpyAsync(m.shots.p, hshots.data(), sizeof(erp_shot) * r,
+                                   hipMemcpyHostToDevice, st));
+
+        // A1: aperture masks, counts
+        PT_.begin(ERP_STAGE_MASK, st, 0.0);
+        mt_launch(m.gA1, direct, st, [&] {
+            hipLaunchKernelGGL(k_illum_mask, dim3(nblk_max, R), dim3(NT), 0, st,
+                               (const float*)nullptr, (int64_t)0, (int64_t)0, dshots, 0.0,
+                               m.mask.as<unsigned long long>(), m.cnt.as<int>(), nblk_max, 0,
+                               dtrk, drows);
+            ERP_CHECK_LAUNCH("k_illum_mask");
+            hipLaunchKernelGGL(k_scan_blocks, dim3(R), dim3(1024), 0, st, m.cnt.as<int>(),
+                               m.off.as<int>(), dV, nblk_max, dtrk, drows);
+            ERP_CHECK_LAUNCH("k_scan_blocks");
+        });
+        PT_.end(st);
+        ERP_HIP_TRY(hipMemcpyAsync(hV.data(), dV, sizeof(int) * r, hipMemcpyDeviceToHost, st));
+        ERP_HIP_TRY(hipStreamSynchronize(st));
+
+        // facet records of the rows whose track has targets
+        int64_t Vtot = 0;
+        for (size_t i = 0; i < r; ++i) {
+            hbase[i] = Vtot;
+            const bool keep = hrows[i].shot >= 0 && tk[hrows[i].trk].T > 0;
+            if (keep && hV[i] >= (1 << 28))
+                throw ErpError{ERP_E_INVALID, "a shot illuminates 2^28 facets or more"};
+            if (keep) Vtot += hV[i];
+        }
+        if (targ && (size_t)Vtot > m.key.reccap) {
+            // more records than the buffers hold: grow by a quarter more, and
+            // capture again the graphs that read the record arrays
+            const size_t cap = (size_t)Vtot + (size_t)Vtot / 4 + 1024;
+            m.rec.ensure(cap * 8 + 3 * cap * 16 + cap * 4 + 5 * 256);
+            m.key.reccap = cap;
+            m.gA2.reset();
+            m.gChunk.reset();
+        }
+        Rec rec{};
+        if (targ) {
+            const size_t cap = m.key.reccap;
+            char* pb = m.rec.as<char>();
+            auto take = [&](size_t n) { char* q = pb; pb += (n + 255) & ~(size_t)255; return q; };
+            rec.itd = (double*)take(cap * 8);
+            rec.qa = (float4*)take(cap * 16);
+            rec.qb = (float4*)take(cap * 16);
+            rec.qc = (float4*)take(cap * 16);
+            rec.qd = (float*)take(cap * 4);
+            rec.cap = (int64_t)cap;
+        }
+        ERP_HIP_TRY(hipMemcpyAsync(dbase, hbase.data(), sizeof(int64_t) * r,
+                                   hipMemcpyHostToDevice, st));
+
+        // A2: surface kernel, surface range compression, target selection
+        double Vsum = 0;
+        for (size_t i = 0; i < r; ++i) Vsum += hV[i];
+        PT_.begin(ERP_STAGE_SURFACE, st, Vsum);
+        mt_launch(m.gA2, direct, st, [&] {
+            zero(m.psurf.as<float2>(), r * P, st);
+            hipLaunchKernelGGL(k_init_bounds, dim3((R + 63) / 64), dim3(64), 0, st, dbounds, R);
+            ERP_CHECK_LAUNCH("k_init_bounds");
+            zero(m.srow.as<float2>(), srow_f2, st);
+            hipLaunchKernelGGL(k_surface, dim3(nblk_max, R), dim3(NT), 0, st,
+                               (const float*)nullptr, (int64_t)0, (int64_t)0, dshots,
+                               m.mask.as<unsigned long long>(), m.off.as<int>(), dV, dbase,
+                               nblk_max, kp0, rec, targ ? 1 : 0, 1, srow, (int64_t)0, dbounds,
+                               det ? 1 : 0, smax, 0, 0, 0, dtrk, drows);
+            ERP_CHECK_LAUNCH("k_surface");
+            if (det) {
+                hipLaunchKernelGGL(k_surface, dim3(nblk_max, R), dim3(NT), 0, st,
+                                   (const float*)nullptr, (int64_t)0, (int64_t)0, dshots,
+                                   m.mask.as<unsigned long long>(), m.off.as<int>(), dV, dbase,
+                                   nblk_max, kp0, rec, 0, 1, srow, (int64_t)0, (Bounds*)nullptr,
+                                   2, smax, 0, 0, 0, dtrk, drows);
+                ERP_CHECK_LAUNCH("k_surface");
+            }
+            hipLaunchKernelGGL(k_surf_rows, dim3((nr + 255) / 256, R), dim3(256), 0, st,
+                               m.psurf.as<float2>(), P, srow, nr, R, det ? 1 : 0, smax, dV);
+            ERP_CHECK_LAUNCH("k_surf_rows");
+            // reflected signal: each row with its track's range kernel
+            float2* ps = m.psurf.as<float2>();
+            fft_forward(ctx, planP, ps, R, st);
+            hipLaunchKernelGGL(k_mul_spec_rows, dim3(grid_for(r * P)), dim3(256), 0, st, ps, ps,
+                               P, R, dtrk, drows, 0);
+            ERP_CHECK_LAUNCH("k_mul_spec_rows");
+            fft_inverse(ctx, planP, ps, R, st);
+            hipLaunchKernelGGL(k_window, dim3(grid_for(r * nr)), dim3(256), 0, st,
+                               m.refl.as<float2>(), nr, ps, P, nr, R, 1.0f, 0);
+            ERP_CHECK_LAUNCH("k_window");
+            if (targ) {
+                hipLaunchKernelGGL(k_target_select, dim3(R), dim3(1024), 0, st,
+                                   (const float*)nullptr, (int64_t)0, 0,
+                                   (const unsigned char*)nullptr, dshots, dV, 0.0, list1, list0,
+                                   m.tcnt.as<int>(), Tcap, dtrk, drows);
+                ERP_CHECK_LAUNCH("k_target_select");
+                zero(m.Sacc.as<float2>(), r * P, st);
+            } else {
+                hipLaunchKernelGGL(k_finalize_rows, dim3(grid_for(r * nr)), dim3(256), 0, st,
+                                   m.refl.as<float2>(), (const float2*)nullptr, dV, nr, R, dtrk,
+                                   drows);
+                ERP_CHECK_LAUNCH("k_finalize_rows");
+            }
+        });
+        PT_.end(st);
+        if (!targ) continue;
+
+        ERP_HIP_TRY(hipMemcpyAsync(htc.data(), m.tcnt.p, sizeof(int) * 2 * r,
+                                   hipMemcpyDeviceToHost, st));
+        ERP_HIP_TRY(hipStreamSynchronize(st));
+        for (size_t i = 0; i < r; ++i) tc1[i] = htc[2 * i];
+
+        // chunks: launches of at most C pairs in (row, target) order
+        float2* A = m.A.as<float2>();
+        float2* Z = m.Z.as<float2>();
+        int cs = 0, cj = 0;
+        for (;;) {
+            fill_chunk(tc1.data(), R, cs, cj, C, GS, det, cf);
+            const int n = cf.n();
+            if (n == 0) break;
+            const int nsl = (int)cf.gb.size(), nseg = (int)cf.sg_s.size();
+            if (nsl > gcap || nseg > segcap)
+                throw ErpError{ERP_E_INVALID, "multi-track launch tables overflow"};
+            std::fill(htab.begin(), htab.end(), 0);
+            std::copy(cf.pshot.begin(), cf.pshot.end(), htab.begin());
+            std::copy(cf.ptgt.begin(), cf.ptgt.end(), htab.begin() + C);
+            std::copy(cf.gb.begin(), cf.gb.end(), htab.begin() + 2 * C);
+            std::copy(cf.ge.begin(), cf.ge.end(), htab.begin() + 2 * C + gcap);
+            const size_t so = 2 * (size_t)C + 2 * gcap;
+            for (int k = 0; k < nseg; ++k) {
+                htab[so + k] = cf.sg_s[k];
+                htab[so + segcap + k] = cf.gsb[k];
+                htab[so + 2 * segcap + k] = cf.gse[k];
+                const float sc = tk[hrows[cf.sg_s[k]].trk].tscale;
+                std::memcpy(&htab[so + 3 * segcap + 1 + k], &sc, sizeof(float));
+            }
+            htab[so + 3 * segcap] = n;
+            ERP_HIP_TRY(hipMemcpyAsync(m.tabs.p, htab.data(), sizeof(int) * tab_ints,
+                                       hipMemcpyHostToDevice, st));
+            double triples = 0;
+            for (int i = 0; i < n; ++i) triples += hV[cf.pshot[i]];
+            PT_.begin(ERP_STAGE_TARGET_ACCUM, st, triples);
+            mt_launch(m.gChunk, direct, st, [&] {
+                hipLaunchKernelGGL(k_pair_targets_rows, dim3((C + 255) / 256), dim3(256), 0, st,
+                                   t_ptgt, t_pshot, list1, Tcap, t_n, d_next, d_G, dtrk, drows);
+                ERP_CHECK_LAUNCH("k_pair_targets_rows");
+                auto kern = det ? (s3 ? k_target_accum_fast<true, PH_F64, ACC_F64, true, true>
+                                      : k_target_accum_fast<true, PH_F64, ACC_F64, false, true>)
+                                : (s3 ? k_target_accum_fast<false, PH_F64, ACC_F64, true, true>
+                                      : k_target_accum_fast<false, PH_F64, ACC_F64, false, true>);
+                // persistent: two workgroups per CU take pairs until *t_n
+                hipLaunchKernelGGL(kern, dim3(std::max(1, std::min(C, 2 * ctx->num_cu))),
+                                   dim3(NTF), 0, st, (const float*)nullptr, (int64_t)0, t_pshot,
+                                   t_ptgt, dshots, dV, dbase, rec, dbounds,
+                                   (const erp_prism*)nullptr, 0, kp0, A, d_pbase, d_pwin, 0, 0,
+                                   d_next, dtrk, drows, (const int*)t_n);
+                ERP_CHECK_LAUNCH("k_target_accum_fast");
+                target_pairs_half(ctx, planP, A, d_pbase, d_pwin, nr, C, nullptr, Z, st, d_G,
+                                  t_n);
+                target_sum_half(planP, Z, A + (size_t)C * P, t_slb, t_sle, gcap, t_sgs, t_sgb,
+                                t_sge, segcap, m.Sacc.as<float2>(), 1.0f, st, t_scale, det);
+            });
+            PT_.end(st);
+        }
+
+        // tail: per row c = IFFT(S)/P; Bluestein DFT_L, crop, IDFT_L, decimate;
+        // then refl + refr into the track's radargram
+        PT_.begin(ERP_STAGE_SHOT_FINAL, st, (double)nb);
+        mt_launch(m.gTail, direct, st, [&] {
+            float2* Sacc = m.Sacc.as<float2>();
+            float2* bu = m.blu.as<float2>();
+            fft_inverse(ctx, planP, Sacc, R, st);
+            hipLaunchKernelGGL(k_blue_in_rows, dim3(grid_for(r * M)), dim3(256), 0, st, bu, M,
+                               Sacc, P, L, 1.0f / (float)P, R, dtrk, drows);
+            ERP_CHECK_LAUNCH("k_blue_in_rows");
+            fft_forward(ctx, planM, bu, R, st);
+            hipLaunchKernelGGL(k_mul_spec_rows, dim3(grid_for(r * M)), dim3(256), 0, st, bu, bu,
+                               M, R, dtrk, drows, 1);
+            ERP_CHECK_LAUNCH("k_mul_spec_rows");
+            fft_inverse(ctx, planM, bu, R, st);
+            hipLaunchKernelGGL(k_blue_mid_rows, dim3(grid_for(r * M)), dim3(256), 0, st, bu, M,
+                               L, R, dtrk, drows);
+            ERP_CHECK_LAUNCH("k_blue_mid_rows");
+            fft_forward(ctx, planM, bu, R, st);
+            hipLaunchKernelGGL(k_mul_spec_rows, dim3(grid_for(r * M)), dim3(256), 0, st, bu, bu,
+                               M, R, dtrk, drows, 1);
+            ERP_CHECK_LAUNCH("k_mul_spec_rows");
+            fft_inverse(ctx, planM, bu, R, st);
+            hipLaunchKernelGGL(k_blue_out_rows, dim3(grid_for(r * nr)), dim3(256), 0, st,
+                               m.refr.as<float2>(), nr, bu, M, 1.0f / (float)L, R, dV, dtrk,
+                               drows);
+            ERP_CHECK_LAUNCH("k_blue_out_rows");
+            hipLaunchKernelGGL(k_finalize_rows, dim3(grid_for(r * nr)), dim3(256), 0, st,
+                               m.refl.as<float2>(), m.refr.as<float2>(), dV, nr, R, dtrk, drows);
+            ERP_CHECK_LAUNCH("k_finalize_rows");
+        });
+        PT_.end(st);
+    }
+}
+
+}  // namespace
+
+void erp_run_tracks_impl(erp_ctx* ctx, const erp_track* tracks, int64_t ntracks, uint32_t flags,
+                         hipStream_t st) {
+    const uint32_t known = ERP_RUN_RAW_SURFACE | ERP_RUN_DETERMINISTIC | ERP_RUN_PHASE_F32 |
+                           ERP_RUN_PHASE_DF | ERP_RUN_ACC_F32 | ERP_RUN_ACC_COMP;
+    if (flags & ~known) throw ErpError{ERP_E_INVALID, "unknown run flags"};
+    const bool det = flags & ERP_RUN_DETERMINISTIC;
+    // groups of one range length and second-medium setting, in track order
+    std::vector<std::pair<std::pair<int, bool>, std::vector<int64_t>>> groups;
+    std::vector<int64_t> solo;
+    for (int64_t i = 0; i < ntracks; ++i) {
+        const erp_track& t = tracks[i];
+        if (!mt_eligible(t, flags)) {
+            solo.push_back(i);
+            continue;
+        }
+        const std::pair<int, bool> gk{t.par->nr, t.par->eps_3 > 0.f};
+        auto it = std::find_if(groups.begin(), groups.end(),
+                               [&](const auto& g) { return g.first == gk; });
+        if (it == groups.end()) groups.push_back({gk, {i}});
+        else it->second.push_back(i);
+    }
+    for (int64_t i : solo) {
+        const erp_track& t = tracks[i];
+        erp_run_impl(ctx, t.par, t.mesh, t.targets, t.shots, t.ns, nullptr, 0,
+                     static_cast<float2*>(t.out_dev), static_cast<int32_t*>(t.nvalid_dev), flags,
+                     st, ctx->work, &ctx->prof);
+    }
+    for (const auto& g : groups) mt_run_group(ctx, tracks, g.second, g.first.first, g.first.second,
+                                              det, st);
 }

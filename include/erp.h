@@ -218,6 +218,34 @@ int erp_run_dev_noalloc(erp_ctx* ctx, const erp_params* par, const erp_mesh* mes
                         const erp_prism* prisms, int64_t nprisms,
                         void* out_dev, void* nvalid_dev, uint32_t flags);
 
+/* One track of erp_run_tracks: the arguments of erp_run_dev for one job
+ * (no attenuation prisms). */
+typedef struct erp_track {
+    const erp_params* par;
+    const erp_mesh* mesh;
+    const erp_targets* targets;  /* or NULL */
+    const erp_shot* shots;       /* host, ns entries */
+    int64_t ns;
+    void* out_dev;               /* device: ns x nr complex64, shot-major */
+    void* nvalid_dev;            /* device: ns int32, or NULL */
+} erp_track;
+
+/* Several tracks as one run.  No counterpart in the reference, whose sim
+ * runs one track per process (sim.cu:615-1008); it is for many small or
+ * uneven jobs that do not fill the device one at a time.  Tracks of the
+ * plain method (convolution and convolution_linear, not specular) with nr
+ * a power of two in [1024, 16384] and no target of type 0 are grouped by
+ * (nr, second medium or not); the (track, shot) rows of a group run in
+ * batches of ERP_TUNE_TRACK_ROWS rows, every launch covering all rows of
+ * the batch with each row reading its own track's mesh, targets, parameters
+ * and range kernels, and each batch's four launch sequences replayed from
+ * HIP graphs captured once.  Every other track runs alone, as erp_run_dev.
+ * flags: 0 or ERP_RUN_DETERMINISTIC for grouped tracks (other flags run
+ * every track alone).  Each radargram is erp_run_dev's on the same inputs:
+ * within the parity tolerance, and with ERP_RUN_DETERMINISTIC the same
+ * values.  Synchronous. */
+int erp_run_tracks(erp_ctx* ctx, const erp_track* tracks, int64_t ntracks, uint32_t flags);
+
 /* Asynchronous form of erp_run: queue a job and return at once.  Jobs run
  * in worker slots (erp_serve_slots; 4 are started at the first submit), each
  * a host thread with its own stream and work buffers, so jobs submitted
@@ -263,16 +291,23 @@ int erp_profile_enable(erp_ctx* ctx, int on);
 int erp_profile_read(erp_ctx* ctx, int stage, double* ms, int64_t* launches, double* units);
 
 /* ---- batch shapes ------------------------------------------------------------
- * Caps on the batch shapes erp_run / erp_run_dev / erp_submit use on this
- * context; 0 restores the default.  They can only lower the defaults, so
- * erp_run_footprint stays an upper bound.  No counterpart in the reference
- * (its sim runs every shot in its own loop iteration, sim.cu:615-1008).
+ * Caps on the batch shapes erp_run / erp_run_dev / erp_submit /
+ * erp_run_tracks use on this context; 0 restores the default.  The first two
+ * can only lower the defaults, so erp_run_footprint stays an upper bound.  No
+ * counterpart in the reference (its sim runs every shot in its own loop
+ * iteration, sim.cu:615-1008).
  *   ERP_TUNE_SHOTS_PER_BATCH : shots per batch (default and maximum 32)
  *   ERP_TUNE_PAIRS_PER_LAUNCH: (shot, target) pairs per target launch
  *                              (default: the pairs whose rows fit 6 GB,
- *                              at most 16384) */
+ *                              at most 16384)
+ *   ERP_TUNE_TRACK_ROWS      : (track, shot) rows per multi-track batch
+ *                              (default 64, at most 256)
+ *   ERP_TUNE_DIRECT_LAUNCH   : > 0: multi-track batches launch their kernels
+ *                              directly instead of replaying HIP graphs */
 #define ERP_TUNE_SHOTS_PER_BATCH  1
 #define ERP_TUNE_PAIRS_PER_LAUNCH 2
+#define ERP_TUNE_TRACK_ROWS       3
+#define ERP_TUNE_DIRECT_LAUNCH    4
 int erp_set_tuning(erp_ctx* ctx, int key, int64_t value);
 /* The value set for key (0 = default). */
 int erp_get_tuning(erp_ctx* ctx, int key, int64_t* value);

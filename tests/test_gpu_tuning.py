@@ -41,7 +41,7 @@ def test_small_batches_and_launches_give_the_same_traces(gpu_ctx, nr):
 
 
 def test_tuning_keys_are_checked(gpu_ctx):
-    for key, value in ((0, 1), (3, 1), (ERP_TUNE_SHOTS_PER_BATCH, -1)):
+    for key, value in ((0, 1), (5, 1), (ERP_TUNE_SHOTS_PER_BATCH, -1)):
         with pytest.raises(ErpError) as ex:
             gpu_ctx.set_tuning(key, value)
         assert ex.value.code == ERP_E_INVALID

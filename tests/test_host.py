@@ -204,6 +204,8 @@ int main(void) {
   printf("%zu %zu\n", offsetof(erp_shot, snx), offsetof(erp_shot, grefr_lin));
   printf("%d %d %d %d %d\n", ERP_MODE_CONVOLUTION, ERP_MODE_SPECULAR, ERP_MODE_DISABLE_SURFACE,
          ERP_RUN_RAW_SURFACE, ERP_E_UNSUPPORTED);
+  printf("%zu %zu %zu %d %d\n", sizeof(erp_track), offsetof(erp_track, ns),
+         offsetof(erp_track, nvalid_dev), ERP_TUNE_TRACK_ROWS, ERP_TUNE_DIRECT_LAUNCH);
   return 0;
 }
 """)
@@ -224,6 +226,10 @@ int main(void) {
     assert consts == [_abi.ERP_MODE_CONVOLUTION, _abi.ERP_MODE_SPECULAR,
                       _abi.ERP_MODE_DISABLE_SURFACE, _abi.ERP_RUN_RAW_SURFACE,
                       _abi.ERP_E_UNSUPPORTED]
+    tr = [int(v) for v in lines[4].split()]
+    T = _abi.ErpTrack
+    assert tr == [ctypes.sizeof(T), T.ns.offset, T.nvalid_dev.offset,
+                  _abi.ERP_TUNE_TRACK_ROWS, _abi.ERP_TUNE_DIRECT_LAUNCH]
 
 
 def test_footprint_is_host_only_and_monotone():
