@@ -398,6 +398,82 @@ def serve_measure(a):
     return jobs, plain, runs
 
 
+def tracks_measure(a):
+    """Multi-track batches (erp_run_tracks) against each track alone, on the
+    batching workload: the first `a.tracks` tracks of the job stream drawn
+    from SeedSequence(a.tracks_seed) (log-uniform F in [1e4, 4e6], Hurst
+    U(0.6, 0.9), ns U{100..2000} at full count, nr in {2048, ..., 16384},
+    1-3 target layers, 9 or 60 MHz, h U(25, 50) km, 10 deg aperture).
+
+    alone : erp_run_dev per track, one after another
+    tracks: one erp_run_tracks call over all of them
+    Meshes and targets are resident and outputs stay in device memory; each
+    timed pass follows an untimed one (the graphs are captured there).  The
+    batch shapes come from the autotuner, run first on a sample (the first
+    4 tracks, 64 shots each).
+    """
+    from europaraypaths_amd import autotune, serve
+    from europaraypaths_amd._native import Context, DeviceTraces
+
+    t_gen = time.perf_counter()
+    jobs = serve.mixed_job_stream(a.tracks, None, seed=a.tracks_seed, reuse=0.0)
+    print(f"# tracks: {len(jobs)} generated in {time.perf_counter() - t_gen:.1f} s",
+          file=sys.stderr, flush=True)
+    ctx = Context(0)
+    handles, runs = [], []
+    try:
+        for j in jobs:
+            m = ctx.upload_mesh(j.scene.soa)
+            t = ctx.upload_targets(*j.scene.targets)
+            handles += [m, t]
+            runs.append((j.par.to_erp(), m, t, j.shots, DeviceTraces(j.ns, j.nr)))
+        tuned = autotune.tune(ctx, autotune.sample_jobs(runs, 4, 64))
+        print(f"# tracks: tuned rows {tuned['rows']}, pairs {tuned['pairs']}", file=sys.stderr,
+              flush=True)
+
+        def alone():
+            for e, m, t, sh, d in runs:
+                ctx.run_dev(e, m, t, sh, d.out.value, d.nvalid.value)
+
+        alone()
+        t0 = time.perf_counter()
+        alone()
+        t_alone = time.perf_counter() - t0
+        ref = [d.read() for *_, d in runs]
+        ctx.run_tracks_dev(runs)
+        ctx.profile(True)
+        t0 = time.perf_counter()
+        ctx.run_tracks_dev(runs)
+        t_tracks = time.perf_counter() - t0
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        got = [d.read() for *_, d in runs]
+    finally:
+        for *_, d in runs:
+            d.free()
+        for h_ in handles:
+            h_.release()
+        ctx.close()
+    pairs = int(sum(int(v.sum()) for _, v in ref))
+    same_counts = all(np.array_equal(v, w) for (_, v), (_, w) in zip(ref, got))
+    ph = max(max_phase_err(g, r) for (g, _), (r, _) in zip(got, ref))
+    print(f"# tracks: alone {t_alone:.2f} s, multi-track {t_tracks:.2f} s "
+          f"({t_alone / t_tracks:.2f}x), {pairs} pairs, phase diff {ph:.2e}", file=sys.stderr,
+          flush=True)
+    return {
+        "workload": (f"batching tracks: the first {len(jobs)} of the job stream of "
+                     f"SeedSequence({a.tracks_seed}), full shot counts"),
+        "tracks": len(jobs), "shots": int(sum(j.ns for j in jobs)),
+        "nr": sorted({j.nr for j in jobs}), "facets": [j.scene.F for j in jobs],
+        "pairs": pairs, "triples": prof["target_accum"][2],
+        "alone_s": t_alone, "tracks_s": t_tracks,
+        "alone_pairs_per_s": pairs / t_alone, "tracks_pairs_per_s": pairs / t_tracks,
+        "tracks_vs_alone": t_alone / t_tracks,
+        "same_illuminated_counts": same_counts, "max_phase_diff_vs_alone_rad": ph,
+        "tuned": tuned, "stage_ms": {k: v[0] for k, v in prof.items()},
+    }
+
+
 def serve_summary(a, jobs, plain, runs):
     """The compact served-vs-plain record of the default bench line."""
     return {"workload": (f"mixed_job_stream: {len(jobs)} jobs, seed {a.serve_seed}, "
@@ -460,6 +536,11 @@ def main():
     ap.add_argument("--serve-budget", type=float, default=250e9)
     ap.add_argument("--no-serve", action="store_true",
                     help="leave the served-vs-plain record out of the default line")
+    ap.add_argument("--tracks", type=int, default=12,
+                    help="tracks of the multi-track record (0: leave it out)")
+    ap.add_argument("--tracks-seed", type=int, default=4)
+    ap.add_argument("--tracks-only", action="store_true",
+                    help="only the multi-track record")
     a = ap.parse_args()
     if a.counters_summary:
         counters_summary(a.counters_summary[:-2], a.counters_summary[-2], a.counters_summary[-1])
@@ -474,6 +555,9 @@ def main():
         sys.exit(2)
     if a.serve:
         serve_bench(a)
+        return
+    if a.tracks_only:
+        print(json.dumps(tracks_measure(a)))
         return
     if a.cpu_protocol:
         p, soa, tg, shots = build_workload(a.workload, a.track_shots)
@@ -599,11 +683,16 @@ def main():
         res["focusing"] = focus_measure(ctx)
     except Exception as ex:  # noqa: BLE001
         res["focusing"] = {"error": str(ex)}
+    mesh.release()
+    if targets is not None:
+        targets.release()
+    ctx.close()
+    if a.tracks > 0:
+        try:
+            res["multi_track"] = tracks_measure(a)
+        except Exception as ex:  # noqa: BLE001
+            res["multi_track"] = {"error": str(ex)}
     if not a.no_serve:
-        mesh.release()
-        if targets is not None:
-            targets.release()
-        ctx.close()
         try:
             res["serving"] = serve_summary(a, *serve_measure(a))
         except Exception as ex:  # noqa: BLE001

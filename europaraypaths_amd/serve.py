@@ -43,6 +43,9 @@ class Scene:
     soa: np.ndarray                      # (12, F) float32
     targets: tuple | None                # ((6, T) float32, (T,) uint8)
     eps_2: float = 3.15
+    # the medium below z_int (0: one medium for every layer)
+    eps_3: float = 0.0
+    z_int: float = 0.0
 
     @property
     def F(self) -> int:
@@ -310,8 +313,11 @@ LAYER_SPACING = 500.0
 def make_scene(mesh_id: int, rng: np.random.Generator, f_range=(1e4, 4e6)) -> Scene:
     """fBm mesh (log-uniform facet count, Hurst U(0.6, 0.9), 10 m RMS, 50 m
     facets) with 1-3 flat target layers at depths U(0.5, 6) km on a 500 m
-    grid over the same extent, and eps_2 ~ U(3.0, 3.4).  The reference models
-    one subsurface medium (file_io.cu:81), so one eps_2 holds for all layers."""
+    grid over the same extent, each layer's medium eps ~ U(3.0, 3.4).  The
+    reference models one subsurface medium (file_io.cu:81); this build adds
+    a second one (erp_params eps_3 below z_int), so the first layer lies in
+    eps_2 and, with two or three layers, the deeper ones in eps_3 below an
+    interface halfway between the first and the second layer."""
     F = float(np.exp(rng.uniform(np.log(f_range[0]), np.log(f_range[1]))))
     n = max(8, int(round(np.sqrt(F))))
     hurst = float(rng.uniform(0.6, 0.9))
@@ -329,7 +335,11 @@ def make_scene(mesh_id: int, rng: np.random.Generator, f_range=(1e4, 4e6)) -> Sc
         xs.append(x)
         ts.append(t)
     tg = (np.concatenate(xs, axis=1), np.concatenate(ts))
-    return Scene(mesh_id, soa, tg, eps_2=float(rng.uniform(3.0, 3.4)))
+    eps = rng.uniform(3.0, 3.4, 2)
+    if nl == 1:
+        return Scene(mesh_id, soa, tg, eps_2=float(eps[0]))
+    return Scene(mesh_id, soa, tg, eps_2=float(eps[0]), eps_3=float(eps[1]),
+                 z_int=-0.5 * float(depths[0] + depths[1]))
 
 
 def make_job(job_id: int, arrival: float, scene: Scene, rng: np.random.Generator,
@@ -346,9 +356,10 @@ def make_job(job_id: int, arrival: float, scene: Scene, rng: np.random.Generator
     extent = n * FACET_SPACING
     sdx = min(FACET_SPACING, 0.8 * extent / max(ns - 1, 1))
     sx0 = -0.5 * sdx * (ns - 1)
+    media = {"eps_3": scene.eps_3, "z_interface": scene.z_int} if scene.eps_3 > 0 else {}
     p = make_params(platform, "planetary_ice", nr=nr, smpl=48e6, rst=h - 1000.0,
                     aperture=10.0, fs=FACET_SPACING, ns=ns, sx0=sx0, sdx=sdx, sy=0.0,
-                    sz=h, eps_2=scene.eps_2)
+                    sz=h, eps_2=scene.eps_2, **media)
     par = parse_parameters(p)
     return SimJob(job_id, arrival, scene, par, make_shots(par))
 
