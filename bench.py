@@ -227,22 +227,42 @@ def ingest_measure(ctx, workload, soa):
     Best of three, host normal computation timed separately."""
     if workload not in ("fbm_two_layer", "fbm_surface_only"):
         return None
-    from europaraypaths_amd.surfaces import fbm_heights, terrain_soa
+    from europaraypaths_amd.surfaces import (device_terrain, fbm_heights, terrain_noise,
+                                             terrain_soa)
     grid, fs = 2048, 50.0
-    z = fbm_heights(grid, grid, fs, 10.0, 0.8, seed=2)
+    o = -fs * grid / 2
     t0 = time.perf_counter()
-    host = terrain_soa(grid, grid, fs, -fs * grid / 2, -fs * grid / 2, z)
+    z = fbm_heights(grid, grid, fs, 10.0, 0.8, seed=2)
+    t_field = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    host = terrain_soa(grid, grid, fs, o, o, z)
     t_host = time.perf_counter() - t0
-    up, dev = [], []
+    t0 = time.perf_counter()
+    noise = terrain_noise("fbm", grid, grid, seed=2)
+    t_draws = time.perf_counter() - t0
+    up, dev, terr, terr_dev = [], [], [], []
     for _ in range(3):
         t0 = time.perf_counter()
         ctx.upload_mesh(host).release()
         up.append(time.perf_counter() - t0)
         t0 = time.perf_counter()
-        ctx.mesh_from_heights(z, fs, -fs * grid / 2, -fs * grid / 2).release()
+        ctx.mesh_from_heights(z, fs, o, o).release()
         dev.append(time.perf_counter() - t0)
-    return {"facets": int(host.shape[1]), "host_normals_s": t_host,
+        t0 = time.perf_counter()
+        m = ctx.mesh_from_terrain(1, grid, grid, fs, o, o, 10.0, hurst=0.8, noise=noise)
+        terr.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        device_terrain(ctx, "fbm", grid, grid, fs, o, o, 10.0, hurst=0.8, seed=2,
+                       host_draws=False).release()
+        terr_dev.append(time.perf_counter() - t0)
+    got = m.download()
+    m.release()
+    return {"facets": int(host.shape[1]), "host_field_s": t_field, "host_normals_s": t_host,
             "soa_upload_s": min(up), "heights_on_device_s": min(dev),
+            "terrain_on_device_host_draws_s": min(terr), "host_draws_s": t_draws,
+            "terrain_on_device_device_draws_s": min(terr_dev),
+            "terrain_max_dz_m": float(np.abs(got[2] - host[2]).max()),
+            "terrain_max_dnormal": float(np.abs(got[3:6] - host[3:6]).max()),
             "same_soa": bool(np.array_equal(host, soa))}
 
 

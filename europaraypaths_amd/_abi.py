@@ -29,6 +29,9 @@ ERP_TUNE_PAIRS_PER_LAUNCH = 2
 ERP_TUNE_TRACK_ROWS = 3
 ERP_TUNE_DIRECT_LAUNCH = 4
 
+ERP_TERRAIN_GAUSSIAN = 0
+ERP_TERRAIN_FBM = 1
+
 _PARAM_FLOATS = ("lam", "k", "rng_res", "P", "fs", "ks", "eps_1", "eps_2", "c",
                  "c_1", "c_2", "nu_1", "nu_2", "alpha1", "alpha2", "aperture",
                  "smpl", "B", "f0", "dr")
@@ -56,6 +59,16 @@ class ErpTrack(ctypes.Structure):
                 ("targets", ctypes.c_void_p), ("shots", ctypes.c_void_p),
                 ("ns", ctypes.c_int64), ("out_dev", ctypes.c_void_p),
                 ("nvalid_dev", ctypes.c_void_p)]
+
+
+class ErpTerrain(ctypes.Structure):
+    """erp_terrain: a synthetic terrain for erp_mesh_from_terrain."""
+    _fields_ = [("kind", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("nx", ctypes.c_int64), ("ny", ctypes.c_int64),
+                ("fs", ctypes.c_double), ("ox", ctypes.c_double), ("oy", ctypes.c_double),
+                ("rms", ctypes.c_double), ("corr_len", ctypes.c_double),
+                ("hurst", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("noise", ctypes.c_void_p)]
 
 
 class ErpPrism(ctypes.Structure):

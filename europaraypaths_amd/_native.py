@@ -11,7 +11,8 @@ import threading
 
 import numpy as np
 
-from ._abi import (ERP_E_NOTREADY, ERP_OK, PRISM_DTYPE, SHOT_DTYPE, ErpParams, ErpTrack)
+from ._abi import (ERP_E_NOTREADY, ERP_OK, PRISM_DTYPE, SHOT_DTYPE, ErpParams, ErpTerrain,
+                   ErpTrack)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liberp.so")
@@ -24,7 +25,7 @@ EXPORTS = ("erp_ctx_create", "erp_ctx_destroy", "erp_last_error", "erp_mesh_uplo
            "erp_run_dev", "erp_submit", "erp_poll", "erp_wait", "erp_run_footprint",
            "erp_profile_enable", "erp_profile_read", "erp_serve_slots",
            "erp_mesh_from_heights", "erp_mesh_download", "erp_focus", "erp_set_tuning",
-           "erp_get_tuning", "erp_run_dev_noalloc", "erp_run_tracks")
+           "erp_get_tuning", "erp_run_dev_noalloc", "erp_run_tracks", "erp_mesh_from_terrain")
 
 STAGES = ("mask", "surface", "surface_conv", "target_select", "target_accum", "target_fft",
           "shot_final", "other", "target_sum")
@@ -60,6 +61,8 @@ def lib() -> ctypes.CDLL:
         L.erp_mesh_from_heights.argtypes = [vp, vp, i64, i64, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_double, P(vp)]
         L.erp_mesh_from_heights.restype = i32
+        L.erp_mesh_from_terrain.argtypes = [vp, P(ErpTerrain), P(vp)]
+        L.erp_mesh_from_terrain.restype = i32
         L.erp_mesh_download.argtypes = [vp, vp]
         L.erp_mesh_download.restype = i32
         L.erp_focus.argtypes = [vp, vp, i64, i64, i64, i64, vp, vp, vp, vp, vp, i64, vp]
@@ -158,10 +161,15 @@ class DeviceTraces:
 class Mesh:
     """A facet mesh resident in HBM (erp_mesh_upload / erp_mesh_from_heights)."""
 
-    def __init__(self, ctx: "Context", soa: np.ndarray | None, heights=None):
+    def __init__(self, ctx: "Context", soa: np.ndarray | None, heights=None, terrain=None):
         self.ctx = ctx
         h = ctypes.c_void_p()
-        if heights is None:
+        if terrain is not None:
+            t, keep = terrain  # ErpTerrain and the host noise it points to
+            self.F = int(t.nx * t.ny)
+            ctx._check(lib().erp_mesh_from_terrain(ctx.handle, ctypes.byref(t), ctypes.byref(h)))
+            del keep
+        elif heights is None:
             soa = np.ascontiguousarray(soa, dtype=np.float32)
             if soa.ndim != 2 or soa.shape[0] != 12:
                 raise ValueError("mesh must be a (12, F) float32 SoA array")
@@ -270,6 +278,20 @@ class Context:
         """Build a mesh on the device from a (ny, nx) height grid with spacing
         fs and origin (ox, oy): the mesh surfaces.terrain_soa would give."""
         return Mesh(self, None, heights=(heights, fs, ox, oy))
+
+    def mesh_from_terrain(self, kind: int, nx: int, ny: int, fs: float, ox: float, oy: float,
+                          rms: float, corr_len: float = 0.0, hurst: float = 0.0,
+                          seed: int = 0, noise=None) -> Mesh:
+        """erp_mesh_from_terrain: a Gaussian-field (ERP_TERRAIN_GAUSSIAN) or fBm
+        (ERP_TERRAIN_FBM) terrain generated on the device, and its mesh.
+        noise: the host draws (see surfaces.terrain_noise), or None for the
+        device's own seeded draws."""
+        if noise is not None:
+            noise = np.ascontiguousarray(noise, dtype=np.float64)
+        t = ErpTerrain(int(kind), 0, int(nx), int(ny), float(fs), float(ox), float(oy),
+                       float(rms), float(corr_len), float(hurst), int(seed) & (2 ** 64 - 1),
+                       noise.ctypes.data if noise is not None else None)
+        return Mesh(self, None, terrain=(t, noise))
 
     def upload_targets(self, xyzn, ttype) -> Targets:
         return Targets(self, xyzn, ttype)

@@ -386,12 +386,16 @@ int erp_mesh_upload(erp_ctx* ctx, const float* soa, int64_t F, erp_mesh** out) {
     return ERP_OK;
 }
 
-int erp_mesh_from_heights(erp_ctx* ctx, const double* heights, int64_t nx, int64_t ny,
-                          double fs, double ox, double oy, erp_mesh** out) {
-    if (!ctx || !out || nx < 0 || ny < 0 || (nx * ny > 0 && !heights)) return ERP_E_INVALID;
+// A mesh built on the device from an ny x nx height grid that fill() writes
+// into device memory (FP64, row-major, x fastest).
+extern "C++" {
+template <class Fill>
+static int mesh_from_device_heights(erp_ctx* ctx, int64_t nx, int64_t ny, double fs, double ox,
+                                    double oy, erp_mesh** out, Fill&& fill) {
     *out = nullptr;
     erp_mesh* m = new (std::nothrow) erp_mesh();
     if (!m) return ERP_E_NOMEM;
+    DevBuf z;
     int rc = guarded(ctx, [&] {
         if (!(fs > 0.0)) throw ErpError{ERP_E_INVALID, "fs must be > 0"};
         if (nx > (int64_t)1 << 20 || ny > (int64_t)1 << 20 || nx * ny > (int64_t)1 << 31)
@@ -404,20 +408,18 @@ int erp_mesh_from_heights(erp_ctx* ctx, const double* heights, int64_t nx, int64
         m->F = F;
         m->Fpad = ((F + 63) / 64) * 64;
         m->soa.ensure(sizeof(float) * 12 * (size_t)std::max<int64_t>(m->Fpad, 64));
-        DevBuf z;
         z.ensure(sizeof(double) * (size_t)std::max<int64_t>(F, 1));
         if (F > 0) {
-            ERP_HIP_TRY(hipMemcpyAsync(z.p, heights, sizeof(double) * F, hipMemcpyHostToDevice,
-                                       ctx->stream));
+            fill(z.as<double>());
             mesh_from_heights(z.as<double>(), nx, ny, fs, ox, oy, m->soa.as<float>(), m->Fpad,
                               ctx->stream);
         }
         ERP_HIP_TRY(hipStreamSynchronize(ctx->stream));
-        z.release();
         // tangents and normal form an orthonormal frame by construction
         m->frame_ok = true;
         m->tile_nx = (nx % 64 == 0 && ny % 32 == 0 && nx <= ((int64_t)1 << 20)) ? (int)nx : 0;
     });
+    z.release();
     if (rc != ERP_OK) {
         m->soa.release();
         delete m;
@@ -425,6 +427,32 @@ int erp_mesh_from_heights(erp_ctx* ctx, const double* heights, int64_t nx, int64
     }
     *out = m;
     return ERP_OK;
+}
+}  // extern "C++"
+
+int erp_mesh_from_heights(erp_ctx* ctx, const double* heights, int64_t nx, int64_t ny,
+                          double fs, double ox, double oy, erp_mesh** out) {
+    if (!ctx || !out || nx < 0 || ny < 0 || (nx * ny > 0 && !heights)) return ERP_E_INVALID;
+    return mesh_from_device_heights(ctx, nx, ny, fs, ox, oy, out, [&](double* z) {
+        ERP_HIP_TRY(hipMemcpyAsync(z, heights, sizeof(double) * nx * ny, hipMemcpyHostToDevice,
+                                   ctx->stream));
+    });
+}
+
+int erp_mesh_from_terrain(erp_ctx* ctx, const erp_terrain* t, erp_mesh** out) {
+    if (!ctx || !out || !t) return ERP_E_INVALID;
+    if (t->kind != ERP_TERRAIN_GAUSSIAN && t->kind != ERP_TERRAIN_FBM) {
+        ctx->last_error = "unknown terrain kind";
+        return ERP_E_INVALID;
+    }
+    if (t->nx < 1 || t->ny < 1 || t->nx > (1 << 16) || t->ny > (1 << 16) ||
+        !(t->rms >= 0.0) || (t->kind == ERP_TERRAIN_GAUSSIAN && !(t->corr_len >= 0.0)) ||
+        (t->kind == ERP_TERRAIN_FBM && !std::isfinite(t->hurst))) {
+        ctx->last_error = "terrain: 1 <= nx, ny <= 65536, rms >= 0, corr_len >= 0, finite hurst";
+        return ERP_E_INVALID;
+    }
+    return mesh_from_device_heights(ctx, t->nx, t->ny, t->fs, t->ox, t->oy, out,
+                                    [&](double* z) { terrain_heights(ctx, *t, z, ctx->stream); });
 }
 
 int erp_mesh_download(const erp_mesh* m, float* soa) {

@@ -317,6 +317,9 @@ void focus_rows(const float2* rdr, int Nr, int Na, int row0, int nrows, const in
 // 12 x Fpad SoA of a height grid (see erp_mesh_from_heights)
 void mesh_from_heights(const double* heights, int64_t nx, int64_t ny, double fs, double ox,
                        double oy, float* soa, int64_t Fpad, hipStream_t st);
+// ny x nx FP64 heights of a synthetic terrain into device memory h (see
+// erp_mesh_from_terrain; erp_terrain.hip); synchronous on st
+void terrain_heights(erp_ctx* ctx, const erp_terrain& t, double* h, hipStream_t st);
 
 inline int next_pow2(int64_t n) {
     int p = 1;

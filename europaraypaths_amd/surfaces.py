@@ -192,6 +192,35 @@ def fbm_heights(nx, ny, fs, rms, hurst, seed=0) -> np.ndarray:
     return z * (rms / s) if s > 0 else z
 
 
+def terrain_noise(kind: str, nx: int, ny: int, seed: int = 0) -> np.ndarray:
+    """The draws of gaussian_heights (kind "gaussian": ny x nx normals) or
+    fbm_heights (kind "fbm": ny x nx phases, then ny x nx normals) for a
+    seed, flattened in that order: handed to the device generator
+    (Context.mesh_from_terrain) they give the host generator's field."""
+    rng = np.random.default_rng(seed)
+    if kind == "gaussian":
+        return rng.standard_normal((ny, nx)).reshape(-1)
+    if kind == "fbm":
+        phase = rng.uniform(0, 2 * np.pi, size=(ny, nx))
+        return np.concatenate([phase.reshape(-1), rng.standard_normal((ny, nx)).reshape(-1)])
+    raise ValueError(f"unknown terrain kind {kind}")
+
+
+def device_terrain(ctx, kind: str, nx: int, ny: int, fs: float, ox: float, oy: float,
+                   rms: float, *, corr_len: float = 0.0, hurst: float = 0.8, seed: int = 0,
+                   host_draws: bool = True):
+    """A terrain mesh generated on the device (erp_mesh_from_terrain): the
+    gaussian_heights / fbm_heights field and its terrain_soa mesh, built
+    without the host FFTs and normals.  host_draws: the host's seeded numpy
+    draws (the same field as the host generators, to FP32 transform
+    rounding); otherwise the device's own seeded draws."""
+    from ._abi import ERP_TERRAIN_FBM, ERP_TERRAIN_GAUSSIAN
+    code = {"gaussian": ERP_TERRAIN_GAUSSIAN, "fbm": ERP_TERRAIN_FBM}[kind]
+    noise = terrain_noise(kind, nx, ny, seed) if host_draws else None
+    return ctx.mesh_from_terrain(code, nx, ny, fs, ox, oy, rms, corr_len=corr_len, hurst=hurst,
+                                 seed=seed, noise=noise)
+
+
 def terrain_soa(nx, ny, fs, ox, oy, heights) -> np.ndarray:
     """(12, F) SoA of a height grid placed at (ox, oy) with spacing fs."""
     params = {"fs": fs, "nx": nx, "ny": ny, "ox": ox, "oy": oy}

@@ -170,6 +170,37 @@ void erp_mesh_release(erp_mesh* mesh);
  * host-side normal computation. */
 int erp_mesh_from_heights(erp_ctx* ctx, const double* heights, int64_t nx, int64_t ny,
                           double fs, double ox, double oy, erp_mesh** out);
+
+/* A synthetic terrain generated on the device.  The reference draws its
+ * terrains on the host (gen_normals, simple_surfaces.py:3-39, over height
+ * fields from its examples' scripts); this is the host generators of the
+ * benchmark meshes (europaraypaths_amd/surfaces.py) on the GPU:
+ *   ERP_TERRAIN_GAUSSIAN: white noise filtered by exp(-K^2 corr_len^2 / 8)
+ *     (K the angular wavenumber on the fftfreq grid): a Gaussian field with
+ *     autocorrelation exp(-r^2 / corr_len^2);
+ *   ERP_TERRAIN_FBM: amplitude |k|^-(hurst + 1) (0 at k = 0) times a normal
+ *     times exp(i phase), phase uniform in [0, 2 pi): fBm, PSD ~ |k|^-(2H+2).
+ * Then the real part of the inverse 2-D DFT, mean removed, rescaled to RMS
+ * `rms`, and the mesh of erp_mesh_from_heights over it.  `noise`, when not
+ * NULL, holds the host's draws in the host generators' order (Gaussian: ny
+ * nx normals; fBm: ny nx phases, then ny nx normals), so that the device
+ * field is the host's to FP32 transform rounding; NULL draws them on the
+ * device from `seed` (SplitMix64 counters, Box-Muller normals).  Any grid
+ * size up to 65536 per side (Bluestein transforms). */
+#define ERP_TERRAIN_GAUSSIAN 0
+#define ERP_TERRAIN_FBM      1
+typedef struct erp_terrain {
+    int32_t kind;
+    int32_t reserved;
+    int64_t nx, ny;
+    double fs, ox, oy;  /* facet spacing and grid origin, as erp_mesh_from_heights */
+    double rms;         /* RMS height after the rescale */
+    double corr_len;    /* ERP_TERRAIN_GAUSSIAN */
+    double hurst;       /* ERP_TERRAIN_FBM */
+    uint64_t seed;      /* device draws (noise NULL) */
+    const double* noise;
+} erp_terrain;
+int erp_mesh_from_terrain(erp_ctx* ctx, const erp_terrain* terrain, erp_mesh** out);
 /* Copy a resident mesh back as the 12 x F float32 SoA of erp_mesh_upload
  * (soa must hold 12 F floats). */
 int erp_mesh_download(const erp_mesh* mesh, float* soa);

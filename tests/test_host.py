@@ -206,6 +206,9 @@ int main(void) {
          ERP_RUN_RAW_SURFACE, ERP_E_UNSUPPORTED);
   printf("%zu %zu %zu %d %d\n", sizeof(erp_track), offsetof(erp_track, ns),
          offsetof(erp_track, nvalid_dev), ERP_TUNE_TRACK_ROWS, ERP_TUNE_DIRECT_LAUNCH);
+  printf("%zu %zu %zu %zu %d %d\n", sizeof(erp_terrain), offsetof(erp_terrain, nx),
+         offsetof(erp_terrain, seed), offsetof(erp_terrain, noise), ERP_TERRAIN_GAUSSIAN,
+         ERP_TERRAIN_FBM);
   return 0;
 }
 """)
@@ -230,6 +233,10 @@ int main(void) {
     T = _abi.ErpTrack
     assert tr == [ctypes.sizeof(T), T.ns.offset, T.nvalid_dev.offset,
                   _abi.ERP_TUNE_TRACK_ROWS, _abi.ERP_TUNE_DIRECT_LAUNCH]
+    te = [int(v) for v in lines[5].split()]
+    R = _abi.ErpTerrain
+    assert te == [ctypes.sizeof(R), R.nx.offset, R.seed.offset, R.noise.offset,
+                  _abi.ERP_TERRAIN_GAUSSIAN, _abi.ERP_TERRAIN_FBM]
 
 
 def test_footprint_is_host_only_and_monotone():
