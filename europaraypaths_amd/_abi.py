@@ -9,6 +9,7 @@ ERP_E_HIP = -2
 ERP_E_NOMEM = -3
 ERP_E_UNSUPPORTED = -4
 ERP_E_NOTREADY = -5
+ERP_E_INTERNAL = -6
 
 ERP_MODE_CONVOLUTION = 0x01
 ERP_MODE_CONV_LINEAR = 0x02
@@ -28,6 +29,7 @@ ERP_TUNE_SHOTS_PER_BATCH = 1
 ERP_TUNE_PAIRS_PER_LAUNCH = 2
 ERP_TUNE_TRACK_ROWS = 3
 ERP_TUNE_DIRECT_LAUNCH = 4
+ERP_TUNE_WINDOW_SHIFT = 5
 
 ERP_TERRAIN_GAUSSIAN = 0
 ERP_TERRAIN_FBM = 1

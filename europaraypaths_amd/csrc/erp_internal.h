@@ -196,6 +196,7 @@ struct WorkBufs {
     DevBuf prisms;
     DevBuf out;        // S x nr complex (device output staging)
     DevBuf nvalid;     // S int32
+    DevBuf oow;        // count of target terms outside their pair's bin window
 };
 
 // stage timers: events recorded on the run stream, resolved after it syncs
@@ -252,6 +253,8 @@ struct erp_ctx {
     // multi-track batches: rows per batch (0 = default), launch directly
     // instead of replaying captured graphs (> 0)
     std::atomic<int64_t> tune_rows{0}, tune_direct{0};
+    // > 0 only in tests: shift every pair's bin window up by this many bins
+    std::atomic<int64_t> tune_shift{0};
     MtState* mt = nullptr;  // buffers and graphs of erp_run_tracks
 };
 

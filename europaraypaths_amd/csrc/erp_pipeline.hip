@@ -91,6 +91,7 @@ struct KPar {
                        // the per-pair LDS chain, P_fft otherwise
     uint32_t mode;
     uint64_t seed;
+    int win_shift;     // 0; > 0 only to test the target kernel's window check
 };
 
 struct ShotK {
@@ -1779,10 +1780,12 @@ __device__ __forceinline__ Tri tri_eval(const RecV& rv, bool frame_ok, double sx
 }
 
 // two-bin split of one term into the LDS window [base, base + len) or, past
-// it, the global row
+// it, the global row.  flag set: the pair's window holds every bin its
+// bounds allow, so a term past it breaks a bound; it is counted in *flag
+// (the run then fails) instead of being added to row bins nothing reads.
 template <int ACC = ACC_F64>
 __device__ __forceinline__ void tri_add(const Tri& t, int base, int len, int nr,
-                                        WinS winA, float2* rowA) {
+                                        WinS winA, float2* rowA, int* flag) {
     const int k0 = t.b - base;
     if constexpr (ACC == ACC_F64) {
         // split in FP64, the window's precision: c1 = c w, c0 = c - c1
@@ -1793,6 +1796,8 @@ __device__ __forceinline__ void tri_add(const Tri& t, int base, int len, int nr,
         if ((unsigned)k0 < (unsigned)len) {
             atomicAdd(&winA.x[k0], c0x);
             atomicAdd(&winA.y[k0], c0y);
+        } else if (flag) {
+            atomicAdd(flag, 1);
         } else {
             glb_add2(rowA, t.b, make_float2((float)c0x, (float)c0y));
         }
@@ -1800,6 +1805,8 @@ __device__ __forceinline__ void tri_add(const Tri& t, int base, int len, int nr,
             if ((unsigned)(k0 + 1) < (unsigned)len) {
                 atomicAdd(&winA.x[k0 + 1], c1x);
                 atomicAdd(&winA.y[k0 + 1], c1y);
+            } else if (flag) {
+                atomicAdd(flag, 1);
             } else {
                 glb_add2(rowA, t.b + 1, make_float2((float)c1x, (float)c1y));
             }
@@ -1809,9 +1816,12 @@ __device__ __forceinline__ void tri_add(const Tri& t, int base, int len, int nr,
     const float wf = (float)t.w;
     const float2 c0 = make_float2(t.c.x * (1.f - wf), t.c.y * (1.f - wf));
     const float2 c1 = make_float2(t.c.x * wf, t.c.y * wf);
-    if (k0 >= 0 && k0 < len) win_add<ACC>(winA, k0, c0); else glb_add2(rowA, t.b, c0);
+    if (k0 >= 0 && k0 < len) win_add<ACC>(winA, k0, c0);
+    else if (flag) atomicAdd(flag, 1);
+    else glb_add2(rowA, t.b, c0);
     if (t.b + 1 < nr) {
         if (k0 + 1 >= 0 && k0 + 1 < len) win_add<ACC>(winA, k0 + 1, c1);
+        else if (flag) atomicAdd(flag, 1);
         else glb_add2(rowA, t.b + 1, c1);
     }
 }
@@ -1841,7 +1851,8 @@ k_target_accum_fast(const float* __restrict__ tg0, int64_t Tpad0, const int* __r
                     int np, KPar kp0, float2* __restrict__ outA, int* __restrict__ pbase,
                     int* __restrict__ pwin, int frame_ok0, int npairs0,
                     int* __restrict__ next_pair, const TrackK* __restrict__ trk,
-                    const RowK* __restrict__ rows, const int* __restrict__ npairs_dev) {
+                    const RowK* __restrict__ rows, const int* __restrict__ npairs_dev,
+                    int* __restrict__ oow) {
     __shared__ double winS[2 * TWF];
     __shared__ int s_pi;
     const WinS winA{winS, winS + TWF};
@@ -1883,7 +1894,9 @@ k_target_accum_fast(const float* __restrict__ tg0, int64_t Tpad0, const int* __r
         const double tmin = sqrt((double)bx * bx + (double)by * by + (double)bz * bz);
         const double itmin = __longlong_as_double((long long)bd.itd_min);
         int base = (int)floor((itmin + n_lo * tmin - rst) / kp.drd) - 2;
-        base = max(base, 0);
+        // win_shift > 0 only to test the out-of-window check: a window that
+        // starts above the lowest bin the bounds allow
+        base = max(base, 0) + kp.win_shift;
         bool fits = false;
         int len = TWF;
         int top = nr;
@@ -1906,6 +1919,10 @@ k_target_accum_fast(const float* __restrict__ tg0, int64_t Tpad0, const int* __r
             pwin[pi] = len;
         }
         float2* rowA = outA + (size_t)pi * P;
+        // out-of-window terms: counted as broken bounds where the window
+        // covers every bin the bounds allow (deterministic: always, so no
+        // float atomic can reach the row)
+        int* const flag = (DET || fits) ? oow : nullptr;
         if (!fits || (DET && !pbase)) {
             // bins outside the window(s) [base, base + len) or, deterministic,
             // [base, top): zero, then reached only by out-of-window atomics
@@ -1954,11 +1971,11 @@ k_target_accum_fast(const float* __restrict__ tg0, int64_t Tpad0, const int* __r
                     const float2 c1 = make_float2(ta.c.x * wf, ta.c.y * wf);
                     const int k0 = ta.b - wb;
                     if (k0 >= 0 && k0 < wl) lds_add2_fix(wux, wuy, k0, c0, scale);
-                    else if ((ta.b < base && first) || (ta.b >= top && last)) glb_add2(rowA, ta.b, c0);
+                    else if ((ta.b < base && first) || (ta.b >= top && last)) atomicAdd(flag, 1);
                     if (ta.b + 1 < nr) {
                         if (k0 + 1 >= 0 && k0 + 1 < wl) lds_add2_fix(wux, wuy, k0 + 1, c1, scale);
                         else if ((ta.b + 1 < base && first) || (ta.b + 1 >= top && last))
-                            glb_add2(rowA, ta.b + 1, c1);
+                            atomicAdd(flag, 1);
                     }
                 };
                 // one facet per thread and step, the next step's record loaded
@@ -2019,8 +2036,8 @@ k_target_accum_fast(const float* __restrict__ tg0, int64_t Tpad0, const int* __r
                         tb = tri_eval<PH, S3>(cb, frame_ok, sh.sx, sh.sy, sh.sz, tx, ty, tz, tnx, tny,
                                           tnz, tn_up, rst, inv_dr, n_d, two_cyc, nr, lossless,
                                           prisms, np, kp, st);
-                    if (ta.valid) tri_add<ACC>(ta, base, len, nr, winA, rowA);
-                    if (tb.valid) tri_add<ACC>(tb, base, len, nr, winA, rowA);
+                    if (ta.valid) tri_add<ACC>(ta, base, len, nr, winA, rowA, flag);
+                    if (tb.valid) tri_add<ACC>(tb, base, len, nr, winA, rowA, flag);
                 };
                 RecV a0 = rec_at(threadIdx.x), b0 = rec_at(threadIdx.x + NTF);
                 for (int k = threadIdx.x; k < V; k += 4 * NTF) {
@@ -2443,6 +2460,7 @@ KPar make_kpar(const erp_params& p, int P_fft) {
     k.rerad = p.rerad_funct;
     k.mode = p.mode;
     k.seed = p.seed;
+    k.win_shift = 0;
     return k;
 }
 
@@ -2555,6 +2573,19 @@ void window_conv(erp_ctx* ctx, const FftPlan& plan, float2* data, int rows, cons
     hipLaunchKernelGGL(k_window, dim3(grid_for((size_t)nr * rows)), dim3(256), 0, st, out, nr,
                        data, P, nr, rows, 1.0f, 0);
     ERP_CHECK_LAUNCH("k_window");
+}
+
+// After a run: the count of target terms that fell outside a bin window
+// their pair's bounds should have covered (k_target_accum_fast); any is a
+// broken bound, whose terms were dropped, so the run fails.
+void check_windows(const int* oow, hipStream_t st) {
+    int n = 0;
+    ERP_HIP_TRY(hipMemcpyAsync(&n, oow, sizeof(int), hipMemcpyDeviceToHost, st));
+    ERP_HIP_TRY(hipStreamSynchronize(st));
+    if (n != 0)
+        throw ErpError{ERP_E_INTERNAL,
+                       std::to_string(n) + " facet x target terms fell outside the bin window "
+                                           "of their pair (a window bound failed)"};
 }
 
 // The (row, target) pairs of one target launch, taken in order from row
@@ -2783,6 +2814,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     const FftPlan planP = fft_plan(P);
     KPar kp = make_kpar(p, P);
     kp.ortho = mesh->frame_ok ? 1 : 0;
+    kp.win_shift = (int)ctx->tune_shift;
     const int L = 2 * nr - 1;
     const int nblk = (int)((F + FB - 1) / FB);
     // facet order of the surface kernels: tiles where a wide aperture lights
@@ -2905,6 +2937,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
             }
             if (any_t0) pb = std::max(pb, sizeof(int) * 2 * s * (size_t)T + 64);
             if (pb) buf(W.pairs, pb, "pair lists");
+            if (any_typed) buf(W.oow, 256, "window check");
             return tot;
         };
         const size_t avail = arena->left();
@@ -2950,6 +2983,14 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     if (T > 0) {
         W.tlist.ensure(sizeof(int) * (size_t)Smax * T * 2);
         W.tcnt.ensure(sizeof(int) * (size_t)Smax * 2);
+    }
+
+    // terms of the plain target kernel that fall outside their pair's bin
+    // window where the window holds every bin the pair's bounds allow
+    const bool check_win = any_typed && !spec;
+    if (check_win) {
+        W.oow.ensure(256);
+        ERP_HIP_TRY(hipMemsetAsync(W.oow.p, 0, sizeof(int), st));
     }
 
     std::vector<int> hV(Smax), htc(2 * Smax);
@@ -3193,7 +3234,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                                            (fused || half) ? dpb : nullptr, dpw,
                                            mesh->frame_ok ? 1 : 0, n, dnext,
                                            (const TrackK*)nullptr, (const RowK*)nullptr,
-                                           (const int*)nullptr);
+                                           (const int*)nullptr, W.oow.as<int>());
                         ERP_CHECK_LAUNCH("k_target_accum_fast");
                     } else {
                         hipLaunchKernelGGL(k_target_accum<1>, dim3(n), dim3(NTT),
@@ -3342,6 +3383,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                            nvalid_dev ? nvalid_dev + s0 : nullptr);
         ERP_CHECK_LAUNCH("k_finalize");
     }
+    if (check_win) check_windows(W.oow.as<int>(), st);
 }
 
 // ---------------------------------------------------------------------------
@@ -3489,6 +3531,7 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
         d = TrackK{};
         d.kp = make_kpar(p, P);
         d.kp.ortho = mesh->frame_ok ? 1 : 0;
+        d.kp.win_shift = (int)ctx->tune_shift;
         d.kp.row_live = nr;
         d.mesh = mesh->soa.as<float>();
         d.Fpad = mesh->Fpad;
@@ -3613,6 +3656,7 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
     for (int k = 0; k < ntr; ++k)
         for (int64_t s = 0; s < tracks[trk_idx[k]].ns; ++s) all.push_back(RowK{k, (int)s});
 
+    if (targ) ERP_HIP_TRY(hipMemsetAsync(d_next + 1, 0, sizeof(int), st));  // window check
     std::vector<RowK> hrows(r);
     std::vector<erp_shot> hshots(r);
     std::vector<int> hV(r), htc(2 * r), tc1(r);
@@ -3785,7 +3829,7 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
                                    dim3(NTF), 0, st, (const float*)nullptr, (int64_t)0, t_pshot,
                                    t_ptgt, dshots, dV, dbase, rec, dbounds,
                                    (const erp_prism*)nullptr, 0, kp0, A, d_pbase, d_pwin, 0, 0,
-                                   d_next, dtrk, drows, (const int*)t_n);
+                                   d_next, dtrk, drows, (const int*)t_n, d_next + 1);
                 ERP_CHECK_LAUNCH("k_target_accum_fast");
                 target_pairs_half(ctx, planP, A, d_pbase, d_pwin, nr, C, nullptr, Z, st, d_G,
                                   t_n);
@@ -3828,6 +3872,7 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
         });
         PT_.end(st);
     }
+    if (targ) check_windows(d_next + 1, st);
 }
 
 }  // namespace

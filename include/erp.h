@@ -47,6 +47,7 @@ extern "C" {
 #define ERP_E_NOMEM       -3  /* device allocation failed */
 #define ERP_E_UNSUPPORTED -4  /* mode the reference itself cannot run */
 #define ERP_E_NOTREADY    -5  /* erp_poll: job still running */
+#define ERP_E_INTERNAL    -6  /* a device-side consistency check failed (message) */
 
 /* ---- processing mode bits (erp_params.mode), file_io.cu:238-246 --------- */
 #define ERP_MODE_CONVOLUTION     0x01u /* "convolution"        */
@@ -334,11 +335,18 @@ int erp_profile_read(erp_ctx* ctx, int stage, double* ms, int64_t* launches, dou
  *   ERP_TUNE_TRACK_ROWS      : (track, shot) rows per multi-track batch
  *                              (default 64, at most 256)
  *   ERP_TUNE_DIRECT_LAUNCH   : > 0: multi-track batches launch their kernels
- *                              directly instead of replaying HIP graphs */
+ *                              directly instead of replaying HIP graphs
+ *   ERP_TUNE_WINDOW_SHIFT    : tests only: shift the bin window of every
+ *                              (shot, target) pair of the plain target kernel
+ *                              up by this many bins, so that its lowest terms
+ *                              fall outside it and the run fails with
+ *                              ERP_E_INTERNAL (the check that a window bound
+ *                              held; 0 = off) */
 #define ERP_TUNE_SHOTS_PER_BATCH  1
 #define ERP_TUNE_PAIRS_PER_LAUNCH 2
 #define ERP_TUNE_TRACK_ROWS       3
 #define ERP_TUNE_DIRECT_LAUNCH    4
+#define ERP_TUNE_WINDOW_SHIFT     5
 int erp_set_tuning(erp_ctx* ctx, int key, int64_t value);
 /* The value set for key (0 = default). */
 int erp_get_tuning(erp_ctx* ctx, int key, int64_t* value);
