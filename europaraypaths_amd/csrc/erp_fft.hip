@@ -649,21 +649,39 @@ __device__ __forceinline__ void fft_regs(float2* s, float2 (&x)[PPT],
     dft_r<16>(x, false);
 }
 
-template <int NTB>
+// x[q] = a[i] (times mod[i]) for i = tid + q NTB inside the pair's window
+// [base, base + win) (base < 0: the whole row), else 0; LIM: also 0 from
+// i = lim on (the row's bins end there)
+template <int NTB, bool LIM = false>
 __device__ __forceinline__ void load_window(float2 (&x)[PPT], const float2* __restrict__ a,
-                                            int base, int win, const float2* __restrict__ mod) {
+                                            int base, int win, const float2* __restrict__ mod,
+                                            int lim = 0) {
     const int tid = opaque_tid();
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
         const int i = tid + q * NTB;
-        const bool live = base < 0 || (i >= base && i < base + win);
+        bool live = base < 0 || (i >= base && i < base + win);
+        if (LIM) live = live && i < lim;
         float2 v = live ? a[i] : make_float2(0.f, 0.f);
         if (mod) v = cmul(v, mod[i]);
         x[q] = v;
     }
 }
 
-template <int NTB, bool MT = false>
+// CROP: nr < n bins (P = 2n, n the next power of two): the kept window
+// z[j] = y[n + j] holds only j < nr, so z is no longer u - conj(m) v over
+// the whole half and the shortcut DFT(IDFT(x)) = n x does not apply; z is
+// formed in the time domain and transformed twice (still six transforms):
+//   0: a          -> E        -> even plane = E; x = conj(E G_e)
+//   1: x          -> conj(u)  -> h = u
+//   2: a m        -> O        -> odd plane = O; x = conj(O G_o)
+//   3: x          -> conj(v)  -> z = u - conj(m) v for j < nr, 0 above;
+//                                x = z, h = z m
+//   4: x          -> Z_e      -> even plane = E Z_e (E read back)
+//   5: h          -> Z_o      -> odd plane = O Z_o (O read back)
+// The range kernel spectrum G is placed for the window at n (kind 2 of
+// k_gen_kernel).  Every thread reads back only what it wrote.
+template <int NTB, bool MT = false, bool CROP = false>
 __global__ void __launch_bounds__(NTB) k_tpair_half(const float2* __restrict__ A, int P,
                                                     const int* __restrict__ pbase,
                                                     const int* __restrict__ pwin,
@@ -671,7 +689,7 @@ __global__ void __launch_bounds__(NTB) k_tpair_half(const float2* __restrict__ A
                                                     float2* __restrict__ out,
                                                     const float2* __restrict__ lad,
                                                     const float4* const* __restrict__ Gpair,
-                                                    const int* __restrict__ npairs) {
+                                                    const int* __restrict__ npairs, int nr) {
     constexpr int n = NTB * PPT;
     __shared__ float2 s[n + n / 16];
     const int p = blockIdx.x;
@@ -687,6 +705,46 @@ __global__ void __launch_bounds__(NTB) k_tpair_half(const float2* __restrict__ A
     float2* o2 = out + (size_t)p * P;  // even plane [0, n), odd plane [n, 2n)
     const float fn = (float)n;
     float2 x[PPT], h[PPT];
+    if constexpr (CROP) {
+#pragma unroll 1
+        for (int ph = 0; ph < 6; ++ph) {
+            if (ph == 0 || ph == 2)
+                load_window<NTB, true>(x, a, base, win, ph == 2 ? mod : nullptr, nr);
+            if (ph == 5) {
+#pragma unroll
+                for (int q = 0; q < PPT; ++q) x[q] = h[q];
+            }
+            fft_regs<NTB>(s, x, lad);
+            const int tid = opaque_tid();
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int i = tid + q * NTB;
+                if (ph == 0) {
+                    const float4 g = Gn[i];
+                    o2[i] = x[q];
+                    x[q] = conjf2(cmul(x[q], make_float2(g.x, g.y)));
+                } else if (ph == 1) {
+                    h[q] = conjf2(x[q]);
+                } else if (ph == 2) {
+                    const float4 g = Gn[i];
+                    o2[n + i] = x[q];
+                    x[q] = conjf2(cmul(x[q], make_float2(g.z, g.w)));
+                } else if (ph == 3) {
+                    // v = conj(x); z = u - conj(m) v = u - conj(m x)
+                    const float2 mx = cmul(mod[i], x[q]);
+                    const float2 z = i < nr ? make_float2(h[q].x - mx.x, h[q].y + mx.y)
+                                            : make_float2(0.f, 0.f);
+                    x[q] = z;
+                    h[q] = cmul(z, mod[i]);
+                } else if (ph == 4) {
+                    o2[i] = cmul(o2[i], x[q]);
+                } else {
+                    o2[n + i] = cmul(o2[n + i], x[q]);
+                }
+            }
+        }
+        return;
+    }
     // Phase: input -> forward transform -> use; an inverse transform is
     // fft(conj(.)) with the conjugates folded into the neighbouring steps
     //   0: a          -> E        -> even plane = E (kept there); x = conj(E G_e)
@@ -942,9 +1000,11 @@ void target_spectra_fused(erp_ctx* ctx, const FftPlan& p, float2* A, float2* Y,
 
 int target_half_slice() { return SLICE; }
 
+// P = 2n with n the next power of two of nr, from 1024 to 16384 (nr below
+// n runs the cropped variant)
 bool target_half_supported(const FftPlan& p, int nr) {
-    return p.N == 2 * nr && (nr == 1024 || nr == 2048 || nr == 4096 || nr == 8192 ||
-                             nr == 16384);
+    const int n = p.N / 2;
+    return p.N == 2 * next_pow2(nr) && n >= 1024 && n <= 16384;
 }
 
 void fft_natural_order(const FftPlan& p, const float2* src, float2* dst, hipStream_t st) {
@@ -965,7 +1025,7 @@ void target_pairs_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const in
                        hipStream_t st, const float4* const* Gpair, const int* npairs_dev) {
     if (!target_half_supported(p, nr))
         throw ErpError{ERP_E_INVALID,
-                       "per-pair LDS chain needs P = 2 nr, nr a power of two in [1024, 16384]"};
+                       "per-pair LDS chain needs P = 2 next_pow2(nr) in [2048, 32768]"};
     const float2* lad = ctx->twiddle.as<float2>();
     const float4* G4 = reinterpret_cast<const float4*>(Gnat);
     float2* o4 = work;
@@ -973,16 +1033,26 @@ void target_pairs_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const in
     // pairs past *npairs_dev exit at once
     auto go = [&](auto kern, int nt) {
         hipLaunchKernelGGL(kern, dim3(n), dim3(nt), 0, st, A, p.N, pbase, pwin, G4, o4, lad,
-                           Gpair, npairs_dev);
+                           Gpair, npairs_dev, nr);
     };
     const bool mt = Gpair != nullptr;
-    switch (nr) {
-        case 1024: mt ? go(k_tpair_half<64, true>, 64) : go(k_tpair_half<64>, 64); break;
-        case 2048: mt ? go(k_tpair_half<128, true>, 128) : go(k_tpair_half<128>, 128); break;
-        case 4096: mt ? go(k_tpair_half<256, true>, 256) : go(k_tpair_half<256>, 256); break;
-        case 8192: mt ? go(k_tpair_half<512, true>, 512) : go(k_tpair_half<512>, 512); break;
-        default: mt ? go(k_tpair_half<1024, true>, 1024) : go(k_tpair_half<1024>, 1024); break;
+    const bool crop = nr < p.N / 2;
+    // the 2 x 2 variants (multi-track, cropped) of one transform size
+    auto pick = [&](auto k00, auto k10, auto k01, auto k11, int nt) {
+        if (crop) mt ? go(k11, nt) : go(k01, nt);
+        else mt ? go(k10, nt) : go(k00, nt);
+    };
+#define ERP_TPAIR_VARIANTS(NTB)                                                       \
+    pick(k_tpair_half<NTB, false, false>, k_tpair_half<NTB, true, false>,            \
+         k_tpair_half<NTB, false, true>, k_tpair_half<NTB, true, true>, NTB)
+    switch (p.N / 2) {
+        case 1024: ERP_TPAIR_VARIANTS(64); break;
+        case 2048: ERP_TPAIR_VARIANTS(128); break;
+        case 4096: ERP_TPAIR_VARIANTS(256); break;
+        case 8192: ERP_TPAIR_VARIANTS(512); break;
+        default: ERP_TPAIR_VARIANTS(1024); break;
     }
+#undef ERP_TPAIR_VARIANTS
     ERP_CHECK_LAUNCH("k_tpair_half");
 }
 

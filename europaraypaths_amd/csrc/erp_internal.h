@@ -273,7 +273,8 @@ void fft_init_twiddles(erp_ctx* ctx);
 // FFT(window(IFFT(FFT(A_p) G))) into S[shot] for each shot segment.  Groups
 // [gbeg, gend) partition the pairs without crossing shots; segment k sums
 // groups (two-pass sizes) or pairs (single-pass sizes) [gseg_beg, gseg_end).
-// one workgroup per (shot, target) pair, whole chain in LDS (P = 2 nr)
+// one workgroup per (shot, target) pair, whole chain in LDS: P = 2n, n the
+// next power of two of nr, from 1024 to 16384 (nr < n: the cropped variant)
 bool target_half_supported(const FftPlan& p, int nr);
 int target_half_slice();
 // Gpair / npairs_dev (multi-track batches): per-pair range kernel spectra

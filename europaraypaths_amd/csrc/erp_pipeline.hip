@@ -87,8 +87,8 @@ struct KPar {
     float tsq;
     double B, smpl;
     int nr, P_fft, rerad;
-    int row_live;      // entries of a pair row the spectral chain reads: nr for
-                       // the per-pair LDS chain, P_fft otherwise
+    int row_live;      // entries of a pair row the spectral chain reads: P_fft / 2
+                       // for the per-pair LDS chain, P_fft otherwise
     uint32_t mode;
     uint64_t seed;
     int win_shift;     // 0; > 0 only to test the target kernel's window check
@@ -1141,21 +1141,27 @@ __global__ void k_gen_kernel(float2* __restrict__ g, int P, int nr, double dr, d
     if (e0 >= P) return;
     float v = 0.f;
     const int e = e0 - 1;
-    if (e >= 0 && e < 2 * nr - 1) {
-        if (kind == 0) {
+    if (kind == 0 || kind == 2) {
+        // output bin j = input bin i + d sits at y[ws + j]: g[ws + d] for
+        // |d| < nr, the 2nr-circular, kc-cropped kernel of the reference;
+        // ws = nr (kind 0) or P / 2 (kind 2, the cropped per-pair chain)
+        const int ws = kind == 0 ? nr : P / 2;
+        const int d = e0 - ws;
+        if (d > -nr && d < nr) {
             const int kc = nr / 2;
-            const int m = (int)(((long long)e + kc - nr + 1 + 2LL * nr) % (2LL * nr));
+            const int m = (int)(((long long)d + kc + 2LL * nr) % (2LL * nr));
             const double x = ((double)(m - kc) * dr) / res;
             v = (fabs(x) < 1e-6) ? 1.f : (float)(sin(M_PI * x) / (M_PI * x));
-        } else {
-            const int m = (int)(((long long)e + 1 - nr + 2LL * nr) % (long long)nr);
-            const double r = rst + (double)m * dr;
-            const double x1 = (r - rst) / res;
-            const double x2 = (r - (rst + (double)nr * dr)) / res;
-            const double a = (fabs(x1) < 1e-6) ? 1.0 : sin(M_PI * x1) / (M_PI * x1);
-            const double b = (fabs(x2) < 1e-6) ? 1.0 : sin(M_PI * x2) / (M_PI * x2);
-            v = (float)(a + b);
         }
+    } else if (e >= 0 && e < 2 * nr - 1) {
+        // kind 1: the nr-circular kernel of one receive window start
+        const int m = (int)(((long long)e + 1 - nr + 2LL * nr) % (long long)nr);
+        const double r = rst + (double)m * dr;
+        const double x1 = (r - rst) / res;
+        const double x2 = (r - (rst + (double)nr * dr)) / res;
+        const double a = (fabs(x1) < 1e-6) ? 1.0 : sin(M_PI * x1) / (M_PI * x1);
+        const double b = (fabs(x2) < 1e-6) ? 1.0 : sin(M_PI * x2) / (M_PI * x2);
+        v = (float)(a + b);
     }
     g[e0] = make_float2(v, 0.f);
 }
@@ -2828,10 +2834,10 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     const float res_t = spec ? p.rng_res : 2.f * p.rng_res;  // sim.cu:853-858
 
     // chirp spectra: cached per context, or (arena) built into the region
-    auto chirp = [&](float res, bool natural) -> const float2* {
-        if (!arena) return get_chirp(ctx, nr, p.dr, res, 0, 0.f, P, planP, st, natural);
+    auto chirp = [&](float res, bool natural, int kind = 0) -> const float2* {
+        if (!arena) return get_chirp(ctx, nr, p.dr, res, kind, 0.f, P, planP, st, natural);
         float2* spec_ = arena->take_n<float2>((size_t)P, "chirp spectrum");
-        build_chirp(ctx, spec_, nr, p.dr, res, 0, 0.f, P, planP, st);
+        build_chirp(ctx, spec_, nr, p.dr, res, kind, 0.f, P, planP, st);
         if (!natural) return spec_;
         float2* nat = arena->take_n<float2>((size_t)P, "chirp spectrum");
         fft_natural_order(planP, spec_, nat, st);
@@ -2841,10 +2847,12 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     if (!spec || (conv && lin)) g_surf = chirp(p.rng_res, false);
     const float2* g_targ = nullptr;
     if (T > 0 && conv && lin) g_targ = chirp(res_t, false);
-    // plain mode: per-pair spectral chain in one workgroup when P = 2 nr fits LDS
+    // plain mode: per-pair spectral chain in one workgroup when P / 2 fits
+    // LDS (nr below P / 2: its cropped variant, whose range kernel has the
+    // kept window at P / 2 and whose rows are read up to P / 2)
     const bool half = !spec && T > 0 && conv && lin && target_half_supported(planP, nr);
-    kp.row_live = half ? nr : P;
-    const float2* g_targ_nat = half ? chirp(res_t, true) : nullptr;
+    kp.row_live = half ? P / 2 : P;
+    const float2* g_targ_nat = half ? chirp(res_t, true, nr < P / 2 ? 2 : 0) : nullptr;
     // circular convolution of type-0 targets: one genChirp table per rx window
     // start; with an arena one table is rebuilt for each shot
     float2* g_circ = (arena && any_t0 && conv && !lin)
@@ -3499,7 +3507,7 @@ bool mt_eligible(const erp_track& t, uint32_t flags) {
     const uint32_t m = p.mode;
     if ((m & ERP_MODE_SPECULAR) || !(m & ERP_MODE_CONVOLUTION) || !(m & ERP_MODE_CONV_LINEAR))
         return false;
-    if (p.nr < 1024 || p.nr > 16384 || (p.nr & (p.nr - 1)) != 0) return false;
+    if (p.nr <= 512 || p.nr > 16384) return false;
     if (t.targets && t.targets->n0 > 0) return false;
     if (t.ns <= 0 || t.mesh->F == 0) return false;
     return true;
@@ -3509,10 +3517,11 @@ bool mt_eligible(const erp_track& t, uint32_t flags) {
 // without the second subsurface medium
 void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64_t>& trk_idx,
                   int nr, bool s3, bool det, hipStream_t st) {
-    const int P = 2 * nr;
+    const int P = next_pow2(2 * (int64_t)nr - 1);
     const FftPlan planP = fft_plan(P);
     if (!target_half_supported(planP, nr))
         throw ErpError{ERP_E_INVALID, "multi-track batch: range length without a pair chain"};
+    const int tkind = nr < P / 2 ? 2 : 0;  // the cropped pair chain's kernel placement
     const int L = 2 * nr - 1;
     const FftPlan planM = fft_plan(next_pow2(2 * (int64_t)L - 1));
     const int M = planM.N;
@@ -3532,7 +3541,7 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
         d.kp = make_kpar(p, P);
         d.kp.ortho = mesh->frame_ok ? 1 : 0;
         d.kp.win_shift = (int)ctx->tune_shift;
-        d.kp.row_live = nr;
+        d.kp.row_live = P / 2;
         d.mesh = mesh->soa.as<float>();
         d.Fpad = mesh->Fpad;
         d.F = mesh->F;
@@ -3546,7 +3555,8 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
             d.Tpad = t.targets->Tpad;
             d.ttype = t.targets->type.as<unsigned char>();
             // sim.cu:853-858: the target range kernel has twice the surface's
-            d.g_targ_nat = get_chirp(ctx, nr, p.dr, 2.f * p.rng_res, 0, 0.f, P, planP, st, true);
+            d.g_targ_nat =
+                get_chirp(ctx, nr, p.dr, 2.f * p.rng_res, tkind, 0.f, P, planP, st, true);
             targ = true;
         }
         // a disabled surface echo: no spectrum, so the row's echo is zero

@@ -266,7 +266,7 @@ typedef struct erp_track {
  * runs one track per process (sim.cu:615-1008); it is for many small or
  * uneven jobs that do not fill the device one at a time.  Tracks of the
  * plain method (convolution and convolution_linear, not specular) with nr
- * a power of two in [1024, 16384] and no target of type 0 are grouped by
+ * in [513, 16384] and no target of type 0 are grouped by
  * (nr, second medium or not); the (track, shot) rows of a group run in
  * batches of ERP_TUNE_TRACK_ROWS rows, every launch covering all rows of
  * the batch with each row reading its own track's mesh, targets, parameters

@@ -34,8 +34,9 @@ def _grid_targets(nx, ny, span, depth, ttype=1, y0=13.0):
 
 
 def _tracks():
-    """(name, par, soa, targets or None, shots): grouped tracks of two range
-    lengths, one with a second medium, and tracks that must run alone."""
+    """(name, par, soa, targets or None, shots): grouped tracks of three range
+    lengths (one not a power of two), one with a second medium, and tracks
+    that must run alone."""
     out = []
     par, soa, shots = scene(rough=0.3, nr=1024, h=3000.0, rst=2600.0, ns=5)
     out.append(("nr1024_a", par, soa, _grid_targets(3, 1, 120.0, 300.0), shots))
@@ -51,14 +52,17 @@ def _tracks():
     out.append(("nr1024_no_surface", par, soa, _grid_targets(2, 2, 80.0, 350.0), shots))
     par, soa, shots = scene(rough=0.3, seed=7, nr=2048, h=3000.0, rst=2600.0, ns=7)
     out.append(("nr2048", par, soa, _grid_targets(4, 1, 150.0, 300.0), shots))
+    # a range length that is not a power of two: the cropped pair chain
+    par, soa, shots = scene(rough=0.3, seed=11, nr=1500, h=3000.0, rst=2600.0, ns=5)
+    out.append(("nr1500", par, soa, _grid_targets(3, 1, 150.0, 300.0), shots))
     par, soa, shots = scene(rough=0.3, seed=8, nr=1024, h=3000.0, rst=2600.0, ns=6, eps_3=3.4,
                             z_interface=-200.0)
     out.append(("nr1024_second_medium", par, soa, _grid_targets(2, 2, 100.0, 320.0), shots))
     # alone: type-0 targets, and a range length without the per-pair chain
     par, soa, shots = scene(rough=0.3, seed=9, nr=1024, h=3000.0, rst=2600.0, ns=3)
     out.append(("type0_alone", par, soa, _grid_targets(3, 1, 120.0, 300.0, ttype=0), shots))
-    par, soa, shots = scene(rough=0.3, seed=10, nr=600, h=3000.0, rst=2600.0, ns=2)
-    out.append(("nr600_alone", par, soa, _grid_targets(2, 1, 100.0, 300.0), shots))
+    par, soa, shots = scene(rough=0.3, seed=10, nr=500, h=3000.0, rst=2600.0, ns=2)
+    out.append(("nr500_alone", par, soa, _grid_targets(2, 1, 100.0, 300.0), shots))
     return out
 
 
