@@ -100,12 +100,13 @@ def test_targets_plain_non_pow2(gpu_ctx):
     compare_traces(g, o)
 
 
-@pytest.mark.parametrize("nr", [2048, 4096, 8192, 16384, 12288, 5000])
+@pytest.mark.parametrize("nr", [2048, 4096, 8192, 16384, 12288, 5000, 20480, 20000])
 def test_targets_plain_range_sizes(gpu_ctx, nr):
-    """Every spectral-chain variant of plain-mode targets: one workgroup per
-    pair (P <= 4096), the per-pair LDS chain (P = 2 nr, nr 4096..16384), the
-    four-kernel two-pass chain (nr 12288: P = 32768, nr a multiple of the
-    256-point rows) and the unfused path (nr 5000)."""
+    """Every spectral-chain variant of plain-mode targets: the per-pair LDS
+    chain (P = 2 nr, nr 2048..16384), its cropped variant for nr below the
+    power of two (12288, 5000), and past 16384 bins the four-kernel two-pass
+    chain (nr 20480: P = 65536, nr a multiple of the 256-point rows) and the
+    unfused path (nr 20000)."""
     print(f"targets plain nr={nr}", flush=True)
     par, soa, shots = scene(rough=0.3, nr=nr, h=3000.0, rst=2600.0, ns=2)
     tg = _targets(par, n=3)
@@ -114,12 +115,12 @@ def test_targets_plain_range_sizes(gpu_ctx, nr):
     compare_traces(g, o)
 
 
-@pytest.mark.parametrize("nr", [1024, 4096, 5000])
+@pytest.mark.parametrize("nr", [1024, 4096, 5000, 20000])
 def test_deterministic_mode_matches_oracle_and_repeats_bitwise(gpu_ctx, nr):
     """ERP_RUN_DETERMINISTIC: fixed-point bins summed with integer atomics.
     Same tolerance against the FP64 oracle as the default mode, and two runs
-    give the same bits.  nr 1024: one workgroup per pair; 4096: the per-pair
-    LDS chain; 5000: the unfused chain that reads whole rows."""
+    give the same bits.  nr 1024 and 4096: the per-pair LDS chain; 5000: its
+    cropped variant; 20000: the unfused chain that reads whole rows."""
     print(f"deterministic nr={nr}", flush=True)
     par, soa, shots = scene(rough=0.3, nr=nr, h=3000.0, rst=2600.0, ns=2)
     tg = _targets(par, n=3)
@@ -296,12 +297,12 @@ def test_tiled_surface_order(gpu_ctx):
     compare_traces(ga, o)
 
 
-@pytest.mark.parametrize("nr", [1024, 4096, 5000])
+@pytest.mark.parametrize("nr", [1024, 4096, 5000, 20000])
 def test_second_subsurface_medium(gpu_ctx, nr):
     """Targets above and below an interface at z = -200 m with eps 3.4 below
     it (the extension of erp_params eps_3 / z_int): against the FP64 oracle
     in the default and the deterministic mode (per-pair chain at 1024 and
-    4096 bins, unfused chain at 5000)."""
+    4096 bins, its cropped variant at 5000, unfused chain at 20000)."""
     print(f"second medium nr={nr}", flush=True)
     par, soa, shots = scene(rough=0.3, nr=nr, h=3000.0, rst=2600.0, ns=2, eps_3=3.4,
                             z_interface=-200.0)
