@@ -103,6 +103,94 @@ def _counter_rows(directory):
     return rows
 
 
+def _csv_rows(directory, suffix):
+    import csv
+    import glob
+    rows = []
+    for path in glob.glob(os.path.join(directory, "**", "*" + suffix), recursive=True):
+        with open(path) as f:
+            rows.extend(csv.DictReader(f))
+    return rows
+
+
+def trace_summary(trace_dir, bench_json, workload_cmd, out_dir):
+    """profiles/<kernel>_<workload>.txt for the hot kernels from one
+    rocprofv3 --kernel-trace --stats run of this script (csv output) and that
+    run's JSON line: rocprof's own statistics, the trace's average over the
+    dispatches of the timed steps (the last ones of the kernel; the timed
+    steps are the only launches after the warm-up), the hipEvent stage timer
+    of the same run, and the HBM traffic per unit from profiles/traffic.json."""
+    with open(bench_json) as f:
+        b = json.loads([l for l in f if l.startswith("{")][-1])
+    wl = b["config"]["workload"]
+    stats = _csv_rows(trace_dir, "kernel_stats.csv")
+    trace = _csv_rows(trace_dir, "kernel_trace.csv")
+    if not stats or not trace:
+        raise SystemExit(f"no kernel_stats / kernel_trace csv under {trace_dir}")
+    tot = sum(float(r["TotalDurationNs"]) for r in stats)
+    top = sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:12]
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            traffic = json.load(f)
+    except (OSError, ValueError):
+        traffic = {}
+    for stage, kern in KERNELS.items():
+        ms, launches, units = (b["stage_ms"][stage], b["stage_launches"][stage],
+                               b["stage_units"][stage])
+        st = [r for r in stats if kern in r["Name"]]
+        tr = sorted((r for r in trace if kern in r["Kernel_Name"]),
+                    key=lambda r: int(r["Start_Timestamp"]))
+        if not st or not tr or launches == 0:
+            continue
+        calls = sum(int(r["Calls"]) for r in st)
+        tsum = sum(float(r["TotalDurationNs"]) for r in st)
+        tmin = min(float(r["MinNs"]) for r in st)
+        tmax = max(float(r["MaxNs"]) for r in st)
+        timed = tr[-launches:]
+        tavg = np.mean([int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in timed]) / 1e6
+        lines = [f"# {kern} on workload {wl} (one MI355X)", "",
+                 "Command (kernel trace + stats):",
+                 "  cd /tmp && export TMPDIR=/tmp && rocprofv3 --kernel-trace --stats -f csv "
+                 f"-d <out>/trace -o run -- python3 $GRAFT_REPO_ROOT/bench.py {workload_cmd}",
+                 "Summary:",
+                 f"  python3 bench.py --trace-summary <out>/trace <out>/trace.json "
+                 f"'{workload_cmd}' profiles",
+                 "Traffic (profiles/traffic.json), one counter pass each, then:",
+                 "  cd /tmp && export TMPDIR=/tmp && timeout -s KILL 150 rocprofv3 --pmc "
+                 "FETCH_SIZE -f csv -d <out>/pmc_fetch -o run -- python3 $GRAFT_REPO_ROOT/bench.py "
+                 "--steps 1 --warmup 0 --shots-per-step 8 --no-cpu --no-serve --tracks 0",
+                 "  (the same with WRITE_SIZE into <out>/pmc_write)",
+                 "  python3 bench.py --ingest-pmc <out>/pmc_fetch <out>/pmc_write "
+                 "<out>/pmc_fetch.json", "",
+                 f"rocprofv3 kernel stats ({kern}): calls {calls}, average "
+                 f"{tsum / calls / 1e6:.4f} ms, min {tmin / 1e6:.4f} ms, max {tmax / 1e6:.4f} ms, "
+                 f"{100 * tsum / tot:.1f} % of GPU time",
+                 f"rocprofv3 kernel trace, the {launches} dispatches of the timed steps only: "
+                 f"average {tavg:.4f} ms",
+                 f"bench.py hipEvent stage timer ({stage}) in the same run: {launches} launches "
+                 f"(timed steps only), average {ms / launches:.4f} ms, "
+                 f"{units / launches:.4g} units per launch"]
+        t = traffic.get(kern)
+        if t and t.get("workload") == wl:
+            lines.append(f"HBM traffic (calibrated FETCH_SIZE + WRITE_SIZE, profiles/traffic.json):"
+                         f" {t['bytes_per_unit']:.4g} B per unit")
+        roof = b.get("roofline") or {}
+        if roof.get("kernel") == kern:
+            lines.append(f"Bench line roofline: achieved {roof['achieved']:.4g} {roof['unit']} of "
+                         f"{roof['peak']} ({100 * roof['frac']:.2f} %)")
+        lines += ["", "Top kernels of the traced run:",
+                  "  kernel                                   calls  total_ms   avg_ms    pct"]
+        for r in top:
+            lines.append(f"  {r['Name'][:40]:40s} {int(r['Calls']):6d} "
+                         f"{float(r['TotalDurationNs']) / 1e6:9.3f} "
+                         f"{float(r['AverageNs']) / 1e6:8.4f} "
+                         f"{100 * float(r['TotalDurationNs']) / tot:6.2f}")
+        path = os.path.join(out_dir, f"{kern}_{wl}.txt")
+        with open(path, "w") as f:
+            f.write("\n".join(lines) + "\n")
+        print(f"wrote {path}: trace avg {tavg:.4f} ms, hipEvent avg {ms / launches:.4f} ms")
+
+
 def ingest_pmc(fetch_dir, write_dir, bench_json, out_path):
     """Turn two rocprofv3 --pmc runs of this script (FETCH_SIZE, WRITE_SIZE;
     csv output) plus that run's JSON line into profiles/traffic.json: HBM
@@ -290,11 +378,12 @@ def counters_summary(dirs, bench_json, out_path):
              f"(one MI355X)", "", "Commands (one counter pass each):"]
     regex = "|".join(KERNELS.values())
     for d in dirs:
-        lines.append(f"  cd /tmp && export TMPDIR=/tmp && timeout -s KILL 120 rocprofv3 --pmc "
+        lines.append(f"  cd /tmp && export TMPDIR=/tmp && timeout -s KILL 150 rocprofv3 --pmc "
                      f"{' '.join(sorted(names.get(d, ())))} --kernel-include-regex '{regex}' "
                      f"-f csv -d <out>/{os.path.basename(os.path.normpath(d))} -o run -- python3 "
                      f"$GRAFT_REPO_ROOT/bench.py --steps {b['steps']} --warmup {b['warmup']} "
-                     f"--shots-per-step {b['config']['shots_per_step']} --no-cpu --no-serve")
+                     f"--shots-per-step {b['config']['shots_per_step']} --no-cpu --no-serve "
+                     f"--tracks 0")
     lines.append(f"  python3 bench.py --counters-summary {' '.join('<out>/' + os.path.basename(os.path.normpath(d)) for d in dirs)} "
                  f"<out>/{os.path.basename(bench_json)} profiles/{os.path.basename(out_path)}")
     lines.append("")
@@ -575,6 +664,10 @@ def main():
     ap.add_argument("--flags", type=int, default=0)
     ap.add_argument("--ingest-pmc", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "BENCH_JSON"),
                     help="write profiles/traffic.json from two rocprofv3 --pmc runs")
+    ap.add_argument("--trace-summary", nargs=4,
+                    metavar=("TRACE_DIR", "BENCH_JSON", "BENCH_ARGS", "OUT_DIR"),
+                    help="write OUT_DIR/<kernel>_<workload>.txt from a rocprofv3 kernel-trace "
+                         "run of this script with BENCH_ARGS")
     ap.add_argument("--counters-summary", nargs="+", metavar="ARG",
                     help="PMC_DIR [PMC_DIR ...] BENCH_JSON OUT_TXT: per-dispatch counter "
                          "averages of the hot kernels from rocprofv3 --pmc runs")
@@ -595,6 +688,9 @@ def main():
     ap.add_argument("--tracks-only", action="store_true",
                     help="only the multi-track record")
     a = ap.parse_args()
+    if a.trace_summary:
+        trace_summary(*a.trace_summary)
+        return
     if a.counters_summary:
         counters_summary(a.counters_summary[:-2], a.counters_summary[-2], a.counters_summary[-1])
         return
