@@ -267,12 +267,20 @@ def cpu_baseline(p, soa, tg, shots, nshots, reps=3):
     warm = time.perf_counter() - t0
     print(f"# cpu baseline warm-up (1 shot): {warm:.1f} s", file=sys.stderr, flush=True)
     runs = []
+    # the block in parts of at most 5 shots (one line of progress each; a
+    # run's time is the sum of its parts)
+    part = 5
     for r in range(reps):
-        t0 = time.perf_counter()
-        _, nv = oracle.run(e, soa, tg, sample, precision=32, nthreads=threads)
-        runs.append(time.perf_counter() - t0)
-        print(f"# cpu baseline run {r + 1}/{reps} ({nshots} shots): {runs[-1]:.1f} s",
-              file=sys.stderr, flush=True)
+        dt_run, nvs = 0.0, []
+        for k in range(0, len(sample), part):
+            t0 = time.perf_counter()
+            _, nv = oracle.run(e, soa, tg, sample[k:k + part], precision=32, nthreads=threads)
+            dt_run += time.perf_counter() - t0
+            nvs.append(nv)
+            print(f"# cpu baseline run {r + 1}/{reps}: shots {k}..{k + len(nv) - 1} done, "
+                  f"{dt_run:.1f} s", file=sys.stderr, flush=True)
+        runs.append(dt_run)
+    nv = np.concatenate(nvs)
     dt = float(np.median(runs))
     pairs = float(nv.sum())
     return {"value": pairs / dt, "unit": "facet·shot pairs/s", "cores": threads,
