@@ -696,6 +696,13 @@ __device__ __forceinline__ double det_unfix(unsigned long long u, double inv_sca
     return (double)(long long)u * inv_scale;
 }
 
+// Scale of a shot's fixed-point type-0 row: V facets times n0 type-0
+// targets terms, each at most the largest component of any pair (bits m),
+// with 1e-4 of margin
+__device__ __forceinline__ double t0_det_scale(int V, int n0, unsigned m) {
+    return det_scale((double)V * (double)max(n0, 1), 1.0001 * (double)__uint_as_float(m));
+}
+
 // Record slot of the illuminated facet of rank i (facet order) among the V
 // of its shot.  The ranks are laid out as 64 rows of equal i mod 64, so
 // consecutive slots hold facets 64 ranks apart.  Neighbouring facets have
@@ -1055,6 +1062,26 @@ __global__ void __launch_bounds__(NT) k_surface(
     }
 }
 
+// deterministic type-0 rows: the fixed-point (int64) pairs of each shot's
+// row (the shot's scale, t0_det_scale) -> FP32 rows of P, zero from nr on
+__global__ void k_fix_rows(float2* __restrict__ dst, const unsigned long long* __restrict__ src,
+                           int P, int nr, int rows, const int* __restrict__ Vs,
+                           const int* __restrict__ tcnt, const unsigned* __restrict__ tmax) {
+    const size_t n = (size_t)P * rows;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / P), k = (int)(i - (size_t)r * P);
+        float2 v = make_float2(0.f, 0.f);
+        if (k < nr) {
+            const double sc = t0_det_scale(Vs[r], tcnt[2 * r + 1], tmax[r]);
+            const double inv = sc > 0.0 ? 1.0 / sc : 0.0;
+            const unsigned long long* q = src + (size_t)r * P;
+            v = make_float2((float)det_unfix(q[2 * k], inv), (float)det_unfix(q[2 * k + 1], inv));
+        }
+        dst[i] = v;
+    }
+}
+
 // surface rows (FP64 or fixed point, stride nr) -> FP32 rows of the FFT
 // buffer (stride P); bins nr..P-1 are left as they are (zero)
 __global__ void k_surf_rows(float2* __restrict__ psurf, int P, const double2* __restrict__ srow,
@@ -1327,13 +1354,23 @@ __device__ __forceinline__ float rerad_f(int fn, float th) {
 // MODE 1: type != 0, specular: inward (A) and outward (B) traces separately.
 // MODE 2: type == 0 (genRefrPhasor): single bin, summed into one per-shot
 //         trace (out row = per shot, global atomics after the LDS window).
-template <int MODE>
+// DET (the deterministic mode; MODE 1 and 2): every term is rounded once to
+// 64-bit fixed point and summed with integer atomics, so no sum depends on
+// the order of arrival.  MODE 1: a first sweep over the facets takes the
+// pair's largest term component (the scale), then each window [wb, wb +
+// WIN) of [base, top) is summed from a sweep of its own; a term outside
+// [base, top) breaks a bound and is counted in *oow.  MODE 2 (all type-0
+// targets of a shot add to the shot's row): pass 1 only takes the largest
+// component of each pair into tmax[shot]; pass 2 adds the terms, scaled by
+// the shot's bound, into the shot's row as fixed-point (int64) pairs.
+template <int MODE, bool DET = false>
 __global__ void __launch_bounds__(NTT) k_target_accum(
     const float* __restrict__ tg, int64_t Tpad, const int* __restrict__ pshot,
     const int* __restrict__ ptgt, const erp_shot* __restrict__ shots,
     const int* __restrict__ Vs, const int64_t* __restrict__ rec_base, Rec rec,
     const Bounds* __restrict__ bounds, const erp_prism* __restrict__ prisms, int np, KPar kp,
-    float2* __restrict__ outA, float2* __restrict__ outB, int* __restrict__ pbase) {
+    float2* __restrict__ outA, float2* __restrict__ outB, int* __restrict__ pbase,
+    unsigned* __restrict__ tmax, const int* __restrict__ tcnt, int pass, int* __restrict__ oow) {
     extern __shared__ double2 lds[];
     constexpr int WIN = (MODE == 1) ? TWIN / 2 : TWIN;  // MODE 1 holds two windows
     double2* winA = lds;
@@ -1364,6 +1401,16 @@ __global__ void __launch_bounds__(NTT) k_target_accum(
     base = max(base, 0);
     // upper bound: Itd_max + n * (distance to the farthest AABB corner)
     bool fits = false;
+    int top = nr;
+    if (DET && MODE == 1) {
+        const float fx = fmaxf(fabsf(o2f(bd.xmin) - txf), fabsf(o2f(bd.xmax) - txf));
+        const float fy = fmaxf(fabsf(o2f(bd.ymin) - tyf), fabsf(o2f(bd.ymax) - tyf));
+        const float fz = fmaxf(fabsf(o2f(bd.zmin) - tzf), fabsf(o2f(bd.zmax) - tzf));
+        const double tmax_d = sqrt((double)fx * fx + (double)fy * fy + (double)fz * fz);
+        const double ithmax = __longlong_as_double((long long)bd.itd_max);
+        const double rhi = ithmax + kp.n_d * tmax_d * (1.0 + 1e-6) + 1.0;
+        top = min((int)ceil((rhi - rst) / kp.drd) + 2, nr);
+    }
     if (MODE == 0 && pbase) {
         const float fx = fmaxf(fabsf(o2f(bd.xmin) - txf), fabsf(o2f(bd.xmax) - txf));
         const float fy = fmaxf(fabsf(o2f(bd.ymin) - tyf), fabsf(o2f(bd.ymax) - tyf));
@@ -1378,25 +1425,11 @@ __global__ void __launch_bounds__(NTT) k_target_accum(
 
     float2* rowA = (MODE == 2) ? outA + (size_t)s * P : outA + (size_t)pi * P;
     float2* rowB = (MODE == 1) ? outB + (size_t)pi * P : nullptr;
-    // zero the output rows outside the window (MODE 2 rows are pre-zeroed);
-    // unnecessary when every contribution provably lands inside the window
-    if (MODE != 2 && !fits) {
-        for (int k = threadIdx.x; k < P; k += NTT) {
-            if (k < base || k >= base + WIN || k >= nr) {
-                rowA[k] = make_float2(0.f, 0.f);
-                if (MODE == 1) rowB[k] = make_float2(0.f, 0.f);
-            }
-        }
-    }
-    for (int k = threadIdx.x; k < WIN; k += NTT) {
-        winA[k] = make_double2(0.0, 0.0);
-        if (MODE == 1) winB[k] = make_double2(0.0, 0.0);
-    }
-    // only the row zeros above must land before other threads' atomics
-    if (MODE != 2 && !fits) __threadfence();
-    __syncthreads();
 
-    for (int k = threadIdx.x; k < V; k += NTT) {
+    // one facet x target term of facet k: add(b, w, c, d) with the bin, the
+    // split weight, the inward term and (MODE 1) the outward term, unless
+    // its range falls outside the bins
+    auto eval = [&](int k, auto&& add) {
         const int64_t r = r0 + k;
         const float fx = rec.geo[r], fy = rec.geo[rec.cap + r], fz = rec.geo[2 * rec.cap + r];
         const float nx = rec.geo[3 * rec.cap + r], ny = rec.geo[4 * rec.cap + r],
@@ -1419,7 +1452,7 @@ __global__ void __launch_bounds__(NTT) k_target_accum(
         }
         int b;
         float w;
-        if (!bin_of(rngt, rst, kp.drd, nr, &b, &w)) continue;
+        if (!bin_of(rngt, rst, kp.drd, nr, &b, &w)) return;
         const float Ttdf = (float)Ttd;
         const double invd = 1.0 / Ttd;
         const double ix = dx * invd, iy = dy * invd, iz = dz * invd;
@@ -1480,9 +1513,8 @@ __global__ void __launch_bounds__(NTT) k_target_accum(
             const float rc = rerad_f(kp.rerad, TTh) * EI * EO * req;
             const float2 ph = phasor_cycles(-2.0 * rngt * kp.inv_cyc);
             const float2 c = make_float2(ph.x * rc, ph.y * rc);
-            const int k0 = b - base;
-            if (k0 >= 0 && k0 < WIN) lds_add2(winA, k0, c); else glb_add2(rowA, b, c);
-            continue;
+            add(b, 0.f, c, make_float2(0.f, 0.f));
+            return;
         }
         // accumulateTarget / radiateTarget amplitude:
         // sqrt(Pray n) = F_f * sqrt(G(Tth,Tph) G(TargetTh,Tph)) / Ttd * sqrt(EI-part)
@@ -1522,37 +1554,168 @@ __global__ void __launch_bounds__(NTT) k_target_accum(
         }
         const float2 ph = phasor_cycles(rngt * kp.inv_cyc);
         const float2 c = make_float2(ph.x * amp, ph.y * amp);
-        const int k0 = b - base;
-        const float2 c0 = make_float2(c.x * (1.f - w), c.y * (1.f - w));
-        const float2 c1 = make_float2(c.x * w, c.y * w);
-        if (k0 >= 0 && k0 < WIN) lds_add2(winA, k0, c0); else glb_add2(rowA, b, c0);
-        if (b + 1 < nr) {
-            if (k0 + 1 >= 0 && k0 + 1 < WIN) lds_add2(winA, k0 + 1, c1);
-            else glb_add2(rowA, b + 1, c1);
-        }
+        float2 d = make_float2(0.f, 0.f);
         if (MODE == 1) {
             // radiateTarget: phasor(rngt - lam/2), P = 1 W
             const float2 pb = phasor_cycles(rngt * kp.inv_cyc - 0.5 * (double)kp.lam * kp.inv_cyc);
-            const float2 d = make_float2(pb.x * ampB, pb.y * ampB);
-            const float2 d0 = make_float2(d.x * (1.f - w), d.y * (1.f - w));
-            const float2 d1 = make_float2(d.x * w, d.y * w);
-            if (k0 >= 0 && k0 < WIN) lds_add2(winB, k0, d0); else glb_add2(rowB, b, d0);
-            if (b + 1 < nr) {
-                if (k0 + 1 >= 0 && k0 + 1 < WIN) lds_add2(winB, k0 + 1, d1);
-                else glb_add2(rowB, b + 1, d1);
+            d = make_float2(pb.x * ampB, pb.y * ampB);
+        }
+        add(b, w, c, d);
+    };
+
+    if constexpr (!DET) {
+        // zero the output rows outside the window (MODE 2 rows are pre-zeroed);
+        // unnecessary when every contribution provably lands inside the window
+        if (MODE != 2 && !fits) {
+            for (int k = threadIdx.x; k < P; k += NTT) {
+                if (k < base || k >= base + WIN || k >= nr) {
+                    rowA[k] = make_float2(0.f, 0.f);
+                    if (MODE == 1) rowB[k] = make_float2(0.f, 0.f);
+                }
             }
         }
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < WIN; k += NTT) {
-        const int b = base + k;
-        if (b >= nr) break;
-        if (MODE == 2) {
-            const double2 v = winA[k];
-            if (v.x != 0.0 || v.y != 0.0) glb_add2(rowA, b, make_float2((float)v.x, (float)v.y));
+        for (int k = threadIdx.x; k < WIN; k += NTT) {
+            winA[k] = make_double2(0.0, 0.0);
+            if (MODE == 1) winB[k] = make_double2(0.0, 0.0);
+        }
+        // only the row zeros above must land before other threads' atomics
+        if (MODE != 2 && !fits) __threadfence();
+        __syncthreads();
+
+        for (int k = threadIdx.x; k < V; k += NTT) {
+            eval(k, [&](int b, float w, float2 c, float2 d) {
+                const int k0 = b - base;
+                if (MODE == 2) {
+                    if (k0 >= 0 && k0 < WIN) lds_add2(winA, k0, c); else glb_add2(rowA, b, c);
+                    return;
+                }
+                const float2 c0 = make_float2(c.x * (1.f - w), c.y * (1.f - w));
+                const float2 c1 = make_float2(c.x * w, c.y * w);
+                if (k0 >= 0 && k0 < WIN) lds_add2(winA, k0, c0); else glb_add2(rowA, b, c0);
+                if (b + 1 < nr) {
+                    if (k0 + 1 >= 0 && k0 + 1 < WIN) lds_add2(winA, k0 + 1, c1);
+                    else glb_add2(rowA, b + 1, c1);
+                }
+                if (MODE == 1) {
+                    const float2 d0 = make_float2(d.x * (1.f - w), d.y * (1.f - w));
+                    const float2 d1 = make_float2(d.x * w, d.y * w);
+                    if (k0 >= 0 && k0 < WIN) lds_add2(winB, k0, d0); else glb_add2(rowB, b, d0);
+                    if (b + 1 < nr) {
+                        if (k0 + 1 >= 0 && k0 + 1 < WIN) lds_add2(winB, k0 + 1, d1);
+                        else glb_add2(rowB, b + 1, d1);
+                    }
+                }
+            });
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < WIN; k += NTT) {
+            const int b = base + k;
+            if (b >= nr) break;
+            if (MODE == 2) {
+                const double2 v = winA[k];
+                if (v.x != 0.0 || v.y != 0.0) glb_add2(rowA, b, make_float2((float)v.x, (float)v.y));
+            } else {
+                rowA[b] = make_float2((float)winA[k].x, (float)winA[k].y);
+                if (MODE == 1) rowB[b] = make_float2((float)winB[k].x, (float)winB[k].y);
+            }
+        }
+    } else {
+        static_assert(MODE != 0, "the deterministic mode of MODE 0 is k_target_accum_fast");
+        // the largest component of this pair's terms (not needed by the
+        // second pass of MODE 2, which reads its shot's bound)
+        __shared__ float red[NTT / 64];
+        float m = 0.f;
+        if (MODE == 1 || pass == 1) {
+            for (int k = threadIdx.x; k < V; k += NTT)
+                eval(k, [&](int, float, float2 c, float2 d) {
+                    m = fmaxf(m, fmaxf(fmaxf(fabsf(c.x), fabsf(c.y)),
+                                       fmaxf(fabsf(d.x), fabsf(d.y))));
+                });
+            for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+            __syncthreads();
+            m = red[0];
+            for (int wv = 1; wv < NTT / 64; ++wv) m = fmaxf(m, red[wv]);
+        }
+        if constexpr (MODE == 2) {
+            if (pass == 1) {
+                if (threadIdx.x == 0 && m > 0.f) atomicMax(&tmax[s], __float_as_uint(m));
+                return;
+            }
+            // pass 2: the shot's scale (V facets x its type-0 targets terms)
+            const double scale = t0_det_scale(V, tcnt[2 * s + 1], tmax[s]);
+            unsigned long long* wx = reinterpret_cast<unsigned long long*>(lds);
+            unsigned long long* wy = wx + WIN;
+            unsigned long long* ru = reinterpret_cast<unsigned long long*>(rowA);
+            for (int k = threadIdx.x; k < WIN; k += NTT) {
+                wx[k] = 0ull;
+                wy[k] = 0ull;
+            }
+            __syncthreads();
+            for (int k = threadIdx.x; k < V; k += NTT)
+                eval(k, [&](int b, float, float2 c, float2) {
+                    const int k0 = b - base;
+                    if (k0 >= 0 && k0 < WIN) lds_add2_fix(wx, wy, k0, c, scale);
+                    else glb_add2_fix(ru, b, c, scale);
+                });
+            __syncthreads();
+            for (int k = threadIdx.x; k < WIN; k += NTT) {
+                const int b = base + k;
+                if (b >= nr) break;
+                const unsigned long long x = wx[k], y = wy[k];
+                if (x | y) {
+                    atomicAdd(&ru[2 * b], x);
+                    atomicAdd(&ru[2 * b + 1], y);
+                }
+            }
         } else {
-            rowA[b] = make_float2((float)winA[k].x, (float)winA[k].y);
-            if (MODE == 1) rowB[b] = make_float2((float)winB[k].x, (float)winB[k].y);
+            // MODE 1: rows outside [base, top) stay zero (no term reaches them)
+            const int hi = max(top, base);
+            for (int k = threadIdx.x; k < P; k += NTT) {
+                if (k < base || k >= hi || k >= nr) {
+                    rowA[k] = make_float2(0.f, 0.f);
+                    rowB[k] = make_float2(0.f, 0.f);
+                }
+            }
+            // 2 V terms per row, each at most m (1e-4 margin for the split)
+            const double scale = det_scale(2.0 * (double)V, 1.0001 * (double)m);
+            const double inv = scale > 0.0 ? 1.0 / scale : 0.0;
+            unsigned long long* wax = reinterpret_cast<unsigned long long*>(lds);
+            unsigned long long* way = wax + WIN;
+            unsigned long long* wbx = way + WIN;
+            unsigned long long* wby = wbx + WIN;
+            for (int wb = base; wb < top || wb == base; wb += WIN) {
+                const int wl = min(WIN, max(0, top - wb));
+                for (int k = threadIdx.x; k < wl; k += NTT) {
+                    wax[k] = 0ull; way[k] = 0ull; wbx[k] = 0ull; wby[k] = 0ull;
+                }
+                __syncthreads();
+                const bool first = wb == base, last = wb + WIN >= top;
+                for (int k = threadIdx.x; k < V; k += NTT)
+                    eval(k, [&](int b, float w, float2 c, float2 d) {
+                        auto put = [&](int bb, float2 v, float2 u) {
+                            const int kk = bb - wb;
+                            if (kk >= 0 && kk < wl) {
+                                lds_add2_fix(wax, way, kk, v, scale);
+                                lds_add2_fix(wbx, wby, kk, u, scale);
+                            } else if ((bb < base && first) || (bb >= top && last)) {
+                                atomicAdd(oow, 1);
+                            }
+                        };
+                        put(b, make_float2(c.x * (1.f - w), c.y * (1.f - w)),
+                            make_float2(d.x * (1.f - w), d.y * (1.f - w)));
+                        if (b + 1 < nr)
+                            put(b + 1, make_float2(c.x * w, c.y * w), make_float2(d.x * w, d.y * w));
+                    });
+                __syncthreads();
+                for (int k = threadIdx.x; k < wl; k += NTT) {
+                    const int b = wb + k;
+                    if (b >= nr) break;
+                    rowA[b] = make_float2((float)det_unfix(wax[k], inv), (float)det_unfix(way[k], inv));
+                    rowB[b] = make_float2((float)det_unfix(wbx[k], inv), (float)det_unfix(wby[k], inv));
+                }
+                __syncthreads();
+            }
         }
     }
 }
@@ -2766,9 +2929,6 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                            ERP_RUN_PHASE_DF | ERP_RUN_ACC_F32 | ERP_RUN_ACC_COMP;
     if (flags & ~known) throw ErpError{ERP_E_INVALID, "unknown run flags"};
     const bool det = flags & ERP_RUN_DETERMINISTIC;
-    if (det && spec)
-        throw ErpError{ERP_E_UNSUPPORTED,
-                       "ERP_RUN_DETERMINISTIC covers the non-specular (convolution) method"};
     // precision-study arithmetic of the target accumulation
     const int ph = (flags & ERP_RUN_PHASE_F32) ? PH_F32 : (flags & ERP_RUN_PHASE_DF) ? PH_DF : PH_F64;
     const int acc = (flags & ERP_RUN_ACC_F32) ? ACC_F32 : (flags & ERP_RUN_ACC_COMP) ? ACC_COMP
@@ -2995,7 +3155,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
 
     // terms of the plain target kernel that fall outside their pair's bin
     // window where the window holds every bin the pair's bounds allow
-    const bool check_win = any_typed && !spec;
+    const bool check_win = any_typed && (!spec || det);
     if (check_win) {
         W.oow.ensure(256);
         ERP_HIP_TRY(hipMemsetAsync(W.oow.p, 0, sizeof(int), st));
@@ -3245,11 +3405,12 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                                            (const int*)nullptr, W.oow.as<int>());
                         ERP_CHECK_LAUNCH("k_target_accum_fast");
                     } else {
-                        hipLaunchKernelGGL(k_target_accum<1>, dim3(n), dim3(NTT),
-                                           sizeof(double2) * TWIN, st,
+                        auto kern1 = det ? k_target_accum<1, true> : k_target_accum<1, false>;
+                        hipLaunchKernelGGL(kern1, dim3(n), dim3(NTT), sizeof(double2) * TWIN, st,
                                            targets->xyzn.as<float>(), Tpad, dps, dpt, dshots, dV,
                                            dbase, rec, dbounds, dpr, (int)nprisms, kp, A, Bs,
-                                           nullptr);
+                                           nullptr, (unsigned*)nullptr, (const int*)nullptr, 0,
+                                           det ? W.oow.as<int>() : nullptr);
                         ERP_CHECK_LAUNCH("k_target_accum<1>");
                     }
                     PT_.end(st);
@@ -3356,11 +3517,36 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                     up.insert(up.end(), pt.begin(), pt.end());
                     ERP_HIP_TRY(hipMemcpyAsync(dps, up.data(), sizeof(int) * up.size(),
                                                hipMemcpyHostToDevice, st));
-                    hipLaunchKernelGGL(k_target_accum<2>, dim3(n), dim3(NTT), sizeof(double2) * TWIN,
-                                       st, targets->xyzn.as<float>(), Tpad, dps, dps + n, dshots,
-                                       dV, dbase, rec, dbounds, dpr, (int)nprisms, kp, t0, nullptr,
-                                       nullptr);
-                    ERP_CHECK_LAUNCH("k_target_accum<2>");
+                    if (!det) {
+                        hipLaunchKernelGGL(k_target_accum<2>, dim3(n), dim3(NTT),
+                                           sizeof(double2) * TWIN, st, targets->xyzn.as<float>(),
+                                           Tpad, dps, dps + n, dshots, dV, dbase, rec, dbounds,
+                                           dpr, (int)nprisms, kp, t0, nullptr, nullptr,
+                                           (unsigned*)nullptr, (const int*)nullptr, 0,
+                                           (int*)nullptr);
+                        ERP_CHECK_LAUNCH("k_target_accum<2>");
+                    } else {
+                        // per-shot bounds (the surface's maxima are consumed), then
+                        // the fixed-point rows, then FP32 rows into the free
+                        // surface buffer
+                        zero(reinterpret_cast<float2*>(smax), ((size_t)S + 1) / 2, st);
+                        auto kern2 = k_target_accum<2, true>;
+                        for (int pass = 1; pass <= 2; ++pass) {
+                            hipLaunchKernelGGL(kern2, dim3(n), dim3(NTT),
+                                               sizeof(double2) * TWIN, st,
+                                               targets->xyzn.as<float>(), Tpad, dps, dps + n,
+                                               dshots, dV, dbase, rec, dbounds, dpr, (int)nprisms,
+                                               kp, t0, nullptr, nullptr, smax,
+                                               (const int*)W.tcnt.as<int>(), pass, (int*)nullptr);
+                            ERP_CHECK_LAUNCH("k_target_accum<2>");
+                        }
+                        hipLaunchKernelGGL(k_fix_rows, dim3(grid_for((size_t)P * S)), dim3(256), 0,
+                                           st, W.psurf.as<float2>(),
+                                           reinterpret_cast<const unsigned long long*>(t0), P, nr,
+                                           S, dV, (const int*)W.tcnt.as<int>(), smax);
+                        ERP_CHECK_LAUNCH("k_fix_rows");
+                        t0 = W.psurf.as<float2>();
+                    }
                     if (lin) {
                         window_conv(ctx, planP, t0, S, g_targ, refr0, nr, st);
                     } else {

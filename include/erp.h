@@ -61,10 +61,11 @@ extern "C" {
 #define ERP_RUN_DETERMINISTIC 0x01u /* bitwise repeatable bins: each term rounded
                                        once to 64-bit fixed point (scaled from a
                                        per-shot / per-target bound) and summed
-                                       with integer atomics; non-specular method
-                                       only.  Replaces the order-dependent float
-                                       atomicAdd binning of radar_kernels.cu:
-                                       754-758 and 821-825                      */
+                                       with integer atomics; every method, and
+                                       type-0 targets summed per shot over a
+                                       bound pass.  Replaces the order-dependent
+                                       float atomicAdd binning of
+                                       radar_kernels.cu:754-758 and 821-825     */
 /* Precision study of the surface echo and the subsurface-target
  * accumulation (non-specular method, not with ERP_RUN_DETERMINISTIC).
  * PHASE_*: arithmetic of Itd, Ttd, rngt = Itd + n Ttd, the bin coordinate

@@ -84,7 +84,7 @@ def test_argument_errors_leave_the_context_usable(gpu_ctx):
     assert ex.value.code == ERP_E_INVALID
     spec = _par(specular=True).to_erp()
     with pytest.raises(ErpError) as ex:
-        gpu_ctx.run(spec, m, None, shots, flags=ERP_RUN_DETERMINISTIC)
+        gpu_ctx.run(spec, m, None, shots, flags=ERP_RUN_PHASE_DF)
     assert ex.value.code == ERP_E_UNSUPPORTED
     other = Context(0)
     try:

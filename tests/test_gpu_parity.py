@@ -203,6 +203,45 @@ def test_specular_noconv(gpu_ctx):
     compare_traces(g, o)
 
 
+@pytest.mark.parametrize("case", ["specular_linear", "specular_circular", "specular_noconv",
+                                  "plain_type0"])
+def test_deterministic_specular_and_type0(gpu_ctx, case):
+    """ERP_RUN_DETERMINISTIC beyond the plain method's typed targets: the
+    specular method (typed targets: per-pair fixed-point windows; type-0
+    targets: per-shot fixed-point rows scaled by a bound pass; the direct
+    sums of the no-convolution method) and type-0 targets under the plain
+    method.  Same tolerance against the FP64 oracle as the default mode, and
+    two runs give the same bits."""
+    print(f"deterministic {case}", flush=True)
+    if case == "specular_linear":
+        par, soa, shots = scene(rough=0.3, nr=1024, h=3000.0, rst=2600.0, specular=True)
+        kinds = (1, 0)
+    elif case == "specular_circular":
+        par, soa, shots = scene(rough=0.3, nr=512, h=3000.0, rst=2600.0, specular=True,
+                                convolution_linear=False)
+        kinds = (0,)
+    elif case == "specular_noconv":
+        par, soa, shots = scene(nx=24, ny=20, rough=0.3, nr=256, h=3000.0, rst=2900.0,
+                                specular=True, convolution=False)
+        kinds = (0,)
+    else:
+        par, soa, shots = scene(rough=0.3, nr=1024, h=3000.0, rst=2600.0)
+        kinds = (1, 0)
+    parts = [_targets(par, n=3, ttype=k) for k in kinds]
+    tg = (np.concatenate([p[0] for p in parts], 1), np.concatenate([p[1] for p in parts]))
+    g, gv, o, ov = run_both(gpu_ctx, par, soa, shots, targets=tg, flags=ERP_RUN_DETERMINISTIC)
+    np.testing.assert_array_equal(gv, ov)
+    print(f"  worst errors {compare_traces(g, o)}", flush=True)
+    m = gpu_ctx.upload_mesh(soa)
+    t = gpu_ctx.upload_targets(*tg)
+    try:
+        g2, _ = gpu_ctx.run(par.to_erp(), m, t, shots, flags=ERP_RUN_DETERMINISTIC)
+    finally:
+        m.release()
+        t.release()
+    assert np.array_equal(g.view(np.uint32), g2.view(np.uint32))
+
+
 def test_dark_shots_are_zero(gpu_ctx):
     par, soa, shots = scene(rough=0.3)
     shots["snz"][1] = -1.0  # antenna pointing up: nothing illuminated
