@@ -4,6 +4,7 @@ device context stands in for the library so these run without a GPU."""
 import time
 
 import numpy as np
+import pytest
 
 from europaraypaths_amd import serve, workloads
 from europaraypaths_amd.params import make_shots, parse_parameters
@@ -191,9 +192,18 @@ def test_job_stream_is_seeded_and_reuses_meshes():
         assert 1 <= j.ns <= 20
         assert 1e3 * 0.8 <= j.scene.F <= 4e3 * 1.2
         assert 25e3 <= float(j.shots["sz"][0]) <= 50e3
-        nl = len(np.unique(j.scene.targets[0][2]))
+        depths = np.unique(-j.scene.targets[0][2])
+        nl = len(depths)
         assert 1 <= nl <= 3
         assert 3.0 <= j.par.eps_2 <= 3.4
+        if nl == 1:
+            assert j.scene.eps_3 == 0.0 and float(j.par.eps_3) == 0.0
+        else:
+            # the layers below the first lie in their own medium, below an
+            # interface halfway between the first two layers
+            assert 3.0 <= j.scene.eps_3 <= 3.4
+            assert float(j.par.eps_3) == pytest.approx(j.scene.eps_3, rel=1e-6)
+            assert float(j.par.z_int) == pytest.approx(-0.5 * (depths[0] + depths[1]), rel=1e-5)
 
 
 def test_all_at_once_stream():
@@ -208,6 +218,6 @@ def test_job_rx_window_holds_surface_and_deepest_layer():
         h = float(j.shots["sz"][0])
         rst = float(j.shots["rst"][0])
         depth = -float(j.scene.targets[0][2].min())
-        n2 = np.sqrt(j.par.eps_2)
-        far = h + n2 * depth
+        n = np.sqrt(max(j.par.eps_2, j.par.eps_3))
+        far = h + n * depth
         assert rst < h and far < rst + j.par.nr * float(j.par.dr)
