@@ -84,3 +84,28 @@ def test_submit_reports_job_errors_and_keeps_serving(gpu_ctx):
     finally:
         m.release()
         t.release()
+
+
+def test_stream_at_full_size_with_poisson_arrivals(gpu_ctx):
+    """The serving workload at its stated sizes: the first four jobs of the
+    job stream (SeedSequence(4): F log-uniform in [1e4, 4e6], U{100..2000}
+    shots at full count, 2048-16384 bins, 1-3 layers in one or two media)
+    arriving as a Poisson stream at 20 jobs/s, in the deterministic mode.
+    Every served radargram equals its job run alone bit for bit, and each
+    latency runs from the job's arrival to its result."""
+    jobs = serve.with_arrivals(serve.mixed_job_stream(4, None, seed=4), 20.0, seed=4)
+    for j in jobs:
+        j.flags = ERP_RUN_DETERMINISTIC
+    srv = serve.Server(gpu_ctx, budget_bytes=250e9, slots=4)
+    rep = srv.run(jobs)
+    srv.clear_cache()
+    print(f"served 4 full-size jobs: {rep.summary()}", flush=True)
+    for j in jobs:
+        r = rep.results[j.job_id]
+        assert r.error is None, r.error
+        assert r.finished >= r.admitted >= r.arrival - 1e-9
+        out, nv = serve.run_plain(gpu_ctx, j)
+        print(f"job {j.job_id}: F={j.scene.F} ns={j.ns} nr={j.nr} pairs={int(nv.sum())}",
+              flush=True)
+        np.testing.assert_array_equal(r.nvalid, nv)
+        assert np.array_equal(r.out.view(np.uint32), out.view(np.uint32))
