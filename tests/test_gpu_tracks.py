@@ -162,3 +162,29 @@ def test_tracks_rejects_bad_arguments(gpu_ctx):
         assert gpu_ctx.run_tracks([]) == []
     finally:
         m.release()
+
+
+def test_tracks_at_full_size_deterministic(gpu_ctx):
+    """The batching workload at its stated sizes: the first three tracks of
+    the job stream (SeedSequence(4), full shot counts, F 2.2e6-3.6e6, two
+    range lengths, one and two media) in one call, deterministic: each
+    radargram equals the track run alone bit for bit."""
+    from europaraypaths_amd import serve
+    jobs = serve.mixed_job_stream(3, None, seed=4, reuse=0.0)
+    handles, runs = [], []
+    try:
+        for j in jobs:
+            m = gpu_ctx.upload_mesh(j.scene.soa)
+            t = gpu_ctx.upload_targets(*j.scene.targets)
+            handles += [m, t]
+            runs.append((j.par.to_erp(), m, t, j.shots))
+        got = gpu_ctx.run_tracks(runs, flags=ERP_RUN_DETERMINISTIC)
+        for j, (e, m, t, sh), (g, gv) in zip(jobs, runs, got):
+            a, av = gpu_ctx.run(e, m, t, sh, flags=ERP_RUN_DETERMINISTIC)
+            print(f"track F={j.scene.F} ns={j.ns} nr={j.nr} eps3={j.scene.eps_3:.3f} "
+                  f"pairs={int(av.sum())}", flush=True)
+            np.testing.assert_array_equal(gv, av)
+            assert np.array_equal(g, a)
+    finally:
+        for h in handles:
+            h.release()
