@@ -166,7 +166,7 @@ def test_tracks_rejects_bad_arguments(gpu_ctx):
 
 def test_tracks_at_full_size_deterministic(gpu_ctx):
     """The batching workload at its stated sizes: the first three tracks of
-    the job stream (SeedSequence(4), full shot counts, F 2.2e6-3.6e6, two
+    the job stream (SeedSequence(4), full shot counts, F 7.3e5-3.6e6, two
     range lengths, one and two media) in one call, deterministic: each
     radargram equals the track run alone bit for bit."""
     from europaraypaths_amd import serve
