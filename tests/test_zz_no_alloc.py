@@ -111,6 +111,26 @@ def test_noalloc_matches_run_dev_two_calls(gpu_ctx):
         assert np.array_equal(g.view(np.uint32), r.view(np.uint32))
 
 
+def test_noalloc_cropped_chain_and_deterministic_specular(gpu_ctx):
+    """The cropped per-pair chain (nr = 3000, whose range kernel spectrum the
+    run builds in the scratch array) and a deterministic specular run with
+    typed and type-0 targets (the fixed-point pair windows and the per-shot
+    type-0 rows): the bits of erp_run_dev."""
+    par, soa, tg, shots = _scene(64, 10.0, 3000, 6, ntg=4)
+    assert par.nr == 3000
+    print("noalloc cropped chain", flush=True)
+    (g, gv), (r, rv) = _both(gpu_ctx, par, soa, tg, shots, flags=ERP_RUN_DETERMINISTIC)
+    np.testing.assert_array_equal(gv, rv)
+    assert np.array_equal(g.view(np.uint32), r.view(np.uint32))
+    par, soa, (xyzn, ttype), shots = _scene(64, 10.0, 1024, 6, ntg=6, specular=True)
+    ttype[::2] = 0
+    print("noalloc deterministic specular", flush=True)
+    (g, gv), (r, rv) = _both(gpu_ctx, par, soa, (xyzn, ttype), shots,
+                             flags=ERP_RUN_DETERMINISTIC)
+    np.testing.assert_array_equal(gv, rv)
+    assert np.array_equal(g.view(np.uint32), r.view(np.uint32))
+
+
 def test_noalloc_shrinks_batches_to_fit(gpu_ctx):
     """nr = 16384 (256 KB pair rows: the launch takes fewer pairs) and a
     mesh of 331 776 facets lit by every shot (their records, 20 MB per shot,
