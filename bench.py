@@ -55,7 +55,7 @@ def target_pair_flops(P: int) -> float:
     return 3 * 5 * P * np.log2(P) + 14 * P
 
 
-KERNELS = {"target_accum": "k_target_accum_fast", "target_fft": "k_tpair_half",
+KERNELS = {"target_accum": "k_target_accum_pair", "target_fft": "k_tpair_half",
            "surface": "k_surface"}
 
 
@@ -670,6 +670,9 @@ def main():
                          "warm-up); writes profiles/cpu_baseline_<workload>.json")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--flags", type=int, default=0)
+    ap.add_argument("--single-shot", action="store_true",
+                    help="the single-shot target kernel (ERP_TUNE_SINGLE_SHOT) instead of "
+                         "the shot-pair kernel")
     ap.add_argument("--ingest-pmc", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "BENCH_JSON"),
                     help="write profiles/traffic.json from two rocprofv3 --pmc runs")
     ap.add_argument("--trace-summary", nargs=4,
@@ -739,6 +742,10 @@ def main():
           file=sys.stderr, flush=True)
 
     ctx = Context(0)
+    if a.single_shot:
+        from europaraypaths_amd._abi import ERP_TUNE_SINGLE_SHOT
+        ctx.set_tuning(ERP_TUNE_SINGLE_SHOT, 1)
+        KERNELS["target_accum"] = "k_target_accum_fast"
     mesh = ctx.upload_mesh(soa)
     targets = ctx.upload_targets(*tg) if tg is not None else None
     h = hip()
@@ -821,7 +828,7 @@ def main():
         "config": {"workload": a.workload, "facets": int(soa.shape[1]),
                    "targets": 0 if tg is None else int(tg[0].shape[1]),
                    "track_shots": len(shots), "shots_per_step": B, "nr": nr,
-                   "flags": a.flags},
+                   "flags": a.flags, "target_kernel": KERNELS["target_accum"]},
         "targets_per_pair": (triples / pairs) if pairs else 0.0,
         "triples_per_s": triples / dt,
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)),

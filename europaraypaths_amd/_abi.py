@@ -30,6 +30,7 @@ ERP_TUNE_PAIRS_PER_LAUNCH = 2
 ERP_TUNE_TRACK_ROWS = 3
 ERP_TUNE_DIRECT_LAUNCH = 4
 ERP_TUNE_WINDOW_SHIFT = 5
+ERP_TUNE_SINGLE_SHOT = 6
 
 ERP_TERRAIN_GAUSSIAN = 0
 ERP_TERRAIN_FBM = 1

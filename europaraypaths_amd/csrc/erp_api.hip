@@ -129,7 +129,8 @@ namespace {
 void release_work(WorkBufs& w) {
     DevBuf* bufs[] = {&w.cnt, &w.off, &w.V, &w.Vd, &w.tcnt, &w.tlist, &w.tcount, &w.rec,
                       &w.bounds, &w.psurf, &w.srow, &w.refl, &w.shotS, &w.blu, &w.tA, &w.tB, &w.tC,
-                      &w.pairs, &w.t0trace, &w.shots, &w.prisms, &w.out, &w.nvalid, &w.oow};
+                      &w.pairs, &w.t0trace, &w.shots, &w.prisms, &w.out, &w.nvalid, &w.oow,
+                      &w.umask, &w.ucnt, &w.uoff, &w.U, &w.ubase, &w.brec, &w.items};
     for (DevBuf* b : bufs) b->release();
 }
 
@@ -719,6 +720,7 @@ int erp_set_tuning(erp_ctx* ctx, int key, int64_t value) {
         case ERP_TUNE_TRACK_ROWS: ctx->tune_rows = value; return ERP_OK;
         case ERP_TUNE_DIRECT_LAUNCH: ctx->tune_direct = value; return ERP_OK;
         case ERP_TUNE_WINDOW_SHIFT: ctx->tune_shift = value; return ERP_OK;
+        case ERP_TUNE_SINGLE_SHOT: ctx->tune_single = value; return ERP_OK;
         default: return ERP_E_INVALID;
     }
 }
@@ -731,6 +733,7 @@ int erp_get_tuning(erp_ctx* ctx, int key, int64_t* value) {
         case ERP_TUNE_TRACK_ROWS: *value = ctx->tune_rows; return ERP_OK;
         case ERP_TUNE_DIRECT_LAUNCH: *value = ctx->tune_direct; return ERP_OK;
         case ERP_TUNE_WINDOW_SHIFT: *value = ctx->tune_shift; return ERP_OK;
+        case ERP_TUNE_SINGLE_SHOT: *value = ctx->tune_single; return ERP_OK;
         default: return ERP_E_INVALID;
     }
 }

@@ -197,6 +197,9 @@ struct WorkBufs {
     DevBuf out;        // S x nr complex (device output staging)
     DevBuf nvalid;     // S int32
     DevBuf oow;        // count of target terms outside their pair's bin window
+    // shot-pair target kernel: union masks, counts, offsets and sizes per
+    // block of two shots; block records; work items
+    DevBuf umask, ucnt, uoff, U, ubase, brec, items;
 };
 
 // stage timers: events recorded on the run stream, resolved after it syncs
@@ -255,6 +258,8 @@ struct erp_ctx {
     std::atomic<int64_t> tune_rows{0}, tune_direct{0};
     // > 0 only in tests: shift every pair's bin window up by this many bins
     std::atomic<int64_t> tune_shift{0};
+    // > 0: the plain target kernel one (shot, target) pair at a time
+    std::atomic<int64_t> tune_single{0};
     MtState* mt = nullptr;  // buffers and graphs of erp_run_tracks
 };
 

@@ -588,6 +588,29 @@ struct Rec {
     int64_t cap;
 };
 
+// Facet records of a block of two consecutive shots for the shot-pair target
+// kernel: the facets that either shot illuminates (the union of the two
+// masks), laid out per block as rec_slot lays out one shot's, with the
+// geometry once and the shot-dependent source leg per shot.
+struct BRec {
+    float4* qa;    // (x, y, z, 0)
+    float4* qb;    // (nx, ny, nz, ux)
+    float4* qc;    // (uy, uz, vx, vy)
+    float* qd;     // vz
+    double2* itd;  // Itd of the block's first and second shot; -1: not lit by it
+    float2* amp;   // F Dmax / pi^4 of the first and second shot
+};
+// Where k_surface writes block records: rec.qa null = none.  umask / uoff:
+// the union masks and their exclusive block offsets per block of shots,
+// U its size, ubase its first record.
+struct BlkRec {
+    BRec rec;
+    const unsigned long long* umask;
+    const int* uoff;
+    const int* U;
+    const int64_t* ubase;
+};
+
 __device__ __forceinline__ void lds_add2(double2* win, int idx, float2 v) {
     atomicAdd(&win[idx].x, (double)v.x);
     atomicAdd(&win[idx].y, (double)v.y);
@@ -722,7 +745,8 @@ __global__ void __launch_bounds__(NT) k_surface(
     const int* __restrict__ Vs, const int64_t* __restrict__ rec_base, int nblk, KPar kp,
     Rec rec, int write_rec, int do_trace, double2* __restrict__ srow, int64_t shot0,
     Bounds* __restrict__ bounds, int det, unsigned* __restrict__ smax, int tile_nx,
-    int prec_ph, int prec_acc, const TrackK* __restrict__ trk, const RowK* __restrict__ rows) {
+    int prec_ph, int prec_acc, const TrackK* __restrict__ trk, const RowK* __restrict__ rows,
+    BlkRec brk) {
     // det 0: FP64 bins (LDS window, FP64 atomics into srow)
     // det 1: first deterministic pass, largest |component| per shot into smax
     // det 2: second deterministic pass, fixed-point bins (srow holds int64)
@@ -755,7 +779,16 @@ __global__ void __launch_bounds__(NT) k_surface(
     __shared__ int wmin_s;
     __shared__ float red_f[6][WPB];
     __shared__ double red_d[2][WPB];
-    if (threadIdx.x < MASKW) m_s[threadIdx.x] = mw[threadIdx.x];
+    // block records: the union mask of the shot's block of two and the
+    // prefix of its words
+    __shared__ unsigned long long um_s[MASKW];
+    __shared__ int upre_s[MASKW];
+    const bool blk_rec = write_rec && brk.rec.qa;
+    if (threadIdx.x < MASKW) {
+        m_s[threadIdx.x] = mw[threadIdx.x];
+        if (blk_rec)
+            um_s[threadIdx.x] = brk.umask[((size_t)(s >> 1) * nblk + blk) * MASKW + threadIdx.x];
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         int t = 0;
@@ -765,6 +798,13 @@ __global__ void __launch_bounds__(NT) k_surface(
         }
         pre_s[MASKW] = t;
         wmin_s = 0x7FFFFFFF;
+        if (blk_rec) {
+            int u = 0;
+            for (int w = 0; w < MASKW; ++w) {
+                upre_s[w] = u;
+                u += __popcll(um_s[w]);
+            }
+        }
     }
     __syncthreads();
     if (pre_s[MASKW] == 0) return;  // dark block (uniform exit)
@@ -858,7 +898,7 @@ __global__ void __launch_bounds__(NT) k_surface(
 
         if (write_rec) {
             const int64_t r = rbase + rec_slot(rank0 + rank, V);
-            rec.itd[r] = Itd;
+            if (rec.itd) rec.itd[r] = Itd;
             // target amplitude factor: sqrt(P n Grefr G1 fRfrC) lam^2/(K Itd)
             const float fac = lossless ? 1.f : fRfrC;
             // NoConv specular keeps fRe here for reflRadarSignal's direct sum
@@ -866,6 +906,20 @@ __global__ void __launch_bounds__(NT) k_surface(
                                  ? fRe
                                  : sqrtf(kp.P * kp.nf * Grefr * G1 * fac) * kp.lam * kp.lam *
                                        kp.invK / Itdf;
+            if (blk_rec) {
+                // the facet's slot among its block's union (both shots of the
+                // block that light it write the same geometry)
+                const int j = s >> 1, k = s & 1;
+                const int ur = brk.uoff[(size_t)j * nblk + blk] + upre_s[w] +
+                               __popcll(um_s[w] & ((1ull << lane) - 1ull));
+                const int64_t u = brk.ubase[j] + rec_slot(ur, brk.U[j]);
+                brk.rec.qa[u] = make_float4(g[0], g[1], g[2], 0.f);
+                brk.rec.qb[u] = make_float4(g[3], g[4], g[5], g[6]);
+                brk.rec.qc[u] = make_float4(g[7], g[8], g[9], g[10]);
+                brk.rec.qd[u] = g[11];
+                reinterpret_cast<double*>(brk.rec.itd + u)[k] = Itd;
+                reinterpret_cast<float*>(brk.rec.amp + u)[k] = Fr * (kp.Dmax * 0.0102659823f);
+            }
             if (rec.geo) {
 #pragma unroll
                 for (int a = 0; a < 12; ++a) rec.geo[(size_t)a * rec.cap + r] = g[a];
@@ -1803,6 +1857,46 @@ __device__ __forceinline__ double below_frac(float fz, const Strat& st) {
     return fma(fma(-q, den, num), r0, q);
 }
 
+// Bin window of a (shot, target) pair from the shot's bounds (the box and
+// the Itd range of its illuminated facets): every term's range lies between
+// Itd_min + n_lo Ttd_min and Itd_max + n_hi Ttd_max, so its bins b and b + 1
+// lie in [base, top).  fits: a window of cap bins at base holds [base, top);
+// len is then its live part, top - base rounded up to 64 bins (the spectral
+// chain reads whole groups of 64), else cap.  need_top false: base only.
+struct PairWin {
+    int base, top, len;
+    bool fits;
+};
+__device__ __forceinline__ PairWin pair_window(const Bounds& bd, float txf, float tyf, float tzf,
+                                               double n_lo, double n_hi, double rst,
+                                               const KPar& kp, int nr, int cap, bool need_top) {
+    PairWin pw;
+    const float bx = fmaxf(fmaxf(o2f(bd.xmin) - txf, txf - o2f(bd.xmax)), 0.f);
+    const float by = fmaxf(fmaxf(o2f(bd.ymin) - tyf, tyf - o2f(bd.ymax)), 0.f);
+    const float bz = fmaxf(fmaxf(o2f(bd.zmin) - tzf, tzf - o2f(bd.zmax)), 0.f);
+    const double tmin = sqrt((double)bx * bx + (double)by * by + (double)bz * bz);
+    const double itmin = __longlong_as_double((long long)bd.itd_min);
+    int base = (int)floor((itmin + n_lo * tmin - rst) / kp.drd) - 2;
+    // win_shift > 0 only to test the out-of-window check: a window that
+    // starts above the lowest bin the bounds allow
+    pw.base = max(base, 0) + kp.win_shift;
+    pw.fits = false;
+    pw.len = cap;
+    pw.top = nr;
+    if (need_top) {
+        const float fx = fmaxf(fabsf(o2f(bd.xmin) - txf), fabsf(o2f(bd.xmax) - txf));
+        const float fy = fmaxf(fabsf(o2f(bd.ymin) - tyf), fabsf(o2f(bd.ymax) - tyf));
+        const float fz = fmaxf(fabsf(o2f(bd.zmin) - tzf), fabsf(o2f(bd.zmax) - tzf));
+        const double tmax = sqrt((double)fx * fx + (double)fy * fy + (double)fz * fz);
+        const double ithmax = __longlong_as_double((long long)bd.itd_max);
+        const double rhi = ithmax + n_hi * tmax * (1.0 + 1e-6) + 1.0;
+        pw.top = min((int)ceil((rhi - rst) / kp.drd) + 2, nr);
+        pw.fits = pw.top - pw.base <= cap;
+        if (pw.fits) pw.len = max(0, min(cap, (pw.top - pw.base + 63) & ~63));
+    }
+    return pw;
+}
+
 template <int PH = PH_F64, bool S3 = false>
 __device__ __forceinline__ Tri tri_eval(const RecV& rv, bool frame_ok, double sx,
                                         double sy, double sz, double tx, double ty, double tz,
@@ -2056,33 +2150,12 @@ k_target_accum_fast(const float* __restrict__ tg0, int64_t Tpad0, const int* __r
         // window from the shot bounds (see k_target_accum); rngt lies between
         // Itd + n_lo Ttd and Itd + n_hi Ttd
         const double n_lo = st.deep ? kp.n_lo : kp.n_d, n_hi = st.deep ? kp.n_hi : kp.n_d;
-        const Bounds bd = bounds[s];
-        const float bx = fmaxf(fmaxf(o2f(bd.xmin) - txf, txf - o2f(bd.xmax)), 0.f);
-        const float by = fmaxf(fmaxf(o2f(bd.ymin) - tyf, tyf - o2f(bd.ymax)), 0.f);
-        const float bz = fmaxf(fmaxf(o2f(bd.zmin) - tzf, tzf - o2f(bd.zmax)), 0.f);
-        const double tmin = sqrt((double)bx * bx + (double)by * by + (double)bz * bz);
-        const double itmin = __longlong_as_double((long long)bd.itd_min);
-        int base = (int)floor((itmin + n_lo * tmin - rst) / kp.drd) - 2;
-        // win_shift > 0 only to test the out-of-window check: a window that
-        // starts above the lowest bin the bounds allow
-        base = max(base, 0) + kp.win_shift;
-        bool fits = false;
-        int len = TWF;
-        int top = nr;
-        if (pbase || DET) {
-            const float fx = fmaxf(fabsf(o2f(bd.xmin) - txf), fabsf(o2f(bd.xmax) - txf));
-            const float fy = fmaxf(fabsf(o2f(bd.ymin) - tyf), fabsf(o2f(bd.ymax) - tyf));
-            const float fz = fmaxf(fabsf(o2f(bd.zmin) - tzf), fabsf(o2f(bd.zmax) - tzf));
-            const double tmax = sqrt((double)fx * fx + (double)fy * fy + (double)fz * fz);
-            const double ithmax = __longlong_as_double((long long)bd.itd_max);
-            const double rhi = ithmax + n_hi * tmax * (1.0 + 1e-6) + 1.0;
-            // every bin written (b and b + 1) is below top
-            top = min((int)ceil((rhi - rst) / kp.drd) + 2, nr);
-            fits = top - base <= TWF;
-            // live window: [base, base + len), len a multiple of 64 (only it is
-            // zeroed, flushed and read by the spectral chain)
-            if (fits) len = max(0, min(TWF, (top - base + 63) & ~63));
-        }
+        // live window: [base, base + len) (only it is zeroed, flushed and
+        // read by the spectral chain)
+        const PairWin pw = pair_window(bounds[s], txf, tyf, tzf, n_lo, n_hi, rst, kp, nr, TWF,
+                                       pbase || DET);
+        const int base = pw.base, top = pw.top, len = pw.len;
+        const bool fits = pw.fits;
         if (pbase && threadIdx.x == 0) {
             pbase[pi] = fits ? base : -1;
             pwin[pi] = len;
@@ -2228,6 +2301,320 @@ k_target_accum_fast(const float* __restrict__ tg0, int64_t Tpad0, const int* __r
         // the next pair zeroes the window this one has just read
         __syncthreads();
     }
+}
+
+// ---------------------------------------------------------------------------
+// 7b. shot-pair target kernel
+// ---------------------------------------------------------------------------
+// Of a facet x target term only the source leg depends on the shot: Itd and
+// the refracted gain F Dmax / pi^4 of the facet record.  The target leg (Ttd,
+// the direction facet -> target in the facet frame, the four sinc factors of
+// facet_G(Tth, Tph) facet_G(TargetTh, Tph), the loss factor, the path below
+// a second interface) depends on the facet and the target alone.
+// Consecutive shots of a track light nearly the same facets, so
+// k_target_accum_pair takes one target with two consecutive shots, evaluates
+// the target leg once per facet that either lights, and adds one term into
+// each shot's bin window: the arithmetic of k_target_accum_fast's PH_F64 /
+// ACC_F64 path, with the product of the gain and the four sinc factors
+// formed as gain x (sincs x loss) instead of (gain x sincs) x loss.
+
+// the target leg of one facet: Ttd, the part of it below the interface times
+// Ttd (L3, second medium only) and the amplitude without the shot's gain
+struct TLeg {
+    double Ttd, L3;
+    float G;
+};
+
+// one block record in registers
+struct BRecV {
+    float4 qa, qb, qc;
+    float qd;
+    double2 itd;
+    float2 amp;
+};
+// the records of one block, based at its first (uniform over the workgroup);
+// record u at 32-bit byte offsets (u < 2^27)
+struct BRecBlk {
+    const char* qa;
+    const char* qb;
+    const char* qc;
+    const char* qd;
+    const char* itd;
+    const char* amp;
+};
+__device__ __forceinline__ BRecV load_brec(const BRecBlk& rb, unsigned u) {
+    BRecV v;
+    const unsigned o16 = u << 4;
+    v.qa = *reinterpret_cast<const float4*>(rb.qa + o16);
+    v.qb = *reinterpret_cast<const float4*>(rb.qb + o16);
+    v.qc = *reinterpret_cast<const float4*>(rb.qc + o16);
+    v.qd = *reinterpret_cast<const float*>(rb.qd + (u << 2));
+    v.itd = *reinterpret_cast<const double2*>(rb.itd + o16);
+    v.amp = *reinterpret_cast<const float2*>(rb.amp + (u << 3));
+    return v;
+}
+
+template <bool S3>
+__device__ __forceinline__ TLeg tri_leg(const BRecV& rv, bool frame_ok, double tx, double ty,
+                                        double tz, float tnx, float tny, float tnz, bool tn_up,
+                                        bool lossless, const erp_prism* __restrict__ prisms,
+                                        int np, const KPar& kp, const Strat& st) {
+    TLeg o;
+    const float fx = rv.qa.x, fy = rv.qa.y, fz = rv.qa.z;
+    const double dx = tx - (double)fx, dy = ty - (double)fy, dz = tz - (double)fz;
+    const double t2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    // Ttd = sqrt(t2) by one Newton step from the hardware reciprocal root
+    // (t2 lies far from the denormal range rsqrtf guards against)
+    const float rs = __builtin_amdgcn_rsqf((float)t2);
+    const double rd = (double)rs, hr = 0.5 * rd;
+    const double y0 = t2 * rd;
+    o.Ttd = fma(fma(-y0, y0, t2), hr, y0);
+    o.L3 = (S3 && st.deep) ? o.Ttd * below_frac(fz, st) : 0.0;
+    const float nx = rv.qb.x, ny = rv.qb.y, nz = rv.qb.z;
+    const float ux = rv.qb.w, uy = rv.qc.x, uz = rv.qc.y;
+    const float vx = rv.qc.z, vy = rv.qc.w, vz = rv.qd;
+    // unit vector facet -> target in FP64 (1/Ttd = rd (1.5 - t2 rd^2 / 2))
+    const double invd = rd * fma(-y0, hr, 1.5);
+    const double ix = dx * invd, iy = dy * invd, iz = dz * invd;
+    const float Ix = (float)ix, Iy = (float)iy, Iz = (float)iz;
+    const float2 csT = cos_sin_hw(ix, iy, iz, nx, ny, nz);
+    const float cT = csT.x, sT = csT.y;
+    float Iu = Ix * ux + Iy * uy + Iz * uz;
+    float Iv = Ix * vx + Iy * vy + Iz * vz;
+    if (!frame_ok) {
+        const float2 e = uv_dots_exact(ix, iy, iz, ux, uy, uz, vx, vy, vz);
+        Iu = e.x;
+        Iv = e.y;
+    }
+    const float sTT = tn_up ? __builtin_amdgcn_sqrtf(fmaxf(0.f, (float)((1.0 + iz) * (1.0 - iz))))
+                            : cos_sin_hw(ix, iy, iz, tnx, tny, tnz).y;
+    // azimuth (cos Tph, sin Tph) = (Iv, Iu) / |(Iu, Iv)|, (1, 0) below 1e-30
+    // (where every sinc argument is < 1e-6 whatever the azimuth)
+    const float q2 = Iu * Iu + Iv * Iv;
+    const bool az = q2 > 1e-30f;
+    const float rinv = az ? __builtin_amdgcn_rsqf(q2) : 0.f;
+    const float cph = az ? Iv * rinv : 1.f;
+    const float sph = Iu * rinv;
+    const float a = kp.piLlam;
+    float n1, d1, n2, d2, n3, d3, n4, d4;
+    pisinc_nd_hw(a * (sT * cph), &n1, &d1);
+    pisinc_nd_hw(a * (sT * sph), &n2, &d2);
+    pisinc_nd_hw(a * sTT * cph, &n3, &d3);
+    pisinc_nd_hw(a * sTT * sph, &n4, &d4);
+    const float den = fabsf((d1 * d2) * (d3 * d4));
+    float g = fabsf((n1 * n2) * (n3 * n4)) * __builtin_amdgcn_rcpf(den) * rs;
+    if (S3 && !lossless && st.deep && (double)fz > st.zi) g *= st.tsq;
+    if (!lossless) {
+        float expo = 0.f, tot = 0.f;
+        for (int k = 0; k < np; ++k) {
+            const float d = prism_len(fx, fy, fz, Ix, Iy, Iz, prisms[k], (float)o.Ttd);
+            tot += d;
+            expo += prisms[k].alpha * d;
+        }
+        expo += kp.alpha2 * ((float)o.Ttd - tot);
+        const float kc = kp.ks * cT;
+        g *= __expf(-2.f * (expo + kc * kc));
+    }
+    o.G = g;
+    return o;
+}
+
+// the source leg of one shot: range Itd + n Ttd (+ the second medium's
+// excess), bin and weight as in tri_eval (the exact FP64 sum within 1e-7 bin
+// of a bin edge), phase, amplitude gain x G
+template <bool S3>
+__device__ __forceinline__ Tri tri_shot(const TLeg& g, double itd, float gain, double rst,
+                                        float fx, float fy, float fz, double tx, double ty,
+                                        double tz, const KPar& kp, const Strat& st, int nr) {
+    Tri o;
+    double rngt = fma(g.Ttd, kp.n_d, itd);
+    if (S3 && st.deep) rngt = fma(g.L3, st.dn, rngt);
+    const double xq = fma(rngt, kp.inv_dr, -rst * kp.inv_dr);
+    o.b = (int)xq;
+    o.w = xq - (double)o.b;
+    if (o.w < 1e-7 || o.w > 1.0 - 1e-7) {
+        // the facet's coordinates pass through an empty asm statement, so
+        // that the exact root below stays on this rare path (the two shots'
+        // copies of it would otherwise be merged and hoisted out of it)
+        float ex = fx, ey = fy, ez = fz;
+        asm volatile("" : "+v"(ex), "+v"(ey), "+v"(ez));
+        const double dx = tx - (double)ex, dy = ty - (double)ey, dz = tz - (double)ez;
+        const double Te = dist3(dx, dy, dz);
+        rngt = madd_nc(itd, Te, kp.n_d);
+        if (S3 && st.deep) {
+#pragma clang fp contract(off)
+            const double L3 = Te * below_frac_exact(ez, st);
+            rngt = madd_nc(rngt, L3, st.dn);
+        }
+        o.valid = bin_of(rngt, rst, kp.drd, nr, &o.b, &o.w);
+    } else {
+        o.valid = (xq > -1.0) && (xq < (double)nr);
+    }
+    const float2 ph = phasor_rc_hw(rngt, kp.inv_cyc);
+    const float amp = gain * g.G;
+    o.c = make_float2(ph.x * amp, ph.y * amp);
+    return o;
+}
+
+// (target, row of the first shot, row of the second, block) per work item;
+// a row of -1: that shot does not take the target
+constexpr int NTB = 1024;  // threads of k_target_accum_pair (one workgroup per CU)
+// bins of each of its two FP64 windows: 2 x 79 KB of the CU's 160 KB of LDS.
+// On the 2048^2 fBm workload the shot bounds allow 2.3e3-5.8e3 bins per pair;
+// 77 % of its pairs fit 5056 bins (28 % fit 4096).
+constexpr int TWB = 5056;
+
+template <bool S3>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, NTB)))
+k_target_accum_pair(const float* __restrict__ tg, int64_t Tpad, const int4* __restrict__ items,
+                    int nitems, int* __restrict__ next_item, const erp_shot* __restrict__ shots,
+                    const int64_t* __restrict__ ubase, const int* __restrict__ Ucnt, BRec rec,
+                    const Bounds* __restrict__ bounds, const erp_prism* __restrict__ prisms,
+                    int np, KPar kp, float2* __restrict__ outA, int* __restrict__ pbase,
+                    int* __restrict__ pwin, int frame_ok, int* __restrict__ oow) {
+    __shared__ double winS[4 * TWB];
+    __shared__ int s_it;
+    const int nr = kp.nr;
+    const bool lossless = kp.mode & ERP_MODE_LOSSLESS;
+    // the windows are zero between items: each item's flush clears what it
+    // read, and terms only reach the bins below an item's len
+    for (int i = threadIdx.x; i < 4 * TWB; i += NTB) winS[i] = 0.0;
+    // persistent: each workgroup takes items from a counter (cleared by the
+    // host before the launch) until none is left; the next item's index is
+    // fetched while the current one runs
+    if (threadIdx.x == 0) s_it = atomicAdd(next_item, 1);
+    __syncthreads();
+    int it = s_it;
+    while (it < nitems) {
+        int nxt = 0;
+        if (threadIdx.x == 0) nxt = atomicAdd(next_item, 1);
+        const int4 item = items[it];
+        const int t = item.x, blk = item.w;
+        const int row[2] = {item.y, item.z};
+        const float txf = tg[t], tyf = tg[Tpad + t], tzf = tg[2 * Tpad + t];
+        const float tnx = tg[3 * Tpad + t], tny = tg[4 * Tpad + t], tnz = tg[5 * Tpad + t];
+        const double tx = txf, ty = tyf, tz = tzf;
+        const bool tn_up = tnx == 0.f && tny == 0.f && tnz == 1.f;
+        const Strat st{S3 && (double)tzf < kp.zi, kp.zi, (double)tzf, kp.dn, kp.tsq};
+        const double n_lo = st.deep ? kp.n_lo : kp.n_d, n_hi = st.deep ? kp.n_hi : kp.n_d;
+
+        // each shot's window (as k_target_accum_fast's); a shot without the
+        // target has an empty one
+        int base[2], len[2];
+        double rst[2];
+        float2* rowp[2];
+        int* flag[2];
+        bool spill = false;
+        WinS win[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            win[k] = WinS{winS + 2 * k * TWB, winS + (2 * k + 1) * TWB};
+            base[k] = 0;
+            len[k] = 0;
+            rst[k] = 0.0;
+            rowp[k] = nullptr;
+            flag[k] = nullptr;
+            if (row[k] < 0) continue;
+            const int s = 2 * blk + k;
+            rst[k] = (double)shots[s].rst;
+            const PairWin pw = pair_window(bounds[s], txf, tyf, tzf, n_lo, n_hi, rst[k], kp, nr,
+                                           TWB, true);
+            base[k] = pw.base;
+            len[k] = pw.len;
+            rowp[k] = outA + (size_t)row[k] * kp.P_fft;
+            // out-of-window terms of a pair whose window covers its bounds
+            // are broken bounds (counted); the others go to the row
+            flag[k] = pw.fits ? oow : nullptr;
+            if (threadIdx.x == 0) {
+                pbase[row[k]] = pw.fits ? pw.base : -1;
+                pwin[row[k]] = pw.len;
+            }
+            if (!pw.fits) {
+                spill = true;
+                const int hi = pw.base + TWB;
+                for (int i = threadIdx.x; i < kp.row_live; i += NTB)
+                    if (i < pw.base || i >= hi || i >= nr) rowp[k][i] = make_float2(0.f, 0.f);
+            }
+        }
+        if (spill) {
+            // the row's zeros land before any thread's out-of-window atomics
+            __threadfence();
+            __syncthreads();
+        }
+
+        const int U = Ucnt[blk];
+        if (U > 0) {
+            const int64_t r0 = ubase[blk];
+            const BRecBlk rb{reinterpret_cast<const char*>(rec.qa + r0),
+                             reinterpret_cast<const char*>(rec.qb + r0),
+                             reinterpret_cast<const char*>(rec.qc + r0),
+                             reinterpret_cast<const char*>(rec.qd + r0),
+                             reinterpret_cast<const char*>(rec.itd + r0),
+                             reinterpret_cast<const char*>(rec.amp + r0)};
+            const unsigned ul = (unsigned)(U - 1);
+            auto step = [&](const BRecV& a) {
+                const TLeg g = tri_leg<S3>(a, frame_ok, tx, ty, tz, tnx, tny, tnz, tn_up,
+                                           lossless, prisms, np, kp, st);
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const double itd = k ? a.itd.y : a.itd.x;
+                    if (row[k] < 0 || !(itd >= 0.0)) continue;
+                    const Tri o = tri_shot<S3>(g, itd, k ? a.amp.y : a.amp.x, rst[k], a.qa.x,
+                                               a.qa.y, a.qa.z, tx, ty, tz, kp, st, nr);
+                    if (o.valid) tri_add<ACC_F64>(o, base[k], len[k], nr, win[k], rowp[k], flag[k]);
+                }
+            };
+            // one facet per thread and step, the next step's record loaded
+            // before this step's arithmetic into the other of two register
+            // sets (clamped, in-bounds indices: no set is copied)
+            BRecV a0 = load_brec(rb, min((unsigned)threadIdx.x, ul));
+            for (int u = threadIdx.x; u < U; u += 2 * NTB) {
+                const BRecV a1 = load_brec(rb, min((unsigned)(u + NTB), ul));
+                step(a0);
+                if (u + NTB >= U) break;
+                a0 = load_brec(rb, min((unsigned)(u + 2 * NTB), ul));
+                step(a1);
+            }
+        }
+        __syncthreads();
+        // flush each window into its row and clear what was read
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+            for (int i = threadIdx.x; i < len[k]; i += NTB) {
+                const int b = base[k] + i;
+                if (b < nr) rowp[k][b] = make_float2((float)win[k].x[i], (float)win[k].y[i]);
+                win[k].x[i] = 0.0;
+                win[k].y[i] = 0.0;
+            }
+        if (threadIdx.x == 0) s_it = nxt;
+        __syncthreads();
+        it = s_it;
+    }
+}
+
+// Union of the aperture masks of shots 2j and 2j + 1 of a batch (the last
+// block of an odd batch: shot 2j alone) and its count per facet block; one
+// workgroup of MASKW threads per (facet block, block of shots)
+__global__ void k_union_mask(const unsigned long long* __restrict__ mask, int nblk, int S,
+                             unsigned long long* __restrict__ umask, int* __restrict__ ucnt) {
+    const int blk = blockIdx.x, j = blockIdx.y, w = threadIdx.x;
+    const int s0 = 2 * j;
+    unsigned long long m = mask[((size_t)s0 * nblk + blk) * MASKW + w];
+    if (s0 + 1 < S) m |= mask[((size_t)(s0 + 1) * nblk + blk) * MASKW + w];
+    umask[((size_t)j * nblk + blk) * MASKW + w] = m;
+    __shared__ int c_s;
+    if (w == 0) c_s = 0;
+    __syncthreads();
+    atomicAdd(&c_s, __popcll(m));
+    __syncthreads();
+    if (w == 0) ucnt[(size_t)j * nblk + blk] = c_s;
+}
+
+// block records not written by a shot read as "not lit": Itd -1
+__global__ void k_fill_itd(double2* __restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = make_double2(-1.0, -1.0);
 }
 
 
@@ -2801,6 +3188,99 @@ void fill_chunk(const int* tc, int rows, int& cs, int& cj, int C, int GS, bool c
     }
 }
 
+// Work of the shot-pair target kernel for the blocks of two shots of a
+// batch: per block, the union of its shots' target lists (each ascending)
+// as (target, index in the first shot's list or -1, index in the second's
+// or -1), ascending.  lists: S rows of pitch ints; tc: per shot its count
+// at stride 2 (the target counts as k_target_select leaves them).
+struct PairIdx {
+    int t, i0, i1;
+};
+void merge_pair_lists(const int* lists, size_t pitch, const int* tc, int S,
+                      std::vector<std::vector<PairIdx>>& out) {
+    const int nbk = (S + 1) / 2;
+    out.resize(nbk);
+    for (int j = 0; j < nbk; ++j) {
+        std::vector<PairIdx>& o = out[j];
+        o.clear();
+        const int* a = lists + (size_t)(2 * j) * pitch;
+        const int na = tc[2 * (2 * j)];
+        const int* b = 2 * j + 1 < S ? lists + (size_t)(2 * j + 1) * pitch : nullptr;
+        const int nb = b ? tc[2 * (2 * j + 1)] : 0;
+        int i = 0, k = 0;
+        while (i < na || k < nb) {
+            if (k >= nb || (i < na && a[i] < b[k])) {
+                o.push_back({a[i], i, -1});
+                ++i;
+            } else if (i >= na || b[k] < a[i]) {
+                o.push_back({b[k], -1, k});
+                ++k;
+            } else {
+                o.push_back({a[i], i, k});
+                ++i;
+                ++k;
+            }
+        }
+    }
+}
+
+// One launch of the shot-pair kernel: whole items in block order while
+// their rows fit C.  Rows are laid out per (block, shot) segment: a block's
+// first-shot rows of this launch in item order, then its second-shot rows.
+struct PairChunk {
+    std::vector<int4> items;          // (target, row of shot 0, row of shot 1, block)
+    std::vector<int> sg_s, gsb, gse;  // per segment: shot, its first and end slice
+    std::vector<int> gb, ge;          // per slice of GS rows: first row, end
+    int rows = 0;
+    double triples = 0;               // facet x target terms (for the stage timer)
+};
+void fill_pair_chunk(const std::vector<std::vector<PairIdx>>& bl, int& jb, size_t& ib, int C,
+                     int GS, const int* V, PairChunk& f) {
+    f.items.clear();
+    f.sg_s.clear(); f.gsb.clear(); f.gse.clear(); f.gb.clear(); f.ge.clear();
+    f.rows = 0;
+    f.triples = 0;
+    auto segment = [&](int shot, int b, int e) {
+        f.sg_s.push_back(shot);
+        f.gsb.push_back((int)f.gb.size());
+        for (int i = b; i < e; i += GS) {
+            f.gb.push_back(i);
+            f.ge.push_back(std::min(i + GS, e));
+        }
+        f.gse.push_back((int)f.gb.size());
+        f.triples += (double)(e - b) * V[shot];
+    };
+    while (jb < (int)bl.size()) {
+        const std::vector<PairIdx>& its = bl[jb];
+        if (ib >= its.size()) {
+            ++jb;
+            ib = 0;
+            continue;
+        }
+        size_t e = ib;
+        int n0 = 0, n1 = 0;
+        while (e < its.size()) {
+            const int a = its[e].i0 >= 0, b = its[e].i1 >= 0;
+            if (f.rows + n0 + n1 + a + b > C) break;
+            n0 += a;
+            n1 += b;
+            ++e;
+        }
+        if (e == ib) break;  // this launch is full
+        const int r = f.rows;
+        int c0 = 0, c1 = 0;
+        for (size_t q = ib; q < e; ++q) {
+            const PairIdx& p = its[q];
+            f.items.push_back(make_int4(p.t, p.i0 >= 0 ? r + c0++ : -1,
+                                        p.i1 >= 0 ? r + n0 + c1++ : -1, jb));
+        }
+        if (n0 > 0) segment(2 * jb, r, r + n0);
+        if (n1 > 0) segment(2 * jb + 1, r + n0, r + n0 + n1);
+        f.rows += n0 + n1;
+        ib = e;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -3161,6 +3641,23 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         ERP_HIP_TRY(hipMemsetAsync(W.oow.p, 0, sizeof(int), st));
     }
 
+    // shot-pair target kernel (k_target_accum_pair): plain runs of the default
+    // arithmetic whose targets all have a type != 0, outside the
+    // allocation-free entry point, unless ERP_TUNE_SINGLE_SHOT asks for the
+    // single-shot kernel
+    const bool pairk = half && !det && ph == PH_F64 && acc == ACC_F64 && !any_t0 && !arena &&
+                       C >= 2 && ctx->tune_single == 0;
+    const int nbk_max = (Smax + 1) / 2;
+    if (pairk) {
+        W.umask.ensure(sizeof(unsigned long long) * (size_t)nbk_max * nblk * MASKW);
+        W.ucnt.ensure(sizeof(int) * (size_t)nbk_max * nblk);
+        W.uoff.ensure(sizeof(int) * (size_t)nbk_max * nblk);
+        W.U.ensure(sizeof(int) * (size_t)nbk_max);
+        W.ubase.ensure(sizeof(int64_t) * (size_t)nbk_max);
+    }
+    std::vector<int> hU(nbk_max);
+    std::vector<int64_t> hubase(nbk_max + 1);
+
     std::vector<int> hV(Smax), htc(2 * Smax);
     std::vector<int64_t> hbase(Smax + 1);
     int* dV = W.V.as<int>();
@@ -3191,6 +3688,19 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                            W.off.as<int>(), dV, nblk, (const TrackK*)nullptr,
                            (const RowK*)nullptr);
         ERP_CHECK_LAUNCH("k_scan_blocks");
+        // blocks of two shots: the union of their masks, its offsets and size
+        const int nbk = (S + 1) / 2;
+        if (pairk) {
+            hipLaunchKernelGGL(k_union_mask, dim3(nblk, nbk), dim3(MASKW), 0, st, dmask, nblk, S,
+                               W.umask.as<unsigned long long>(), W.ucnt.as<int>());
+            ERP_CHECK_LAUNCH("k_union_mask");
+            hipLaunchKernelGGL(k_scan_blocks, dim3(nbk), dim3(1024), 0, st, W.ucnt.as<int>(),
+                               W.uoff.as<int>(), W.U.as<int>(), nblk, (const TrackK*)nullptr,
+                               (const RowK*)nullptr);
+            ERP_CHECK_LAUNCH("k_scan_blocks");
+            ERP_HIP_TRY(hipMemcpyAsync(hU.data(), W.U.p, sizeof(int) * nbk,
+                                       hipMemcpyDeviceToHost, st));
+        }
         PT_.end(st);
         ERP_HIP_TRY(hipMemcpyAsync(hV.data(), dV, sizeof(int) * S, hipMemcpyDeviceToHost, st));
         ERP_HIP_TRY(hipStreamSynchronize(st));
@@ -3219,7 +3729,40 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         ERP_HIP_TRY(hipMemcpyAsync(dbase, hbase.data(), sizeof(int64_t) * S,
                                    hipMemcpyHostToDevice, st));
         Rec rec{};
-        if (need_rec && Vtot > 0) {
+        BlkRec brk{};
+        if (pairk && Vtot > 0) {
+            // block records: 76 bytes per facet of a block's union
+            hubase[0] = 0;
+            for (int j = 0; j < nbk; ++j) {
+                if (hU[j] >= (1 << 28))
+                    throw ErpError{ERP_E_INVALID, "two shots illuminate 2^28 facets or more"};
+                hubase[j + 1] = hubase[j] + hU[j];
+            }
+            const size_t cap = (size_t)hubase[nbk];
+            auto rnd = [](size_t n) { return (n + 255) & ~(size_t)255; };
+            const size_t bytes = 4 * rnd(cap * 16) + rnd(cap * 4) + rnd(cap * 8);
+            W.brec.ensure_grow(bytes, (size_t)nbk * (size_t)F * 76 + 256 * 6);
+            char* b = W.brec.as<char>();
+            auto take = [&](size_t n) { char* r = b; b += rnd(n); return r; };
+            BRec& br = brk.rec;
+            br.qa = (float4*)take(cap * 16);
+            br.qb = (float4*)take(cap * 16);
+            br.qc = (float4*)take(cap * 16);
+            br.qd = (float*)take(cap * 4);
+            br.itd = (double2*)take(cap * 16);
+            br.amp = (float2*)take(cap * 8);
+            brk.umask = W.umask.as<unsigned long long>();
+            brk.uoff = W.uoff.as<int>();
+            brk.U = W.U.as<int>();
+            brk.ubase = W.ubase.as<int64_t>();
+            ERP_HIP_TRY(hipMemcpyAsync(W.ubase.p, hubase.data(), sizeof(int64_t) * nbk,
+                                       hipMemcpyHostToDevice, st));
+            if (cap > 0) {
+                hipLaunchKernelGGL(k_fill_itd, dim3(grid_for(cap)), dim3(256), 0, st, br.itd,
+                                   (int64_t)cap);
+                ERP_CHECK_LAUNCH("k_fill_itd");
+            }
+        } else if (need_rec && Vtot > 0) {
             const size_t cap = (size_t)Vtot;
             // at most every facet of every shot of the batch (the bound that
             // erp_run_footprint counts)
@@ -3259,7 +3802,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         hipLaunchKernelGGL(k_surface, dim3(nblk, S), dim3(NT), 0, st, mesh->soa.as<float>(),
                            mesh->Fpad, F, dshots, dmask, W.off.as<int>(), dV, dbase, nblk, kp, rec,
                            write, do_trace ? 1 : 0, srow, s0, dbounds, det ? 1 : 0, smax,
-                           tile_nx, ph, acc, (const TrackK*)nullptr, (const RowK*)nullptr);
+                           tile_nx, ph, acc, (const TrackK*)nullptr, (const RowK*)nullptr, brk);
         ERP_CHECK_LAUNCH("k_surface");
         if (det && do_trace) {
             // second pass: the same terms as fixed-point values scaled by the
@@ -3267,7 +3810,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
             hipLaunchKernelGGL(k_surface, dim3(nblk, S), dim3(NT), 0, st, mesh->soa.as<float>(),
                                mesh->Fpad, F, dshots, dmask, W.off.as<int>(), dV, dbase, nblk, kp,
                                rec, 0, 1, srow, s0, (Bounds*)nullptr, 2, smax, tile_nx, 0,
-                               0, (const TrackK*)nullptr, (const RowK*)nullptr);
+                               0, (const TrackK*)nullptr, (const RowK*)nullptr, BlkRec{});
             ERP_CHECK_LAUNCH("k_surface");
         }
         if (do_trace) {
@@ -3353,12 +3896,74 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                 int* dge = dgb + gcap;
                 int* dseg = dge + gcap;              // segment shot / begin / end
                 int* dpw = dseg + 3 * S;             // per-pair window length (fused)
+                if (pairk) {
+                    // the shot-pair kernel: items of a target and two
+                    // consecutive shots, from the shots' target lists
+                    int maxc = 0;
+                    for (int s = 0; s < S; ++s) maxc = std::max(maxc, htc[2 * s]);
+                    std::vector<int> hl((size_t)S * maxc);
+                    if (maxc > 0) {
+                        ERP_HIP_TRY(hipMemcpy2DAsync(hl.data(), sizeof(int) * maxc, list1,
+                                                     sizeof(int) * Tcap, sizeof(int) * maxc, S,
+                                                     hipMemcpyDeviceToHost, st));
+                        ERP_HIP_TRY(hipStreamSynchronize(st));
+                    }
+                    std::vector<std::vector<PairIdx>> bl;
+                    merge_pair_lists(hl.data(), (size_t)maxc, htc.data(), S, bl);
+                    W.items.ensure(sizeof(int4) * (size_t)C);
+                    int4* ditems = W.items.as<int4>();
+                    int* dnext = dpw + C;  // item counter, in the spare tail of the list
+                    PairChunk pc;
+                    int jb = 0;
+                    size_t ib = 0;
+                    for (;;) {
+                        fill_pair_chunk(bl, jb, ib, C, GS, hV.data(), pc);
+                        const int n = pc.rows;
+                        if (n == 0) break;
+                        const int nseg = (int)pc.sg_s.size();
+                        const int ngroups = (int)pc.gb.size();
+                        const int nit = (int)pc.items.size();
+                        // slices and segments at the offsets the chain reads them
+                        std::vector<int> up(2 * gcap + 3 * (size_t)S, 0);
+                        std::copy(pc.gb.begin(), pc.gb.end(), up.begin());
+                        std::copy(pc.ge.begin(), pc.ge.end(), up.begin() + gcap);
+                        std::copy(pc.sg_s.begin(), pc.sg_s.end(), up.begin() + 2 * gcap);
+                        std::copy(pc.gsb.begin(), pc.gsb.end(), up.begin() + 2 * gcap + S);
+                        std::copy(pc.gse.begin(), pc.gse.end(), up.begin() + 2 * gcap + 2 * S);
+                        ERP_HIP_TRY(hipMemcpyAsync(dgb, up.data(), sizeof(int) * up.size(),
+                                                   hipMemcpyHostToDevice, st));
+                        ERP_HIP_TRY(hipMemcpyAsync(ditems, pc.items.data(), sizeof(int4) * nit,
+                                                   hipMemcpyHostToDevice, st));
+                        ERP_HIP_TRY(hipMemsetAsync(dnext, 0, sizeof(int), st));
+                        PT_.begin(ERP_STAGE_TARGET_ACCUM, st, pc.triples);
+                        // persistent grid: one workgroup per CU (its two windows
+                        // take 128 KB of LDS), each taking items until none is left
+                        const int grid = std::max(1, std::min(nit, ctx->num_cu));
+                        auto kern = kp.has3 ? k_target_accum_pair<true> : k_target_accum_pair<false>;
+                        hipLaunchKernelGGL(kern, dim3(grid), dim3(NTB), 0, st,
+                                           targets->xyzn.as<float>(), Tpad,
+                                           (const int4*)ditems, nit, dnext, dshots,
+                                           brk.ubase, brk.U, brk.rec, dbounds, dpr, (int)nprisms,
+                                           kp, A, dpb, dpw, mesh->frame_ok ? 1 : 0,
+                                           W.oow.as<int>());
+                        ERP_CHECK_LAUNCH("k_target_accum_pair");
+                        PT_.end(st);
+                        PT_.begin(ERP_STAGE_TARGET_FFT, st, (double)n);
+                        target_pairs_half(ctx, planP, A, dpb, dpw, nr, n, g_targ_nat, Z, st);
+                        PT_.end(st);
+                        PT_.begin(ERP_STAGE_TARGET_SUM, st, (double)n);
+                        target_sum_half(planP, Z, A + (size_t)C * P, dgb, dge, ngroups, dseg,
+                                        dseg + S, dseg + 2 * S, nseg, Sacc, -1.0f / kp.sqrtP, st,
+                                        nullptr, false);
+                        PT_.end(st);
+                    }
+                }
                 // iterate pairs in (shot, target) order
                 std::vector<int> tc1((size_t)S);
                 for (int s = 0; s < S; ++s) tc1[s] = htc[2 * s];
                 ChunkFill cf;
                 int cs = 0, cj = 0;
-                while (cs < S) {
+                while (!pairk && cs < S) {
                     // deterministic: every shot's slices independent of the launches
                     fill_chunk(tc1.data(), S, cs, cj, C, GS, det && half, cf);
                     const int n = cf.n();
@@ -3937,14 +4542,14 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
                                (const float*)nullptr, (int64_t)0, (int64_t)0, dshots,
                                m.mask.as<unsigned long long>(), m.off.as<int>(), dV, dbase,
                                nblk_max, kp0, rec, targ ? 1 : 0, 1, srow, (int64_t)0, dbounds,
-                               det ? 1 : 0, smax, 0, 0, 0, dtrk, drows);
+                               det ? 1 : 0, smax, 0, 0, 0, dtrk, drows, BlkRec{});
             ERP_CHECK_LAUNCH("k_surface");
             if (det) {
                 hipLaunchKernelGGL(k_surface, dim3(nblk_max, R), dim3(NT), 0, st,
                                    (const float*)nullptr, (int64_t)0, (int64_t)0, dshots,
                                    m.mask.as<unsigned long long>(), m.off.as<int>(), dV, dbase,
                                    nblk_max, kp0, rec, 0, 1, srow, (int64_t)0, (Bounds*)nullptr,
-                                   2, smax, 0, 0, 0, dtrk, drows);
+                                   2, smax, 0, 0, 0, dtrk, drows, BlkRec{});
                 ERP_CHECK_LAUNCH("k_surface");
             }
             hipLaunchKernelGGL(k_surf_rows, dim3((nr + 255) / 256, R), dim3(256), 0, st,

@@ -342,12 +342,18 @@ int erp_profile_read(erp_ctx* ctx, int stage, double* ms, int64_t* launches, dou
  *                              up by this many bins, so that its lowest terms
  *                              fall outside it and the run fails with
  *                              ERP_E_INTERNAL (the check that a window bound
- *                              held; 0 = off) */
+ *                              held; 0 = off)
+ *   ERP_TUNE_SINGLE_SHOT     : > 0: the plain target kernel takes each
+ *                              (shot, target) pair on its own instead of a
+ *                              target with two consecutive shots (which
+ *                              shares the facet -> target leg of their common
+ *                              facets; the default) */
 #define ERP_TUNE_SHOTS_PER_BATCH  1
 #define ERP_TUNE_PAIRS_PER_LAUNCH 2
 #define ERP_TUNE_TRACK_ROWS       3
 #define ERP_TUNE_DIRECT_LAUNCH    4
 #define ERP_TUNE_WINDOW_SHIFT     5
+#define ERP_TUNE_SINGLE_SHOT      6
 int erp_set_tuning(erp_ctx* ctx, int key, int64_t value);
 /* The value set for key (0 = default). */
 int erp_get_tuning(erp_ctx* ctx, int key, int64_t* value);

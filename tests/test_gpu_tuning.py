@@ -7,7 +7,7 @@ import pytest
 
 from europaraypaths_amd._abi import (ERP_E_INTERNAL, ERP_E_INVALID, ERP_RUN_DETERMINISTIC,
                                     ERP_TUNE_PAIRS_PER_LAUNCH, ERP_TUNE_SHOTS_PER_BATCH,
-                                    ERP_TUNE_WINDOW_SHIFT)
+                                    ERP_TUNE_SINGLE_SHOT, ERP_TUNE_WINDOW_SHIFT)
 from europaraypaths_amd._native import ErpError
 from oracle import oracle
 from tests.helpers import compare_traces
@@ -42,7 +42,7 @@ def test_small_batches_and_launches_give_the_same_traces(gpu_ctx, nr):
 
 
 def test_tuning_keys_are_checked(gpu_ctx):
-    for key, value in ((0, 1), (6, 1), (ERP_TUNE_SHOTS_PER_BATCH, -1)):
+    for key, value in ((0, 1), (7, 1), (ERP_TUNE_SHOTS_PER_BATCH, -1)):
         with pytest.raises(ErpError) as ex:
             gpu_ctx.set_tuning(key, value)
         assert ex.value.code == ERP_E_INVALID
@@ -53,8 +53,9 @@ def test_window_check_fails_a_run_whose_window_misses_terms(gpu_ctx):
     """A pair's bin window must hold every term its bounds allow.  Shifting
     every window up by 16 bins (ERP_TUNE_WINDOW_SHIFT, a test-only knob)
     leaves the lowest terms outside: the run must fail with ERP_E_INTERNAL
-    (single track, deterministic mode, multi-track batch) instead of
-    dropping them, and pass again once the shift is off."""
+    (single track with the shot-pair and the single-shot kernel,
+    deterministic mode, multi-track batch) instead of dropping them, and
+    pass again once the shift is off."""
     par, soa, shots = scene(rough=0.3, nr=1024, h=3000.0, rst=2600.0, ns=3)
     tg = _targets(par, n=5)
     e = par.to_erp()
@@ -63,16 +64,19 @@ def test_window_check_fails_a_run_whose_window_misses_terms(gpu_ctx):
     try:
         gpu_ctx.set_tuning(ERP_TUNE_WINDOW_SHIFT, 16)
         try:
-            for flags in (0, ERP_RUN_DETERMINISTIC):
+            for flags, single in ((0, 0), (0, 1), (ERP_RUN_DETERMINISTIC, 0)):
+                gpu_ctx.set_tuning(ERP_TUNE_SINGLE_SHOT, single)
                 with pytest.raises(ErpError) as ex:
                     gpu_ctx.run(e, m, t, shots, flags=flags)
                 assert ex.value.code == ERP_E_INTERNAL, ex.value
                 assert "window" in str(ex.value)
+            gpu_ctx.set_tuning(ERP_TUNE_SINGLE_SHOT, 0)
             with pytest.raises(ErpError) as ex:
                 gpu_ctx.run_tracks([(e, m, t, shots)])
             assert ex.value.code == ERP_E_INTERNAL, ex.value
         finally:
             gpu_ctx.set_tuning(ERP_TUNE_WINDOW_SHIFT, 0)
+            gpu_ctx.set_tuning(ERP_TUNE_SINGLE_SHOT, 0)
         o, ov = oracle.run(e, soa, tg, shots)
         for g, gv in (gpu_ctx.run(e, m, t, shots), gpu_ctx.run_tracks([(e, m, t, shots)])[0]):
             np.testing.assert_array_equal(gv, ov)
