@@ -2464,18 +2464,22 @@ constexpr int NTB = 1024;  // threads of k_target_accum_pair (one workgroup per 
 // 77 % of its pairs fit 5056 bins (28 % fit 4096).
 constexpr int TWB = 5056;
 
-template <bool S3>
+// MT: a multi-track batch: an item's block is two rows of one track, whose
+// targets, parameters and facet frames it uses; the item count is read from
+// device memory (the launch is replayed from a graph).
+template <bool S3, bool MT = false>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, NTB)))
-k_target_accum_pair(const float* __restrict__ tg, int64_t Tpad, const int4* __restrict__ items,
-                    int nitems, int* __restrict__ next_item, const erp_shot* __restrict__ shots,
+k_target_accum_pair(const float* __restrict__ tg0, int64_t Tpad0, const int4* __restrict__ items,
+                    int nitems0, int* __restrict__ next_item, const erp_shot* __restrict__ shots,
                     const int64_t* __restrict__ ubase, const int* __restrict__ Ucnt, BRec rec,
                     const Bounds* __restrict__ bounds, const erp_prism* __restrict__ prisms,
-                    int np, KPar kp, float2* __restrict__ outA, int* __restrict__ pbase,
-                    int* __restrict__ pwin, int frame_ok, int* __restrict__ oow) {
+                    int np, KPar kp0, float2* __restrict__ outA, int* __restrict__ pbase,
+                    int* __restrict__ pwin, int frame_ok0, int* __restrict__ oow,
+                    const TrackK* __restrict__ trk, const RowK* __restrict__ rows,
+                    const int* __restrict__ nitems_dev) {
     __shared__ double winS[4 * TWB];
     __shared__ int s_it;
-    const int nr = kp.nr;
-    const bool lossless = kp.mode & ERP_MODE_LOSSLESS;
+    const int nitems = MT ? *nitems_dev : nitems0;
     // the windows are zero between items: each item's flush clears what it
     // read, and terms only reach the bins below an item's len
     for (int i = threadIdx.x; i < 4 * TWB; i += NTB) winS[i] = 0.0;
@@ -2491,6 +2495,13 @@ k_target_accum_pair(const float* __restrict__ tg, int64_t Tpad, const int4* __re
         const int4 item = items[it];
         const int t = item.x, blk = item.w;
         const int row[2] = {item.y, item.z};
+        const TrackK* tkp = MT ? &trk[rows[2 * blk].trk] : nullptr;
+        const KPar& kp = MT ? tkp->kp : kp0;
+        const float* __restrict__ tg = MT ? tkp->tg : tg0;
+        const int64_t Tpad = MT ? tkp->Tpad : Tpad0;
+        const int frame_ok = MT ? tkp->frame_ok : frame_ok0;
+        const int nr = kp.nr;
+        const bool lossless = kp.mode & ERP_MODE_LOSSLESS;
         const float txf = tg[t], tyf = tg[Tpad + t], tzf = tg[2 * Tpad + t];
         const float tnx = tg[3 * Tpad + t], tny = tg[4 * Tpad + t], tnz = tg[5 * Tpad + t];
         const double tx = txf, ty = tyf, tz = tzf;
@@ -2594,13 +2605,21 @@ k_target_accum_pair(const float* __restrict__ tg, int64_t Tpad, const int4* __re
 
 // Union of the aperture masks of shots 2j and 2j + 1 of a batch (the last
 // block of an odd batch: shot 2j alone) and its count per facet block; one
-// workgroup of MASKW threads per (facet block, block of shots)
+// workgroup of MASKW threads per (facet block, block of shots).  Rows of a
+// multi-track batch (trk, rows): a padding row, or a block past its track's
+// mesh, adds nothing (k_illum_mask leaves their masks unwritten).
 __global__ void k_union_mask(const unsigned long long* __restrict__ mask, int nblk, int S,
-                             unsigned long long* __restrict__ umask, int* __restrict__ ucnt) {
+                             unsigned long long* __restrict__ umask, int* __restrict__ ucnt,
+                             const TrackK* __restrict__ trk, const RowK* __restrict__ rows) {
     const int blk = blockIdx.x, j = blockIdx.y, w = threadIdx.x;
     const int s0 = 2 * j;
-    unsigned long long m = mask[((size_t)s0 * nblk + blk) * MASKW + w];
-    if (s0 + 1 < S) m |= mask[((size_t)(s0 + 1) * nblk + blk) * MASKW + w];
+    auto lit = [&](int s) {
+        if (s >= S) return false;
+        return !trk || (rows[s].shot >= 0 && blk < trk[rows[s].trk].nblk);
+    };
+    unsigned long long m = 0;
+    if (lit(s0)) m = mask[((size_t)s0 * nblk + blk) * MASKW + w];
+    if (lit(s0 + 1)) m |= mask[((size_t)(s0 + 1) * nblk + blk) * MASKW + w];
     umask[((size_t)j * nblk + blk) * MASKW + w] = m;
     __shared__ int c_s;
     if (w == 0) c_s = 0;
@@ -3229,6 +3248,7 @@ void merge_pair_lists(const int* lists, size_t pitch, const int* tc, int S,
 // first-shot rows of this launch in item order, then its second-shot rows.
 struct PairChunk {
     std::vector<int4> items;          // (target, row of shot 0, row of shot 1, block)
+    std::vector<int> rshot, ridx;     // per row: its shot, the target's index in its list
     std::vector<int> sg_s, gsb, gse;  // per segment: shot, its first and end slice
     std::vector<int> gb, ge;          // per slice of GS rows: first row, end
     int rows = 0;
@@ -3237,6 +3257,7 @@ struct PairChunk {
 void fill_pair_chunk(const std::vector<std::vector<PairIdx>>& bl, int& jb, size_t& ib, int C,
                      int GS, const int* V, PairChunk& f) {
     f.items.clear();
+    f.rshot.clear(); f.ridx.clear();
     f.sg_s.clear(); f.gsb.clear(); f.gse.clear(); f.gb.clear(); f.ge.clear();
     f.rows = 0;
     f.triples = 0;
@@ -3268,11 +3289,23 @@ void fill_pair_chunk(const std::vector<std::vector<PairIdx>>& bl, int& jb, size_
         }
         if (e == ib) break;  // this launch is full
         const int r = f.rows;
+        f.rshot.resize((size_t)(r + n0 + n1));
+        f.ridx.resize((size_t)(r + n0 + n1));
         int c0 = 0, c1 = 0;
         for (size_t q = ib; q < e; ++q) {
             const PairIdx& p = its[q];
-            f.items.push_back(make_int4(p.t, p.i0 >= 0 ? r + c0++ : -1,
-                                        p.i1 >= 0 ? r + n0 + c1++ : -1, jb));
+            int r0 = -1, r1 = -1;
+            if (p.i0 >= 0) {
+                r0 = r + c0++;
+                f.rshot[r0] = 2 * jb;
+                f.ridx[r0] = p.i0;
+            }
+            if (p.i1 >= 0) {
+                r1 = r + n0 + c1++;
+                f.rshot[r1] = 2 * jb + 1;
+                f.ridx[r1] = p.i1;
+            }
+            f.items.push_back(make_int4(p.t, r0, r1, jb));
         }
         if (n0 > 0) segment(2 * jb, r, r + n0);
         if (n1 > 0) segment(2 * jb + 1, r + n0, r + n0 + n1);
@@ -3692,7 +3725,8 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         const int nbk = (S + 1) / 2;
         if (pairk) {
             hipLaunchKernelGGL(k_union_mask, dim3(nblk, nbk), dim3(MASKW), 0, st, dmask, nblk, S,
-                               W.umask.as<unsigned long long>(), W.ucnt.as<int>());
+                               W.umask.as<unsigned long long>(), W.ucnt.as<int>(),
+                               (const TrackK*)nullptr, (const RowK*)nullptr);
             ERP_CHECK_LAUNCH("k_union_mask");
             hipLaunchKernelGGL(k_scan_blocks, dim3(nbk), dim3(1024), 0, st, W.ucnt.as<int>(),
                                W.uoff.as<int>(), W.U.as<int>(), nblk, (const TrackK*)nullptr,
@@ -3945,7 +3979,8 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                                            (const int4*)ditems, nit, dnext, dshots,
                                            brk.ubase, brk.U, brk.rec, dbounds, dpr, (int)nprisms,
                                            kp, A, dpb, dpw, mesh->frame_ok ? 1 : 0,
-                                           W.oow.as<int>());
+                                           W.oow.as<int>(), (const TrackK*)nullptr,
+                                           (const RowK*)nullptr, (const int*)nullptr);
                         ERP_CHECK_LAUNCH("k_target_accum_pair");
                         PT_.end(st);
                         PT_.begin(ERP_STAGE_TARGET_FFT, st, (double)n);
@@ -4210,11 +4245,11 @@ namespace {
 struct MtKey {
     int R = 0, nblk = 0, nr = 0, Tcap = 0, C = 0, ntrk = 0;
     size_t reccap = 0;
-    bool det = false, s3 = false, targ = false;
+    bool det = false, s3 = false, targ = false, pair = false;
     bool operator==(const MtKey& o) const {
         return R == o.R && nblk == o.nblk && nr == o.nr && Tcap == o.Tcap && C == o.C &&
                ntrk == o.ntrk && reccap == o.reccap && det == o.det && s3 == o.s3 &&
-               targ == o.targ;
+               targ == o.targ && pair == o.pair;
     }
 };
 
@@ -4235,6 +4270,9 @@ struct MtState {
     MtKey key;
     DevBuf trk, rows, shots, cnt, off, V, mask, bounds, base, psurf, srow, refl, refr, Sacc, blu,
         list, tcnt, A, Z, tabs, dtabs, rec;
+    // shot-pair kernel: union masks, counts, offsets, sizes, first records
+    // and first rows per block of two rows; work items and their count
+    DevBuf umask, ucnt, uoff, U, ubase, brows, items;
     MtGraph gA1, gA2, gChunk, gTail;
     void reset_graphs() {
         gA1.reset();
@@ -4245,7 +4283,8 @@ struct MtState {
     void release() {
         reset_graphs();
         DevBuf* b[] = {&trk, &rows, &shots, &cnt, &off, &V, &mask, &bounds, &base, &psurf, &srow,
-                       &refl, &refr, &Sacc, &blu, &list, &tcnt, &A, &Z, &tabs, &dtabs, &rec};
+                       &refl, &refr, &Sacc, &blu, &list, &tcnt, &A, &Z, &tabs, &dtabs, &rec,
+                       &umask, &ucnt, &uoff, &U, &ubase, &brows, &items};
         for (DevBuf* d : b) d->release();
         key = MtKey{};
     }
@@ -4380,6 +4419,12 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
     int C = (int)std::max<int64_t>(1, kPairRowBytes / ((int64_t)P * 8 * 2));
     C = std::min<int>(C, kMaxPairsPerLaunch);
     if (ctx->tune_pairs > 0) C = (int)std::min<int64_t>(C, ctx->tune_pairs);
+    // the shot-pair kernel (default arithmetic, unless ERP_TUNE_SINGLE_SHOT):
+    // blocks of two rows of one track, so R is even and a track with an odd
+    // shot count ends on a padding row
+    const bool pairk = targ && !det && C >= 2 && ctx->tune_single == 0;
+    if (pairk && (R & 1)) ++R;
+    const int nbk = R / 2;
     const int gcap = C / GS + R + 2;
     const int segcap = R;
     const bool direct = ctx->tune_direct > 0;
@@ -4388,7 +4433,7 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
     MtState& m = *ctx->mt;
     MtKey key;
     key.R = R; key.nblk = nblk_max; key.nr = nr; key.Tcap = Tcap; key.C = C; key.ntrk = ntr;
-    key.reccap = m.key.reccap; key.det = det; key.s3 = s3; key.targ = targ;
+    key.reccap = m.key.reccap; key.det = det; key.s3 = s3; key.targ = targ; key.pair = pairk;
     if (!(key == m.key)) {
         m.reset_graphs();
         m.key = key;
@@ -4423,6 +4468,17 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
     // device-written: pair window bases and lengths, the pair counter, the
     // per-pair range kernel spectra
     m.dtabs.ensure(sizeof(int) * (2 * (size_t)C + 64) + sizeof(void*) * (size_t)C);
+    if (pairk) {
+        m.umask.ensure(sizeof(unsigned long long) * (size_t)nbk * nblk_max * MASKW);
+        m.ucnt.ensure(sizeof(int) * (size_t)nbk * nblk_max);
+        m.uoff.ensure(sizeof(int) * (size_t)nbk * nblk_max);
+        m.U.ensure(sizeof(int) * (size_t)nbk);
+        m.ubase.ensure(sizeof(int64_t) * (size_t)nbk);
+        m.brows.ensure(sizeof(RowK) * (size_t)nbk);
+        m.items.ensure(sizeof(int4) * (size_t)C + 64);  // items, then their count
+    }
+    int4* d_items = m.items.as<int4>();
+    int* d_nitems = reinterpret_cast<int*>(d_items + C);
     int* t_pshot = m.tabs.as<int>();
     int* t_ptgt = t_pshot + C;
     int* t_slb = t_ptgt + C;
@@ -4454,14 +4510,21 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
     // all rows, track-major
     std::vector<RowK> all;
     all.reserve((size_t)nrows);
-    for (int k = 0; k < ntr; ++k)
-        for (int64_t s = 0; s < tracks[trk_idx[k]].ns; ++s) all.push_back(RowK{k, (int)s});
+    for (int k = 0; k < ntr; ++k) {
+        const int64_t ns_k = tracks[trk_idx[k]].ns;
+        for (int64_t s = 0; s < ns_k; ++s) all.push_back(RowK{k, (int)s});
+        if (pairk && (ns_k & 1)) all.push_back(RowK{k, -1});
+    }
 
     if (targ) ERP_HIP_TRY(hipMemsetAsync(d_next + 1, 0, sizeof(int), st));  // window check
     std::vector<RowK> hrows(r);
     std::vector<erp_shot> hshots(r);
-    std::vector<int> hV(r), htc(2 * r), tc1(r);
-    std::vector<int64_t> hbase(r);
+    std::vector<int> hV(r), htc(2 * r), tc1(r), hU(nbk);
+    std::vector<int64_t> hbase(r), hubase(nbk);
+    std::vector<RowK> hbrows(nbk);
+    std::vector<std::vector<PairIdx>> bl;
+    PairChunk pc;
+    std::vector<int4> hitems;
     std::vector<int> htab(tab_ints);
     StageTimer none_;
     StageTimer& PT_ = ctx->prof.on ? ctx->prof : none_;
@@ -4479,6 +4542,12 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
                                    hipMemcpyHostToDevice, st));
         ERP_HIP_TRY(hipMemcpyAsync(m.shots.p, hshots.data(), sizeof(erp_shot) * r,
                                    hipMemcpyHostToDevice, st));
+        if (pairk) {
+            // a block's rows share a track: its first row names it
+            for (int j = 0; j < nbk; ++j) hbrows[j] = hrows[2 * j];
+            ERP_HIP_TRY(hipMemcpyAsync(m.brows.p, hbrows.data(), sizeof(RowK) * nbk,
+                                       hipMemcpyHostToDevice, st));
+        }
 
         // A1: aperture masks, counts
         PT_.begin(ERP_STAGE_MASK, st, 0.0);
@@ -4491,9 +4560,23 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
             hipLaunchKernelGGL(k_scan_blocks, dim3(R), dim3(1024), 0, st, m.cnt.as<int>(),
                                m.off.as<int>(), dV, nblk_max, dtrk, drows);
             ERP_CHECK_LAUNCH("k_scan_blocks");
+            if (pairk) {
+                hipLaunchKernelGGL(k_union_mask, dim3(nblk_max, nbk), dim3(MASKW), 0, st,
+                                   m.mask.as<unsigned long long>(), nblk_max, R,
+                                   m.umask.as<unsigned long long>(), m.ucnt.as<int>(), dtrk,
+                                   drows);
+                ERP_CHECK_LAUNCH("k_union_mask");
+                hipLaunchKernelGGL(k_scan_blocks, dim3(nbk), dim3(1024), 0, st, m.ucnt.as<int>(),
+                                   m.uoff.as<int>(), m.U.as<int>(), nblk_max, dtrk,
+                                   m.brows.as<RowK>());
+                ERP_CHECK_LAUNCH("k_scan_blocks");
+            }
         });
         PT_.end(st);
         ERP_HIP_TRY(hipMemcpyAsync(hV.data(), dV, sizeof(int) * r, hipMemcpyDeviceToHost, st));
+        if (pairk)
+            ERP_HIP_TRY(hipMemcpyAsync(hU.data(), m.U.p, sizeof(int) * nbk, hipMemcpyDeviceToHost,
+                                       st));
         ERP_HIP_TRY(hipStreamSynchronize(st));
 
         // facet records of the rows whose track has targets
@@ -4503,19 +4586,48 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
             const bool keep = hrows[i].shot >= 0 && tk[hrows[i].trk].T > 0;
             if (keep && hV[i] >= (1 << 28))
                 throw ErpError{ERP_E_INVALID, "a shot illuminates 2^28 facets or more"};
-            if (keep) Vtot += hV[i];
+            if (keep && !pairk) Vtot += hV[i];
         }
+        if (pairk) {
+            // block records: the union of each block's two rows
+            for (int j = 0; j < nbk; ++j) {
+                hubase[j] = Vtot;
+                const bool keep = hrows[2 * j].shot >= 0 && tk[hrows[2 * j].trk].T > 0;
+                if (keep && hU[j] >= (1 << 28))
+                    throw ErpError{ERP_E_INVALID, "two shots illuminate 2^28 facets or more"};
+                if (keep) Vtot += hU[j];
+            }
+            ERP_HIP_TRY(hipMemcpyAsync(m.ubase.p, hubase.data(), sizeof(int64_t) * nbk,
+                                       hipMemcpyHostToDevice, st));
+        }
+        auto rnd = [](size_t n) { return (n + 255) & ~(size_t)255; };
         if (targ && (size_t)Vtot > m.key.reccap) {
             // more records than the buffers hold: grow by a quarter more, and
             // capture again the graphs that read the record arrays
             const size_t cap = (size_t)Vtot + (size_t)Vtot / 4 + 1024;
-            m.rec.ensure(cap * 8 + 3 * cap * 16 + cap * 4 + 5 * 256);
+            if (pairk) m.rec.ensure(4 * rnd(cap * 16) + rnd(cap * 4) + rnd(cap * 8));
+            else m.rec.ensure(cap * 8 + 3 * cap * 16 + cap * 4 + 5 * 256);
             m.key.reccap = cap;
             m.gA2.reset();
             m.gChunk.reset();
         }
         Rec rec{};
-        if (targ) {
+        BlkRec brk{};
+        if (targ && pairk) {
+            const size_t cap = m.key.reccap;
+            char* pb = m.rec.as<char>();
+            auto take = [&](size_t n) { char* q = pb; pb += rnd(n); return q; };
+            brk.rec.qa = (float4*)take(cap * 16);
+            brk.rec.qb = (float4*)take(cap * 16);
+            brk.rec.qc = (float4*)take(cap * 16);
+            brk.rec.qd = (float*)take(cap * 4);
+            brk.rec.itd = (double2*)take(cap * 16);
+            brk.rec.amp = (float2*)take(cap * 8);
+            brk.umask = m.umask.as<unsigned long long>();
+            brk.uoff = m.uoff.as<int>();
+            brk.U = m.U.as<int>();
+            brk.ubase = m.ubase.as<int64_t>();
+        } else if (targ) {
             const size_t cap = m.key.reccap;
             char* pb = m.rec.as<char>();
             auto take = [&](size_t n) { char* q = pb; pb += (n + 255) & ~(size_t)255; return q; };
@@ -4534,6 +4646,11 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
         for (size_t i = 0; i < r; ++i) Vsum += hV[i];
         PT_.begin(ERP_STAGE_SURFACE, st, Vsum);
         mt_launch(m.gA2, direct, st, [&] {
+            if (pairk) {
+                hipLaunchKernelGGL(k_fill_itd, dim3(grid_for(m.key.reccap)), dim3(256), 0, st,
+                                   brk.rec.itd, (int64_t)m.key.reccap);
+                ERP_CHECK_LAUNCH("k_fill_itd");
+            }
             zero(m.psurf.as<float2>(), r * P, st);
             hipLaunchKernelGGL(k_init_bounds, dim3((R + 63) / 64), dim3(64), 0, st, dbounds, R);
             ERP_CHECK_LAUNCH("k_init_bounds");
@@ -4542,7 +4659,7 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
                                (const float*)nullptr, (int64_t)0, (int64_t)0, dshots,
                                m.mask.as<unsigned long long>(), m.off.as<int>(), dV, dbase,
                                nblk_max, kp0, rec, targ ? 1 : 0, 1, srow, (int64_t)0, dbounds,
-                               det ? 1 : 0, smax, 0, 0, 0, dtrk, drows, BlkRec{});
+                               det ? 1 : 0, smax, 0, 0, 0, dtrk, drows, brk);
             ERP_CHECK_LAUNCH("k_surface");
             if (det) {
                 hipLaunchKernelGGL(k_surface, dim3(nblk_max, R), dim3(NT), 0, st,
@@ -4587,11 +4704,84 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
         ERP_HIP_TRY(hipStreamSynchronize(st));
         for (size_t i = 0; i < r; ++i) tc1[i] = htc[2 * i];
 
-        // chunks: launches of at most C pairs in (row, target) order
         float2* A = m.A.as<float2>();
         float2* Z = m.Z.as<float2>();
+        if (pairk) {
+            // shot-pair kernel: items of a target and a block's two rows, from
+            // the rows' target lists; launches of whole items of at most C rows
+            int maxc = 0;
+            for (size_t i = 0; i < r; ++i) maxc = std::max(maxc, htc[2 * i]);
+            std::vector<int> hl(r * (size_t)maxc);
+            if (maxc > 0) {
+                ERP_HIP_TRY(hipMemcpy2DAsync(hl.data(), sizeof(int) * maxc, list1,
+                                             sizeof(int) * Tcap, sizeof(int) * maxc, r,
+                                             hipMemcpyDeviceToHost, st));
+                ERP_HIP_TRY(hipStreamSynchronize(st));
+            }
+            merge_pair_lists(hl.data(), (size_t)maxc, htc.data(), R, bl);
+            int jb = 0;
+            size_t ib = 0;
+            for (;;) {
+                fill_pair_chunk(bl, jb, ib, C, GS, hV.data(), pc);
+                const int n = pc.rows;
+                if (n == 0) break;
+                const int nsl = (int)pc.gb.size(), nseg = (int)pc.sg_s.size();
+                const int nit = (int)pc.items.size();
+                if (nsl > gcap || nseg > segcap)
+                    throw ErpError{ERP_E_INVALID, "multi-track launch tables overflow"};
+                std::fill(htab.begin(), htab.end(), 0);
+                std::copy(pc.rshot.begin(), pc.rshot.end(), htab.begin());
+                std::copy(pc.ridx.begin(), pc.ridx.end(), htab.begin() + C);
+                std::copy(pc.gb.begin(), pc.gb.end(), htab.begin() + 2 * C);
+                std::copy(pc.ge.begin(), pc.ge.end(), htab.begin() + 2 * C + gcap);
+                const size_t so = 2 * (size_t)C + 2 * gcap;
+                for (int k = 0; k < nseg; ++k) {
+                    htab[so + k] = pc.sg_s[k];
+                    htab[so + segcap + k] = pc.gsb[k];
+                    htab[so + 2 * segcap + k] = pc.gse[k];
+                    const float sc = tk[hrows[pc.sg_s[k]].trk].tscale;
+                    std::memcpy(&htab[so + 3 * segcap + 1 + k], &sc, sizeof(float));
+                }
+                htab[so + 3 * segcap] = n;
+                ERP_HIP_TRY(hipMemcpyAsync(m.tabs.p, htab.data(), sizeof(int) * tab_ints,
+                                           hipMemcpyHostToDevice, st));
+                hitems.resize((size_t)C + 1);
+                std::copy(pc.items.begin(), pc.items.end(), hitems.begin());
+                hitems[C] = make_int4(nit, 0, 0, 0);
+                ERP_HIP_TRY(hipMemcpyAsync(d_items, hitems.data(), sizeof(int4) * nit,
+                                           hipMemcpyHostToDevice, st));
+                ERP_HIP_TRY(hipMemcpyAsync(d_items + C, hitems.data() + C, sizeof(int4),
+                                           hipMemcpyHostToDevice, st));
+                PT_.begin(ERP_STAGE_TARGET_ACCUM, st, pc.triples);
+                mt_launch(m.gChunk, direct, st, [&] {
+                    // range kernel spectra per row, the item counter cleared
+                    hipLaunchKernelGGL(k_pair_targets_rows, dim3((C + 255) / 256), dim3(256), 0,
+                                       st, t_ptgt, t_pshot, list1, Tcap, t_n, d_next, d_G, dtrk,
+                                       drows);
+                    ERP_CHECK_LAUNCH("k_pair_targets_rows");
+                    auto kern = s3 ? k_target_accum_pair<true, true>
+                                   : k_target_accum_pair<false, true>;
+                    // persistent: one workgroup per CU takes items until *d_nitems
+                    hipLaunchKernelGGL(kern, dim3(std::max(1, std::min(C, ctx->num_cu))),
+                                       dim3(NTB), 0, st, (const float*)nullptr, (int64_t)0,
+                                       (const int4*)d_items, 0, d_next, dshots,
+                                       (const int64_t*)m.ubase.as<int64_t>(),
+                                       (const int*)m.U.as<int>(), brk.rec, dbounds,
+                                       (const erp_prism*)nullptr, 0, kp0, A, d_pbase, d_pwin, 0,
+                                       d_next + 1, dtrk, drows, (const int*)d_nitems);
+                    ERP_CHECK_LAUNCH("k_target_accum_pair");
+                    target_pairs_half(ctx, planP, A, d_pbase, d_pwin, nr, C, nullptr, Z, st, d_G,
+                                      t_n);
+                    target_sum_half(planP, Z, A + (size_t)C * P, t_slb, t_sle, gcap, t_sgs, t_sgb,
+                                    t_sge, segcap, m.Sacc.as<float2>(), 1.0f, st, t_scale, false);
+                });
+                PT_.end(st);
+            }
+        }
+
+        // chunks: launches of at most C pairs in (row, target) order
         int cs = 0, cj = 0;
-        for (;;) {
+        for (; !pairk;) {
             fill_chunk(tc1.data(), R, cs, cj, C, GS, det, cf);
             const int n = cf.n();
             if (n == 0) break;

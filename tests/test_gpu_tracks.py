@@ -14,7 +14,8 @@ import numpy as np
 import pytest
 
 from europaraypaths_amd._abi import (ERP_RUN_DETERMINISTIC, ERP_TUNE_DIRECT_LAUNCH,
-                                    ERP_TUNE_PAIRS_PER_LAUNCH, ERP_TUNE_TRACK_ROWS)
+                                    ERP_TUNE_PAIRS_PER_LAUNCH, ERP_TUNE_SINGLE_SHOT,
+                                    ERP_TUNE_TRACK_ROWS)
 from oracle import oracle
 from tests.helpers import compare_traces
 from tests.test_gpu_parity import scene
@@ -104,6 +105,33 @@ def test_tracks_match_the_oracle(small_batches):
             np.testing.assert_array_equal(gv, ov, err_msg=name)
             print(f"  {name}: {compare_traces(g, o)}", flush=True)
     finally:
+        up.release()
+
+
+def test_tracks_shot_pair_and_single_shot_kernels(gpu_ctx):
+    """The default arithmetic's multi-track batches pair each track's
+    consecutive rows for the shot-pair kernel (a track with an odd shot count
+    ends on a padding row; 15 rows per batch become 16) and replay the
+    captured graphs; with ERP_TUNE_SINGLE_SHOT every (row, target) pair runs
+    on its own.  Both against the FP64 oracle, replays included."""
+    tracks = _tracks()
+    up = _Uploaded(gpu_ctx, tracks)
+    try:
+        gpu_ctx.set_tuning(ERP_TUNE_TRACK_ROWS, 15)
+        gpu_ctx.set_tuning(ERP_TUNE_PAIRS_PER_LAUNCH, 60)
+        runs = []
+        for single in (0, 0, 1):
+            gpu_ctx.set_tuning(ERP_TUNE_SINGLE_SHOT, single)
+            print(f"multi-track run, single-shot kernel {single}", flush=True)
+            runs.append(gpu_ctx.run_tracks(up.jobs))
+        for (name, par, soa, tg, shots), (e, _, _, _), *res in zip(tracks, up.jobs, *runs):
+            o, ov = oracle.run(e, soa, tg, shots)
+            for g, gv in res:
+                np.testing.assert_array_equal(gv, ov, err_msg=name)
+                compare_traces(g, o)
+    finally:
+        for key in (ERP_TUNE_TRACK_ROWS, ERP_TUNE_PAIRS_PER_LAUNCH, ERP_TUNE_SINGLE_SHOT):
+            gpu_ctx.set_tuning(key, 0)
         up.release()
 
 
