@@ -591,14 +591,17 @@ struct Rec {
 // Facet records of a block of two consecutive shots for the shot-pair target
 // kernel: the facets that either shot illuminates (the union of the two
 // masks), laid out per block as rec_slot lays out one shot's, with the
-// geometry once and the shot-dependent source leg per shot.
+// geometry once and the shot-dependent source leg per shot: Itd, and the
+// gain F Dmax / pi^4 times the source leg's phasor exp(-i 2 pi Itd / lam')
+// (lam' = 1 / inv_cyc), so that a term's phasor is this one times the
+// target leg's.
 struct BRec {
     float4* qa;    // (x, y, z, 0)
     float4* qb;    // (nx, ny, nz, ux)
     float4* qc;    // (uy, uz, vx, vy)
     float* qd;     // vz
     double2* itd;  // Itd of the block's first and second shot; -1: not lit by it
-    float2* amp;   // F Dmax / pi^4 of the first and second shot
+    float4* src;   // gain x source phasor of the first shot (x, y), the second (z, w)
 };
 // Where k_surface writes block records: rec.qa null = none.  umask / uoff:
 // the union masks and their exclusive block offsets per block of shots,
@@ -918,7 +921,10 @@ __global__ void __launch_bounds__(NT) k_surface(
                 brk.rec.qc[u] = make_float4(g[7], g[8], g[9], g[10]);
                 brk.rec.qd[u] = g[11];
                 reinterpret_cast<double*>(brk.rec.itd + u)[k] = Itd;
-                reinterpret_cast<float*>(brk.rec.amp + u)[k] = Fr * (kp.Dmax * 0.0102659823f);
+                const float gain = Fr * (kp.Dmax * 0.0102659823f);
+                const float2 ph = phasor_rc_hw(Itd, kp.inv_cyc);
+                reinterpret_cast<float2*>(brk.rec.src + u)[k] = make_float2(ph.x * gain,
+                                                                            ph.y * gain);
             }
             if (rec.geo) {
 #pragma unroll
@@ -2318,11 +2324,12 @@ k_target_accum_fast(const float* __restrict__ tg0, int64_t Tpad0, const int* __r
 // ACC_F64 path, with the product of the gain and the four sinc factors
 // formed as gain x (sincs x loss) instead of (gain x sincs) x loss.
 
-// the target leg of one facet: Ttd, the part of it below the interface times
-// Ttd (L3, second medium only) and the amplitude without the shot's gain
+// the target leg of one facet: Ttd, its optical path RT = n2 Ttd (+ (n3 -
+// n2) times the part below a second interface) and its amplitude times its
+// phasor exp(-i 2 pi RT / lam'), without the shot's gain
 struct TLeg {
-    double Ttd, L3;
-    float G;
+    double Ttd, RT;
+    float2 Gc;
 };
 
 // one block record in registers
@@ -2330,7 +2337,7 @@ struct BRecV {
     float4 qa, qb, qc;
     float qd;
     double2 itd;
-    float2 amp;
+    float4 src;
 };
 // the records of one block, based at its first (uniform over the workgroup);
 // record u at 32-bit byte offsets (u < 2^27)
@@ -2340,7 +2347,7 @@ struct BRecBlk {
     const char* qc;
     const char* qd;
     const char* itd;
-    const char* amp;
+    const char* src;
 };
 __device__ __forceinline__ BRecV load_brec(const BRecBlk& rb, unsigned u) {
     BRecV v;
@@ -2350,7 +2357,7 @@ __device__ __forceinline__ BRecV load_brec(const BRecBlk& rb, unsigned u) {
     v.qc = *reinterpret_cast<const float4*>(rb.qc + o16);
     v.qd = *reinterpret_cast<const float*>(rb.qd + (u << 2));
     v.itd = *reinterpret_cast<const double2*>(rb.itd + o16);
-    v.amp = *reinterpret_cast<const float2*>(rb.amp + (u << 3));
+    v.src = *reinterpret_cast<const float4*>(rb.src + o16);
     return v;
 }
 
@@ -2369,7 +2376,8 @@ __device__ __forceinline__ TLeg tri_leg(const BRecV& rv, bool frame_ok, double t
     const double rd = (double)rs, hr = 0.5 * rd;
     const double y0 = t2 * rd;
     o.Ttd = fma(fma(-y0, y0, t2), hr, y0);
-    o.L3 = (S3 && st.deep) ? o.Ttd * below_frac(fz, st) : 0.0;
+    o.RT = o.Ttd * kp.n_d;
+    if (S3 && st.deep) o.RT = fma(o.Ttd * below_frac(fz, st), st.dn, o.RT);
     const float nx = rv.qb.x, ny = rv.qb.y, nz = rv.qb.z;
     const float ux = rv.qb.w, uy = rv.qc.x, uz = rv.qc.y;
     const float vx = rv.qc.z, vy = rv.qc.w, vz = rv.qd;
@@ -2405,30 +2413,33 @@ __device__ __forceinline__ TLeg tri_leg(const BRecV& rv, bool frame_ok, double t
     float g = fabsf((n1 * n2) * (n3 * n4)) * __builtin_amdgcn_rcpf(den) * rs;
     if (S3 && !lossless && st.deep && (double)fz > st.zi) g *= st.tsq;
     if (!lossless) {
+        // exp(-2 (alpha path + (ks cT)^2)) as exp2 with -2 log2(e) folded into
+        // the coefficients
+        constexpr float m2l2e = -2.f * 1.44269504088896341f;
         float expo = 0.f, tot = 0.f;
         for (int k = 0; k < np; ++k) {
             const float d = prism_len(fx, fy, fz, Ix, Iy, Iz, prisms[k], (float)o.Ttd);
             tot += d;
-            expo += prisms[k].alpha * d;
+            expo += (m2l2e * prisms[k].alpha) * d;
         }
-        expo += kp.alpha2 * ((float)o.Ttd - tot);
-        const float kc = kp.ks * cT;
-        g *= __expf(-2.f * (expo + kc * kc));
+        expo = fmaf(m2l2e * kp.alpha2, (float)o.Ttd - tot, expo);
+        const float kc2 = (m2l2e * kp.ks * kp.ks) * cT;
+        g *= __builtin_amdgcn_exp2f(fmaf(kc2, cT, expo));
     }
-    o.G = g;
+    const float2 ph = phasor_rc_hw(o.RT, kp.inv_cyc);
+    o.Gc = make_float2(ph.x * g, ph.y * g);
     return o;
 }
 
-// the source leg of one shot: range Itd + n Ttd (+ the second medium's
-// excess), bin and weight as in tri_eval (the exact FP64 sum within 1e-7 bin
-// of a bin edge), phase, amplitude gain x G
+// the source leg of one shot: range Itd + RT, bin and weight as in tri_eval
+// (the exact FP64 sum within 1e-7 bin of a bin edge), the term the shot's
+// gain x source phasor times the target leg's
 template <bool S3>
-__device__ __forceinline__ Tri tri_shot(const TLeg& g, double itd, float gain, double rst,
+__device__ __forceinline__ Tri tri_shot(const TLeg& g, double itd, float2 src, double rst,
                                         float fx, float fy, float fz, double tx, double ty,
                                         double tz, const KPar& kp, const Strat& st, int nr) {
     Tri o;
-    double rngt = fma(g.Ttd, kp.n_d, itd);
-    if (S3 && st.deep) rngt = fma(g.L3, st.dn, rngt);
+    double rngt = itd + g.RT;
     const double xq = fma(rngt, kp.inv_dr, -rst * kp.inv_dr);
     o.b = (int)xq;
     o.w = xq - (double)o.b;
@@ -2450,9 +2461,7 @@ __device__ __forceinline__ Tri tri_shot(const TLeg& g, double itd, float gain, d
     } else {
         o.valid = (xq > -1.0) && (xq < (double)nr);
     }
-    const float2 ph = phasor_rc_hw(rngt, kp.inv_cyc);
-    const float amp = gain * g.G;
-    o.c = make_float2(ph.x * amp, ph.y * amp);
+    o.c = cmul(src, g.Gc);
     return o;
 }
 
@@ -2561,7 +2570,7 @@ k_target_accum_pair(const float* __restrict__ tg0, int64_t Tpad0, const int4* __
                              reinterpret_cast<const char*>(rec.qc + r0),
                              reinterpret_cast<const char*>(rec.qd + r0),
                              reinterpret_cast<const char*>(rec.itd + r0),
-                             reinterpret_cast<const char*>(rec.amp + r0)};
+                             reinterpret_cast<const char*>(rec.src + r0)};
             const unsigned ul = (unsigned)(U - 1);
             auto step = [&](const BRecV& a) {
                 const TLeg g = tri_leg<S3>(a, frame_ok, tx, ty, tz, tnx, tny, tnz, tn_up,
@@ -2570,8 +2579,10 @@ k_target_accum_pair(const float* __restrict__ tg0, int64_t Tpad0, const int4* __
                 for (int k = 0; k < 2; ++k) {
                     const double itd = k ? a.itd.y : a.itd.x;
                     if (row[k] < 0 || !(itd >= 0.0)) continue;
-                    const Tri o = tri_shot<S3>(g, itd, k ? a.amp.y : a.amp.x, rst[k], a.qa.x,
-                                               a.qa.y, a.qa.z, tx, ty, tz, kp, st, nr);
+                    const float2 src = k ? make_float2(a.src.z, a.src.w)
+                                         : make_float2(a.src.x, a.src.y);
+                    const Tri o = tri_shot<S3>(g, itd, src, rst[k], a.qa.x, a.qa.y, a.qa.z, tx,
+                                               ty, tz, kp, st, nr);
                     if (o.valid) tri_add<ACC_F64>(o, base[k], len[k], nr, win[k], rowp[k], flag[k]);
                 }
             };
@@ -3765,7 +3776,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         Rec rec{};
         BlkRec brk{};
         if (pairk && Vtot > 0) {
-            // block records: 76 bytes per facet of a block's union
+            // block records: 84 bytes per facet of a block's union
             hubase[0] = 0;
             for (int j = 0; j < nbk; ++j) {
                 if (hU[j] >= (1 << 28))
@@ -3774,8 +3785,8 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
             }
             const size_t cap = (size_t)hubase[nbk];
             auto rnd = [](size_t n) { return (n + 255) & ~(size_t)255; };
-            const size_t bytes = 4 * rnd(cap * 16) + rnd(cap * 4) + rnd(cap * 8);
-            W.brec.ensure_grow(bytes, (size_t)nbk * (size_t)F * 76 + 256 * 6);
+            const size_t bytes = 5 * rnd(cap * 16) + rnd(cap * 4);
+            W.brec.ensure_grow(bytes, (size_t)nbk * (size_t)F * 84 + 256 * 6);
             char* b = W.brec.as<char>();
             auto take = [&](size_t n) { char* r = b; b += rnd(n); return r; };
             BRec& br = brk.rec;
@@ -3784,7 +3795,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
             br.qc = (float4*)take(cap * 16);
             br.qd = (float*)take(cap * 4);
             br.itd = (double2*)take(cap * 16);
-            br.amp = (float2*)take(cap * 8);
+            br.src = (float4*)take(cap * 16);
             brk.umask = W.umask.as<unsigned long long>();
             brk.uoff = W.uoff.as<int>();
             brk.U = W.U.as<int>();
@@ -4601,12 +4612,18 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
                                        hipMemcpyHostToDevice, st));
         }
         auto rnd = [](size_t n) { return (n + 255) & ~(size_t)255; };
-        if (targ && (size_t)Vtot > m.key.reccap) {
+        // bytes of cap records in this group's layout: block records (84 B)
+        // or a shot's packed records (60 B); one buffer serves both, so a
+        // capacity reached in one layout does not hold in the other
+        auto rec_bytes = [&](size_t cap) {
+            return pairk ? 5 * rnd(cap * 16) + rnd(cap * 4)
+                         : cap * 8 + 3 * cap * 16 + cap * 4 + 5 * 256;
+        };
+        if (targ && ((size_t)Vtot > m.key.reccap || rec_bytes(m.key.reccap) > m.rec.bytes)) {
             // more records than the buffers hold: grow by a quarter more, and
             // capture again the graphs that read the record arrays
-            const size_t cap = (size_t)Vtot + (size_t)Vtot / 4 + 1024;
-            if (pairk) m.rec.ensure(4 * rnd(cap * 16) + rnd(cap * 4) + rnd(cap * 8));
-            else m.rec.ensure(cap * 8 + 3 * cap * 16 + cap * 4 + 5 * 256);
+            const size_t cap = std::max((size_t)Vtot + (size_t)Vtot / 4 + 1024, m.key.reccap);
+            m.rec.ensure(rec_bytes(cap));
             m.key.reccap = cap;
             m.gA2.reset();
             m.gChunk.reset();
@@ -4622,7 +4639,7 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
             brk.rec.qc = (float4*)take(cap * 16);
             brk.rec.qd = (float*)take(cap * 4);
             brk.rec.itd = (double2*)take(cap * 16);
-            brk.rec.amp = (float2*)take(cap * 8);
+            brk.rec.src = (float4*)take(cap * 16);
             brk.umask = m.umask.as<unsigned long long>();
             brk.uoff = m.uoff.as<int>();
             brk.U = m.U.as<int>();

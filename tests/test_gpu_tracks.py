@@ -135,6 +135,35 @@ def test_tracks_shot_pair_and_single_shot_kernels(gpu_ctx):
         up.release()
 
 
+def test_tracks_record_buffer_grows_for_each_layout():
+    """One context's multi-track record buffer serves both record layouts:
+    a shot's packed records (deterministic mode, 60 B each) and the
+    shot-pair kernel's block records (84 B each).  A deterministic call that
+    sizes it, then a default call with fewer records than that capacity,
+    then the deterministic call again: the buffer must grow for the larger
+    layout (a capacity counted in records alone would not), and every call
+    must match the oracle and the first bit for bit."""
+    from europaraypaths_amd._native import Context
+    ctx = Context(0)
+    tracks = _tracks()[:4]
+    up = _Uploaded(ctx, tracks)
+    try:
+        print("deterministic, default, deterministic on a fresh context", flush=True)
+        a = ctx.run_tracks(up.jobs, flags=ERP_RUN_DETERMINISTIC)
+        b = ctx.run_tracks(up.jobs)
+        c = ctx.run_tracks(up.jobs, flags=ERP_RUN_DETERMINISTIC)
+        for (name, par, soa, tg, shots), (e, *_), (ga, _), (gb, vb), (gc, _) in zip(
+                tracks, up.jobs, a, b, c):
+            o, ov = oracle.run(e, soa, tg, shots)
+            np.testing.assert_array_equal(vb, ov, err_msg=name)
+            compare_traces(ga, o)
+            compare_traces(gb, o)
+            assert np.array_equal(ga.view(np.uint32), gc.view(np.uint32)), name
+    finally:
+        up.release()
+        ctx.close()
+
+
 def test_tracks_deterministic_equal_each_track_alone(small_batches):
     ctx = small_batches
     tracks = _tracks()
