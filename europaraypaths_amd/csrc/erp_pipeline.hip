@@ -2403,14 +2403,25 @@ __device__ __forceinline__ TLeg tri_leg(const BRecV& rv, bool frame_ok, double t
     const float rinv = az ? __builtin_amdgcn_rsqf(q2) : 0.f;
     const float cph = az ? Iv * rinv : 1.f;
     const float sph = Iu * rinv;
-    const float a = kp.piLlam;
+    // pi-sincs from half their arguments h = x / 2 in revolutions, where the
+    // hardware sine's own reduction is as accurate as an explicit one
+    // (1.2e-7 absolute up to |h| = 260, checked against double on gfx950;
+    // the host takes this kernel only for pi fs / lam <= 512, so |h| <= 256):
+    // sin(pi |x|) / |x| = sin(2 pi |h|) / (2 |h|), the four factors 2 in
+    // the final 1/16
+    const float a = 0.5f * kp.piLlam;
     float n1, d1, n2, d2, n3, d3, n4, d4;
-    pisinc_nd_hw(a * (sT * cph), &n1, &d1);
-    pisinc_nd_hw(a * (sT * sph), &n2, &d2);
-    pisinc_nd_hw(a * sTT * cph, &n3, &d3);
-    pisinc_nd_hw(a * sTT * sph, &n4, &d4);
+    auto pisinc_h = [](float h, float* num, float* den) {
+        const float ha = fmaxf(fabsf(h), 5e-7f);
+        *num = __builtin_amdgcn_sinf(ha);
+        *den = ha;
+    };
+    pisinc_h(a * (sT * cph), &n1, &d1);
+    pisinc_h(a * (sT * sph), &n2, &d2);
+    pisinc_h(a * sTT * cph, &n3, &d3);
+    pisinc_h(a * sTT * sph, &n4, &d4);
     const float den = fabsf((d1 * d2) * (d3 * d4));
-    float g = fabsf((n1 * n2) * (n3 * n4)) * __builtin_amdgcn_rcpf(den) * rs;
+    float g = fabsf((n1 * n2) * (n3 * n4)) * __builtin_amdgcn_rcpf(den) * (0.0625f * rs);
     if (S3 && !lossless && st.deep && (double)fz > st.zi) g *= st.tsq;
     if (!lossless) {
         // exp(-2 (alpha path + (ks cT)^2)) as exp2 with -2 log2(e) folded into
@@ -3690,7 +3701,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     // allocation-free entry point, unless ERP_TUNE_SINGLE_SHOT asks for the
     // single-shot kernel
     const bool pairk = half && !det && ph == PH_F64 && acc == ACC_F64 && !any_t0 && !arena &&
-                       C >= 2 && ctx->tune_single == 0;
+                       C >= 2 && ctx->tune_single == 0 && kp.piLlam <= 512.f;
     const int nbk_max = (Smax + 1) / 2;
     if (pairk) {
         W.umask.ensure(sizeof(unsigned long long) * (size_t)nbk_max * nblk * MASKW);
@@ -4433,7 +4444,8 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
     // the shot-pair kernel (default arithmetic, unless ERP_TUNE_SINGLE_SHOT):
     // blocks of two rows of one track, so R is even and a track with an odd
     // shot count ends on a padding row
-    const bool pairk = targ && !det && C >= 2 && ctx->tune_single == 0;
+    bool pairk = targ && !det && C >= 2 && ctx->tune_single == 0;
+    for (int k = 0; k < ntr; ++k) pairk = pairk && tk[k].kp.piLlam <= 512.f;
     if (pairk && (R & 1)) ++R;
     const int nbk = R / 2;
     const int gcap = C / GS + R + 2;
