@@ -59,6 +59,15 @@ KERNELS = {"target_accum": "k_target_accum_pair", "target_fft": "k_tpair_half",
            "surface": "k_surface"}
 
 
+def kernels_of(b):
+    """The hot kernels of a bench run: its JSON line names the target
+    kernel it ran (the shot-pair kernel, or with --single-shot the
+    single-shot one)."""
+    k = dict(KERNELS)
+    k["target_accum"] = b.get("config", {}).get("target_kernel", k["target_accum"])
+    return k
+
+
 def hip():
     for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
         try:
@@ -123,6 +132,7 @@ def trace_summary(trace_dir, bench_json, workload_cmd, out_dir):
     with open(bench_json) as f:
         b = json.loads([l for l in f if l.startswith("{")][-1])
     wl = b["config"]["workload"]
+    kernels = kernels_of(b)
     stats = _csv_rows(trace_dir, "kernel_stats.csv")
     trace = _csv_rows(trace_dir, "kernel_trace.csv")
     if not stats or not trace:
@@ -134,7 +144,7 @@ def trace_summary(trace_dir, bench_json, workload_cmd, out_dir):
             traffic = json.load(f)
     except (OSError, ValueError):
         traffic = {}
-    for stage, kern in KERNELS.items():
+    for stage, kern in kernels.items():
         ms, launches, units = (b["stage_ms"][stage], b["stage_launches"][stage],
                                b["stage_units"][stage])
         st = [r for r in stats if kern in r["Name"]]
@@ -198,6 +208,7 @@ def ingest_pmc(fetch_dir, write_dir, bench_json, out_path):
     the 256 MiB calibration copy."""
     with open(bench_json) as f:
         b = json.loads([l for l in f if l.startswith("{")][-1])
+    kernels = kernels_of(b)
     sums = {}
     calib = {}
     for d, cname in ((fetch_dir, "FETCH_SIZE"), (write_dir, "WRITE_SIZE")):
@@ -208,7 +219,7 @@ def ingest_pmc(fetch_dir, write_dir, bench_json, out_path):
             k = r["Kernel_Name"]
             if "copyBuffer" in k:
                 calib[cname] = max(calib.get(cname, 0.0), v)
-            for stage, kern in KERNELS.items():
+            for stage, kern in kernels.items():
                 if kern in k:
                     sums.setdefault(kern, {"FETCH_SIZE": 0.0, "WRITE_SIZE": 0.0, "n": 0,
                                            "stage": stage})
@@ -218,9 +229,15 @@ def ingest_pmc(fetch_dir, write_dir, bench_json, out_path):
         raise SystemExit("calibration copy not found in the counter runs")
     kib = CALIB_BYTES / 1024.0
     sf, sw = kib / calib["FETCH_SIZE"], kib / calib["WRITE_SIZE"]
-    out = {"_calibration": {"copy_bytes": CALIB_BYTES, "fetch_kib_read": calib["FETCH_SIZE"],
-                            "write_kib_read": calib["WRITE_SIZE"], "fetch_scale": sf,
-                            "write_scale": sw}}
+    # kernels of other runs (the single-shot target kernel) keep their entries
+    try:
+        with open(out_path) as f:
+            out = json.load(f)
+    except (OSError, ValueError):
+        out = {}
+    out["_calibration"] = {"copy_bytes": CALIB_BYTES, "fetch_kib_read": calib["FETCH_SIZE"],
+                           "write_kib_read": calib["WRITE_SIZE"], "fetch_scale": sf,
+                           "write_scale": sw}
     for kern, e in sums.items():
         units = b["stage_units"][e["stage"]]
         byts = (e["FETCH_SIZE"] * sf + e["WRITE_SIZE"] * sw) * 1024.0
@@ -373,18 +390,19 @@ def counters_summary(dirs, bench_json, out_path):
     and LDS bank-conflict cycles per active LDS cycle."""
     with open(bench_json) as f:
         b = json.loads([l for l in f if l.startswith("{")][-1])
+    kernels = kernels_of(b)
     agg, disp, names = {}, {}, {}
     for d in dirs:
         for r in _counter_rows(d):
             names.setdefault(d, set()).add(r["Counter_Name"])
-            for stage, kern in KERNELS.items():
+            for stage, kern in kernels.items():
                 if kern in r["Kernel_Name"]:
                     c = agg.setdefault(kern, {})
                     c[r["Counter_Name"]] = c.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
                     disp.setdefault((kern, r["Counter_Name"]), set()).add(r["Dispatch_Id"])
     lines = [f"# Hardware counters of the hot kernels, workload {b['config']['workload']} "
              f"(one MI355X)", "", "Commands (one counter pass each):"]
-    regex = "|".join(KERNELS.values())
+    regex = "|".join(kernels.values())
     for d in dirs:
         lines.append(f"  cd /tmp && export TMPDIR=/tmp && timeout -s KILL 150 rocprofv3 --pmc "
                      f"{' '.join(sorted(names.get(d, ())))} --kernel-include-regex '{regex}' "
@@ -396,7 +414,7 @@ def counters_summary(dirs, bench_json, out_path):
                  f"<out>/{os.path.basename(bench_json)} profiles/{os.path.basename(out_path)}")
     lines.append("")
     for kern, c in agg.items():
-        stage = [s for s, k in KERNELS.items() if k == kern][0]
+        stage = [s for s, k in kernels.items() if k == kern][0]
         units = b["stage_units"][stage] / max(b["stage_launches"][stage], 1)
         lines.append(f"{kern}: {units:.4g} units per dispatch")
         avg = {}
