@@ -130,7 +130,7 @@ void release_work(WorkBufs& w) {
     DevBuf* bufs[] = {&w.cnt, &w.off, &w.V, &w.Vd, &w.tcnt, &w.tlist, &w.tcount, &w.rec,
                       &w.bounds, &w.psurf, &w.srow, &w.refl, &w.shotS, &w.blu, &w.tA, &w.tB, &w.tC,
                       &w.pairs, &w.t0trace, &w.shots, &w.prisms, &w.out, &w.nvalid, &w.oow,
-                      &w.umask, &w.ucnt, &w.uoff, &w.U, &w.ubase, &w.brec, &w.items};
+                      &w.umask, &w.ucnt, &w.uoff, &w.U, &w.ubase, &w.brec, &w.items, &w.bbox};
     for (DevBuf* b : bufs) b->release();
 }
 

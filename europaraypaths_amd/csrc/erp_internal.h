@@ -200,6 +200,7 @@ struct WorkBufs {
     // shot-pair target kernel: union masks, counts, offsets and sizes per
     // block of two shots; block records; work items
     DevBuf umask, ucnt, uoff, U, ubase, brec, items;
+    DevBuf bbox;       // per (shot, facet block) box of the lit facets
 };
 
 // stage timers: events recorded on the run stream, resolved after it syncs

@@ -364,6 +364,9 @@ __device__ __forceinline__ float o2f(unsigned u) {
 struct Bounds {
     unsigned long long itd_min, itd_max;  // double bits (positive)
     unsigned xmin, ymin, zmin, xmax, ymax, zmax;  // f2o-encoded
+    // a sphere holding every lit facet: centre (the box's) and radius, or
+    // rad < 0 where k_shot_sphere has not run
+    float cx, cy, cz, rad;
 };
 
 // Arithmetic of the range, bin and phase of a triple (precision study):
@@ -450,6 +453,54 @@ __global__ void k_init_bounds(Bounds* b, int S) {
         b[s].itd_min = ~0ull; b[s].itd_max = 0ull;
         b[s].xmin = b[s].ymin = b[s].zmin = 0xFFFFFFFFu;
         b[s].xmax = b[s].ymax = b[s].zmax = 0u;
+        b[s].cx = b[s].cy = b[s].cz = 0.f;
+        b[s].rad = -1.f;
+    }
+}
+
+// Bounding sphere of each shot's lit facets: centre c = the centre of the
+// shot's box, radius = the largest distance from c to a corner of a lit
+// facet block's box (every lit facet lies in its block's box).  Each facet
+// block (a mesh row or tile) is a thin box, so the sphere is close to the
+// lit set; a pair's window takes the nearer of the box and sphere bounds.
+// Rows of a multi-track batch: a padding row and blocks past a row's mesh
+// have no boxes.
+__global__ void __launch_bounds__(256) k_shot_sphere(Bounds* __restrict__ bounds,
+                                                     const float4* __restrict__ bbox,
+                                                     const int* __restrict__ cnt, int nblk,
+                                                     const TrackK* __restrict__ trk,
+                                                     const RowK* __restrict__ rows) {
+    const int s = blockIdx.x;
+    int n = nblk;
+    if (trk) {
+        if (rows[s].shot < 0) return;
+        n = trk[rows[s].trk].nblk;
+    }
+    Bounds& bd = bounds[s];
+    const float cx = 0.5f * (o2f(bd.xmin) + o2f(bd.xmax));
+    const float cy = 0.5f * (o2f(bd.ymin) + o2f(bd.ymax));
+    const float cz = 0.5f * (o2f(bd.zmin) + o2f(bd.zmax));
+    double m = 0.0;
+    for (int b = threadIdx.x; b < n; b += 256) {
+        const size_t i = (size_t)s * nblk + b;
+        if (cnt[i] == 0) continue;
+        const float4 lo = bbox[2 * i], hi = bbox[2 * i + 1];
+        const double ex = fmax(fabs((double)lo.x - cx), fabs((double)hi.x - cx));
+        const double ey = fmax(fabs((double)lo.y - cy), fabs((double)hi.y - cy));
+        const double ez = fmax(fabs((double)lo.z - cz), fabs((double)hi.z - cz));
+        m = fmax(m, ex * ex + ey * ey + ez * ez);
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        bd.cx = cx;
+        bd.cy = cy;
+        bd.cz = cz;
+        // rounded up: the float radius is at least the FP64 one
+        bd.rad = (float)sqrt(m) * (1.f + 1e-6f);
     }
 }
 
@@ -749,7 +800,7 @@ __global__ void __launch_bounds__(NT) k_surface(
     Rec rec, int write_rec, int do_trace, double2* __restrict__ srow, int64_t shot0,
     Bounds* __restrict__ bounds, int det, unsigned* __restrict__ smax, int tile_nx,
     int prec_ph, int prec_acc, const TrackK* __restrict__ trk, const RowK* __restrict__ rows,
-    BlkRec brk) {
+    BlkRec brk, float4* __restrict__ bbox) {
     // det 0: FP64 bins (LDS window, FP64 atomics into srow)
     // det 1: first deterministic pass, largest |component| per shot into smax
     // det 2: second deterministic pass, fixed-point bins (srow holds int64)
@@ -1041,6 +1092,12 @@ __global__ void __launch_bounds__(NT) k_surface(
             d0 = fmin(d0, red_d[0][w]); d1 = fmax(d1, red_d[1][w]);
         }
         Bounds* bd = bounds + s;
+        if (bbox) {
+            // this block's box, for the shot's bounding sphere (k_shot_sphere)
+            float4* q = bbox + 2 * ((size_t)s * nblk + blk);
+            q[0] = make_float4(a[0], a[1], a[2], 0.f);
+            q[1] = make_float4(a[3], a[4], a[5], 0.f);
+        }
         atomicMin(&bd->xmin, f2o(a[0])); atomicMin(&bd->ymin, f2o(a[1]));
         atomicMin(&bd->zmin, f2o(a[2])); atomicMax(&bd->xmax, f2o(a[3]));
         atomicMax(&bd->ymax, f2o(a[4])); atomicMax(&bd->zmax, f2o(a[5]));
@@ -1893,7 +1950,13 @@ __device__ __forceinline__ PairWin pair_window(const Bounds& bd, float txf, floa
         const float fx = fmaxf(fabsf(o2f(bd.xmin) - txf), fabsf(o2f(bd.xmax) - txf));
         const float fy = fmaxf(fabsf(o2f(bd.ymin) - tyf), fabsf(o2f(bd.ymax) - tyf));
         const float fz = fmaxf(fabsf(o2f(bd.zmin) - tzf), fabsf(o2f(bd.zmax) - tzf));
-        const double tmax = sqrt((double)fx * fx + (double)fy * fy + (double)fz * fz);
+        double tmax = sqrt((double)fx * fx + (double)fy * fy + (double)fz * fz);
+        if (bd.rad >= 0.f) {
+            // |f - t| <= |c - t| + rad for every lit facet f
+            const double dx = (double)txf - bd.cx, dy = (double)tyf - bd.cy,
+                         dz = (double)tzf - bd.cz;
+            tmax = fmin(tmax, sqrt(dx * dx + dy * dy + dz * dz) + (double)bd.rad);
+        }
         const double ithmax = __longlong_as_double((long long)bd.itd_max);
         const double rhi = ithmax + n_hi * tmax * (1.0 + 1e-6) + 1.0;
         pw.top = min((int)ceil((rhi - rst) / kp.drd) + 2, nr);
@@ -3441,6 +3504,7 @@ int64_t erp_footprint_impl(const erp_params* par, int64_t F, int64_t T, int64_t 
         C = std::min<int64_t>(C, kMaxPairsPerLaunch);
         const int64_t gcap = C / 16 + S + 2;
         b += S * T * 8 + S * 8;               // target lists and counts
+        b += S * nblk * 32;                   // facet block boxes (bounding spheres)
         b += P * 8 * (C * nbuf + gcap);       // per-target chunk rows and partial sums
         b += 4 * (C * 4 + 2 * gcap + 3 * S + 16);
     }
@@ -3629,6 +3693,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
             if (T > 0) {
                 buf(W.tlist, sizeof(int) * s * (size_t)T * 2, "target lists");
                 buf(W.tcnt, sizeof(int) * s * 2, "target counts");
+                buf(W.bbox, sizeof(float4) * 2 * s * nblk, "facet block boxes");
             }
             size_t pb = 0;
             if (any_typed) {
@@ -3686,6 +3751,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     if (T > 0) {
         W.tlist.ensure(sizeof(int) * (size_t)Smax * T * 2);
         W.tcnt.ensure(sizeof(int) * (size_t)Smax * 2);
+        W.bbox.ensure(sizeof(float4) * 2 * (size_t)Smax * nblk);
     }
 
     // terms of the plain target kernel that fall outside their pair's bin
@@ -3858,15 +3924,23 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         hipLaunchKernelGGL(k_surface, dim3(nblk, S), dim3(NT), 0, st, mesh->soa.as<float>(),
                            mesh->Fpad, F, dshots, dmask, W.off.as<int>(), dV, dbase, nblk, kp, rec,
                            write, do_trace ? 1 : 0, srow, s0, dbounds, det ? 1 : 0, smax,
-                           tile_nx, ph, acc, (const TrackK*)nullptr, (const RowK*)nullptr, brk);
+                           tile_nx, ph, acc, (const TrackK*)nullptr, (const RowK*)nullptr, brk,
+                           T > 0 ? W.bbox.as<float4>() : nullptr);
         ERP_CHECK_LAUNCH("k_surface");
+        if (T > 0) {
+            hipLaunchKernelGGL(k_shot_sphere, dim3(S), dim3(256), 0, st, dbounds,
+                               (const float4*)W.bbox.as<float4>(), (const int*)W.cnt.as<int>(),
+                               nblk, (const TrackK*)nullptr, (const RowK*)nullptr);
+            ERP_CHECK_LAUNCH("k_shot_sphere");
+        }
         if (det && do_trace) {
             // second pass: the same terms as fixed-point values scaled by the
             // per-shot maxima of the first
             hipLaunchKernelGGL(k_surface, dim3(nblk, S), dim3(NT), 0, st, mesh->soa.as<float>(),
                                mesh->Fpad, F, dshots, dmask, W.off.as<int>(), dV, dbase, nblk, kp,
                                rec, 0, 1, srow, s0, (Bounds*)nullptr, 2, smax, tile_nx, 0,
-                               0, (const TrackK*)nullptr, (const RowK*)nullptr, BlkRec{});
+                               0, (const TrackK*)nullptr, (const RowK*)nullptr, BlkRec{},
+                               (float4*)nullptr);
             ERP_CHECK_LAUNCH("k_surface");
         }
         if (do_trace) {
@@ -4295,6 +4369,7 @@ struct MtState {
     // shot-pair kernel: union masks, counts, offsets, sizes, first records
     // and first rows per block of two rows; work items and their count
     DevBuf umask, ucnt, uoff, U, ubase, brows, items;
+    DevBuf bbox;  // facet block boxes of the rows (bounding spheres)
     MtGraph gA1, gA2, gChunk, gTail;
     void reset_graphs() {
         gA1.reset();
@@ -4306,7 +4381,7 @@ struct MtState {
         reset_graphs();
         DevBuf* b[] = {&trk, &rows, &shots, &cnt, &off, &V, &mask, &bounds, &base, &psurf, &srow,
                        &refl, &refr, &Sacc, &blu, &list, &tcnt, &A, &Z, &tabs, &dtabs, &rec,
-                       &umask, &ucnt, &uoff, &U, &ubase, &brows, &items};
+                       &umask, &ucnt, &uoff, &U, &ubase, &brows, &items, &bbox};
         for (DevBuf* d : b) d->release();
         key = MtKey{};
     }
@@ -4481,6 +4556,7 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
         m.blu.ensure(sizeof(float2) * r * M);
         m.list.ensure(sizeof(int) * r * (size_t)Tcap * 2);
         m.tcnt.ensure(sizeof(int) * r * 2);
+        m.bbox.ensure(sizeof(float4) * 2 * r * nblk_max);
         m.A.ensure(sizeof(float2) * (size_t)P * ((size_t)C + gcap));  // pair rows, slice sums
         m.Z.ensure(sizeof(float2) * (size_t)P * C);                   // per-pair products
     }
@@ -4688,14 +4764,21 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
                                (const float*)nullptr, (int64_t)0, (int64_t)0, dshots,
                                m.mask.as<unsigned long long>(), m.off.as<int>(), dV, dbase,
                                nblk_max, kp0, rec, targ ? 1 : 0, 1, srow, (int64_t)0, dbounds,
-                               det ? 1 : 0, smax, 0, 0, 0, dtrk, drows, brk);
+                               det ? 1 : 0, smax, 0, 0, 0, dtrk, drows, brk,
+                               targ ? m.bbox.as<float4>() : nullptr);
             ERP_CHECK_LAUNCH("k_surface");
+            if (targ) {
+                hipLaunchKernelGGL(k_shot_sphere, dim3(R), dim3(256), 0, st, dbounds,
+                                   (const float4*)m.bbox.as<float4>(),
+                                   (const int*)m.cnt.as<int>(), nblk_max, dtrk, drows);
+                ERP_CHECK_LAUNCH("k_shot_sphere");
+            }
             if (det) {
                 hipLaunchKernelGGL(k_surface, dim3(nblk_max, R), dim3(NT), 0, st,
                                    (const float*)nullptr, (int64_t)0, (int64_t)0, dshots,
                                    m.mask.as<unsigned long long>(), m.off.as<int>(), dV, dbase,
                                    nblk_max, kp0, rec, 0, 1, srow, (int64_t)0, (Bounds*)nullptr,
-                                   2, smax, 0, 0, 0, dtrk, drows, BlkRec{});
+                                   2, smax, 0, 0, 0, dtrk, drows, BlkRec{}, (float4*)nullptr);
                 ERP_CHECK_LAUNCH("k_surface");
             }
             hipLaunchKernelGGL(k_surf_rows, dim3((nr + 255) / 256, R), dim3(256), 0, st,
