@@ -54,7 +54,10 @@ constexpr int NTT = 512;       // threads per workgroup of the target kernels
 // workgroups a small part of each launch and need fewer host round trips.
 constexpr int64_t kPairRowBytes = (int64_t)6144 << 20;
 constexpr int kMaxPairsPerLaunch = 16384;
-constexpr int TWF = 4096;      // bin window of k_target_accum_fast (FP64, 64 KB of LDS)
+// bin window of k_target_accum_fast: FP64, 79 KB of LDS, so two workgroups
+// fill a CU's 160 KB; the shot bounds of the 2048^2 fBm workload allow at
+// most about 4.7e3 bins per pair
+constexpr int TWF = 5056;
 constexpr int NTF = 512;       // threads of k_target_accum_fast (two workgroups per CU)
 // Bin windows accumulate in FP64: LDS atomic adds of doubles (ds_add_f64) run
 // at more than twice the rate of FP32 ones (ds_add_f32) on gfx950 and remove
