@@ -2553,7 +2553,13 @@ constexpr int TWB = 5056;
 // MT: a multi-track batch: an item's block is two rows of one track, whose
 // targets, parameters and facet frames it uses; the item count is read from
 // device memory (the launch is replayed from a graph).
-template <bool S3, bool MT = false>
+// DET: the deterministic mode.  Each shot's terms are summed as 64-bit fixed
+// point, as in the single-shot kernel's deterministic path.  The scale comes
+// from a bound of the shot's own terms (its lit facets only), so a row's
+// bits do not depend on the shot it is paired with, or on whether it is
+// paired at all; a pair whose window cannot hold its bounds is summed in
+// several passes of one window each.
+template <bool S3, bool MT = false, bool DET = false>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, NTB)))
 k_target_accum_pair(const float* __restrict__ tg0, int64_t Tpad0, const int4* __restrict__ items,
                     int nitems0, int* __restrict__ next_item, const erp_shot* __restrict__ shots,
@@ -2562,19 +2568,22 @@ k_target_accum_pair(const float* __restrict__ tg0, int64_t Tpad0, const int4* __
                     int np, KPar kp0, float2* __restrict__ outA, int* __restrict__ pbase,
                     int* __restrict__ pwin, int frame_ok0, int* __restrict__ oow,
                     const TrackK* __restrict__ trk, const RowK* __restrict__ rows,
-                    const int* __restrict__ nitems_dev) {
+                    const int* __restrict__ nitems_dev, const int* __restrict__ Vs) {
     __shared__ double winS[4 * TWB];
     __shared__ int s_it;
+    __shared__ float s_m[NTB / 64][2];
     const int nitems = MT ? *nitems_dev : nitems0;
-    // the windows are zero between items: each item's flush clears what it
-    // read, and terms only reach the bins below an item's len
+    // the windows are zero between items (and passes): each flush clears
+    // what it read, and terms only reach the bins below a window's length
     for (int i = threadIdx.x; i < 4 * TWB; i += NTB) winS[i] = 0.0;
     // persistent: each workgroup takes items from a counter (cleared by the
     // host before the launch) until none is left; the next item's index is
     // fetched while the current one runs
     if (threadIdx.x == 0) s_it = atomicAdd(next_item, 1);
     __syncthreads();
-    int it = s_it;
+    // the item index is uniform over the workgroup: what derives from it
+    // stays in scalar registers
+    int it = __builtin_amdgcn_readfirstlane(s_it);
     while (it < nitems) {
         int nxt = 0;
         if (threadIdx.x == 0) nxt = atomicAdd(next_item, 1);
@@ -2597,7 +2606,8 @@ k_target_accum_pair(const float* __restrict__ tg0, int64_t Tpad0, const int4* __
 
         // each shot's window (as k_target_accum_fast's); a shot without the
         // target has an empty one
-        int base[2], len[2];
+        int base[2], len[2], top[2];
+        bool fits[2];
         double rst[2];
         float2* rowp[2];
         int* flag[2];
@@ -2608,6 +2618,8 @@ k_target_accum_pair(const float* __restrict__ tg0, int64_t Tpad0, const int4* __
             win[k] = WinS{winS + 2 * k * TWB, winS + (2 * k + 1) * TWB};
             base[k] = 0;
             len[k] = 0;
+            top[k] = 0;
+            fits[k] = true;
             rst[k] = 0.0;
             rowp[k] = nullptr;
             flag[k] = nullptr;
@@ -2618,37 +2630,47 @@ k_target_accum_pair(const float* __restrict__ tg0, int64_t Tpad0, const int4* __
                                            TWB, true);
             base[k] = pw.base;
             len[k] = pw.len;
+            top[k] = pw.top;
+            fits[k] = pw.fits;
             rowp[k] = outA + (size_t)row[k] * kp.P_fft;
             // out-of-window terms of a pair whose window covers its bounds
-            // are broken bounds (counted); the others go to the row
-            flag[k] = pw.fits ? oow : nullptr;
+            // (deterministic: of every pair) are broken bounds (counted);
+            // the others go to the row
+            flag[k] = (DET || pw.fits) ? oow : nullptr;
             if (threadIdx.x == 0) {
                 pbase[row[k]] = pw.fits ? pw.base : -1;
                 pwin[row[k]] = pw.len;
             }
             if (!pw.fits) {
                 spill = true;
-                const int hi = pw.base + TWB;
+                // bins outside what the window passes write: zero
+                const int hi = DET ? max(pw.top, pw.base) : pw.base + TWB;
                 for (int i = threadIdx.x; i < kp.row_live; i += NTB)
                     if (i < pw.base || i >= hi || i >= nr) rowp[k][i] = make_float2(0.f, 0.f);
             }
         }
-        if (spill) {
-            // the row's zeros land before any thread's out-of-window atomics
+        // the row's zeros land before any thread's out-of-window atomics
+        // (none in the deterministic mode)
+        if (!DET && spill) {
             __threadfence();
             __syncthreads();
         }
 
         const int U = Ucnt[blk];
-        if (U > 0) {
-            const int64_t r0 = ubase[blk];
-            const BRecBlk rb{reinterpret_cast<const char*>(rec.qa + r0),
-                             reinterpret_cast<const char*>(rec.qb + r0),
-                             reinterpret_cast<const char*>(rec.qc + r0),
-                             reinterpret_cast<const char*>(rec.qd + r0),
-                             reinterpret_cast<const char*>(rec.itd + r0),
-                             reinterpret_cast<const char*>(rec.src + r0)};
-            const unsigned ul = (unsigned)(U - 1);
+        const int64_t r0 = ubase[blk];
+        const BRecBlk rb{reinterpret_cast<const char*>(rec.qa + r0),
+                         reinterpret_cast<const char*>(rec.qb + r0),
+                         reinterpret_cast<const char*>(rec.qc + r0),
+                         reinterpret_cast<const char*>(rec.qd + r0),
+                         reinterpret_cast<const char*>(rec.itd + r0),
+                         reinterpret_cast<const char*>(rec.src + r0)};
+        const unsigned ul = (unsigned)max(U - 1, 0);
+        // every union facet through add(k, term): one facet per thread and
+        // step, the next step's record loaded before this step's arithmetic
+        // into the other of two register sets (clamped, in-bounds indices:
+        // no set is copied)
+        auto for_terms = [&](auto&& add) {
+            if (U <= 0) return;
             auto step = [&](const BRecV& a) {
                 const TLeg g = tri_leg<S3>(a, frame_ok, tx, ty, tz, tnx, tny, tnz, tn_up,
                                            lossless, prisms, np, kp, st);
@@ -2660,12 +2682,9 @@ k_target_accum_pair(const float* __restrict__ tg0, int64_t Tpad0, const int4* __
                                          : make_float2(a.src.x, a.src.y);
                     const Tri o = tri_shot<S3>(g, itd, src, rst[k], a.qa.x, a.qa.y, a.qa.z, tx,
                                                ty, tz, kp, st, nr);
-                    if (o.valid) tri_add<ACC_F64>(o, base[k], len[k], nr, win[k], rowp[k], flag[k]);
+                    if (o.valid) add(k, o);
                 }
             };
-            // one facet per thread and step, the next step's record loaded
-            // before this step's arithmetic into the other of two register
-            // sets (clamped, in-bounds indices: no set is copied)
             BRecV a0 = load_brec(rb, min((unsigned)threadIdx.x, ul));
             for (int u = threadIdx.x; u < U; u += 2 * NTB) {
                 const BRecV a1 = load_brec(rb, min((unsigned)(u + NTB), ul));
@@ -2674,20 +2693,109 @@ k_target_accum_pair(const float* __restrict__ tg0, int64_t Tpad0, const int4* __
                 a0 = load_brec(rb, min((unsigned)(u + 2 * NTB), ul));
                 step(a1);
             }
-        }
-        __syncthreads();
-        // flush each window into its row and clear what was read
+        };
+
+        if constexpr (!DET) {
+            for_terms([&](int k, const Tri& o) {
+                tri_add<ACC_F64>(o, base[k], len[k], nr, win[k], rowp[k], flag[k]);
+            });
+            __syncthreads();
+            // flush each window into its row and clear what was read
 #pragma unroll
-        for (int k = 0; k < 2; ++k)
-            for (int i = threadIdx.x; i < len[k]; i += NTB) {
-                const int b = base[k] + i;
-                if (b < nr) rowp[k][b] = make_float2((float)win[k].x[i], (float)win[k].y[i]);
-                win[k].x[i] = 0.0;
-                win[k].y[i] = 0.0;
+            for (int k = 0; k < 2; ++k)
+                for (int i = threadIdx.x; i < len[k]; i += NTB) {
+                    const int b = base[k] + i;
+                    if (b < nr) rowp[k][b] = make_float2((float)win[k].x[i], (float)win[k].y[i]);
+                    win[k].x[i] = 0.0;
+                    win[k].y[i] = 0.0;
+                }
+        } else {
+            // bound of each shot's terms: |component| <= |gain source
+            // phasor| x (four pi-sincs, each <= pi, times the loss factor and
+            // the split weights, each <= 1) / Ttd; max(|re|, |im|) x 1.4143
+            // bounds the phasor's modulus
+            float m[2] = {0.f, 0.f};
+            for (int u = threadIdx.x; u < U; u += NTB) {
+                const unsigned o16 = (unsigned)u << 4;
+                const float4 qa = *reinterpret_cast<const float4*>(rb.qa + o16);
+                const double2 itd = *reinterpret_cast<const double2*>(rb.itd + o16);
+                const float4 sr = *reinterpret_cast<const float4*>(rb.src + o16);
+                const double dx = tx - (double)qa.x, dy = ty - (double)qa.y, dz = tz - (double)qa.z;
+                const float rs = __builtin_amdgcn_rsqf((float)fma(dx, dx, fma(dy, dy, dz * dz)));
+                if (row[0] >= 0 && itd.x >= 0.0)
+                    m[0] = fmaxf(m[0], fmaxf(fabsf(sr.x), fabsf(sr.y)) * rs);
+                if (row[1] >= 0 && itd.y >= 0.0)
+                    m[1] = fmaxf(m[1], fmaxf(fabsf(sr.z), fabsf(sr.w)) * rs);
             }
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                for (int o = 32; o > 0; o >>= 1) m[k] = fmaxf(m[k], __shfl_xor(m[k], o));
+                if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6][k] = m[k];
+            }
+            __syncthreads();
+            double scale[2], inv[2];
+            int npass = 1;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                float mk = 0.f;
+                for (int w = 0; w < NTB / 64; ++w) mk = fmaxf(mk, s_m[w][k]);
+                // 2x margin for the rounding of the sinc quotients; pi^4 =
+                // 97.409...
+                const int V = row[k] >= 0 ? Vs[2 * blk + k] : 0;
+                scale[k] = det_scale((double)V, 2.0 * 1.4143 * (double)mk * 97.40909103400244);
+                inv[k] = scale[k] > 0.0 ? 1.0 / scale[k] : 0.0;
+                if (row[k] >= 0 && !fits[k])
+                    npass = max(npass, (top[k] - base[k] + TWB - 1) / TWB);
+            }
+            // shot k's fixed-point window: real parts at 2k TWB, imaginary
+            // parts at (2k + 1) TWB of the LDS array
+            unsigned long long* const wu0 = reinterpret_cast<unsigned long long*>(winS);
+            auto wux = [&](int k) { return wu0 + 2 * k * TWB; };
+            auto wuy = [&](int k) { return wu0 + (2 * k + 1) * TWB; };
+            // one window per shot and pass: [base + p TWB, ...) below top
+            for (int p = 0; p < npass; ++p) {
+                int wb[2], wl[2];
+                bool first[2], last[2];
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    wb[k] = base[k] + p * TWB;
+                    wl[k] = row[k] < 0 ? 0 : fits[k] ? (p == 0 ? len[k] : 0)
+                                                     : min(TWB, max(0, top[k] - wb[k]));
+                    first[k] = p == 0;
+                    last[k] = fits[k] || wb[k] + TWB >= top[k];
+                }
+                for_terms([&](int k, const Tri& o) {
+                    const float wf = (float)o.w;
+                    const float2 c0 = make_float2(o.c.x * (1.f - wf), o.c.y * (1.f - wf));
+                    const float2 c1 = make_float2(o.c.x * wf, o.c.y * wf);
+                    const int k0 = o.b - wb[k];
+                    if (k0 >= 0 && k0 < wl[k]) lds_add2_fix(wux(k), wuy(k), k0, c0, scale[k]);
+                    else if ((o.b < base[k] && first[k]) || (o.b >= top[k] && last[k]))
+                        atomicAdd(flag[k], 1);
+                    if (o.b + 1 < nr) {
+                        if (k0 + 1 >= 0 && k0 + 1 < wl[k])
+                            lds_add2_fix(wux(k), wuy(k), k0 + 1, c1, scale[k]);
+                        else if ((o.b + 1 < base[k] && first[k]) || (o.b + 1 >= top[k] && last[k]))
+                            atomicAdd(flag[k], 1);
+                    }
+                });
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < 2; ++k)
+                    for (int i = threadIdx.x; i < wl[k]; i += NTB) {
+                        const int b = wb[k] + i;
+                        if (b < nr)
+                            rowp[k][b] = make_float2((float)det_unfix(wux(k)[i], inv[k]),
+                                                     (float)det_unfix(wuy(k)[i], inv[k]));
+                        wux(k)[i] = 0ull;
+                        wuy(k)[i] = 0ull;
+                    }
+                if (p + 1 < npass) __syncthreads();
+            }
+        }
         if (threadIdx.x == 0) s_it = nxt;
         __syncthreads();
-        it = s_it;
+        it = __builtin_amdgcn_readfirstlane(s_it);
     }
 }
 
@@ -3402,6 +3510,114 @@ void fill_pair_chunk(const std::vector<std::vector<PairIdx>>& bl, int& jb, size_
     }
 }
 
+// Deterministic launches of the shot-pair kernel: every shot's rows are cut
+// only at multiples of GS rows of its own target index, so that its slices
+// (and their sums, added to the shot in order) do not depend on the launches.
+// A block whose rows fit one launch is taken whole, as merged items; a larger
+// one is taken as items of one shot each, its first shot's rows, then its
+// second's.  A row's values are the same either way.
+struct PairCursor {
+    int jb = 0;     // block
+    int sk = -1;    // -1: the block not split; 0, 1: the split block's shot
+    size_t ib = 0;  // next item of the split block
+};
+void fill_pair_chunk_canon(const std::vector<std::vector<PairIdx>>& bl, PairCursor& cur, int C,
+                           int GS, const int* V, PairChunk& f) {
+    f.items.clear();
+    f.rshot.clear(); f.ridx.clear();
+    f.sg_s.clear(); f.gsb.clear(); f.gse.clear(); f.gb.clear(); f.ge.clear();
+    f.rows = 0;
+    f.triples = 0;
+    auto segment = [&](int shot, int b, int e) {
+        f.sg_s.push_back(shot);
+        f.gsb.push_back((int)f.gb.size());
+        for (int i = b; i < e; i += GS) {
+            f.gb.push_back(i);
+            f.ge.push_back(std::min(i + GS, e));
+        }
+        f.gse.push_back((int)f.gb.size());
+        f.triples += (double)(e - b) * V[shot];
+    };
+    const bool aligned = C >= GS;
+    while (cur.jb < (int)bl.size()) {
+        const std::vector<PairIdx>& its = bl[cur.jb];
+        int n0 = 0, n1 = 0;
+        for (const PairIdx& p : its) {
+            n0 += p.i0 >= 0;
+            n1 += p.i1 >= 0;
+        }
+        if (cur.sk < 0) {
+            if (its.empty()) {
+                ++cur.jb;
+                continue;
+            }
+            if (n0 + n1 <= C) {
+                if (f.rows + n0 + n1 > C) break;  // the next launch takes it whole
+                const int r = f.rows;
+                f.rshot.resize((size_t)(r + n0 + n1));
+                f.ridx.resize((size_t)(r + n0 + n1));
+                int c0 = 0, c1 = 0;
+                for (const PairIdx& p : its) {
+                    int r0 = -1, r1 = -1;
+                    if (p.i0 >= 0) {
+                        r0 = r + c0++;
+                        f.rshot[r0] = 2 * cur.jb;
+                        f.ridx[r0] = p.i0;
+                    }
+                    if (p.i1 >= 0) {
+                        r1 = r + n0 + c1++;
+                        f.rshot[r1] = 2 * cur.jb + 1;
+                        f.ridx[r1] = p.i1;
+                    }
+                    f.items.push_back(make_int4(p.t, r0, r1, cur.jb));
+                }
+                if (n0 > 0) segment(2 * cur.jb, r, r + n0);
+                if (n1 > 0) segment(2 * cur.jb + 1, r + n0, r + n0 + n1);
+                f.rows += n0 + n1;
+                ++cur.jb;
+                continue;
+            }
+            cur.sk = 0;
+            cur.ib = 0;
+        }
+        // the split block: shot sk's rows from item ib on
+        const int sk = cur.sk;
+        auto idx = [&](size_t q) { return sk ? its[q].i1 : its[q].i0; };
+        while (cur.ib < its.size() && idx(cur.ib) < 0) ++cur.ib;
+        if (cur.ib >= its.size()) {
+            if (sk == 0) {
+                cur.sk = 1;
+                cur.ib = 0;
+            } else {
+                cur.sk = -1;
+                cur.ib = 0;
+                ++cur.jb;
+            }
+            continue;
+        }
+        const int nk = sk ? n1 : n0;
+        const int start = idx(cur.ib);
+        int take = std::min(nk - start, C - f.rows);
+        if (aligned && take < nk - start) take -= take % GS;
+        if (take <= 0) break;  // this launch is full
+        const int r = f.rows;
+        f.rshot.resize((size_t)(r + take));
+        f.ridx.resize((size_t)(r + take));
+        int c = 0;
+        for (; cur.ib < its.size() && c < take; ++cur.ib) {
+            const PairIdx& p = its[cur.ib];
+            const int i = sk ? p.i1 : p.i0;
+            if (i < 0) continue;
+            f.rshot[r + c] = 2 * cur.jb + sk;
+            f.ridx[r + c] = i;
+            f.items.push_back(make_int4(p.t, sk ? -1 : r + c, sk ? r + c : -1, cur.jb));
+            ++c;
+        }
+        segment(2 * cur.jb + sk, r, r + take);
+        f.rows += take;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -3654,6 +3870,11 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     int C = (int)std::max<int64_t>(1, kPairRowBytes / ((int64_t)P * 8 * nbuf));
     C = std::min<int>(C, kMaxPairsPerLaunch);
     if (ctx->tune_pairs > 0) C = (int)std::min<int64_t>(C, ctx->tune_pairs);
+    // shot-pair target kernel (k_target_accum_pair): plain runs of the
+    // product's arithmetic whose targets all have a type != 0, unless
+    // ERP_TUNE_SINGLE_SHOT asks for the single-shot kernel
+    const bool pairk = half && ph == PH_F64 && acc == ACC_F64 && !any_t0 && C >= 2 &&
+                       ctx->tune_single == 0 && kp.piLlam <= 512.f;
 
     // bytes of the facet records of cap illuminated facets (only the layouts
     // the run reads; each array 256-byte aligned, as the batch loop carves them)
@@ -3709,13 +3930,22 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
             if (any_t0) pb = std::max(pb, sizeof(int) * 2 * s * (size_t)T + 64);
             if (pb) buf(W.pairs, pb, "pair lists");
             if (any_typed) buf(W.oow, 256, "window check");
+            if (pairk) {
+                const size_t nb = (s + 1) / 2;
+                buf(W.umask, sizeof(unsigned long long) * nb * nblk * MASKW, "union masks");
+                buf(W.ucnt, sizeof(int) * nb * nblk, "union counts");
+                buf(W.uoff, sizeof(int) * nb * nblk, "union offsets");
+                buf(W.U, sizeof(int) * nb, "union sizes");
+                buf(W.ubase, sizeof(int64_t) * nb, "block record bases");
+                buf(W.items, sizeof(int4) * (size_t)C_, "pair items");
+            }
             return tot;
         };
         const size_t avail = arena->left();
         while (layout(Smax, C, false) > avail / 4 * 3) {
             if (any_typed && C > 256) C /= 2;
             else if (Smax > 1) Smax = (Smax + 1) / 2;
-            else if (any_typed && C > 1) C /= 2;
+            else if (any_typed && C > (pairk ? 2 : 1)) C /= 2;
             else
                 throw ErpError{ERP_E_NOMEM,
                                "job too large for the allocation-free entry point: one shot's "
@@ -3724,7 +3954,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                                    " left after the tables"};
         }
         layout(Smax, C, true);
-        if (need_rec) arena->lend(W.rec, arena->left(), "facet records");
+        if (need_rec) arena->lend(pairk ? W.brec : W.rec, arena->left(), "facet records");
     }
 
     W.shots.ensure(sizeof(erp_shot) * (size_t)Smax);
@@ -3765,12 +3995,8 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         ERP_HIP_TRY(hipMemsetAsync(W.oow.p, 0, sizeof(int), st));
     }
 
-    // shot-pair target kernel (k_target_accum_pair): plain runs of the default
-    // arithmetic whose targets all have a type != 0, outside the
-    // allocation-free entry point, unless ERP_TUNE_SINGLE_SHOT asks for the
-    // single-shot kernel
-    const bool pairk = half && !det && ph == PH_F64 && acc == ACC_F64 && !any_t0 && !arena &&
-                       C >= 2 && ctx->tune_single == 0 && kp.piLlam <= 512.f;
+    // the shot-pair kernel's buffers: union masks and records per block of
+    // two shots
     const int nbk_max = (Smax + 1) / 2;
     if (pairk) {
         W.umask.ensure(sizeof(unsigned long long) * (size_t)nbk_max * nblk * MASKW);
@@ -3813,7 +4039,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                            (const RowK*)nullptr);
         ERP_CHECK_LAUNCH("k_scan_blocks");
         // blocks of two shots: the union of their masks, its offsets and size
-        const int nbk = (S + 1) / 2;
+        int nbk = (S + 1) / 2;
         if (pairk) {
             hipLaunchKernelGGL(k_union_mask, dim3(nblk, nbk), dim3(MASKW), 0, st, dmask, nblk, S,
                                W.umask.as<unsigned long long>(), W.ucnt.as<int>(),
@@ -3829,7 +4055,43 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         PT_.end(st);
         ERP_HIP_TRY(hipMemcpyAsync(hV.data(), dV, sizeof(int) * S, hipMemcpyDeviceToHost, st));
         ERP_HIP_TRY(hipStreamSynchronize(st));
-        if (arena && need_rec) {
+        auto brec_need = [](size_t cap) {
+            return 5 * Arena::round(cap * 16) + Arena::round(cap * 4);
+        };
+        if (arena && need_rec && pairk) {
+            // keep the leading blocks whose union records fit the region (or,
+            // failing that, the first shot alone); the other shots are masked
+            // again as the next batch
+            int fitb = 0;
+            size_t acc = 0;
+            while (fitb < nbk && brec_need(acc + (size_t)hU[fitb]) <= W.brec.bytes)
+                acc += hU[fitb++];
+            if (fitb == 0 && S > 1) {
+                S = 1;
+                nbk = 1;
+                hipLaunchKernelGGL(k_union_mask, dim3(nblk, 1), dim3(MASKW), 0, st, dmask, nblk, 1,
+                                   W.umask.as<unsigned long long>(), W.ucnt.as<int>(),
+                                   (const TrackK*)nullptr, (const RowK*)nullptr);
+                ERP_CHECK_LAUNCH("k_union_mask");
+                hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, W.ucnt.as<int>(),
+                                   W.uoff.as<int>(), W.U.as<int>(), nblk, (const TrackK*)nullptr,
+                                   (const RowK*)nullptr);
+                ERP_CHECK_LAUNCH("k_scan_blocks");
+                ERP_HIP_TRY(hipMemcpyAsync(hU.data(), W.U.p, sizeof(int), hipMemcpyDeviceToHost,
+                                           st));
+                ERP_HIP_TRY(hipStreamSynchronize(st));
+                if (brec_need((size_t)hU[0]) <= W.brec.bytes) fitb = 1;
+            }
+            if (fitb == 0)
+                throw ErpError{ERP_E_NOMEM,
+                               "shot " + std::to_string(s0) + " illuminates " +
+                                   std::to_string(hV[0]) + " facets: their records need " +
+                                   std::to_string(brec_need((size_t)hU[0])) +
+                                   " bytes, the allocation-free scratch has " +
+                                   std::to_string(W.brec.bytes) + " for records"};
+            S = std::min(S, 2 * fitb);
+            nbk = (S + 1) / 2;
+        } else if (arena && need_rec) {
             // keep the leading shots whose records fit the region; the others
             // are masked again as the next batch
             int fit = 0;
@@ -3865,8 +4127,9 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
             }
             const size_t cap = (size_t)hubase[nbk];
             auto rnd = [](size_t n) { return (n + 255) & ~(size_t)255; };
-            const size_t bytes = 5 * rnd(cap * 16) + rnd(cap * 4);
-            W.brec.ensure_grow(bytes, (size_t)nbk * (size_t)F * 84 + 256 * 6);
+            const size_t bytes = brec_need(cap);
+            if (arena) W.brec.ensure(bytes);  // fits: the batch was cut to fit
+            else W.brec.ensure_grow(bytes, (size_t)nbk * (size_t)F * 84 + 256 * 6);
             char* b = W.brec.as<char>();
             auto take = [&](size_t n) { char* r = b; b += rnd(n); return r; };
             BRec& br = brk.rec;
@@ -4049,8 +4312,11 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                     PairChunk pc;
                     int jb = 0;
                     size_t ib = 0;
+                    PairCursor cur;
                     for (;;) {
-                        fill_pair_chunk(bl, jb, ib, C, GS, hV.data(), pc);
+                        // deterministic: every shot's slices independent of the launches
+                        if (det) fill_pair_chunk_canon(bl, cur, C, GS, hV.data(), pc);
+                        else fill_pair_chunk(bl, jb, ib, C, GS, hV.data(), pc);
                         const int n = pc.rows;
                         if (n == 0) break;
                         const int nseg = (int)pc.sg_s.size();
@@ -4072,14 +4338,18 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                         // persistent grid: one workgroup per CU (its two windows
                         // take 128 KB of LDS), each taking items until none is left
                         const int grid = std::max(1, std::min(nit, ctx->num_cu));
-                        auto kern = kp.has3 ? k_target_accum_pair<true> : k_target_accum_pair<false>;
+                        auto kern = det ? (kp.has3 ? k_target_accum_pair<true, false, true>
+                                                   : k_target_accum_pair<false, false, true>)
+                                        : (kp.has3 ? k_target_accum_pair<true>
+                                                   : k_target_accum_pair<false>);
                         hipLaunchKernelGGL(kern, dim3(grid), dim3(NTB), 0, st,
                                            targets->xyzn.as<float>(), Tpad,
                                            (const int4*)ditems, nit, dnext, dshots,
                                            brk.ubase, brk.U, brk.rec, dbounds, dpr, (int)nprisms,
                                            kp, A, dpb, dpw, mesh->frame_ok ? 1 : 0,
                                            W.oow.as<int>(), (const TrackK*)nullptr,
-                                           (const RowK*)nullptr, (const int*)nullptr);
+                                           (const RowK*)nullptr, (const int*)nullptr,
+                                           (const int*)dV);
                         ERP_CHECK_LAUNCH("k_target_accum_pair");
                         PT_.end(st);
                         PT_.begin(ERP_STAGE_TARGET_FFT, st, (double)n);
@@ -4088,7 +4358,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                         PT_.begin(ERP_STAGE_TARGET_SUM, st, (double)n);
                         target_sum_half(planP, Z, A + (size_t)C * P, dgb, dge, ngroups, dseg,
                                         dseg + S, dseg + 2 * S, nseg, Sacc, -1.0f / kp.sqrtP, st,
-                                        nullptr, false);
+                                        nullptr, det);
                         PT_.end(st);
                     }
                 }
@@ -4522,7 +4792,7 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
     // the shot-pair kernel (default arithmetic, unless ERP_TUNE_SINGLE_SHOT):
     // blocks of two rows of one track, so R is even and a track with an odd
     // shot count ends on a padding row
-    bool pairk = targ && !det && C >= 2 && ctx->tune_single == 0;
+    bool pairk = targ && C >= 2 && ctx->tune_single == 0;
     for (int k = 0; k < ntr; ++k) pairk = pairk && tk[k].kp.piLlam <= 512.f;
     if (pairk && (R & 1)) ++R;
     const int nbk = R / 2;
@@ -4836,8 +5106,11 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
             merge_pair_lists(hl.data(), (size_t)maxc, htc.data(), R, bl);
             int jb = 0;
             size_t ib = 0;
+            PairCursor cur;
             for (;;) {
-                fill_pair_chunk(bl, jb, ib, C, GS, hV.data(), pc);
+                // deterministic: every row's slices independent of the launches
+                if (det) fill_pair_chunk_canon(bl, cur, C, GS, hV.data(), pc);
+                else fill_pair_chunk(bl, jb, ib, C, GS, hV.data(), pc);
                 const int n = pc.rows;
                 if (n == 0) break;
                 const int nsl = (int)pc.gb.size(), nseg = (int)pc.sg_s.size();
@@ -4874,8 +5147,10 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
                                        st, t_ptgt, t_pshot, list1, Tcap, t_n, d_next, d_G, dtrk,
                                        drows);
                     ERP_CHECK_LAUNCH("k_pair_targets_rows");
-                    auto kern = s3 ? k_target_accum_pair<true, true>
-                                   : k_target_accum_pair<false, true>;
+                    auto kern = det ? (s3 ? k_target_accum_pair<true, true, true>
+                                          : k_target_accum_pair<false, true, true>)
+                                    : (s3 ? k_target_accum_pair<true, true>
+                                          : k_target_accum_pair<false, true>);
                     // persistent: one workgroup per CU takes items until *d_nitems
                     hipLaunchKernelGGL(kern, dim3(std::max(1, std::min(C, ctx->num_cu))),
                                        dim3(NTB), 0, st, (const float*)nullptr, (int64_t)0,
@@ -4883,12 +5158,13 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
                                        (const int64_t*)m.ubase.as<int64_t>(),
                                        (const int*)m.U.as<int>(), brk.rec, dbounds,
                                        (const erp_prism*)nullptr, 0, kp0, A, d_pbase, d_pwin, 0,
-                                       d_next + 1, dtrk, drows, (const int*)d_nitems);
+                                       d_next + 1, dtrk, drows, (const int*)d_nitems,
+                                       (const int*)dV);
                     ERP_CHECK_LAUNCH("k_target_accum_pair");
                     target_pairs_half(ctx, planP, A, d_pbase, d_pwin, nr, C, nullptr, Z, st, d_G,
                                       t_n);
                     target_sum_half(planP, Z, A + (size_t)C * P, t_slb, t_sle, gcap, t_sgs, t_sgb,
-                                    t_sge, segcap, m.Sacc.as<float2>(), 1.0f, st, t_scale, false);
+                                    t_sge, segcap, m.Sacc.as<float2>(), 1.0f, st, t_scale, det);
                 });
                 PT_.end(st);
             }
