@@ -711,6 +711,20 @@ __device__ __forceinline__ void lds_add2_fix(unsigned long long* wx, unsigned lo
     atomicAdd(&wx[idx], det_fix(v.x, scale));
     atomicAdd(&wy[idx], det_fix(v.y, scale));
 }
+// det_fix for |v scale| < 2^51: v scale is exact (scale is a power of two),
+// and adding 1.5 * 2^52 rounds it to the nearest integer, ties to even (as
+// __double2ll_rn), held in the low mantissa bits: the same value in three
+// instructions instead of a full double -> int64 conversion
+__device__ __forceinline__ unsigned long long det_fix51(float v, double scale) {
+    const double magic = 6755399441055744.0;  // 1.5 * 2^52
+    const double x = (double)v * scale + magic;
+    return (unsigned long long)(__double_as_longlong(x) - __double_as_longlong(magic));
+}
+__device__ __forceinline__ void lds_add2_fix51(unsigned long long* wx, unsigned long long* wy,
+                                               int idx, float2 v, double scale) {
+    atomicAdd(&wx[idx], det_fix51(v.x, scale));
+    atomicAdd(&wy[idx], det_fix51(v.y, scale));
+}
 // FP32 window add; ACC_COMP also accumulates the rounding error of the
 // addition the atomic performed (TwoSum of the returned old value and v)
 template <int ACC>
@@ -2734,6 +2748,9 @@ k_target_accum_pair(const float* __restrict__ tg0, int64_t Tpad0, const int4* __
             }
             __syncthreads();
             double scale[2], inv[2];
+            // a shot of at least 1024 lit facets bounds every term below
+            // 2^61 / 2048 = 2^50 in fixed point: det_fix51 applies
+            bool fast[2];
             int npass = 1;
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
@@ -2744,6 +2761,7 @@ k_target_accum_pair(const float* __restrict__ tg0, int64_t Tpad0, const int4* __
                 const int V = row[k] >= 0 ? Vs[2 * blk + k] : 0;
                 scale[k] = det_scale((double)V, 2.0 * 1.4143 * (double)mk * 97.40909103400244);
                 inv[k] = scale[k] > 0.0 ? 1.0 / scale[k] : 0.0;
+                fast[k] = V >= 1024;
                 if (row[k] >= 0 && !fits[k])
                     npass = max(npass, (top[k] - base[k] + TWB - 1) / TWB);
             }
@@ -2769,12 +2787,16 @@ k_target_accum_pair(const float* __restrict__ tg0, int64_t Tpad0, const int4* __
                     const float2 c0 = make_float2(o.c.x * (1.f - wf), o.c.y * (1.f - wf));
                     const float2 c1 = make_float2(o.c.x * wf, o.c.y * wf);
                     const int k0 = o.b - wb[k];
-                    if (k0 >= 0 && k0 < wl[k]) lds_add2_fix(wux(k), wuy(k), k0, c0, scale[k]);
+                    auto put = [&](int i, float2 c) {
+                        if (fast[k]) lds_add2_fix51(wux(k), wuy(k), i, c, scale[k]);
+                        else lds_add2_fix(wux(k), wuy(k), i, c, scale[k]);
+                    };
+                    if (k0 >= 0 && k0 < wl[k]) put(k0, c0);
                     else if ((o.b < base[k] && first[k]) || (o.b >= top[k] && last[k]))
                         atomicAdd(flag[k], 1);
                     if (o.b + 1 < nr) {
                         if (k0 + 1 >= 0 && k0 + 1 < wl[k])
-                            lds_add2_fix(wux(k), wuy(k), k0 + 1, c1, scale[k]);
+                            put(k0 + 1, c1);
                         else if ((o.b + 1 < base[k] && first[k]) || (o.b + 1 >= top[k] && last[k]))
                             atomicAdd(flag[k], 1);
                     }
