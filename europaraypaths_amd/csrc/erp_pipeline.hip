@@ -3535,13 +3535,16 @@ void fill_pair_chunk(const std::vector<std::vector<PairIdx>>& bl, int& jb, size_
 // Deterministic launches of the shot-pair kernel: every shot's rows are cut
 // only at multiples of GS rows of its own target index, so that its slices
 // (and their sums, added to the shot in order) do not depend on the launches.
-// A block whose rows fit one launch is taken whole, as merged items; a larger
-// one is taken as items of one shot each, its first shot's rows, then its
-// second's.  A row's values are the same either way.
+// Each shot of a block keeps its own place in the block's items.  A launch
+// takes merged items as far as its room allows, then pads each shot's rows
+// with items of that shot alone up to its next multiple of GS; the next
+// launch resumes each shot where it stopped (the padded items then carry the
+// other shot alone).  A block that cannot fit one merged item and its pads
+// into a whole launch (launches of under 2 GS rows) is taken one shot at a
+// time.  A row's values are the same either way.
 struct PairCursor {
-    int jb = 0;     // block
-    int sk = -1;    // -1: the block not split; 0, 1: the split block's shot
-    size_t ib = 0;  // next item of the split block
+    int jb = 0;          // block
+    size_t p0 = 0, p1 = 0;  // per shot of the block: its first item not yet taken
 };
 void fill_pair_chunk_canon(const std::vector<std::vector<PairIdx>>& bl, PairCursor& cur, int C,
                            int GS, const int* V, PairChunk& f) {
@@ -3561,82 +3564,107 @@ void fill_pair_chunk_canon(const std::vector<std::vector<PairIdx>>& bl, PairCurs
         f.triples += (double)(e - b) * V[shot];
     };
     const bool aligned = C >= GS;
+    std::vector<int> c0, c1;  // rows of each shot before item q
     while (cur.jb < (int)bl.size()) {
         const std::vector<PairIdx>& its = bl[cur.jb];
-        int n0 = 0, n1 = 0;
-        for (const PairIdx& p : its) {
-            n0 += p.i0 >= 0;
-            n1 += p.i1 >= 0;
+        const size_t ni = its.size();
+        c0.assign(ni + 1, 0);
+        c1.assign(ni + 1, 0);
+        for (size_t q = 0; q < ni; ++q) {
+            c0[q + 1] = c0[q] + (its[q].i0 >= 0);
+            c1[q + 1] = c1[q] + (its[q].i1 >= 0);
         }
-        if (cur.sk < 0) {
-            if (its.empty()) {
-                ++cur.jb;
-                continue;
-            }
-            if (n0 + n1 <= C) {
-                if (f.rows + n0 + n1 > C) break;  // the next launch takes it whole
-                const int r = f.rows;
-                f.rshot.resize((size_t)(r + n0 + n1));
-                f.ridx.resize((size_t)(r + n0 + n1));
-                int c0 = 0, c1 = 0;
-                for (const PairIdx& p : its) {
-                    int r0 = -1, r1 = -1;
-                    if (p.i0 >= 0) {
-                        r0 = r + c0++;
-                        f.rshot[r0] = 2 * cur.jb;
-                        f.ridx[r0] = p.i0;
-                    }
-                    if (p.i1 >= 0) {
-                        r1 = r + n0 + c1++;
-                        f.rshot[r1] = 2 * cur.jb + 1;
-                        f.ridx[r1] = p.i1;
-                    }
-                    f.items.push_back(make_int4(p.t, r0, r1, cur.jb));
-                }
-                if (n0 > 0) segment(2 * cur.jb, r, r + n0);
-                if (n1 > 0) segment(2 * cur.jb + 1, r + n0, r + n0 + n1);
-                f.rows += n0 + n1;
-                ++cur.jb;
-                continue;
-            }
-            cur.sk = 0;
-            cur.ib = 0;
-        }
-        // the split block: shot sk's rows from item ib on
-        const int sk = cur.sk;
-        auto idx = [&](size_t q) { return sk ? its[q].i1 : its[q].i0; };
-        while (cur.ib < its.size() && idx(cur.ib) < 0) ++cur.ib;
-        if (cur.ib >= its.size()) {
-            if (sk == 0) {
-                cur.sk = 1;
-                cur.ib = 0;
-            } else {
-                cur.sk = -1;
-                cur.ib = 0;
-                ++cur.jb;
-            }
+        const int m0 = c0[ni], m1 = c1[ni];
+        const int a0 = c0[std::min(cur.p0, ni)], a1 = c1[std::min(cur.p1, ni)];
+        if (a0 >= m0 && a1 >= m1) {
+            ++cur.jb;
+            cur.p0 = cur.p1 = 0;
             continue;
         }
-        const int nk = sk ? n1 : n0;
-        const int start = idx(cur.ib);
-        int take = std::min(nk - start, C - f.rows);
-        if (aligned && take < nk - start) take -= take % GS;
-        if (take <= 0) break;  // this launch is full
-        const int r = f.rows;
-        f.rshot.resize((size_t)(r + take));
-        f.ridx.resize((size_t)(r + take));
-        int c = 0;
-        for (; cur.ib < its.size() && c < take; ++cur.ib) {
-            const PairIdx& p = its[cur.ib];
-            const int i = sk ? p.i1 : p.i0;
-            if (i < 0) continue;
-            f.rshot[r + c] = 2 * cur.jb + sk;
-            f.ridx[r + c] = i;
-            f.items.push_back(make_int4(p.t, sk ? -1 : r + c, sk ? r + c : -1, cur.jb));
-            ++c;
+        const int room = C - f.rows;
+        // rows of each shot when the merged items end at item e, with its pad
+        auto pad = [&](int a, int m) {
+            return (!aligned || a % GS == 0 || a >= m) ? 0 : std::min(GS - a % GS, m - a);
+        };
+        const size_t qs = std::min(cur.p0, cur.p1);
+        size_t e = qs;
+        int n0 = 0, n1 = 0, d0 = 0, d1 = 0;
+        for (size_t q = ni; q > qs; --q) {
+            const int x0 = c0[std::max(q, cur.p0)] - a0, x1 = c1[std::max(q, cur.p1)] - a1;
+            const int y0 = pad(a0 + x0, m0), y1 = pad(a1 + x1, m1);
+            if (x0 + x1 + y0 + y1 <= room && x0 + x1 > 0) {
+                e = q;
+                n0 = x0; n1 = x1; d0 = y0; d1 = y1;
+                break;
+            }
         }
-        segment(2 * cur.jb + sk, r, r + take);
-        f.rows += take;
+        if (e == qs) {
+            if (f.rows > 0) break;  // the next launch starts here
+            // one shot at a time: the first shot with rows left, cut at a
+            // multiple of GS
+            const int k = a0 < m0 ? 0 : 1;
+            const int left = k ? m1 - a1 : m0 - a0;
+            int take = std::min(left, C);
+            if (aligned && take < left) take -= take % GS;
+            size_t& pk = k ? cur.p1 : cur.p0;
+            f.rshot.resize((size_t)take);
+            f.ridx.resize((size_t)take);
+            int c = 0;
+            for (; pk < ni && c < take; ++pk) {
+                const PairIdx& p = its[pk];
+                const int i = k ? p.i1 : p.i0;
+                if (i < 0) continue;
+                f.rshot[c] = 2 * cur.jb + k;
+                f.ridx[c] = i;
+                f.items.push_back(make_int4(p.t, k ? -1 : c, k ? c : -1, cur.jb));
+                ++c;
+            }
+            segment(2 * cur.jb + k, 0, take);
+            f.rows = take;
+            break;
+        }
+        // shot 0's rows [r, r + n0 + d0), then shot 1's
+        const int r = f.rows, w0 = n0 + d0, w1 = n1 + d1;
+        f.rshot.resize((size_t)(r + w0 + w1));
+        f.ridx.resize((size_t)(r + w0 + w1));
+        int k0 = 0, k1 = 0;
+        for (size_t q = qs; q < e; ++q) {
+            const PairIdx& p = its[q];
+            int r0 = -1, r1 = -1;
+            if (q >= cur.p0 && p.i0 >= 0) {
+                r0 = r + k0++;
+                f.rshot[r0] = 2 * cur.jb;
+                f.ridx[r0] = p.i0;
+            }
+            if (q >= cur.p1 && p.i1 >= 0) {
+                r1 = r + w0 + k1++;
+                f.rshot[r1] = 2 * cur.jb + 1;
+                f.ridx[r1] = p.i1;
+            }
+            if (r0 >= 0 || r1 >= 0) f.items.push_back(make_int4(p.t, r0, r1, cur.jb));
+        }
+        cur.p0 = std::max(cur.p0, e);
+        cur.p1 = std::max(cur.p1, e);
+        for (; k0 < w0; ++cur.p0) {  // the pads
+            const PairIdx& p = its[cur.p0];
+            if (p.i0 < 0) continue;
+            f.rshot[r + k0] = 2 * cur.jb;
+            f.ridx[r + k0] = p.i0;
+            f.items.push_back(make_int4(p.t, r + k0, -1, cur.jb));
+            ++k0;
+        }
+        for (; k1 < w1; ++cur.p1) {
+            const PairIdx& p = its[cur.p1];
+            if (p.i1 < 0) continue;
+            f.rshot[r + w0 + k1] = 2 * cur.jb + 1;
+            f.ridx[r + w0 + k1] = p.i1;
+            f.items.push_back(make_int4(p.t, -1, r + w0 + k1, cur.jb));
+            ++k1;
+        }
+        if (w0 > 0) segment(2 * cur.jb, r, r + w0);
+        if (w1 > 0) segment(2 * cur.jb + 1, r + w0, r + w0 + w1);
+        f.rows += w0 + w1;
+        if (e < ni) break;  // the launch is full
     }
 }
 

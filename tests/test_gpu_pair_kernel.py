@@ -10,15 +10,15 @@ illuminated counts exactly.  Cases: odd shot counts (a block with one
 shot), consecutive shots whose apertures light different facets and take
 different targets, a dark shot inside a block, launches that cut a
 block's items and batches of odd size, prisms, general facet frames,
-tilted target normals, the second medium, and pair windows wider than the
-kernel's 4096 bins.
+tilted target normals, the second medium, pair windows wider than the
+kernel's 5056 bins, and deterministic runs under several launch shapes.
 """
 import numpy as np
 import pytest
 
 from europaraypaths_amd import surfaces, workloads
-from europaraypaths_amd._abi import (ERP_TUNE_PAIRS_PER_LAUNCH, ERP_TUNE_SHOTS_PER_BATCH,
-                                    ERP_TUNE_SINGLE_SHOT, PRISM_DTYPE)
+from europaraypaths_amd._abi import (ERP_RUN_DETERMINISTIC, ERP_TUNE_PAIRS_PER_LAUNCH,
+                                    ERP_TUNE_SHOTS_PER_BATCH, ERP_TUNE_SINGLE_SHOT, PRISM_DTYPE)
 from europaraypaths_amd.params import make_shots, parse_parameters, prism_alpha
 from oracle import oracle
 from tests.helpers import compare_traces
@@ -127,7 +127,7 @@ def test_prisms_frames_tilted_targets_and_second_medium(gpu_ctx):
 
 def test_windows_wider_than_the_kernel(gpu_ctx):
     """85 deg aperture over a 30 km mesh from 3 km: a pair's ranges span
-    about 6200 bins, past the kernel's 4096-bin windows (the row takes the
+    about 6200 bins, past the kernel's 5056-bin windows (the row takes the
     rest through atomics)."""
     print("wide windows", flush=True)
     par, soa, shots = _scene(nx=300, ny=40, fs=100.0, rough=0.3, nr=16384, aperture=85.0,
@@ -136,10 +136,45 @@ def test_windows_wider_than_the_kernel(gpu_ctx):
     _both_kernels(gpu_ctx, par, soa, shots, tg)
 
 
+@pytest.mark.parametrize("pairs,shots_per_batch", [(40, 0), (70, 0), (150, 0), (333, 0),
+                                                   (70, 3), (100, 1)])
+def test_deterministic_launch_shapes_give_the_same_bits(gpu_ctx, pairs, shots_per_batch):
+    """ERP_RUN_DETERMINISTIC with the shot-pair kernel: 7 shots 40 m apart
+    at 5 deg aperture over a 40 x 5 target grid (about 92 targets per shot,
+    the sets shifting from one shot to the next, so the two shots' row
+    counts of a block differ).  Launches of fewer rows than a block holds
+    cut its items; each shot's rows are still sliced at multiples of 32 of
+    its own target index, so every launch shape and batch size gives the
+    bits of the default run (40 rows: below two slices, one shot at a time;
+    70 to 333: merged items and per-shot pads; 3 and 1 shots per batch:
+    other pairings), which matches the FP64 oracle."""
+    print(f"deterministic, pairs per launch {pairs}, shots per batch {shots_per_batch}",
+          flush=True)
+    par, soa, shots = _scene(aperture=5.0, ns=7, sdx=40.0)
+    tg = _grid(40, 5, 600.0, 40.0, 280.0)
+    e = par.to_erp()
+    o, ov = oracle.run(e, soa, tg, shots)
+    m = gpu_ctx.upload_mesh(soa)
+    t = gpu_ctx.upload_targets(*tg)
+    try:
+        a, av = gpu_ctx.run(e, m, t, shots, flags=ERP_RUN_DETERMINISTIC)
+        gpu_ctx.set_tuning(ERP_TUNE_PAIRS_PER_LAUNCH, pairs)
+        gpu_ctx.set_tuning(ERP_TUNE_SHOTS_PER_BATCH, shots_per_batch)
+        b, bv = gpu_ctx.run(e, m, t, shots, flags=ERP_RUN_DETERMINISTIC)
+    finally:
+        gpu_ctx.set_tuning(ERP_TUNE_PAIRS_PER_LAUNCH, 0)
+        gpu_ctx.set_tuning(ERP_TUNE_SHOTS_PER_BATCH, 0)
+        m.release()
+        t.release()
+    np.testing.assert_array_equal(av, ov)
+    np.testing.assert_array_equal(bv, ov)
+    print(f"  worst {compare_traces(a, o)}", flush=True)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 def test_full_size_consecutive_shots(gpu_ctx):
     """Shots 1000-1002 of the 2048 x 2048 fBm workload with its two target
-    layers (about 9.8e4 lit facets and 2.3e3 targets per shot, most pair
-    windows wider than 4096 bins by the shot bounds): the shot-pair kernel
+    layers (about 9.8e4 lit facets and 2.3e3 targets per shot): the shot-pair kernel
     against the single-shot kernel, each to the parity tolerance of the
     other."""
     p, soa, tg, shots = workloads.fbm_two_layer(ns=2000)
