@@ -137,20 +137,23 @@ def test_tracks_shot_pair_and_single_shot_kernels(gpu_ctx):
 
 def test_tracks_record_buffer_grows_for_each_layout():
     """One context's multi-track record buffer serves both record layouts:
-    a shot's packed records (deterministic mode, 60 B each) and the
-    shot-pair kernel's block records (84 B each).  A deterministic call that
-    sizes it, then a default call with fewer records than that capacity,
-    then the deterministic call again: the buffer must grow for the larger
-    layout (a capacity counted in records alone would not), and every call
-    must match the oracle and the first bit for bit."""
+    the single-shot kernel's packed records (60 B each) and the shot-pair
+    kernel's block records (84 B each).  A deterministic single-shot call
+    that sizes it, then a shot-pair call with fewer records than that
+    capacity, then the first call again: the buffer must grow for the larger
+    layout (a capacity counted in records alone would not), every call must
+    match the oracle, and the third the first bit for bit."""
     from europaraypaths_amd._native import Context
     ctx = Context(0)
     tracks = _tracks()[:4]
     up = _Uploaded(ctx, tracks)
     try:
-        print("deterministic, default, deterministic on a fresh context", flush=True)
+        print("single-shot, shot-pair, single-shot on a fresh context", flush=True)
+        ctx.set_tuning(ERP_TUNE_SINGLE_SHOT, 1)
         a = ctx.run_tracks(up.jobs, flags=ERP_RUN_DETERMINISTIC)
+        ctx.set_tuning(ERP_TUNE_SINGLE_SHOT, 0)
         b = ctx.run_tracks(up.jobs)
+        ctx.set_tuning(ERP_TUNE_SINGLE_SHOT, 1)
         c = ctx.run_tracks(up.jobs, flags=ERP_RUN_DETERMINISTIC)
         for (name, par, soa, tg, shots), (e, *_), (ga, _), (gb, vb), (gc, _) in zip(
                 tracks, up.jobs, a, b, c):
