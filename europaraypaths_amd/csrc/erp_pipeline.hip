@@ -2560,8 +2560,8 @@ __device__ __forceinline__ Tri tri_shot(const TLeg& g, double itd, float2 src, d
 // a row of -1: that shot does not take the target
 constexpr int NTB = 1024;  // threads of k_target_accum_pair (one workgroup per CU)
 // bins of each of its two FP64 windows: 2 x 79 KB of the CU's 160 KB of LDS.
-// On the 2048^2 fBm workload the shot bounds allow 2.3e3-5.8e3 bins per pair;
-// 77 % of its pairs fit 5056 bins (28 % fit 4096).
+// On the 2048^2 fBm workload every pair's bounds (the nearer of the shot's
+// box and bounding-sphere bounds) fit 5056 bins.
 constexpr int TWB = 5056;
 
 // MT: a multi-track batch: an item's block is two rows of one track, whose
