@@ -3,8 +3,9 @@
 The allocation-free entry point runs the same kernels with every work buffer
 carved from the library's per-device scratch array; batch shapes shrink to
 fit it.  Its traces must match erp_run_dev's within the parity tolerance
-(tests/helpers.py::compare_traces) and, where the batch shapes stay the
-same, bit for bit in the deterministic mode.  Calls follow each other with
+(tests/helpers.py::compare_traces) and, in the deterministic mode, bit for
+bit, also where its batches and launches shrink (each shot's fixed-point
+scale and 32-row slices do not depend on them).  Calls follow each other with
 smaller inputs (the array keeps the previous call's contents), a job whose
 per-shot buffers cannot shrink below ERP_SCRATCH_BYTES is rejected with
 ERP_E_NOMEM, and the entry point still works after that rejection.
@@ -134,7 +135,9 @@ def test_noalloc_cropped_chain_and_deterministic_specular(gpu_ctx):
 def test_noalloc_shrinks_batches_to_fit(gpu_ctx):
     """nr = 16384 (256 KB pair rows: the launch takes fewer pairs) and a
     mesh of 331 776 facets lit by every shot (their records, 20 MB per shot,
-    cut each batch to the shots that fit), then a small call again."""
+    cut each batch to the shots that fit, which pairs other shots), then a
+    small call again; each also in the deterministic mode, where the
+    smaller batches and launches must give erp_run_dev's bits."""
     cases = [_scene(128, 10.0, 16384, 5, ntg=40),
              _scene(576, 4.0, 2048, 12, aperture=80.0, ntg=3),
              _scene(32, 10.0, 2048, 3, ntg=2)]
@@ -145,6 +148,10 @@ def test_noalloc_shrinks_batches_to_fit(gpu_ctx):
         np.testing.assert_array_equal(gv, rv)
         assert (gv > 0).all()
         compare_traces(g, r)
+        print(f"noalloc shrink call {k}, deterministic", flush=True)
+        (g, gv), (r, rv) = _both(gpu_ctx, par, soa, tg, shots, flags=ERP_RUN_DETERMINISTIC)
+        np.testing.assert_array_equal(gv, rv)
+        assert np.array_equal(g.view(np.uint32), r.view(np.uint32))
 
 
 def test_noalloc_rejects_a_shot_that_cannot_fit(gpu_ctx):
