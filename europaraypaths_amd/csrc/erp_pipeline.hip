@@ -3920,6 +3920,10 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     int C = (int)std::max<int64_t>(1, kPairRowBytes / ((int64_t)P * 8 * nbuf));
     C = std::min<int>(C, kMaxPairsPerLaunch);
     if (ctx->tune_pairs > 0) C = (int)std::min<int64_t>(C, ctx->tune_pairs);
+    // deterministic: launches of at least one slice, so that every shot's
+    // rows are cut at multiples of GS of its own targets whatever the tuning
+    const int Cmin_det = det && half ? GS : 1;
+    C = std::max(C, Cmin_det);
     // shot-pair target kernel (k_target_accum_pair): plain runs of the
     // product's arithmetic whose targets all have a type != 0, unless
     // ERP_TUNE_SINGLE_SHOT asks for the single-shot kernel
@@ -3993,9 +3997,10 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         };
         const size_t avail = arena->left();
         while (layout(Smax, C, false) > avail / 4 * 3) {
+            const int Cmin = std::max(Cmin_det, pairk ? 2 : 1);
             if (any_typed && C > 256) C /= 2;
             else if (Smax > 1) Smax = (Smax + 1) / 2;
-            else if (any_typed && C > (pairk ? 2 : 1)) C /= 2;
+            else if (any_typed && C > Cmin) C = std::max(C / 2, Cmin);
             else
                 throw ErpError{ERP_E_NOMEM,
                                "job too large for the allocation-free entry point: one shot's "
@@ -4839,6 +4844,8 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
     int C = (int)std::max<int64_t>(1, kPairRowBytes / ((int64_t)P * 8 * 2));
     C = std::min<int>(C, kMaxPairsPerLaunch);
     if (ctx->tune_pairs > 0) C = (int)std::min<int64_t>(C, ctx->tune_pairs);
+    // deterministic: launches of at least one slice (as in a single track)
+    if (det) C = std::max(C, GS);
     // the shot-pair kernel (default arithmetic, unless ERP_TUNE_SINGLE_SHOT):
     // blocks of two rows of one track, so R is even and a track with an odd
     // shot count ends on a padding row

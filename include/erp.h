@@ -332,7 +332,9 @@ int erp_profile_read(erp_ctx* ctx, int stage, double* ms, int64_t* launches, dou
  *   ERP_TUNE_SHOTS_PER_BATCH : shots per batch (default and maximum 32)
  *   ERP_TUNE_PAIRS_PER_LAUNCH: (shot, target) pairs per target launch
  *                              (default: the pairs whose rows fit 6 GB,
- *                              at most 16384)
+ *                              at most 16384; with ERP_RUN_DETERMINISTIC
+ *                              at least 32, so that the results do not
+ *                              depend on it)
  *   ERP_TUNE_TRACK_ROWS      : (track, shot) rows per multi-track batch
  *                              (default 64, at most 256)
  *   ERP_TUNE_DIRECT_LAUNCH   : > 0: multi-track batches launch their kernels

@@ -136,8 +136,8 @@ def test_windows_wider_than_the_kernel(gpu_ctx):
     _both_kernels(gpu_ctx, par, soa, shots, tg)
 
 
-@pytest.mark.parametrize("pairs,shots_per_batch", [(40, 0), (70, 0), (150, 0), (333, 0),
-                                                   (70, 3), (100, 1)])
+@pytest.mark.parametrize("pairs,shots_per_batch", [(7, 0), (40, 0), (70, 0), (150, 0),
+                                                   (333, 0), (70, 3), (100, 1)])
 def test_deterministic_launch_shapes_give_the_same_bits(gpu_ctx, pairs, shots_per_batch):
     """ERP_RUN_DETERMINISTIC with the shot-pair kernel: 7 shots 40 m apart
     at 5 deg aperture over a 40 x 5 target grid (about 92 targets per shot,
@@ -145,9 +145,10 @@ def test_deterministic_launch_shapes_give_the_same_bits(gpu_ctx, pairs, shots_pe
     counts of a block differ).  Launches of fewer rows than a block holds
     cut its items; each shot's rows are still sliced at multiples of 32 of
     its own target index, so every launch shape and batch size gives the
-    bits of the default run (40 rows: below two slices, one shot at a time;
-    70 to 333: merged items and per-shot pads; 3 and 1 shots per batch:
-    other pairings), which matches the FP64 oracle."""
+    bits of the default run (7 rows: raised to one slice of 32 in this
+    mode; 40 rows: below two slices, one shot at a time; 70 to 333: merged
+    items and per-shot pads; 3 and 1 shots per batch: other pairings),
+    which matches the FP64 oracle."""
     print(f"deterministic, pairs per launch {pairs}, shots per batch {shots_per_batch}",
           flush=True)
     par, soa, shots = _scene(aperture=5.0, ns=7, sdx=40.0)
