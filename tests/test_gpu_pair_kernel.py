@@ -123,6 +123,24 @@ def test_prisms_frames_tilted_targets_and_second_medium(gpu_ctx):
     pr[0] = (prism_alpha(par, 1e-2), -200, -200, -250, 0, 200, 0)
     pr[1] = (prism_alpha(par, 5e-3), 0, -200, -400, 200, 200, -100)
     _both_kernels(gpu_ctx, par, soa, shots, (xyzn, ty), prisms=pr)
+    # the deterministic variant of the shot-pair kernel on the same scene:
+    # its per-shot bound must hold with the prisms' and the second medium's
+    # factors (a broken bound would overflow the fixed point or trip the
+    # window check); against the oracle, and two runs give the same bits
+    print("  deterministic", flush=True)
+    e = par.to_erp()
+    o, ov = oracle.run(e, soa, (xyzn, ty), shots, prisms=pr)
+    m = gpu_ctx.upload_mesh(soa)
+    t = gpu_ctx.upload_targets(xyzn, ty)
+    try:
+        a, av = gpu_ctx.run(e, m, t, shots, prisms=pr, flags=ERP_RUN_DETERMINISTIC)
+        b, _ = gpu_ctx.run(e, m, t, shots, prisms=pr, flags=ERP_RUN_DETERMINISTIC)
+    finally:
+        m.release()
+        t.release()
+    np.testing.assert_array_equal(av, ov)
+    print(f"  worst {compare_traces(a, o)}", flush=True)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
 def test_windows_wider_than_the_kernel(gpu_ctx):
