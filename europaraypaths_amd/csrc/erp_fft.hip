@@ -789,6 +789,263 @@ __global__ void __launch_bounds__(NTB) k_tpair_half(const float2* __restrict__ A
     }
 }
 
+// ---------------------------------------------------------------------------
+// Wave-local transforms of the 16384-point pair chain (1024 threads: 16
+// waves x 64 lanes x 16 registers).  With t = 64 w + l the thread and its
+// register r holding element t + 1024 r (the natural layout), a transform
+// is 16 (registers) x 16 (waves) x 64 (lanes):
+//   F, natural -> permuted: DFT16 over r, twiddle W_n^(t k1); one exchange
+//     across waves (element (t, k1) at k1 1024 + t) and DFT16 over w, twiddle
+//     W_1024^(l kw); then two exchanges inside the wave's own 1024 slots:
+//     DFT16 over l_hi (l = l_lo + 4 l_hi), twiddle W_64^(l_lo ka), and four
+//     DFT4 over l_lo.  Thread (w, l = g + 4 kw) register h + 4 kb then holds
+//     element w + 16 kw + 1024 g + 256 h + 4096 kb (wl_perm).
+//   G, permuted -> natural: the same steps transposed in reverse order.
+// Both compute the forward DFT (the DFT matrix is symmetric).  F takes three
+// workgroup barriers and G two, against six for fft_regs; the exchanges
+// inside a wave need only a wave-level fence, as LDS operations of one wave
+// complete in order.  A row position p = t + 1024 r holds element p in the
+// natural layout and element wl_perm(p) in the permuted one, so tables read
+// in the permuted layout are permuted copies (k_wl_perm_table).
+// ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ int wl_perm(int p) {
+    const int rho = p >> 10, t = p & 1023, w = t >> 6, mu = t & 63;
+    return w + 16 * (mu >> 2) + 1024 * (mu & 3) + 256 * (rho & 3) + 4096 * (rho >> 2);
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// padded slot of element e: pad16(e); for the constant strides below
+// pad16(a + 64 j) = pad16(a) + 68 j and pad16(a + 1024 j) = pad16(a) + 1088 j
+__device__ __forceinline__ void fft_wl_F(float2* s, float2 (&x)[16],
+                                         const float2* __restrict__ lad) {
+    int tid = opaque_tid();
+    dft_r<16>(x, false);
+    apply_twiddles<16>(x, lad, 16384, tid, false);
+    __syncthreads();  // earlier readers of s are done
+    {
+        float2* d = s + pad16(tid);
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) d[k1 * 1088] = x[k1];
+    }
+    __syncthreads();
+    tid = opaque_tid();
+    const int w = tid >> 6, l = tid & 63;
+    {
+        const float2* src = s + pad16(w * 1024 + l);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) x[j] = src[j * 68];
+    }
+    dft_r<16>(x, false);
+    apply_twiddles<16>(x, lad, 1024, l, false);
+    __syncthreads();  // every wave's reads across waves done before the wave-local writes
+    float2* R = s + w * 1088;  // this wave's slots: elements [1024 w, 1024 w + 1024)
+    {
+        float2* d = R + pad16(l);
+#pragma unroll
+        for (int kw = 0; kw < 16; ++kw) d[kw * 68] = x[kw];
+    }
+    wave_sync();
+    const int lo = l & 3, kq = l >> 2;  // l_lo and kw of this lane's next steps
+    {
+        // element kq 64 + lo + 4 j at kq 68 + lo + 4 j + j / 4
+        const float2* src = R + kq * 68 + lo;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) x[j] = src[4 * j + (j >> 2)];
+    }
+    dft_r<16>(x, false);
+    apply_twiddles<16>(x, lad, 64, lo, false);
+    wave_sync();
+    {
+        float2* d = R + kq * 68 + lo;
+#pragma unroll
+        for (int ka = 0; ka < 16; ++ka) d[4 * ka + (ka >> 2)] = x[ka];
+    }
+    wave_sync();
+    {
+        // element kq 64 + 16 g + 4 h + m at kq 68 + 17 g + 4 h + m (g = lo)
+        const float2* src = R + kq * 68 + 17 * lo;
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            float2 y0 = src[4 * h], y1 = src[4 * h + 1], y2 = src[4 * h + 2], y3 = src[4 * h + 3];
+            dft4(y0, y1, y2, y3, false);
+            x[h] = y0;
+            x[h + 4] = y1;
+            x[h + 8] = y2;
+            x[h + 12] = y3;
+        }
+    }
+}
+
+__device__ __forceinline__ void fft_wl_G(float2* s, float2 (&x)[16],
+                                         const float2* __restrict__ lad) {
+    int tid = opaque_tid();
+    const int w = tid >> 6, l = tid & 63;
+    const int lo = l & 3, kq = l >> 2;
+    float2* R = s + w * 1088;
+    {
+        float2* d = R + kq * 68 + 17 * lo;
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            float2 y0 = x[h], y1 = x[h + 4], y2 = x[h + 8], y3 = x[h + 12];
+            dft4(y0, y1, y2, y3, false);
+            d[4 * h] = y0;
+            d[4 * h + 1] = y1;
+            d[4 * h + 2] = y2;
+            d[4 * h + 3] = y3;
+        }
+    }
+    wave_sync();
+    {
+        const float2* src = R + kq * 68 + lo;
+#pragma unroll
+        for (int ka = 0; ka < 16; ++ka) x[ka] = src[4 * ka + (ka >> 2)];
+    }
+    apply_twiddles<16>(x, lad, 64, lo, false);
+    dft_r<16>(x, false);
+    wave_sync();
+    {
+        float2* d = R + kq * 68 + lo;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) d[4 * j + (j >> 2)] = x[j];
+    }
+    wave_sync();
+    {
+        const float2* src = R + pad16(l);
+#pragma unroll
+        for (int kw = 0; kw < 16; ++kw) x[kw] = src[kw * 68];
+    }
+    apply_twiddles<16>(x, lad, 1024, l, false);
+    dft_r<16>(x, false);
+    __syncthreads();  // every wave's own-slot reads done before the writes across waves
+    {
+        float2* d = s + pad16(w * 1024 + l);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) d[j * 68] = x[j];
+    }
+    __syncthreads();
+    tid = opaque_tid();
+    {
+        const float2* src = s + pad16(tid);
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) x[k1] = src[k1 * 1088];
+    }
+    apply_twiddles<16>(x, lad, 16384, tid, false);
+    dft_r<16>(x, false);
+}
+
+// Gp[p] = Gn[wl_perm(p)], p < 16384 (the (G[2k], G[2k+1]) pairs of a range
+// kernel spectrum in the permuted layout)
+__global__ void k_wl_perm_table(float4* __restrict__ Gp, const float4* __restrict__ Gn) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p < 16384) Gp[p] = Gn[wl_perm(p)];
+}
+
+// k_tpair_half for n = 16384 with the wave-local transforms: the same
+// phases, with F where a transform takes natural input and G where it takes
+// permuted input (plain: F G F F G F; cropped: F G F G F F).  Gn is the
+// permuted spectrum (k_wl_perm_table), the output planes are written in the
+// permuted layout, and the inputs, m(j) and the cropped z stay natural.
+template <bool MT = false, bool CROP = false>
+__global__ void __launch_bounds__(1024) k_tpair_wl(const float2* __restrict__ A, int P,
+                                                   const int* __restrict__ pbase,
+                                                   const int* __restrict__ pwin,
+                                                   const float4* __restrict__ Gn,
+                                                   float2* __restrict__ out,
+                                                   const float2* __restrict__ lad,
+                                                   const float4* const* __restrict__ Gpair,
+                                                   const int* __restrict__ npairs, int nr) {
+    constexpr int NTB = 1024, n = 16384;
+    __shared__ float2 s[n + n / 16];
+    const int p = blockIdx.x;
+    if (MT) {
+        if (p >= *npairs) return;
+        Gn = Gpair[p];
+    }
+    const int base = pbase[p], win = pwin[p];
+    const float2* a = A + (size_t)p * P;
+    const float2* mod = lad + 2 * n;
+    float2* o2 = out + (size_t)p * P;
+    const float fn = (float)n;
+    float2 x[PPT], h[PPT];
+    if constexpr (CROP) {
+#pragma unroll 1
+        for (int ph = 0; ph < 6; ++ph) {
+            if (ph == 0 || ph == 2)
+                load_window<NTB, true>(x, a, base, win, ph == 2 ? mod : nullptr, nr);
+            if (ph == 5) {
+#pragma unroll
+                for (int q = 0; q < PPT; ++q) x[q] = h[q];
+            }
+            if (ph == 1 || ph == 3) fft_wl_G(s, x, lad);
+            else fft_wl_F(s, x, lad);
+            const int tid = opaque_tid();
+#pragma unroll
+            for (int q = 0; q < PPT; ++q) {
+                const int i = tid + q * NTB;
+                if (ph == 0) {
+                    const float4 g = Gn[i];
+                    o2[i] = x[q];
+                    x[q] = conjf2(cmul(x[q], make_float2(g.x, g.y)));
+                } else if (ph == 1) {
+                    h[q] = conjf2(x[q]);
+                } else if (ph == 2) {
+                    const float4 g = Gn[i];
+                    o2[n + i] = x[q];
+                    x[q] = conjf2(cmul(x[q], make_float2(g.z, g.w)));
+                } else if (ph == 3) {
+                    const float2 mx = cmul(mod[i], x[q]);
+                    const float2 z = i < nr ? make_float2(h[q].x - mx.x, h[q].y + mx.y)
+                                            : make_float2(0.f, 0.f);
+                    x[q] = z;
+                    h[q] = cmul(z, mod[i]);
+                } else if (ph == 4) {
+                    o2[i] = cmul(o2[i], x[q]);
+                } else {
+                    o2[n + i] = cmul(o2[n + i], x[q]);
+                }
+            }
+        }
+        return;
+    }
+#pragma unroll 1
+    for (int ph = 0; ph < 7; ++ph) {
+        if (ph == 0 || ph == 3) load_window<NTB>(x, a, base, win, ph == 3 ? mod : nullptr);
+        if (ph == 1 || ph == 4) fft_wl_G(s, x, lad);
+        else if (ph != 6) fft_wl_F(s, x, lad);
+        const int tid = opaque_tid();
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            const int i = tid + q * NTB;
+            if (ph == 0) {
+                const float4 g = Gn[i];
+                o2[i] = x[q];
+                x[q] = conjf2(cmul(x[q], make_float2(g.x, g.y)));
+            } else if (ph == 1) {
+                x[q] = cmul(conjf2(x[q]), mod[i]);
+            } else if (ph == 2 || ph == 5) {
+                h[q] = x[q];
+            } else if (ph == 3) {
+                const float4 g = Gn[i];
+                const float2 og = cmul(x[q], make_float2(g.z, g.w));
+                o2[n + i] = cmul(x[q], make_float2(h[q].x - fn * og.x, h[q].y - fn * og.y));
+                x[q] = conjf2(og);
+            } else if (ph == 4) {
+                x[q] = conjf2(cmul(x[q], mod[i]));
+            } else {
+                const float4 g = Gn[i];
+                const float2 e = o2[i];
+                const float2 eg = cmul(e, make_float2(g.x, g.y));
+                o2[i] = cmul(e, make_float2(fn * eg.x - h[q].x, fn * eg.y - h[q].y));
+            }
+        }
+    }
+}
+
 // Two-level deterministic sum of per-pair rows (P complex each, as the two
 // planes written by k_tpair_half: [0, P/2) even, [P/2, P) odd frequencies).
 // Level 1: partial[slice] = sum of the rows of one slice (at most SLICE rows
@@ -842,13 +1099,17 @@ __global__ void __launch_bounds__(256) k_sum_final(float2* __restrict__ S,
                                                    const int* __restrict__ seg_beg,
                                                    const int* __restrict__ seg_end, int P,
                                                    float scale, int pN1, int pN2,
-                                                   const float* __restrict__ seg_scale, int seq) {
+                                                   const float* __restrict__ seg_scale, int seq,
+                                                   int wl) {
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= P) return;
     const int sg = blockIdx.y;
     if (seg_beg[sg] == seg_end[sg]) return;  // padding segment
     if (seg_scale) scale = seg_scale[sg];   // segments of several tracks
-    const int kd = pN1 > 0 ? (k % pN2) * pN1 + k / pN2 : k;
+    // wl: rows in the permuted layout of k_tpair_wl (planes of n = P / 2:
+    // position k / 2 of plane k % 2 holds frequency 2 wl_perm(k / 2) + k % 2)
+    const int f = wl ? 2 * wl_perm(k >> 1) + (k & 1) : k;
+    const int kd = pN1 > 0 ? (f % pN2) * pN1 + f / pN2 : f;
     float2* d = S + (size_t)seg_shot[sg] * P + kd;
     if (seq) {
         float2 a = *d;
@@ -1007,6 +1268,14 @@ void target_spectra_fused(erp_ctx* ctx, const FftPlan& p, float2* A, float2* Y,
 
 int target_half_slice() { return SLICE; }
 
+bool target_wl(const FftPlan& p) { return p.N == 32768; }
+
+void wl_perm_table(const float2* Gnat, float2* Gwl, hipStream_t st) {
+    hipLaunchKernelGGL(k_wl_perm_table, dim3(16384 / 256), dim3(256), 0, st,
+                       reinterpret_cast<float4*>(Gwl), reinterpret_cast<const float4*>(Gnat));
+    ERP_CHECK_LAUNCH("k_wl_perm_table");
+}
+
 // P = 2n with n the next power of two of nr, from 1024 to 16384 (nr below
 // n runs the cropped variant)
 bool target_half_supported(const FftPlan& p, int nr) {
@@ -1052,6 +1321,13 @@ void target_pairs_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const in
 #define ERP_TPAIR_VARIANTS(NTB)                                                       \
     pick(k_tpair_half<NTB, false, false>, k_tpair_half<NTB, true, false>,            \
          k_tpair_half<NTB, false, true>, k_tpair_half<NTB, true, true>, NTB)
+    if (target_wl(p)) {
+        // the wave-local transforms (Gnat in their layout)
+        pick(k_tpair_wl<false, false>, k_tpair_wl<true, false>, k_tpair_wl<false, true>,
+             k_tpair_wl<true, true>, 1024);
+        ERP_CHECK_LAUNCH("k_tpair_wl");
+        return;
+    }
     switch (p.N / 2) {
         case 1024: ERP_TPAIR_VARIANTS(64); break;
         case 2048: ERP_TPAIR_VARIANTS(128); break;
@@ -1070,13 +1346,13 @@ void target_pairs_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const in
 void target_sum_half(const FftPlan& p, const float2* work, float2* partial, const int* sl_beg,
                      const int* sl_end, int nslices, const int* seg_shot, const int* seg_beg,
                      const int* seg_end, int nseg, float2* S, float scale, hipStream_t st,
-                     const float* seg_scale, bool seq) {
+                     const float* seg_scale, bool seq, bool wl) {
     const int P2 = p.N / 2;
     hipLaunchKernelGGL(k_sum_slices, dim3((P2 + 255) / 256, nslices), dim3(256), 0, st,
                        reinterpret_cast<float4*>(partial), work, sl_beg, sl_end, P2);
     ERP_CHECK_LAUNCH("k_sum_slices");
     hipLaunchKernelGGL(k_sum_final, dim3((p.N + 255) / 256, nseg), dim3(256), 0, st, S, partial,
                        seg_shot, seg_beg, seg_end, p.N, scale, p.two_pass ? p.N1 : 0,
-                       p.two_pass ? p.N2 : 0, seg_scale, seq ? 1 : 0);
+                       p.two_pass ? p.N2 : 0, seg_scale, seq ? 1 : 0, wl ? 1 : 0);
     ERP_CHECK_LAUNCH("k_sum_final");
 }

@@ -170,6 +170,7 @@ struct ChirpSpec {
     ChirpKey key;
     DevBuf spec;  // P complex64, permuted forward order, scaled by 1/P
     DevBuf nat;   // the same spectrum in natural order (filled on first use)
+    DevBuf wl;    // the natural one in the layout of k_tpair_wl (first use)
 };
 
 struct BlueTables {
@@ -283,6 +284,12 @@ void fft_init_twiddles(erp_ctx* ctx);
 // next power of two of nr, from 1024 to 16384 (nr < n: the cropped variant)
 bool target_half_supported(const FftPlan& p, int nr);
 int target_half_slice();
+// P = 32768: the chain runs with the wave-local transforms (k_tpair_wl); its
+// range kernel spectra are then in that kernel's layout (wl_perm_table) and
+// its rows are summed with target_sum_half(..., wl = true)
+bool target_wl(const FftPlan& p);
+// Gwl = Gnat in the layout of k_tpair_wl (P complex64 each)
+void wl_perm_table(const float2* Gnat, float2* Gwl, hipStream_t st);
 // Gpair / npairs_dev (multi-track batches): per-pair range kernel spectra
 // and the pair count on the device, n then being the grid's bound
 void target_pairs_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const int* pbase,
@@ -295,7 +302,7 @@ void target_pairs_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const in
 void target_sum_half(const FftPlan& p, const float2* work, float2* partial, const int* sl_beg,
                      const int* sl_end, int nslices, const int* seg_shot, const int* seg_beg,
                      const int* seg_end, int nseg, float2* S, float scale, hipStream_t st,
-                     const float* seg_scale = nullptr, bool seq = false);
+                     const float* seg_scale = nullptr, bool seq = false, bool wl = false);
 // natural-order copy of a forward spectrum produced by fft_forward
 void fft_natural_order(const FftPlan& p, const float2* src, float2* dst, hipStream_t st);
 void target_spectra_fused(erp_ctx* ctx, const FftPlan& p, float2* A, float2* Y,

@@ -115,7 +115,8 @@ struct TrackK {
     int T;
     const unsigned char* ttype;
     const float2* g_surf;          // surface range kernel spectrum (fft order)
-    const float2* g_targ_nat;      // target range kernel spectrum (natural order)
+    const float2* g_targ_nat;      // target range kernel spectrum (natural order; at
+                                   // P = 32768 in the layout of k_tpair_wl)
     const float2* blue_w;          // Bluestein tables of the track's band
     const float2* blue_Bw;
     const unsigned char* blue_mask;
@@ -3273,8 +3274,10 @@ void build_chirp(erp_ctx* ctx, float2* dst, int nr, float dr, float res, int kin
 // returns it in natural frequency order (the per-pair LDS chain), otherwise
 // in the order fft_forward produces.  Only kind-1 entries (one per receive
 // window start) are evicted, so kind-0 pointers held by a run stay valid.
+// layout: 0 the forward FFT's order, 1 natural, 2 natural in the layout of
+// k_tpair_wl (wl_perm_table)
 const float2* get_chirp(erp_ctx* ctx, int nr, float dr, float res, int kind, float rst, int P,
-                        const FftPlan& plan, hipStream_t st, bool natural = false) {
+                        const FftPlan& plan, hipStream_t st, int layout = 0) {
     ChirpKey key{nr, dr, res, kind, kind == 1 ? rst : 0.f};
     std::lock_guard<std::recursive_mutex> lk(ctx->cache_mu);
     ChirpSpec* c = nullptr;
@@ -3301,13 +3304,19 @@ const float2* get_chirp(erp_ctx* ctx, int nr, float dr, float res, int kind, flo
         }
         ctx->chirps.push_back(c);
     }
-    if (!natural) return c->spec.as<float2>();
+    if (layout == 0) return c->spec.as<float2>();
     if (!c->nat.p) {
         c->nat.ensure(sizeof(float2) * (size_t)P);
         fft_natural_order(plan, c->spec.as<float2>(), c->nat.as<float2>(), st);
         ERP_HIP_TRY(hipStreamSynchronize(st));
     }
-    return c->nat.as<float2>();
+    if (layout == 1) return c->nat.as<float2>();
+    if (!c->wl.p) {
+        c->wl.ensure(sizeof(float2) * (size_t)P);
+        wl_perm_table(c->nat.as<float2>(), c->wl.as<float2>(), st);
+        ERP_HIP_TRY(hipStreamSynchronize(st));
+    }
+    return c->wl.as<float2>();
 }
 
 // Bluestein tables of b (L, M, smpl, B set; w, Bw and mask sized) built on st
@@ -3862,25 +3871,30 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     const float res_t = spec ? p.rng_res : 2.f * p.rng_res;  // sim.cu:853-858
 
     // chirp spectra: cached per context, or (arena) built into the region
-    auto chirp = [&](float res, bool natural, int kind = 0) -> const float2* {
-        if (!arena) return get_chirp(ctx, nr, p.dr, res, kind, 0.f, P, planP, st, natural);
+    auto chirp = [&](float res, int layout, int kind = 0) -> const float2* {
+        if (!arena) return get_chirp(ctx, nr, p.dr, res, kind, 0.f, P, planP, st, layout);
         float2* spec_ = arena->take_n<float2>((size_t)P, "chirp spectrum");
         build_chirp(ctx, spec_, nr, p.dr, res, kind, 0.f, P, planP, st);
-        if (!natural) return spec_;
+        if (layout == 0) return spec_;
         float2* nat = arena->take_n<float2>((size_t)P, "chirp spectrum");
         fft_natural_order(planP, spec_, nat, st);
-        return nat;
+        if (layout == 1) return nat;
+        float2* wl = arena->take_n<float2>((size_t)P, "chirp spectrum");
+        wl_perm_table(nat, wl, st);
+        return wl;
     };
     const float2* g_surf = nullptr;
-    if (!spec || (conv && lin)) g_surf = chirp(p.rng_res, false);
+    if (!spec || (conv && lin)) g_surf = chirp(p.rng_res, 0);
     const float2* g_targ = nullptr;
-    if (T > 0 && conv && lin) g_targ = chirp(res_t, false);
+    if (T > 0 && conv && lin) g_targ = chirp(res_t, 0);
     // plain mode: per-pair spectral chain in one workgroup when P / 2 fits
     // LDS (nr below P / 2: its cropped variant, whose range kernel has the
     // kept window at P / 2 and whose rows are read up to P / 2)
     const bool half = !spec && T > 0 && conv && lin && target_half_supported(planP, nr);
     kp.row_live = half ? P / 2 : P;
-    const float2* g_targ_nat = half ? chirp(res_t, true, nr < P / 2 ? 2 : 0) : nullptr;
+    // the per-pair chain's spectrum: natural, or in the wave-local layout
+    const bool wl = half && target_wl(planP);
+    const float2* g_targ_nat = half ? chirp(res_t, wl ? 2 : 1, nr < P / 2 ? 2 : 0) : nullptr;
     // circular convolution of type-0 targets: one genChirp table per rx window
     // start; with an arena one table is rebuilt for each shot
     float2* g_circ = (arena && any_t0 && conv && !lin)
@@ -4413,7 +4427,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                         PT_.begin(ERP_STAGE_TARGET_SUM, st, (double)n);
                         target_sum_half(planP, Z, A + (size_t)C * P, dgb, dge, ngroups, dseg,
                                         dseg + S, dseg + 2 * S, nseg, Sacc, -1.0f / kp.sqrtP, st,
-                                        nullptr, det);
+                                        nullptr, det, wl);
                         PT_.end(st);
                     }
                 }
@@ -4486,7 +4500,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
                         PT_.begin(ERP_STAGE_TARGET_SUM, st, (double)n);
                         target_sum_half(planP, Z, A + (size_t)C * P, dgb, dge, ngroups, dseg,
                                         dseg + S, dseg + 2 * S, nseg, Sacc, -1.0f / kp.sqrtP, st,
-                                        nullptr, det);
+                                        nullptr, det, wl);
                         PT_.end(st);
                         continue;
                     }
@@ -4810,8 +4824,8 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
             d.Tpad = t.targets->Tpad;
             d.ttype = t.targets->type.as<unsigned char>();
             // sim.cu:853-858: the target range kernel has twice the surface's
-            d.g_targ_nat =
-                get_chirp(ctx, nr, p.dr, 2.f * p.rng_res, tkind, 0.f, P, planP, st, true);
+            d.g_targ_nat = get_chirp(ctx, nr, p.dr, 2.f * p.rng_res, tkind, 0.f, P, planP, st,
+                                     target_wl(planP) ? 2 : 1);
             targ = true;
         }
         // a disabled surface echo: no spectrum, so the row's echo is zero
@@ -5221,7 +5235,8 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
                     target_pairs_half(ctx, planP, A, d_pbase, d_pwin, nr, C, nullptr, Z, st, d_G,
                                       t_n);
                     target_sum_half(planP, Z, A + (size_t)C * P, t_slb, t_sle, gcap, t_sgs, t_sgb,
-                                    t_sge, segcap, m.Sacc.as<float2>(), 1.0f, st, t_scale, det);
+                                    t_sge, segcap, m.Sacc.as<float2>(), 1.0f, st, t_scale, det,
+                                    target_wl(planP));
                 });
                 PT_.end(st);
             }
@@ -5273,7 +5288,8 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
                 target_pairs_half(ctx, planP, A, d_pbase, d_pwin, nr, C, nullptr, Z, st, d_G,
                                   t_n);
                 target_sum_half(planP, Z, A + (size_t)C * P, t_slb, t_sle, gcap, t_sgs, t_sgb,
-                                t_sge, segcap, m.Sacc.as<float2>(), 1.0f, st, t_scale, det);
+                                t_sge, segcap, m.Sacc.as<float2>(), 1.0f, st, t_scale, det,
+                                target_wl(planP));
             });
             PT_.end(st);
         }
