@@ -59,12 +59,21 @@ KERNELS = {"target_accum": "k_target_accum_pair", "target_fft": "k_tpair_half",
            "surface": "k_surface"}
 
 
+def chain_kernel(nr: int) -> str:
+    """The per-pair spectral chain's kernel: the wave-local transforms at
+    P = 32768 (nr from 8193 to 16384), the register/LDS chain below."""
+    n = 1 << max(0, int(nr - 1).bit_length())
+    return "k_tpair_wl" if n == 16384 else "k_tpair_half"
+
+
 def kernels_of(b):
     """The hot kernels of a bench run: its JSON line names the target
     kernel it ran (the shot-pair kernel, or with --single-shot the
-    single-shot one)."""
+    single-shot one) and the per-pair chain's kernel."""
     k = dict(KERNELS)
-    k["target_accum"] = b.get("config", {}).get("target_kernel", k["target_accum"])
+    cfg = b.get("config", {})
+    k["target_accum"] = cfg.get("target_kernel", k["target_accum"])
+    k["target_fft"] = cfg.get("chain_kernel", k["target_fft"])
     return k
 
 
@@ -759,6 +768,7 @@ def main():
           f"ns={len(shots)} nr={nr} built in {time.perf_counter() - t_build:.1f} s",
           file=sys.stderr, flush=True)
 
+    KERNELS["target_fft"] = chain_kernel(nr)
     ctx = Context(0)
     if a.single_shot:
         from europaraypaths_amd._abi import ERP_TUNE_SINGLE_SHOT
@@ -846,7 +856,8 @@ def main():
         "config": {"workload": a.workload, "facets": int(soa.shape[1]),
                    "targets": 0 if tg is None else int(tg[0].shape[1]),
                    "track_shots": len(shots), "shots_per_step": B, "nr": nr,
-                   "flags": a.flags, "target_kernel": KERNELS["target_accum"]},
+                   "flags": a.flags, "target_kernel": KERNELS["target_accum"],
+                   "chain_kernel": KERNELS["target_fft"]},
         "targets_per_pair": (triples / pairs) if pairs else 0.0,
         "triples_per_s": triples / dt,
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)),
