@@ -801,10 +801,14 @@ __global__ void __launch_bounds__(NTB) k_tpair_half(const float2* __restrict__ A
 //     DFT4 over l_lo.  Thread (w, l = g + 4 kw) register h + 4 kb then holds
 //     element w + 16 kw + 1024 g + 256 h + 4096 kb (wl_perm).
 //   G, permuted -> natural: the same steps transposed in reverse order.
-// Both compute the forward DFT (the DFT matrix is symmetric).  F takes three
-// workgroup barriers and G two, against six for fft_regs; the exchanges
-// inside a wave need only a wave-level fence, as LDS operations of one wave
-// complete in order.  A row position p = t + 1024 r holds element p in the
+// Both compute the forward DFT (the DFT matrix is symmetric).  Only F's
+// writes cross waves (each wave then reads its own 1024 slots) and only G's
+// reads do (each wave has written its own slots), so F takes two workgroup
+// barriers (before its writes, whose slots the previous transform's
+// readers may still hold, and between its writes and reads) and G one,
+// against six for fft_regs; every exchange inside a wave needs only a
+// wave-level fence, as LDS operations of one wave complete in order.  G
+// always follows an F in the chain.  A row position p = t + 1024 r holds element p in the
 // natural layout and element wl_perm(p) in the permuted one, so tables read
 // in the permuted layout are permuted copies (k_wl_perm_table).
 // ---------------------------------------------------------------------------
@@ -842,7 +846,10 @@ __device__ __forceinline__ void fft_wl_F(float2* s, float2 (&x)[16],
     }
     dft_r<16>(x, false);
     apply_twiddles<16>(x, lad, 1024, l, false);
-    __syncthreads();  // every wave's reads across waves done before the wave-local writes
+    // the reads above were of this wave's own slots (elements 1024 w + l +
+    // 64 j), which no other wave touches until the next transform's first
+    // barrier: a wave-level fence orders them before the writes below
+    wave_sync();
     float2* R = s + w * 1088;  // this wave's slots: elements [1024 w, 1024 w + 1024)
     {
         float2* d = R + pad16(l);
@@ -921,7 +928,9 @@ __device__ __forceinline__ void fft_wl_G(float2* s, float2 (&x)[16],
     }
     apply_twiddles<16>(x, lad, 1024, l, false);
     dft_r<16>(x, false);
-    __syncthreads();  // every wave's own-slot reads done before the writes across waves
+    // these writes go to this wave's own slots too (elements 1024 w + l +
+    // 64 j); no other wave has read them since the last F's barrier
+    wave_sync();
     {
         float2* d = s + pad16(w * 1024 + l);
 #pragma unroll
