@@ -803,12 +803,12 @@ __global__ void __launch_bounds__(NTB) k_tpair_half(const float2* __restrict__ A
 //   G, permuted -> natural: the same steps transposed in reverse order.
 // Both compute the forward DFT (the DFT matrix is symmetric).  Only F's
 // writes cross waves (each wave then reads its own 1024 slots) and only G's
-// reads do (each wave has written its own slots), so F takes two workgroup
-// barriers (before its writes, whose slots the previous transform's
-// readers may still hold, and between its writes and reads) and G one,
-// against six for fft_regs; every exchange inside a wave needs only a
-// wave-level fence, as LDS operations of one wave complete in order.  G
-// always follows an F in the chain.  A row position p = t + 1024 r holds element p in the
+// reads do (each wave has written its own slots).  So G takes one
+// workgroup barrier (between its writes and reads) and F one (between its
+// writes and reads) plus, after another F, one before its writes (whose
+// slots that F's readers may still hold), against six for fft_regs; every
+// exchange inside a wave needs only a wave-level fence, as LDS operations
+// of one wave complete in order.  G always follows an F in the chain.  A row position p = t + 1024 r holds element p in the
 // natural layout and element wl_perm(p) in the permuted one, so tables read
 // in the permuted layout are permuted copies (k_wl_perm_table).
 // ---------------------------------------------------------------------------
@@ -825,12 +825,16 @@ __device__ __forceinline__ void wave_sync() {
 
 // padded slot of element e: pad16(e); for the constant strides below
 // pad16(a + 64 j) = pad16(a) + 68 j and pad16(a + 1024 j) = pad16(a) + 1088 j
+// after_f: the previous transform was an F, whose last reads (each wave in
+// its own slots) the writes below may overtake.  After a G no barrier is
+// needed: these writes go to the very slots this thread read last (G's
+// final read, element k1 1024 + t), which no other thread reads or writes.
 __device__ __forceinline__ void fft_wl_F(float2* s, float2 (&x)[16],
-                                         const float2* __restrict__ lad) {
+                                         const float2* __restrict__ lad, bool after_f) {
     int tid = opaque_tid();
     dft_r<16>(x, false);
     apply_twiddles<16>(x, lad, 16384, tid, false);
-    __syncthreads();  // earlier readers of s are done
+    if (after_f) __syncthreads();
     {
         float2* d = s + pad16(tid);
 #pragma unroll
@@ -991,7 +995,7 @@ __global__ void __launch_bounds__(1024) k_tpair_wl(const float2* __restrict__ A,
                 for (int q = 0; q < PPT; ++q) x[q] = h[q];
             }
             if (ph == 1 || ph == 3) fft_wl_G(s, x, lad);
-            else fft_wl_F(s, x, lad);
+            else fft_wl_F(s, x, lad, ph == 5);
             const int tid = opaque_tid();
 #pragma unroll
             for (int q = 0; q < PPT; ++q) {
@@ -1025,7 +1029,7 @@ __global__ void __launch_bounds__(1024) k_tpair_wl(const float2* __restrict__ A,
     for (int ph = 0; ph < 7; ++ph) {
         if (ph == 0 || ph == 3) load_window<NTB>(x, a, base, win, ph == 3 ? mod : nullptr);
         if (ph == 1 || ph == 4) fft_wl_G(s, x, lad);
-        else if (ph != 6) fft_wl_F(s, x, lad);
+        else if (ph != 6) fft_wl_F(s, x, lad, ph == 3);
         const int tid = opaque_tid();
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
