@@ -60,10 +60,10 @@ KERNELS = {"target_accum": "k_target_accum_pair", "target_fft": "k_tpair_half",
 
 
 def chain_kernel(nr: int) -> str:
-    """The per-pair spectral chain's kernel: the wave-local transforms at
-    P = 32768 (nr from 8193 to 16384), the register/LDS chain below."""
+    """The per-pair spectral chain's kernel: the wave-local transforms for
+    n = nextpow2(nr) of 8192 and 16384, the register/LDS chain below."""
     n = 1 << max(0, int(nr - 1).bit_length())
-    return "k_tpair_wl" if n == 16384 else "k_tpair_half"
+    return "k_tpair_wl" if n in (8192, 16384) else "k_tpair_half"
 
 
 def kernels_of(b):

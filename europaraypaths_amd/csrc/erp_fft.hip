@@ -790,16 +790,17 @@ __global__ void __launch_bounds__(NTB) k_tpair_half(const float2* __restrict__ A
 }
 
 // ---------------------------------------------------------------------------
-// Wave-local transforms of the 16384-point pair chain (1024 threads: 16
-// waves x 64 lanes x 16 registers).  With t = 64 w + l the thread and its
-// register r holding element t + 1024 r (the natural layout), a transform
-// is 16 (registers) x 16 (waves) x 64 (lanes):
+// Wave-local transforms of the per-pair chain: n = 1024 W points held by W
+// waves of 64 lanes, 16 registers each.  With t = 64 w + l the thread and
+// its register r holding element t + 64 W r (the natural layout), a
+// transform is 16 (registers) x W (waves) x 64 (lanes):
 //   F, natural -> permuted: DFT16 over r, twiddle W_n^(t k1); one exchange
-//     across waves (element (t, k1) at k1 1024 + t) and DFT16 over w, twiddle
-//     W_1024^(l kw); then two exchanges inside the wave's own 1024 slots:
-//     DFT16 over l_hi (l = l_lo + 4 l_hi), twiddle W_64^(l_lo ka), and four
-//     DFT4 over l_lo.  Thread (w, l = g + 4 kw) register h + 4 kb then holds
-//     element w + 16 kw + 1024 g + 256 h + 4096 kb (wl_perm).
+//     across waves (element (t, k1) at k1 64 W + t; wave w then holds the 16
+//     / W values of k1 in [16 w / W, 16 (w + 1) / W), each over every wave
+//     index) and DFT_W over the wave index, twiddle W_(64 W)^(l kw); then
+//     two exchanges inside the wave's own 1024 slots: DFT16 over l_hi
+//     (l = l_lo + 4 l_hi), twiddle W_64^(l_lo ka), and four DFT4 over l_lo.
+//     Thread (w, l = g + 4 c) register h + 4 kb then holds element wl_perm.
 //   G, permuted -> natural: the same steps transposed in reverse order.
 // Both compute the forward DFT (the DFT matrix is symmetric).  Only F's
 // writes cross waves (each wave then reads its own 1024 slots) and only G's
@@ -808,13 +809,19 @@ __global__ void __launch_bounds__(NTB) k_tpair_half(const float2* __restrict__ A
 // writes and reads) plus, after another F, one before its writes (whose
 // slots that F's readers may still hold), against six for fft_regs; every
 // exchange inside a wave needs only a wave-level fence, as LDS operations
-// of one wave complete in order.  G always follows an F in the chain.  A row position p = t + 1024 r holds element p in the
-// natural layout and element wl_perm(p) in the permuted one, so tables read
-// in the permuted layout are permuted copies (k_wl_perm_table).
+// of one wave complete in order.  G always follows an F in the chain.  A
+// row position p = t + 64 W r holds element p in the natural layout and
+// element wl_perm(p) in the permuted one, so tables read in the permuted
+// layout are permuted copies (k_wl_perm_table).
 // ---------------------------------------------------------------------------
-__host__ __device__ __forceinline__ int wl_perm(int p) {
-    const int rho = p >> 10, t = p & 1023, w = t >> 6, mu = t & 63;
-    return w + 16 * (mu >> 2) + 1024 * (mu & 3) + 256 * (rho & 3) + 4096 * (rho >> 2);
+constexpr int clog2(int v) { return v <= 1 ? 0 : 1 + clog2(v >> 1); }
+
+__host__ __device__ __forceinline__ int wl_perm(int p, int lw) {
+    // lw = log2 W
+    const int rho = p >> (6 + lw), t = p & ((64 << lw) - 1), w = t >> 6, mu = t & 63;
+    const int g = mu & 3, c = mu >> 2, h = rho & 3, kb = rho >> 2;
+    const int k1 = (w << (4 - lw)) + (c >> lw), kw = c & ((1 << lw) - 1);
+    return k1 + 16 * (kw + (1 << lw) * (4 * g + h + 16 * kb));
 }
 
 __device__ __forceinline__ void wave_sync() {
@@ -824,32 +831,40 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // padded slot of element e: pad16(e); for the constant strides below
-// pad16(a + 64 j) = pad16(a) + 68 j and pad16(a + 1024 j) = pad16(a) + 1088 j
+// pad16(a + 64 j) = pad16(a) + 68 j and pad16(a + 64 W k) = pad16(a) + 68 W k.
 // after_f: the previous transform was an F, whose last reads (each wave in
 // its own slots) the writes below may overtake.  After a G no barrier is
 // needed: these writes go to the very slots this thread read last (G's
-// final read, element k1 1024 + t), which no other thread reads or writes.
+// final read, element k1 64 W + t), which no other thread reads or writes.
+template <int W>
 __device__ __forceinline__ void fft_wl_F(float2* s, float2 (&x)[16],
                                          const float2* __restrict__ lad, bool after_f) {
+    constexpr int NTB = 64 * W, n = 1024 * W;
     int tid = opaque_tid();
     dft_r<16>(x, false);
-    apply_twiddles<16>(x, lad, 16384, tid, false);
+    apply_twiddles<16>(x, lad, n, tid, false);
     if (after_f) __syncthreads();
     {
         float2* d = s + pad16(tid);
 #pragma unroll
-        for (int k1 = 0; k1 < 16; ++k1) d[k1 * 1088] = x[k1];
+        for (int k1 = 0; k1 < 16; ++k1) d[k1 * (NTB + NTB / 16)] = x[k1];
     }
     __syncthreads();
     tid = opaque_tid();
     const int w = tid >> 6, l = tid & 63;
     {
+        // this wave's own slots: register j = c1 W + (wave index)
         const float2* src = s + pad16(w * 1024 + l);
 #pragma unroll
         for (int j = 0; j < 16; ++j) x[j] = src[j * 68];
     }
-    dft_r<16>(x, false);
-    apply_twiddles<16>(x, lad, 1024, l, false);
+    if constexpr (W > 1) {
+#pragma unroll
+        for (int c1 = 0; c1 < 16 / W; ++c1) {
+            dft_r<W>(x + c1 * W, false);
+            apply_twiddles<W>(x + c1 * W, lad, NTB, l, false);
+        }
+    }
     // the reads above were of this wave's own slots (elements 1024 w + l +
     // 64 j), which no other wave touches until the next transform's first
     // barrier: a wave-level fence orders them before the writes below
@@ -858,10 +873,10 @@ __device__ __forceinline__ void fft_wl_F(float2* s, float2 (&x)[16],
     {
         float2* d = R + pad16(l);
 #pragma unroll
-        for (int kw = 0; kw < 16; ++kw) d[kw * 68] = x[kw];
+        for (int c = 0; c < 16; ++c) d[c * 68] = x[c];
     }
     wave_sync();
-    const int lo = l & 3, kq = l >> 2;  // l_lo and kw of this lane's next steps
+    const int lo = l & 3, kq = l >> 2;  // l_lo and the combination c of this lane's next steps
     {
         // element kq 64 + lo + 4 j at kq 68 + lo + 4 j + j / 4
         const float2* src = R + kq * 68 + lo;
@@ -892,8 +907,10 @@ __device__ __forceinline__ void fft_wl_F(float2* s, float2 (&x)[16],
     }
 }
 
+template <int W>
 __device__ __forceinline__ void fft_wl_G(float2* s, float2 (&x)[16],
                                          const float2* __restrict__ lad) {
+    constexpr int NTB = 64 * W, n = 1024 * W;
     int tid = opaque_tid();
     const int w = tid >> 6, l = tid & 63;
     const int lo = l & 3, kq = l >> 2;
@@ -928,10 +945,15 @@ __device__ __forceinline__ void fft_wl_G(float2* s, float2 (&x)[16],
     {
         const float2* src = R + pad16(l);
 #pragma unroll
-        for (int kw = 0; kw < 16; ++kw) x[kw] = src[kw * 68];
+        for (int c = 0; c < 16; ++c) x[c] = src[c * 68];
     }
-    apply_twiddles<16>(x, lad, 1024, l, false);
-    dft_r<16>(x, false);
+    if constexpr (W > 1) {
+#pragma unroll
+        for (int c1 = 0; c1 < 16 / W; ++c1) {
+            apply_twiddles<W>(x + c1 * W, lad, NTB, l, false);
+            dft_r<W>(x + c1 * W, false);
+        }
+    }
     // these writes go to this wave's own slots too (elements 1024 w + l +
     // 64 j); no other wave has read them since the last F's barrier
     wave_sync();
@@ -945,26 +967,27 @@ __device__ __forceinline__ void fft_wl_G(float2* s, float2 (&x)[16],
     {
         const float2* src = s + pad16(tid);
 #pragma unroll
-        for (int k1 = 0; k1 < 16; ++k1) x[k1] = src[k1 * 1088];
+        for (int k1 = 0; k1 < 16; ++k1) x[k1] = src[k1 * (NTB + NTB / 16)];
     }
-    apply_twiddles<16>(x, lad, 16384, tid, false);
+    apply_twiddles<16>(x, lad, n, tid, false);
     dft_r<16>(x, false);
 }
 
-// Gp[p] = Gn[wl_perm(p)], p < 16384 (the (G[2k], G[2k+1]) pairs of a range
-// kernel spectrum in the permuted layout)
-__global__ void k_wl_perm_table(float4* __restrict__ Gp, const float4* __restrict__ Gn) {
+// Gp[p] = Gn[wl_perm(p)], p < n = 1024 W (the (G[2k], G[2k+1]) pairs of a
+// range kernel spectrum in the permuted layout)
+__global__ void k_wl_perm_table(float4* __restrict__ Gp, const float4* __restrict__ Gn, int n,
+                                int lw) {
     const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p < 16384) Gp[p] = Gn[wl_perm(p)];
+    if (p < n) Gp[p] = Gn[wl_perm(p, lw)];
 }
 
-// k_tpair_half for n = 16384 with the wave-local transforms: the same
+// k_tpair_half with the wave-local transforms (n = 1024 W): the same
 // phases, with F where a transform takes natural input and G where it takes
 // permuted input (plain: F G F F G F; cropped: F G F G F F).  Gn is the
 // permuted spectrum (k_wl_perm_table), the output planes are written in the
 // permuted layout, and the inputs, m(j) and the cropped z stay natural.
-template <bool MT = false, bool CROP = false>
-__global__ void __launch_bounds__(1024) k_tpair_wl(const float2* __restrict__ A, int P,
+template <int W, bool MT = false, bool CROP = false>
+__global__ void __launch_bounds__(64 * W) k_tpair_wl(const float2* __restrict__ A, int P,
                                                    const int* __restrict__ pbase,
                                                    const int* __restrict__ pwin,
                                                    const float4* __restrict__ Gn,
@@ -972,7 +995,7 @@ __global__ void __launch_bounds__(1024) k_tpair_wl(const float2* __restrict__ A,
                                                    const float2* __restrict__ lad,
                                                    const float4* const* __restrict__ Gpair,
                                                    const int* __restrict__ npairs, int nr) {
-    constexpr int NTB = 1024, n = 16384;
+    constexpr int NTB = 64 * W, n = 1024 * W;
     __shared__ float2 s[n + n / 16];
     const int p = blockIdx.x;
     if (MT) {
@@ -994,8 +1017,8 @@ __global__ void __launch_bounds__(1024) k_tpair_wl(const float2* __restrict__ A,
 #pragma unroll
                 for (int q = 0; q < PPT; ++q) x[q] = h[q];
             }
-            if (ph == 1 || ph == 3) fft_wl_G(s, x, lad);
-            else fft_wl_F(s, x, lad, ph == 5);
+            if (ph == 1 || ph == 3) fft_wl_G<W>(s, x, lad);
+            else fft_wl_F<W>(s, x, lad, ph == 5);
             const int tid = opaque_tid();
 #pragma unroll
             for (int q = 0; q < PPT; ++q) {
@@ -1028,8 +1051,8 @@ __global__ void __launch_bounds__(1024) k_tpair_wl(const float2* __restrict__ A,
 #pragma unroll 1
     for (int ph = 0; ph < 7; ++ph) {
         if (ph == 0 || ph == 3) load_window<NTB>(x, a, base, win, ph == 3 ? mod : nullptr);
-        if (ph == 1 || ph == 4) fft_wl_G(s, x, lad);
-        else if (ph != 6) fft_wl_F(s, x, lad, ph == 3);
+        if (ph == 1 || ph == 4) fft_wl_G<W>(s, x, lad);
+        else if (ph != 6) fft_wl_F<W>(s, x, lad, ph == 3);
         const int tid = opaque_tid();
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
@@ -1119,9 +1142,10 @@ __global__ void __launch_bounds__(256) k_sum_final(float2* __restrict__ S,
     const int sg = blockIdx.y;
     if (seg_beg[sg] == seg_end[sg]) return;  // padding segment
     if (seg_scale) scale = seg_scale[sg];   // segments of several tracks
-    // wl: rows in the permuted layout of k_tpair_wl (planes of n = P / 2:
-    // position k / 2 of plane k % 2 holds frequency 2 wl_perm(k / 2) + k % 2)
-    const int f = wl ? 2 * wl_perm(k >> 1) + (k & 1) : k;
+    // wl = 1 + log2 W: rows in the permuted layout of k_tpair_wl (planes of
+    // n = P / 2: position k / 2 of plane k % 2 holds frequency
+    // 2 wl_perm(k / 2) + k % 2)
+    const int f = wl ? 2 * wl_perm(k >> 1, wl - 1) + (k & 1) : k;
     const int kd = pN1 > 0 ? (f % pN2) * pN1 + f / pN2 : f;
     float2* d = S + (size_t)seg_shot[sg] * P + kd;
     if (seq) {
@@ -1281,11 +1305,17 @@ void target_spectra_fused(erp_ctx* ctx, const FftPlan& p, float2* A, float2* Y,
 
 int target_half_slice() { return SLICE; }
 
-bool target_wl(const FftPlan& p) { return p.N == 32768; }
+// the wave-local chain from n = 8192 (one or two workgroups per CU, whose
+// barriers idle it); below, several workgroups share a CU and hide the
+// barriers of k_tpair_half, which is then 2-3 % faster (fBm geometry,
+// 50 shots: n = 8192 13.4 against 14.7 ms; 4096 6.58 against 6.39)
+bool target_wl(const FftPlan& p) { return p.N / 2 >= 8192 && p.N / 2 <= 16384; }
 
-void wl_perm_table(const float2* Gnat, float2* Gwl, hipStream_t st) {
-    hipLaunchKernelGGL(k_wl_perm_table, dim3(16384 / 256), dim3(256), 0, st,
-                       reinterpret_cast<float4*>(Gwl), reinterpret_cast<const float4*>(Gnat));
+void wl_perm_table(const float2* Gnat, float2* Gwl, int P, hipStream_t st) {
+    const int n = P / 2;
+    hipLaunchKernelGGL(k_wl_perm_table, dim3((n + 255) / 256), dim3(256), 0, st,
+                       reinterpret_cast<float4*>(Gwl), reinterpret_cast<const float4*>(Gnat), n,
+                       clog2(n / 1024));
     ERP_CHECK_LAUNCH("k_wl_perm_table");
 }
 
@@ -1336,8 +1366,17 @@ void target_pairs_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const in
          k_tpair_half<NTB, false, true>, k_tpair_half<NTB, true, true>, NTB)
     if (target_wl(p)) {
         // the wave-local transforms (Gnat in their layout)
-        pick(k_tpair_wl<false, false>, k_tpair_wl<true, false>, k_tpair_wl<false, true>,
-             k_tpair_wl<true, true>, 1024);
+#define ERP_TPAIR_WL(W)                                                                \
+    pick(k_tpair_wl<W, false, false>, k_tpair_wl<W, true, false>,                     \
+         k_tpair_wl<W, false, true>, k_tpair_wl<W, true, true>, 64 * W)
+        switch (p.N / 2) {
+            case 1024: ERP_TPAIR_WL(1); break;
+            case 2048: ERP_TPAIR_WL(2); break;
+            case 4096: ERP_TPAIR_WL(4); break;
+            case 8192: ERP_TPAIR_WL(8); break;
+            default: ERP_TPAIR_WL(16); break;
+        }
+#undef ERP_TPAIR_WL
         ERP_CHECK_LAUNCH("k_tpair_wl");
         return;
     }
@@ -1366,6 +1405,7 @@ void target_sum_half(const FftPlan& p, const float2* work, float2* partial, cons
     ERP_CHECK_LAUNCH("k_sum_slices");
     hipLaunchKernelGGL(k_sum_final, dim3((p.N + 255) / 256, nseg), dim3(256), 0, st, S, partial,
                        seg_shot, seg_beg, seg_end, p.N, scale, p.two_pass ? p.N1 : 0,
-                       p.two_pass ? p.N2 : 0, seg_scale, seq ? 1 : 0, wl ? 1 : 0);
+                       p.two_pass ? p.N2 : 0, seg_scale, seq ? 1 : 0,
+                       wl ? 1 + clog2(p.N / 2048) : 0);
     ERP_CHECK_LAUNCH("k_sum_final");
 }

@@ -284,12 +284,12 @@ void fft_init_twiddles(erp_ctx* ctx);
 // next power of two of nr, from 1024 to 16384 (nr < n: the cropped variant)
 bool target_half_supported(const FftPlan& p, int nr);
 int target_half_slice();
-// P = 32768: the chain runs with the wave-local transforms (k_tpair_wl); its
-// range kernel spectra are then in that kernel's layout (wl_perm_table) and
-// its rows are summed with target_sum_half(..., wl = true)
+// the chain runs with the wave-local transforms (k_tpair_wl): its range
+// kernel spectra are then in that kernel's layout (wl_perm_table) and its
+// rows are summed with target_sum_half(..., wl = true)
 bool target_wl(const FftPlan& p);
 // Gwl = Gnat in the layout of k_tpair_wl (P complex64 each)
-void wl_perm_table(const float2* Gnat, float2* Gwl, hipStream_t st);
+void wl_perm_table(const float2* Gnat, float2* Gwl, int P, hipStream_t st);
 // Gpair / npairs_dev (multi-track batches): per-pair range kernel spectra
 // and the pair count on the device, n then being the grid's bound
 void target_pairs_half(erp_ctx* ctx, const FftPlan& p, const float2* A, const int* pbase,

@@ -115,8 +115,8 @@ struct TrackK {
     int T;
     const unsigned char* ttype;
     const float2* g_surf;          // surface range kernel spectrum (fft order)
-    const float2* g_targ_nat;      // target range kernel spectrum (natural order; at
-                                   // P = 32768 in the layout of k_tpair_wl)
+    const float2* g_targ_nat;      // target range kernel spectrum (natural order, or
+                                   // in the layout of k_tpair_wl where it runs)
     const float2* blue_w;          // Bluestein tables of the track's band
     const float2* blue_Bw;
     const unsigned char* blue_mask;
@@ -3313,7 +3313,7 @@ const float2* get_chirp(erp_ctx* ctx, int nr, float dr, float res, int kind, flo
     if (layout == 1) return c->nat.as<float2>();
     if (!c->wl.p) {
         c->wl.ensure(sizeof(float2) * (size_t)P);
-        wl_perm_table(c->nat.as<float2>(), c->wl.as<float2>(), st);
+        wl_perm_table(c->nat.as<float2>(), c->wl.as<float2>(), P, st);
         ERP_HIP_TRY(hipStreamSynchronize(st));
     }
     return c->wl.as<float2>();
@@ -3880,7 +3880,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         fft_natural_order(planP, spec_, nat, st);
         if (layout == 1) return nat;
         float2* wl = arena->take_n<float2>((size_t)P, "chirp spectrum");
-        wl_perm_table(nat, wl, st);
+        wl_perm_table(nat, wl, P, st);
         return wl;
     };
     const float2* g_surf = nullptr;
