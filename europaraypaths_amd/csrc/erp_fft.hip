@@ -858,7 +858,11 @@ __device__ __forceinline__ void fft_wl_F(float2* s, float2 (&x)[16],
 #pragma unroll
         for (int j = 0; j < 16; ++j) x[j] = src[j * 68];
     }
-    if constexpr (W > 1) {
+    // (W = 16 on the whole array: the grouped form costs it registers)
+    if constexpr (W == 16) {
+        dft_r<16>(x, false);
+        apply_twiddles<16>(x, lad, NTB, l, false);
+    } else if constexpr (W > 1) {
 #pragma unroll
         for (int c1 = 0; c1 < 16 / W; ++c1) {
             dft_r<W>(x + c1 * W, false);
@@ -947,7 +951,10 @@ __device__ __forceinline__ void fft_wl_G(float2* s, float2 (&x)[16],
 #pragma unroll
         for (int c = 0; c < 16; ++c) x[c] = src[c * 68];
     }
-    if constexpr (W > 1) {
+    if constexpr (W == 16) {
+        apply_twiddles<16>(x, lad, NTB, l, false);
+        dft_r<16>(x, false);
+    } else if constexpr (W > 1) {
 #pragma unroll
         for (int c1 = 0; c1 < 16 / W; ++c1) {
             apply_twiddles<W>(x + c1 * W, lad, NTB, l, false);
