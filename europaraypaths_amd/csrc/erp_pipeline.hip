@@ -524,63 +524,78 @@ __device__ __forceinline__ int64_t facet_of(int blk, int j, int tile_nx) {
     return ((int64_t)trow * 32 + (j >> 6)) * tile_nx + (int64_t)tcol * 64 + (j & 63);
 }
 
+// sb shots per workgroup (blockIdx.y: shots [sb y, sb y + sb) of the batch):
+// each facet is loaded once for all of them.  Multi-track rows each have
+// their own mesh and take sb = 1.
+constexpr int MASK_SB = 8;
 __global__ void __launch_bounds__(NT) k_illum_mask(
     const float* __restrict__ mesh, int64_t Fpad, int64_t F,
     const erp_shot* __restrict__ shots, double cos_ap,
     unsigned long long* __restrict__ mask, int* __restrict__ cnt, int nblk, int tile_nx,
-    const TrackK* __restrict__ trk, const RowK* __restrict__ rows) {
-    const int blk = blockIdx.x, s = blockIdx.y;
+    const TrackK* __restrict__ trk, const RowK* __restrict__ rows, int S, int sb) {
+    const int blk = blockIdx.x;
+    const int s0 = blockIdx.y * sb, s1 = min(s0 + sb, S);
     if (trk) {
         // multi-track: the row's mesh; blocks past it and padding rows
         // (shot -1) hold nothing
-        const TrackK& t = trk[rows[s].trk];
-        if (blk >= t.nblk || rows[s].shot < 0) return;
+        const TrackK& t = trk[rows[s0].trk];
+        if (blk >= t.nblk || rows[s0].shot < 0) return;
         mesh = t.mesh;
         Fpad = t.Fpad;
         F = t.F;
         cos_ap = t.kp.cos_ap;
         tile_nx = t.tile_nx;
     }
-    const ShotK sh = load_shot(shots, s);
-    const float sxf = (float)sh.sx, syf = (float)sh.sy, szf = (float)sh.sz;
     const float cosf_ap = (float)cos_ap;
     const float* X = mesh;
     const float* Y = mesh + Fpad;
     const float* Z = mesh + 2 * Fpad;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    __shared__ int wc[WPB];
-    int total = 0;
-    const float srcmag = fabsf(sxf) + fabsf(syf) + fabsf(szf);
+    __shared__ int wc[MASK_SB][WPB];
+    float fx[FPT], fy[FPT], fz[FPT];
+    bool in[FPT];
 #pragma unroll
     for (int q = 0; q < FPT; ++q) {
         const int64_t i = facet_of(blk, q * NT + threadIdx.x, tile_nx);
-        bool p = false;
-        if (i < F) {
-            const float fx = X[i], fy = Y[i], fz = Z[i];
-            const float dx = fx - sxf, dy = fy - syf, dz = fz - szf;
-            const float d2 = dx * dx + dy * dy + dz * dz;
-            const float rinv = rsqrtf(d2);
-            const float c = -(dx * sh.snx + dy * sh.sny + dz * sh.snz) * rinv;
-            const float margin =
-                1e-5f + 4e-7f * (srcmag + fabsf(fx) + fabsf(fy) + fabsf(fz)) * rinv;
-            if (c < cosf_ap - margin) {
-                p = false;
-            } else if (c > cosf_ap + margin && c < 1.0f - margin) {
-                p = true;
-            } else {
-                p = illum_exact(fx, fy, fz, sh, cos_ap);
-            }
-        }
-        const unsigned long long b = __ballot(p);
-        if (lane == 0) mask[((size_t)s * nblk + blk) * MASKW + q * WPB + wv] = b;
-        total += __popcll(b);
+        in[q] = i < F;
+        fx[q] = in[q] ? X[i] : 0.f;
+        fy[q] = in[q] ? Y[i] : 0.f;
+        fz[q] = in[q] ? Z[i] : 0.f;
     }
-    if (lane == 0) wc[wv] = total;
+    for (int s = s0; s < s1; ++s) {
+        const ShotK sh = load_shot(shots, s);
+        const float sxf = (float)sh.sx, syf = (float)sh.sy, szf = (float)sh.sz;
+        const float srcmag = fabsf(sxf) + fabsf(syf) + fabsf(szf);
+        int total = 0;
+#pragma unroll
+        for (int q = 0; q < FPT; ++q) {
+            bool p = false;
+            if (in[q]) {
+                const float dx = fx[q] - sxf, dy = fy[q] - syf, dz = fz[q] - szf;
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                const float rinv = rsqrtf(d2);
+                const float c = -(dx * sh.snx + dy * sh.sny + dz * sh.snz) * rinv;
+                const float margin =
+                    1e-5f + 4e-7f * (srcmag + fabsf(fx[q]) + fabsf(fy[q]) + fabsf(fz[q])) * rinv;
+                if (c < cosf_ap - margin) {
+                    p = false;
+                } else if (c > cosf_ap + margin && c < 1.0f - margin) {
+                    p = true;
+                } else {
+                    p = illum_exact(fx[q], fy[q], fz[q], sh, cos_ap);
+                }
+            }
+            const unsigned long long b = __ballot(p);
+            if (lane == 0) mask[((size_t)s * nblk + blk) * MASKW + q * WPB + wv] = b;
+            total += __popcll(b);
+        }
+        if (lane == 0) wc[s - s0][wv] = total;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < s1 - s0) {
         int t = 0;
-        for (int w = 0; w < WPB; ++w) t += wc[w];
-        cnt[(size_t)s * nblk + blk] = t;
+        for (int w = 0; w < WPB; ++w) t += wc[threadIdx.x][w];
+        cnt[(size_t)(s0 + threadIdx.x) * nblk + blk] = t;
     }
 }
 
@@ -4099,9 +4114,10 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
         StageTimer& PT_ = timer ? *timer : none_;
         // 1-2: aperture mask, counts, offsets
         PT_.begin(ERP_STAGE_MASK, st, (double)F * S);
-        hipLaunchKernelGGL(k_illum_mask, dim3(nblk, S), dim3(NT), 0, st, mesh->soa.as<float>(),
-                           mesh->Fpad, F, dshots, p.cos_ap, dmask, W.cnt.as<int>(), nblk,
-                           tile_nx, (const TrackK*)nullptr, (const RowK*)nullptr);
+        hipLaunchKernelGGL(k_illum_mask, dim3(nblk, (S + MASK_SB - 1) / MASK_SB), dim3(NT), 0, st,
+                           mesh->soa.as<float>(), mesh->Fpad, F, dshots, p.cos_ap, dmask,
+                           W.cnt.as<int>(), nblk, tile_nx, (const TrackK*)nullptr,
+                           (const RowK*)nullptr, S, MASK_SB);
         ERP_CHECK_LAUNCH("k_illum_mask");
         hipLaunchKernelGGL(k_scan_blocks, dim3(S), dim3(1024), 0, st, W.cnt.as<int>(),
                            W.off.as<int>(), dV, nblk, (const TrackK*)nullptr,
@@ -4998,7 +5014,7 @@ void mt_run_group(erp_ctx* ctx, const erp_track* tracks, const std::vector<int64
             hipLaunchKernelGGL(k_illum_mask, dim3(nblk_max, R), dim3(NT), 0, st,
                                (const float*)nullptr, (int64_t)0, (int64_t)0, dshots, 0.0,
                                m.mask.as<unsigned long long>(), m.cnt.as<int>(), nblk_max, 0,
-                               dtrk, drows);
+                               dtrk, drows, R, 1);
             ERP_CHECK_LAUNCH("k_illum_mask");
             hipLaunchKernelGGL(k_scan_blocks, dim3(R), dim3(1024), 0, st, m.cnt.as<int>(),
                                m.off.as<int>(), dV, nblk_max, dtrk, drows);
