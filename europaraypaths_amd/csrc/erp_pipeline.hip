@@ -1401,14 +1401,17 @@ __global__ void __launch_bounds__(1024) k_target_select(
         cos_ap = tk.kp.cos_ap;
     }
     const ShotK sh = load_shot(shots, s);
-    __shared__ int buf1[1024], buf0[1024];
-    __shared__ int c1, c0;
-    if (threadIdx.x == 0) { c1 = 0; c0 = 0; }
-    __syncthreads();
+    // each chunk of 1024 targets: a wave's rank from its ballots, the waves'
+    // counts through LDS (two buffers by chunk parity, so one barrier per
+    // chunk), the running totals in registers
+    __shared__ int wn[2][2][16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    int run1 = 0, run0 = 0;
     const bool live = Vs[s] > 0;
-    for (int base = 0; base < T; base += 1024) {
+    for (int base = 0, it = 0; base < T; base += 1024, ++it) {
         const int t = base + threadIdx.x;
-        int in1 = 0, in0 = 0;
+        bool in1 = false, in0 = false;
         if (live && t < T) {
             const double vx = (double)tg[t] - sh.sx, vy = (double)tg[Tpad + t] - sh.sy,
                          vz = (double)tg[2 * Tpad + t] - sh.sz;
@@ -1417,29 +1420,34 @@ __global__ void __launch_bounds__(1024) k_target_select(
             const double dp = (-(double)sh.snx) * ux + (-(double)sh.sny) * uy +
                               (-(double)sh.snz) * uz;
             if (!(dp < cos_ap)) {
-                if (ttype[t] != 0) in1 = 1; else in0 = 1;
+                if (ttype[t] != 0) in1 = true; else in0 = true;
             }
         }
-        buf1[threadIdx.x] = in1;
-        buf0[threadIdx.x] = in0;
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            const int a = (threadIdx.x >= o) ? buf1[threadIdx.x - o] : 0;
-            const int b = (threadIdx.x >= o) ? buf0[threadIdx.x - o] : 0;
-            __syncthreads();
-            buf1[threadIdx.x] += a;
-            buf0[threadIdx.x] += b;
-            __syncthreads();
+        const unsigned long long b1 = __ballot(in1), b0 = __ballot(in0);
+        int* n1 = wn[it & 1][0];
+        int* n0 = wn[it & 1][1];
+        if (lane == 0) {
+            n1[wv] = __popcll(b1);
+            n0[wv] = __popcll(b0);
         }
-        if (in1) list1[(size_t)s * Tcap + c1 + buf1[threadIdx.x] - 1] = t;
-        if (in0) list0[(size_t)s * Tcap + c0 + buf0[threadIdx.x] - 1] = t;
         __syncthreads();
-        if (threadIdx.x == 1023) { c1 += buf1[1023]; c0 += buf0[1023]; }
-        __syncthreads();
+        int o1 = run1 + __popcll(b1 & below), o0 = run0 + __popcll(b0 & below);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int a = n1[k], b = n0[k];
+            if (k < wv) {
+                o1 += a;
+                o0 += b;
+            }
+            run1 += a;
+            run0 += b;
+        }
+        if (in1) list1[(size_t)s * Tcap + o1] = t;
+        if (in0) list0[(size_t)s * Tcap + o0] = t;
     }
     if (threadIdx.x == 0) {
-        tcnt[2 * s] = c1;
-        tcnt[2 * s + 1] = c0;
+        tcnt[2 * s] = run1;
+        tcnt[2 * s + 1] = run0;
     }
 }
 
