@@ -132,6 +132,21 @@ def test_noalloc_cropped_chain_and_deterministic_specular(gpu_ctx):
     assert np.array_equal(g.view(np.uint32), r.view(np.uint32))
 
 
+def test_noalloc_wave_local_chain(gpu_ctx):
+    """nr = 6000 (n = 8192, the cropped wave-local chain, whose permuted
+    range kernel spectrum the run builds in the scratch array): erp_run_dev's
+    traces within tolerance, and its bits in the deterministic mode."""
+    par, soa, tg, shots = _scene(64, 10.0, 6000, 6, ntg=4)
+    assert par.nr == 6000
+    print("noalloc wave-local chain, nr 6000", flush=True)
+    (g, gv), (r, rv) = _both(gpu_ctx, par, soa, tg, shots)
+    np.testing.assert_array_equal(gv, rv)
+    compare_traces(g, r)
+    (g, gv), (r, rv) = _both(gpu_ctx, par, soa, tg, shots, flags=ERP_RUN_DETERMINISTIC)
+    np.testing.assert_array_equal(gv, rv)
+    assert np.array_equal(g.view(np.uint32), r.view(np.uint32))
+
+
 def test_noalloc_shrinks_batches_to_fit(gpu_ctx):
     """nr = 16384 (256 KB pair rows: the launch takes fewer pairs) and a
     mesh of 331 776 facets lit by every shot (their records, 20 MB per shot,
