@@ -59,7 +59,10 @@ def _tracks():
     par, soa, shots = scene(rough=0.3, seed=8, nr=1024, h=3000.0, rst=2600.0, ns=6, eps_3=3.4,
                             z_interface=-200.0)
     out.append(("nr1024_second_medium", par, soa, _grid_targets(2, 2, 100.0, 320.0), shots))
-    # 16384 bins: the multi-track chain with the wave-local transforms
+    # 8192 and 16384 bins: the multi-track chain with the wave-local
+    # transforms (8 and 16 waves)
+    par, soa, shots = scene(rough=0.3, seed=13, nr=8192, h=3000.0, rst=2600.0, ns=3)
+    out.append(("nr8192", par, soa, _grid_targets(2, 1, 100.0, 300.0), shots))
     par, soa, shots = scene(rough=0.3, seed=12, nr=16384, h=3000.0, rst=2600.0, ns=3)
     out.append(("nr16384", par, soa, _grid_targets(2, 1, 100.0, 300.0), shots))
     # alone: type-0 targets, and a range length without the per-pair chain
