@@ -700,6 +700,8 @@ def main():
     ap.add_argument("--single-shot", action="store_true",
                     help="the single-shot target kernel (ERP_TUNE_SINGLE_SHOT) instead of "
                          "the shot-pair kernel")
+    ap.add_argument("--shots-per-batch", type=int, default=0,
+                    help="cap on shots per batch (ERP_TUNE_SHOTS_PER_BATCH; 0: the default)")
     ap.add_argument("--ingest-pmc", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "BENCH_JSON"),
                     help="write profiles/traffic.json from two rocprofv3 --pmc runs")
     ap.add_argument("--trace-summary", nargs=4,
@@ -774,6 +776,9 @@ def main():
         from europaraypaths_amd._abi import ERP_TUNE_SINGLE_SHOT
         ctx.set_tuning(ERP_TUNE_SINGLE_SHOT, 1)
         KERNELS["target_accum"] = "k_target_accum_fast"
+    if a.shots_per_batch:
+        from europaraypaths_amd._abi import ERP_TUNE_SHOTS_PER_BATCH
+        ctx.set_tuning(ERP_TUNE_SHOTS_PER_BATCH, a.shots_per_batch)
     mesh = ctx.upload_mesh(soa)
     targets = ctx.upload_targets(*tg) if tg is not None else None
     h = hip()

@@ -54,6 +54,10 @@ constexpr int NTT = 512;       // threads per workgroup of the target kernels
 // workgroups a small part of each launch and need fewer host round trips.
 constexpr int64_t kPairRowBytes = (int64_t)6144 << 20;
 constexpr int kMaxPairsPerLaunch = 16384;
+// Shots per batch (default and maximum).  Each batch waits on the host a
+// few times (illuminated counts, target counts and lists), so a 50-shot
+// step runs as one batch rather than two.
+constexpr int kShotsPerBatch = 64;
 // bin window of k_target_accum_fast: FP64, 79 KB of LDS, so two workgroups
 // fill a CU's 160 KB; the shot bounds of the 2048^2 fBm workload allow at
 // most about 4.7e3 bins per pair
@@ -3784,14 +3788,14 @@ TargetAccumFn pick_target_accum(bool det, int ph, int acc, bool s3) {
 }
 
 // Device bytes of one run, following the allocations of erp_run_impl: the
-// work buffers of one batch of up to 32 shots (facet records counted for
+// work buffers of one batch of up to kShotsPerBatch shots (facet records counted for
 // every facet illuminated by every shot of the batch, an upper bound) plus
 // the ns x nr output and counts.
 int64_t erp_footprint_impl(const erp_params* par, int64_t F, int64_t T, int64_t ns) {
     const int64_t nr = par->nr;
     const int64_t P = next_pow2(2 * nr - 1);
     const int64_t M = next_pow2(2 * (2 * nr - 1) - 1);
-    const int64_t S = std::max<int64_t>(1, std::min<int64_t>(ns, 32));
+    const int64_t S = std::max<int64_t>(1, std::min<int64_t>(ns, kShotsPerBatch));
     const int64_t nblk = (F + FB - 1) / FB;
     const bool spec = par->mode & ERP_MODE_SPECULAR;
     int64_t b = 0;
@@ -3944,7 +3948,7 @@ void erp_run_impl(erp_ctx* ctx, const erp_params* par, const erp_mesh* mesh,
     }
 
     // batch size: bounded by memory of per-shot buffers
-    int Smax = 32;
+    int Smax = kShotsPerBatch;
     if (ctx->tune_shots > 0) Smax = (int)std::min<int64_t>(Smax, ctx->tune_shots);
     if ((int64_t)P * 8 * 4 * Smax > (int64_t)1 << 30) Smax = std::max<int>(1, (int)(((int64_t)1 << 30) / ((int64_t)P * 32)));
     const int64_t Tpad = targets ? targets->Tpad : 0;

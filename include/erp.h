@@ -329,7 +329,7 @@ int erp_profile_read(erp_ctx* ctx, int stage, double* ms, int64_t* launches, dou
  * can only lower the defaults, so erp_run_footprint stays an upper bound.  No
  * counterpart in the reference (its sim runs every shot in its own loop
  * iteration, sim.cu:615-1008).
- *   ERP_TUNE_SHOTS_PER_BATCH : shots per batch (default and maximum 32)
+ *   ERP_TUNE_SHOTS_PER_BATCH : shots per batch (default and maximum 64)
  *   ERP_TUNE_PAIRS_PER_LAUNCH: (shot, target) pairs per target launch
  *                              (default: the pairs whose rows fit 6 GB,
  *                              at most 16384; with ERP_RUN_DETERMINISTIC
